@@ -1,0 +1,257 @@
+"""Seeded synthetic inputs for the spatio-temporal Gaussian model.
+
+The reference ships its example inputs as git-lfs stubs, so every workload
+here is generated.  The construction follows the reference's model family
+(spatio-temporal SPDE submodel + regression submodel + Gaussian likelihood,
+`core/model.py:40-304`, `submodels/spatio_temporal.py:20-60`,
+`submodels/regression.py:14-26`):
+
+* spatial mesh: an m x m regular grid on the unit square, every cell split
+  into two right triangles along the (0,0)-(1,1) diagonal; node index
+  ``s = iy * m + ix``;
+* P1 FEM: lumped mass ``c0`` (diagonal, area/3 per incident triangle),
+  stiffness ``g1``, ``g2 = g1 c0^-1 g1``, ``g3 = g1 c0^-1 g2`` (the reference
+  loads all four, `spatio_temporal.py:27-30`);
+* temporal matrices for nt steps with dt = 1: ``m0`` the 1-D P1 mass,
+  ``m1 = diag(1/2, 0, ..., 0, 1/2)`` (boundary term), ``m2`` the 1-D
+  stiffness;
+* observations at uniform points in [0,1]^2 x [0, nt); the spatio-temporal
+  design row holds barycentric weights times linear-in-time weights (6
+  entries), the regression design block holds N(0,1) covariates;
+* ``y = A_st x_st + X beta + eps`` with a smooth random space-time field
+  ``x_st`` (a sum of random Fourier modes; drawing from the SPDE prior at
+  the full size would need a factorisation of Q_prior just to make data),
+  ``beta ~ N(0,1)`` and ``eps ~ N(0, exp(-theta_obs))``.
+
+Everything here is host-side numpy/scipy set-up data, built once per model.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+import scipy.sparse as sps
+
+
+@dataclasses.dataclass
+class PriorSpec:
+    """Hyper-prior of one hyperparameter (`prior_hyperparameters/*.py`)."""
+
+    kind: str  # "penalized_complexity" or "gaussian"
+    alpha: float = 0.0
+    u: float = 0.0
+    mean: float = 0.0
+    precision: float = 0.5
+
+
+@dataclasses.dataclass
+class STModelData:
+    """One spatio-temporal Gaussian model in DALIA's latent ordering.
+
+    Latent vector: ``x = [x_st (time-major, index t*ns + s), beta (nb)]``,
+    i.e. the spatio-temporal submodel first and the fixed effects last, as
+    `core/model.py:49-55` orders them so that the fixed effects form the
+    arrowhead.
+    """
+
+    ns: int
+    nt: int
+    nb: int
+    c0: sps.csr_matrix
+    g1: sps.csr_matrix
+    g2: sps.csr_matrix
+    g3: sps.csr_matrix
+    m0: sps.csr_matrix
+    m1: sps.csr_matrix
+    m2: sps.csr_matrix
+    a_st: sps.csr_matrix  # n_obs x (ns*nt)
+    x_cov: np.ndarray  # n_obs x nb
+    y: np.ndarray  # n_obs
+    theta: np.ndarray  # (r_s, r_t, sigma_st, prec_o)
+    priors: list  # PriorSpec per theta entry, same order
+    fe_prec: float = 1e-3
+    manifold: str = "plane"
+
+    @property
+    def n_obs(self) -> int:
+        return int(self.y.shape[0])
+
+    @property
+    def n_latent(self) -> int:
+        return self.ns * self.nt + self.nb
+
+    def a_full(self) -> sps.csr_matrix:
+        """Design matrix ``[A_st | X]`` (`core/model.py:212-233`)."""
+        return sps.hstack([self.a_st, sps.csr_matrix(self.x_cov)], format="csr")
+
+
+# --------------------------------------------------------------------------
+# mesh and FEM
+# --------------------------------------------------------------------------
+
+def grid_triangles(m: int) -> tuple[np.ndarray, np.ndarray]:
+    """Node coordinates (m*m, 2) and triangles (2*(m-1)^2, 3) of the grid."""
+    h = 1.0 / (m - 1)
+    ix, iy = np.meshgrid(np.arange(m), np.arange(m), indexing="xy")
+    pts = np.stack([ix.ravel() * h, iy.ravel() * h], axis=1)
+    cx, cy = np.meshgrid(np.arange(m - 1), np.arange(m - 1), indexing="xy")
+    cx = cx.ravel()
+    cy = cy.ravel()
+    a = cy * m + cx
+    b = a + 1
+    c = a + m
+    d = c + 1
+    tri = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)], 0)
+    return pts, tri
+
+
+def p1_fem(pts: np.ndarray, tri: np.ndarray):
+    """Lumped mass c0 and stiffness g1 of P1 elements (CSR, symmetric)."""
+    n = pts.shape[0]
+    p0, p1, p2 = pts[tri[:, 0]], pts[tri[:, 1]], pts[tri[:, 2]]
+    e0 = p2 - p1  # edge opposite vertex 0
+    e1 = p0 - p2
+    e2 = p1 - p0
+    area = 0.5 * np.abs(e2[:, 0] * (-e1[:, 1]) - e2[:, 1] * (-e1[:, 0]))
+    edges = [e0, e1, e2]
+    rows, cols, vals = [], [], []
+    for i in range(3):
+        for j in range(3):
+            kij = (edges[i][:, 0] * edges[j][:, 0] + edges[i][:, 1] * edges[j][:, 1]) / (4.0 * area)
+            rows.append(tri[:, i])
+            cols.append(tri[:, j])
+            vals.append(kij)
+    g1 = sps.coo_matrix(
+        (np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n)
+    ).tocsr()
+    g1.sum_duplicates()
+    lump = np.zeros(n)
+    for i in range(3):
+        np.add.at(lump, tri[:, i], area / 3.0)
+    c0 = sps.diags(lump, format="csr")
+    g1 = 0.5 * (g1 + g1.T)
+    return c0.tocsr(), g1.tocsr()
+
+
+def temporal_matrices(nt: int, dt: float = 1.0):
+    """1-D P1 mass m0, boundary m1, stiffness m2 on nt nodes (CSR)."""
+    main0 = np.full(nt, 2.0 * dt / 3.0)
+    main0[0] = main0[-1] = dt / 3.0
+    off0 = np.full(nt - 1, dt / 6.0)
+    m0 = sps.diags([off0, main0, off0], [-1, 0, 1], format="csr")
+    d1 = np.zeros(nt)
+    d1[0] = d1[-1] = 0.5
+    m1 = sps.diags(d1, format="csr")
+    main2 = np.full(nt, 2.0 / dt)
+    main2[0] = main2[-1] = 1.0 / dt
+    off2 = np.full(nt - 1, -1.0 / dt)
+    m2 = sps.diags([off2, main2, off2], [-1, 0, 1], format="csr")
+    return m0, m1, m2
+
+
+# --------------------------------------------------------------------------
+# observations
+# --------------------------------------------------------------------------
+
+def design_matrix(m: int, nt: int, s_xy: np.ndarray, t: np.ndarray) -> sps.csr_matrix:
+    """Barycentric-in-space times linear-in-time interpolation rows."""
+    ns = m * m
+    n_obs = s_xy.shape[0]
+    fx = np.clip(s_xy[:, 0] * (m - 1), 0.0, m - 1 - 1e-12)
+    fy = np.clip(s_xy[:, 1] * (m - 1), 0.0, m - 1 - 1e-12)
+    cx = np.minimum(np.floor(fx).astype(np.int64), m - 2)
+    cy = np.minimum(np.floor(fy).astype(np.int64), m - 2)
+    u = fx - cx
+    v = fy - cy
+    a = cy * m + cx
+    b = a + 1
+    c = a + m
+    d = c + 1
+    lower = u >= v
+    n1 = np.where(lower, b, c)
+    w0 = np.where(lower, 1.0 - u, 1.0 - v)
+    w1 = np.where(lower, u - v, v - u)
+    w2 = np.where(lower, v, u)
+    nodes = np.stack([a, n1, d], 1)
+    ws = np.stack([w0, w1, w2], 1)
+    t0 = np.minimum(np.floor(t).astype(np.int64), nt - 2)
+    ft = np.clip(t - t0, 0.0, 1.0)
+    wt = np.stack([1.0 - ft, ft], 1)
+    tt = np.stack([t0, t0 + 1], 1)
+    cols = (tt[:, :, None] * ns + nodes[:, None, :]).reshape(n_obs, 6)
+    vals = (wt[:, :, None] * ws[:, None, :]).reshape(n_obs, 6)
+    rows = np.repeat(np.arange(n_obs), 6)
+    A = sps.csr_matrix((vals.ravel(), (rows, cols.ravel())), shape=(n_obs, ns * nt))
+    A.sum_duplicates()
+    return A
+
+
+DEFAULT_THETA = np.array([-1.0, 0.7, 0.0, 1.5])
+
+#: PC hyper-priors with the values of the reference's spatio-temporal example
+#: (`examples/gst_small/run.py:33-35`, `:56-60`).
+DEFAULT_PRIORS = [
+    PriorSpec("penalized_complexity", alpha=0.01, u=0.5),
+    PriorSpec("penalized_complexity", alpha=0.01, u=5.0),
+    PriorSpec("penalized_complexity", alpha=0.01, u=3.0),
+    PriorSpec("penalized_complexity", alpha=0.01, u=4.0),
+]
+
+
+def make_st_model(m: int, nt: int, nb: int, seed: int, obs_per_node: int = 4,
+                  theta=None, priors=None) -> STModelData:
+    """Build the seeded spatio-temporal model on an m x m grid."""
+    rng = np.random.default_rng(seed)
+    theta = np.array(DEFAULT_THETA if theta is None else theta, dtype=np.float64)
+    pts, tri = grid_triangles(m)
+    c0, g1 = p1_fem(pts, tri)
+    c0inv = sps.diags(1.0 / c0.diagonal(), format="csr")
+    g2 = (g1 @ c0inv @ g1).tocsr()
+    g3 = (g1 @ c0inv @ g2).tocsr()
+    g2 = (0.5 * (g2 + g2.T)).tocsr()
+    g3 = (0.5 * (g3 + g3.T)).tocsr()
+    m0, m1, m2 = temporal_matrices(nt)
+    ns = m * m
+    n_obs = obs_per_node * ns * nt
+    s_xy = rng.uniform(0.0, 1.0, size=(n_obs, 2))
+    t = rng.uniform(0.0, nt, size=n_obs)
+    a_st = design_matrix(m, nt, s_xy, t)
+    x_cov = rng.standard_normal((n_obs, nb))
+    beta = rng.standard_normal(nb)
+    # smooth space-time field: random Fourier modes
+    n_modes = 8
+    kx = rng.integers(1, 4, n_modes)
+    ky = rng.integers(1, 4, n_modes)
+    kt = rng.uniform(0.0, 2.0 * math.pi / max(nt, 2) * 3.0, n_modes)
+    amp = rng.standard_normal(n_modes) / math.sqrt(n_modes)
+    ph = rng.uniform(0.0, 2.0 * math.pi, n_modes)
+    px = pts[:, 0][None, :]
+    py = pts[:, 1][None, :]
+    tn = np.arange(nt, dtype=np.float64)[:, None]
+    field = np.zeros((nt, ns))
+    for k in range(n_modes):
+        field += amp[k] * np.cos(math.pi * (kx[k] * px + ky[k] * py) + kt[k] * tn + ph[k])
+    x_st = field.ravel()
+    eps = rng.standard_normal(n_obs) * math.exp(-0.5 * theta[3])
+    y = a_st @ x_st + x_cov @ beta + eps
+    return STModelData(
+        ns=ns, nt=nt, nb=nb, c0=c0, g1=g1.tocsr(), g2=g2, g3=g3,
+        m0=m0, m1=m1, m2=m2, a_st=a_st, x_cov=x_cov, y=y, theta=theta,
+        priors=list(DEFAULT_PRIORS if priors is None else priors),
+    )
+
+
+#: The workloads, by name: (grid side m, nt, nb, seed).
+WORKLOADS = {
+    # CPU-runnable case: 20x20 grid (ns=400), nt=12, nb=4, seed 0
+    "st_ns400_nt12_nb4": (20, 12, 4, 0),
+    # plain drop-in and adjoint case: 45x45 grid (ns=2025), nt=100, nb=6, seed 1
+    "st_ns2025_nt100_nb6": (45, 100, 6, 1),
+}
+
+
+def make_workload(name: str) -> STModelData:
+    m, nt, nb, seed = WORKLOADS[name]
+    return make_st_model(m, nt, nb, seed)

@@ -1,0 +1,138 @@
+/*
+ * dalia_hip.h -- C ABI of the MI355X (gfx950) inner-solve library.
+ *
+ * One handle owns the dense storage of one block-tridiagonal-with-arrowhead
+ * (BTA) SPD matrix of n diagonal blocks of size b and an arrowhead of size a,
+ * and runs on one HIP stream.  Every entry point is FP64, takes plain device
+ * pointers, returns an int status (0 = ok, < 0 = bad argument or HIP error)
+ * and is asynchronous on the handle's stream unless its comment says it
+ * synchronises.  A handle is not thread safe.
+ *
+ * Storage: blocks are row-major with a leading dimension padded to a multiple
+ * of 64 (bp >= b); the arrowhead is stored with ap = 64 rows (a <= 64).
+ * Padding rows/columns hold the identity on the diagonal blocks and the tip
+ * and zeros elsewhere, so the factorisation, solve and selected inversion of
+ * the padded matrix equal those of the logical one.  The logical blocks are
+ * the [b][b], [a][b] and [a][a] sub-views at offset 0 (see dh_layout).
+ *
+ * Reference interfaces replaced (DALIA, affifboudaoud/DALIA):
+ *   dh_create / dh_destroy   SerinvSolver.__init__   src/dalia/solvers/structured_solver.py:22-70
+ *   dh_assemble              Model.construct_Q_prior / construct_Q_conditional +
+ *                            SerinvSolver._spmatrix_to_structured
+ *                            src/dalia/core/model.py:306-423,
+ *                            src/dalia/submodels/spatio_temporal.py:114-149,
+ *                            src/dalia/solvers/structured_solver.py:192-468
+ *   dh_cholesky              SerinvSolver.cholesky (pobtaf / pobtf)   structured_solver.py:72-100
+ *   dh_solve                 SerinvSolver.solve (pobtas N then C)     structured_solver.py:102-149
+ *   dh_logdet                SerinvSolver.logdet                      structured_solver.py:151-166
+ *   dh_selected_inversion    SerinvSolver.selected_inversion (pobtasi) structured_solver.py:168-190
+ *   dh_qc_matvec             Q_conditional @ x inside DALIA._evaluate_conditional_latent_parameters
+ *                            src/dalia/core/dalia.py:1161
+ *   dh_scale, dh_dot         the vector algebra of core/model.py:446 and core/dalia.py:1161
+ *   dh_contract_fem, dh_contract_fem_vec, dh_contract_ata, dh_prior_eig
+ *                            no counterpart: the adjoint gradient that replaces the
+ *                            finite-difference gradient of DALIA._objective_function
+ *                            src/dalia/core/dalia.py:513-601
+ */
+#ifndef DALIA_HIP_H
+#define DALIA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DH_SPARSITY_BT 0
+#define DH_SPARSITY_BTA 1
+
+typedef struct dh_handle dh_handle;
+
+typedef struct dh_layout {
+  int n, b, a;        /* logical sizes */
+  int bp, ap;         /* padded block size (multiple of 64) and arrow rows (64 or 0) */
+  double* diag;       /* [n][bp][bp]   diagonal blocks (L_ii, later X_ii) */
+  double* lower;      /* [n-1][bp][bp] sub-diagonal blocks (A_{i+1,i}); NULL when n == 1 */
+  double* arrow;      /* [n][ap][bp]   arrowhead bottom blocks; NULL when a == 0 */
+  double* tip;        /* [ap][ap]      arrowhead tip; NULL when a == 0 */
+  double* dinv;       /* [n][bp]       1 / diag(L_ii) after dh_cholesky */
+} dh_layout;
+
+/* Device-resident, theta-independent model data of the spatio-temporal
+ * Gaussian model; all pointers are device pointers owned by the caller and
+ * must outlive every call that receives the struct. */
+typedef struct dh_model {
+  int ns, nt, nb;
+  int fem_nnz;
+  const int* fem_rowptr;     /* [ns+1] union pattern of c0, g1, g2, g3 */
+  const int* fem_col;        /* [fem_nnz] */
+  const double* fem_val;     /* [4][fem_nnz]  c0, g1, g2, g3 on that pattern */
+  const double* m_diag;      /* [3][nt]    diagonals of M0, M1, M2 */
+  const double* m_off;       /* [3][nt-1]  sub-diagonals of M0, M1, M2 */
+  long long ata_nnz;
+  const long long* ata_rowptr; /* [ns*nt+1] A_st^T A_st, CSR, global columns */
+  const int* ata_col;
+  const double* ata_val;
+  const double* ata_arrow;   /* [nb][ns*nt] X^T A_st */
+  const double* ata_tip;     /* [nb][nb]    X^T X */
+  double fe_prec;            /* prior precision of the fixed effects */
+} dh_model;
+
+int dh_version(void);
+
+/* Allocates the BTA storage and scratch on the current device. */
+int dh_create(int n, int b, int a, void* hip_stream, dh_handle** out);
+int dh_destroy(dh_handle* h);
+int dh_get_layout(const dh_handle* h, dh_layout* out);
+int dh_set_stream(dh_handle* h, void* hip_stream);
+/* Bytes of device memory the handle owns. */
+long long dh_memory_bytes(const dh_handle* h);
+
+/* Fill the blocks with Q = sum_k sum_m coef[k*4+m] (M_k kron G_m)
+ * (+ fe_prec I on the tip for BTA) (+ tau_scale A^T A when with_likelihood).
+ * coef: [3][4] Kronecker coefficients.  sparsity BT drops the arrowhead. */
+int dh_assemble(dh_handle* h, const dh_model* m, const double* coef, double tau_scale,
+                int sparsity, int with_likelihood);
+
+/* In-place BTA (pobtaf) or BT (pobtf) Cholesky.  A failing pivot shows up as
+ * NaN in dh_logdet and as a positive value from dh_get_info. */
+int dh_cholesky(dh_handle* h, int sparsity);
+
+/* In-place solve A x = rhs with the factor; rhs is a device vector of
+ * length n*b (+ a for BTA), logical (unpadded) layout. */
+int dh_solve(dh_handle* h, double* rhs, int sparsity);
+
+/* logdet = 2 sum log diag(L).  out_dev: device double (async). */
+int dh_logdet(dh_handle* h, int sparsity, double* out_dev);
+
+/* Overwrite the factor with the blocks of A^-1 on the BTA pattern; the
+ * diagonal blocks and the tip are stored full symmetric. */
+int dh_selected_inversion(dh_handle* h, int sparsity);
+
+/* Synchronises the stream; *info = 0 or 1 + global row of the first failing pivot. */
+int dh_get_info(dh_handle* h, int* info);
+
+/* y_prior = Q_prior x (spatio-temporal part + fe_prec on the fixed effects),
+ * y_like = tau_scale A^T A x; vectors of length ns*nt + nb (device). */
+int dh_qc_matvec(dh_handle* h, const dh_model* m, const double* coef, double tau_scale,
+                 const double* x, double* y_prior, double* y_like);
+
+/* y = s * x (device vectors of length len). */
+int dh_scale(dh_handle* h, const double* x, double s, double* y, long long len);
+/* *out_dev = x . y */
+int dh_dot(dh_handle* h, const double* x, const double* y, long long len, double* out_dev);
+
+/* After dh_selected_inversion (BTA or BT): out_dev[(2n-1)*4] =
+ * <X_block, G_m>_F for the n diagonal blocks then the n-1 sub-diagonal blocks. */
+int dh_contract_fem(dh_handle* h, const dh_model* m, double* out_dev);
+/* out_dev[(2n-1)*4] = x_t^T G_m x_t for t < n, then x_{t+1}^T G_m x_t. */
+int dh_contract_fem_vec(dh_handle* h, const dh_model* m, const double* x, double* out_dev);
+/* After dh_selected_inversion (BTA): *out_dev = <X, A^T A> over all blocks. */
+int dh_contract_ata(dh_handle* h, const dh_model* m, double* out_dev);
+/* Kronecker-eigen closed form of the prior log-determinant:
+ * out_dev[4] = (sum_j logdet T_j, d/dgamma_s, d/dgamma_t, d/dgamma_st). */
+int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const double* gam,
+                 double* out_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
