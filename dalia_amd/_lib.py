@@ -1,0 +1,100 @@
+"""ctypes binding of `libdalia_hip.so` (the C ABI declared in include/dalia_hip.h).
+
+The library is built in-tree by `dalia_amd._build.build()`.  There is no
+fallback: if the shared object is missing or a GPU is absent, loading fails
+with an error that says so.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdalia_hip.so")
+
+SPARSITY = {"bt": 0, "bta": 1}
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class DhLayout(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("b", ctypes.c_int), ("a", ctypes.c_int),
+        ("bp", ctypes.c_int), ("ap", ctypes.c_int),
+        ("diag", ctypes.c_void_p), ("lower", ctypes.c_void_p), ("arrow", ctypes.c_void_p),
+        ("tip", ctypes.c_void_p), ("dinv", ctypes.c_void_p),
+    ]
+
+
+class DhModel(ctypes.Structure):
+    _fields_ = [
+        ("ns", ctypes.c_int), ("nt", ctypes.c_int), ("nb", ctypes.c_int),
+        ("fem_nnz", ctypes.c_int),
+        ("fem_rowptr", ctypes.c_void_p), ("fem_col", ctypes.c_void_p), ("fem_val", ctypes.c_void_p),
+        ("m_diag", ctypes.c_void_p), ("m_off", ctypes.c_void_p),
+        ("ata_nnz", ctypes.c_longlong),
+        ("ata_rowptr", ctypes.c_void_p), ("ata_col", ctypes.c_void_p), ("ata_val", ctypes.c_void_p),
+        ("ata_arrow", ctypes.c_void_p), ("ata_tip", ctypes.c_void_p),
+        ("fe_prec", ctypes.c_double),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_LL = ctypes.c_longlong
+
+#: name -> (restype, argtypes); every symbol include/dalia_hip.h declares.
+SIGNATURES = {
+    "dh_version": (_I, []),
+    "dh_create": (_I, [_I, _I, _I, _P, ctypes.POINTER(_P)]),
+    "dh_destroy": (_I, [_P]),
+    "dh_get_layout": (_I, [_P, ctypes.POINTER(DhLayout)]),
+    "dh_set_stream": (_I, [_P, _P]),
+    "dh_memory_bytes": (_LL, [_P]),
+    "dh_assemble": (_I, [_P, ctypes.POINTER(DhModel), c_double_p, _D, _I, _I]),
+    "dh_scatter_coo": (_I, [_P, _LL, _P, _P, _P, _I]),
+    "dh_cholesky": (_I, [_P, _I]),
+    "dh_solve": (_I, [_P, _P, _I]),
+    "dh_logdet": (_I, [_P, _I, _P]),
+    "dh_selected_inversion": (_I, [_P, _I]),
+    "dh_get_info": (_I, [_P, ctypes.POINTER(_I)]),
+    "dh_qc_matvec": (_I, [_P, ctypes.POINTER(DhModel), c_double_p, _D, _P, _P, _P]),
+    "dh_scale": (_I, [_P, _P, _D, _P, _LL]),
+    "dh_dot": (_I, [_P, _P, _P, _LL, _P]),
+    "dh_contract_fem": (_I, [_P, ctypes.POINTER(DhModel), _P]),
+    "dh_contract_fem_vec": (_I, [_P, ctypes.POINTER(DhModel), _P, _P]),
+    "dh_contract_ata": (_I, [_P, ctypes.POINTER(DhModel), _P]),
+    "dh_prior_eig": (_I, [_P, ctypes.POINTER(DhModel), _P, c_double_p, _P]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -m dalia_amd._build` "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class DhError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        names = {-1: "bad argument", -2: "HIP error", -3: "wrong state"}
+        raise DhError(f"{what} failed: {names.get(rc, rc)}")

@@ -1,0 +1,1372 @@
+// MI355X (gfx950) inner solve of one f(theta) evaluation on the BTA precision.
+//
+// Kernels (all FP64):
+//   assembly      k_assemble_rows / k_assemble_arrow / k_assemble_tip
+//   pobtaf/pobtf  k_leaf, k_panel, k_update, k_syrk  (right-looking tall-panel
+//                 block Cholesky, 64-wide panels, FP64 MFMA tiles)
+//   inverse       k_trtri_t, k_trtri_inv (batched recursive inverse of L_ii,
+//                 packed into the strict upper triangle of each diagonal block)
+//   pobtas        k_pack/k_unpack, k_gemv_row, k_gemv_col_part, k_fin_*, k_upk_*
+//   pobtasi       k_gemm (generic MFMA tile GEMM, up to 3 terms) + k_symm_copy
+//   logdet        k_logdet_part / k_logdet_fin
+//   adjoint       k_qp_matvec, k_ata_matvec*, k_contract_*, k_prior_eig
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/dalia_hip.h"
+#include "bta_tiles.h"
+
+#define DH_OK 0
+#define DH_EARG -1
+#define DH_EHIP -2
+#define DH_ESTATE -3
+
+#define HIPCHK(x)                          \
+  do {                                     \
+    if ((x) != hipSuccess) return DH_EHIP; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double dh_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Sum over the 256 threads of a workgroup; result valid in every thread.
+__device__ double dh_block_sum(double v) {
+  __shared__ double red[4];
+  v = dh_wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// out[j] = sum_i part[i*stride + j] for j < nout, fixed order.
+__global__ __launch_bounds__(256) void k_sum_parts(const double* part, int nparts, int stride, int nout,
+                                                   double* out) {
+  for (int j = threadIdx.x; j < nout; j += blockDim.x) {
+    double s = 0.0, c = 0.0;
+    for (int i = 0; i < nparts; ++i) {  // Kahan
+      double y = part[(long)i * stride + j] - c;
+      double t = s + y;
+      c = (t - s) - y;
+      s = t;
+    }
+    out[j] = s;
+  }
+}
+
+// out[q*w + j] = sum_r part[(q*nr + r)*w + j], one thread per (q, j).
+__global__ __launch_bounds__(256) void k_sum_seg(const double* part, int nseg, int nr, int w, double* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nseg * w) return;
+  const int q = e / w, j = e % w;
+  double s = 0.0;
+  for (int r = 0; r < nr; ++r) s += part[((long)q * nr + r) * w + j];
+  out[e] = s;
+}
+
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+// ---------------------------------------------------------------------------
+// assembly
+// ---------------------------------------------------------------------------
+
+struct Coef {
+  double c[12];
+};
+
+// One workgroup per row of a diagonal or sub-diagonal block; the row is built
+// in LDS (zero, FEM Kronecker entries, tau A^T A entries) and streamed out.
+__global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* lower, int n, int b, int bp,
+                                                       dh_model m, Coef cf, double tau, int with_like) {
+  extern __shared__ double row[];
+  const long r = blockIdx.x;
+  const long ndiag = (long)n * bp;
+  int t, s, trow, tcol;
+  bool isdiag;
+  double* out;
+  if (r < ndiag) {
+    isdiag = true;
+    t = (int)(r / bp);
+    s = (int)(r % bp);
+    trow = tcol = t;
+    out = diag + (long)t * bp * bp + (long)s * bp;
+  } else {
+    isdiag = false;
+    long rr = r - ndiag;
+    t = (int)(rr / bp);
+    s = (int)(rr % bp);
+    trow = t + 1;
+    tcol = t;
+    out = lower + (long)t * bp * bp + (long)s * bp;
+  }
+  for (int c = threadIdx.x; c < bp; c += blockDim.x) row[c] = 0.0;
+  __syncthreads();
+  if (s < b) {
+    const int nt = m.nt;
+    double w[4];
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double mk = isdiag ? m.m_diag[k * nt + t] : m.m_off[k * (nt - 1) + t];
+        acc += cf.c[k * 4 + mm] * mk;
+      }
+      w[mm] = acc;
+    }
+    const int e0 = m.fem_rowptr[s], e1 = m.fem_rowptr[s + 1];
+    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+      const long nn = m.fem_nnz;
+      row[m.fem_col[e]] = w[0] * m.fem_val[e] + w[1] * m.fem_val[nn + e] + w[2] * m.fem_val[2 * nn + e] +
+                          w[3] * m.fem_val[3 * nn + e];
+    }
+    __syncthreads();
+    if (with_like) {
+      const long R = (long)trow * m.ns + s;
+      const long a0 = m.ata_rowptr[R], a1 = m.ata_rowptr[R + 1];
+      for (long e = a0 + threadIdx.x; e < a1; e += blockDim.x) {
+        const int c = m.ata_col[e];
+        const int ct = c / m.ns;
+        if (ct == tcol) row[c - ct * m.ns] += tau * m.ata_val[e];
+      }
+    }
+  } else if (isdiag) {
+    if (threadIdx.x == 0) row[s] = 1.0;
+  }
+  __syncthreads();
+  double2* o2 = reinterpret_cast<double2*>(out);
+  for (int c = threadIdx.x; c < bp / 2; c += blockDim.x) o2[c] = make_double2(row[2 * c], row[2 * c + 1]);
+}
+
+__global__ __launch_bounds__(256) void k_assemble_arrow(double* arrow, int n, int b, int a, int bp, int ap,
+                                                        dh_model m, double tau, int with_like) {
+  const long r = blockIdx.x;  // t*ap + j
+  const int t = (int)(r / ap), j = (int)(r % ap);
+  double* out = arrow + r * bp;
+  const long nst = (long)m.ns * m.nt;
+  for (int s = threadIdx.x; s < bp; s += blockDim.x) {
+    double v = 0.0;
+    if (with_like && j < a && s < b) v = tau * m.ata_arrow[(long)j * nst + (long)t * m.ns + s];
+    out[s] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_assemble_tip(double* tip, int a, int ap, dh_model m, double tau,
+                                                      int with_like) {
+  for (int e = threadIdx.x; e < ap * ap; e += blockDim.x) {
+    const int j = e / ap, k = e % ap;
+    double v;
+    if (j < a && k < a) {
+      v = (j == k) ? m.fe_prec : 0.0;
+      if (with_like) v += tau * m.ata_tip[j * a + k];
+    } else {
+      v = (j == k) ? 1.0 : 0.0;
+    }
+    tip[e] = v;
+  }
+}
+
+// Zero every block, identity on the padded diagonal of the diagonal blocks
+// and of the tip.
+__global__ __launch_bounds__(256) void k_fill_bta(double* diag, double* lower, double* arrow, double* tip, int n,
+                                                  int b, int bp, int a, int ap) {
+  const long bb = (long)bp * bp;
+  const long nd = n * bb, nl = (n - 1) * bb, na = (long)n * ap * bp, ntp = (long)ap * ap;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < nd + nl + na + ntp;
+       e += (long)gridDim.x * blockDim.x) {
+    if (e < nd) {
+      const long w = e % bb;
+      const int r = (int)(w / bp), c = (int)(w % bp);
+      diag[e] = (r == c && r >= b) ? 1.0 : 0.0;
+    } else if (e < nd + nl) {
+      lower[e - nd] = 0.0;
+    } else if (e < nd + nl + na) {
+      arrow[e - nd - nl] = 0.0;
+    } else {
+      const long w = e - nd - nl - na;
+      const int r = (int)(w / ap), c = (int)(w % ap);
+      tip[w] = (r == c && r >= a) ? 1.0 : 0.0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scatter_coo(double* diag, double* lower, double* arrow, double* tip, int n,
+                                                     int b, int bp, int a, int ap, long nnz, const long long* rows,
+                                                     const long long* cols, const double* vals, int bta) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  const long nb = (long)n * b;
+  const long r = rows[e], c = cols[e];
+  const long br = r < nb ? r / b : n, bc = c < nb ? c / b : n;
+  const int lr = (int)(r < nb ? r % b : r - nb), lc = (int)(c < nb ? c % b : c - nb);
+  const long bb = (long)bp * bp;
+  if (br < n && bc == br) diag[br * bb + (long)lr * bp + lc] = vals[e];
+  else if (br < n && bc == br - 1) lower[bc * bb + (long)lr * bp + lc] = vals[e];
+  else if (bta && br == n && bc < n) arrow[(bc * ap + lr) * (long)bp + lc] = vals[e];
+  else if (bta && br == n && bc == n) tip[lr * ap + lc] = vals[e];
+}
+
+// ---------------------------------------------------------------------------
+// pobtaf
+// ---------------------------------------------------------------------------
+
+struct BtaPtrs {
+  double* D;    // block i diagonal
+  double* L;    // block i sub-diagonal (block (i+1, i)), or null
+  double* R;    // block i arrow rows, or null
+  double* Dn;   // block i+1 diagonal, or null
+  double* Rn;   // block i+1 arrow rows, or null
+  double* T;    // tip, or null
+  double* pinv; // 64x64 dense inverse of the current diagonal tile
+  double* dinv; // block i 1/diag(L)
+  double* dinvn;// block i+1 1/diag(L)
+  int* info;
+  int bp, nbt, row_base, row_base_n;
+};
+
+__global__ __launch_bounds__(256) void k_leaf(double* tile, long ld, double* dinv, double* inv_out, int* info,
+                                              int row_base) {
+  dh_leaf_potrf(tile, ld, dinv, inv_out, info, row_base);
+}
+
+// Row tile q of the tall panel below diagonal tile p: diagonal-block rows
+// p+1..nbt-1, then the nbt rows of the sub-diagonal block, then the arrow.
+__device__ __forceinline__ double* panel_row(const BtaPtrs& P, int p, int q) {
+  const int nd = P.nbt - 1 - p;
+  if (q < nd) return P.D + (long)(p + 1 + q) * DH_TILE * P.bp;
+  q -= nd;
+  if (P.L) {
+    if (q < P.nbt) return P.L + (long)q * DH_TILE * P.bp;
+    q -= P.nbt;
+  }
+  return P.R;  // q == 0
+}
+
+// X_q,p = A_q,p inv(L_pp)^T for every row tile q below the diagonal tile p.
+__global__ __launch_bounds__(256) void k_panel(BtaPtrs P, int p) {
+  double* base = panel_row(P, p, blockIdx.x) + p * DH_TILE;
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  Opnd A{base, P.bp, M_ROW, nullptr};
+  Opnd B{P.pinv, DH_TILE, M_ROW, nullptr};  // opB[k][j] = inv[j][k]
+  dh_mma(acc, A, 0, B, 0, 0, DH_TILE);
+  dh_store(base, P.bp, 0, 0, acc, 1.0, 0.0, false);
+}
+
+// Trailing update of panel p: tile (q, J) -= X_q,p X_J,p^T for J = p+1..nbt-1
+// and every row tile q of the tall panel at or below J; the workgroup that
+// updates diagonal tile (p+1, p+1) factors it afterwards.
+__global__ __launch_bounds__(256) void k_update(BtaPtrs P, int p) {
+  const int J = p + 1 + blockIdx.y;
+  const int q = blockIdx.x;  // row tile index in the tall panel of p
+  const int nd = P.nbt - 1 - p;
+  if (q < nd && (p + 1 + q) < J) return;  // above the diagonal
+  double* rowp = panel_row(P, p, q);
+  Opnd A{rowp + p * DH_TILE, P.bp, M_ROW, nullptr};
+  Opnd B{P.D + (long)J * DH_TILE * P.bp + p * DH_TILE, P.bp, M_ROW, nullptr};
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  dh_mma(acc, A, 0, B, 0, 0, DH_TILE);
+  double* C = rowp + J * DH_TILE;
+  dh_store(C, P.bp, 0, 0, acc, -1.0, 1.0, false);
+  if (q == 0 && J == p + 1) {
+    __syncthreads();
+    __threadfence();
+    dh_leaf_potrf(C, P.bp, P.dinv + J * DH_TILE, P.pinv, P.info, P.row_base + J * DH_TILE);
+  }
+}
+
+// Schur update of the next block: D_{i+1} -= L_i L_i^T (lower tiles),
+// R_{i+1} -= R_i L_i^T, T -= R_i R_i^T; then factor tile (0,0) of D_{i+1}.
+__global__ __launch_bounds__(256) void k_syrk(BtaPtrs P) {
+  int e = blockIdx.x;
+  const int nbt = P.nbt;
+  const int K = P.bp;
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  const int ntri = P.L ? nbt * (nbt + 1) / 2 : 0;
+  const int narr = (P.L && P.R) ? nbt : 0;
+  if (e < ntri) {
+    int I = 0;
+    while ((I + 1) * (I + 2) / 2 <= e) ++I;
+    const int J = e - I * (I + 1) / 2;
+    Opnd A{P.L, P.bp, M_ROW, nullptr};
+    dh_mma(acc, A, I * DH_TILE, A, J * DH_TILE, 0, K);
+    double* C = P.Dn;
+    dh_store(C, P.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 1.0, false);
+    if (e == 0) {
+      __syncthreads();
+      __threadfence();
+      dh_leaf_potrf(C, P.bp, P.dinvn, P.pinv, P.info, P.row_base_n);
+    }
+    return;
+  }
+  e -= ntri;
+  if (e < narr) {
+    Opnd A{P.R, P.bp, M_ROW, nullptr};
+    Opnd B{P.L, P.bp, M_ROW, nullptr};
+    dh_mma(acc, A, 0, B, e * DH_TILE, 0, K);
+    dh_store(P.Rn, P.bp, 0, e * DH_TILE, acc, -1.0, 1.0, false);
+    return;
+  }
+  // tip
+  Opnd A{P.R, P.bp, M_ROW, nullptr};
+  dh_mma(acc, A, 0, A, 0, 0, K);
+  dh_store(P.T, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
+}
+
+// ---------------------------------------------------------------------------
+// batched inverse of the diagonal factors (packed into the strict upper part)
+// ---------------------------------------------------------------------------
+
+#define DH_MAXNODES 64
+struct NodeList {
+  int cnt;
+  int o[DH_MAXNODES], s1[DH_MAXNODES], s[DH_MAXNODES];
+};
+
+// T = L[o+s1 .. o+s, o .. o+s1] * inv(L11)  (T stored at the same position in W)
+__global__ __launch_bounds__(256) void k_trtri_t(double* Dbase, double* dinvbase, double* Wbase, int bp,
+                                                 int blk0, NodeList nl) {
+  const int nd = blockIdx.y;
+  if (nd >= nl.cnt) return;
+  const int o = nl.o[nd], s1 = nl.s1[nd], s = nl.s[nd];
+  const int s2 = s - s1;
+  const int x = blockIdx.x;
+  if (x >= s2 * s1) return;
+  const int I = x / s1, J = x % s1;
+  const long g = blk0 + blockIdx.z;
+  double* D = Dbase + g * (long)bp * bp;
+  double* W = Wbase + (long)blockIdx.z * bp * bp;
+  const double* dinv = dinvbase + g * bp;
+  const long off = (long)o * DH_TILE;
+  Opnd A{D + (off + (long)s1 * DH_TILE) * bp + off, bp, M_ROW, nullptr};
+  Opnd B{D + off * bp + off, bp, M_UPK_ROW, dinv + off};  // opB = inv(L11)
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  dh_mma(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, s1 * DH_TILE);
+  double* C = W + (off + (long)s1 * DH_TILE) * bp + off;
+  dh_store(C, bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
+}
+
+// inv21 = -inv(L22) T, stored transposed into the strict upper triangle.
+__global__ __launch_bounds__(256) void k_trtri_inv(double* Dbase, double* dinvbase, double* Wbase, int bp,
+                                                   int blk0, NodeList nl) {
+  const int nd = blockIdx.y;
+  if (nd >= nl.cnt) return;
+  const int o = nl.o[nd], s1 = nl.s1[nd], s = nl.s[nd];
+  const int s2 = s - s1;
+  const int x = blockIdx.x;
+  if (x >= s2 * s1) return;
+  const int I = x / s1, J = x % s1;
+  const long g = blk0 + blockIdx.z;
+  double* D = Dbase + g * (long)bp * bp;
+  double* W = Wbase + (long)blockIdx.z * bp * bp;
+  const double* dinv = dinvbase + g * bp;
+  const long off = (long)o * DH_TILE;
+  const long off2 = off + (long)s1 * DH_TILE;
+  Opnd A{D + off2 * bp + off2, bp, M_UPK_COL, dinv + off2};  // opA = inv(L22)
+  Opnd B{W + off2 * bp + off, bp, M_COL, nullptr};            // opB = T
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  dh_mma(acc, A, I * DH_TILE, B, J * DH_TILE, 0, (I + 1) * DH_TILE);
+  // C[i][j] = inv21[i][j] lives at D[off + j][off2 + i]
+  double* C = D + off * bp + off2;
+  dh_store(C, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
+}
+
+// ---------------------------------------------------------------------------
+// generic tile GEMM: C = sum_t alpha_t opA_t opB_t + beta C
+// ---------------------------------------------------------------------------
+
+enum { KLO_0 = 0, KLO_I = 1, KLO_J = 2, KLO_MAXIJ = 3 };
+enum { KHI_K = 0, KHI_I = 1 };
+
+struct GTerm {
+  Opnd A, B;
+  int K, klo, khi;
+  double alpha;
+};
+struct GArgs {
+  GTerm t[3];
+  int nterm;
+  double* C;
+  long ldc;
+  double beta;
+  int mt, ntl;
+  int lower;
+};
+
+__global__ __launch_bounds__(256) void k_gemm(GArgs g) {
+  int I, J;
+  int e = blockIdx.x;
+  if (g.lower) {
+    I = 0;
+    while ((I + 1) * (I + 2) / 2 <= e) ++I;
+    J = e - I * (I + 1) / 2;
+  } else {
+    I = e / g.ntl;
+    J = e % g.ntl;
+  }
+  dh_d4 tot[2][2];
+  dh_zero(tot);
+  for (int ti = 0; ti < g.nterm; ++ti) {
+    const GTerm& t = g.t[ti];
+    int klo = 0, khi = t.K;
+    if (t.klo == KLO_I) klo = I * DH_TILE;
+    else if (t.klo == KLO_J) klo = J * DH_TILE;
+    else if (t.klo == KLO_MAXIJ) klo = (I > J ? I : J) * DH_TILE;
+    if (t.khi == KHI_I) khi = min(khi, (I + 1) * DH_TILE);
+    dh_d4 acc[2][2];
+    dh_zero(acc);
+    dh_mma(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) tot[a][b] += t.alpha * acc[a][b];
+  }
+  dh_store(g.C, g.ldc, I * DH_TILE, J * DH_TILE, tot, 1.0, g.beta, false);
+}
+
+// D[r][c] = S[max(r,c)][min(r,c)] (lower tiles of S mirrored).
+__global__ __launch_bounds__(256) void k_symm_copy(double* D, const double* S, int bp) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)bp * bp) return;
+  const int r = (int)(e / bp), c = (int)(e % bp);
+  D[e] = (r >= c) ? S[(long)r * bp + c] : S[(long)c * bp + r];
+}
+
+// ---------------------------------------------------------------------------
+// pobtas helpers (HBM-streaming GEMV on the factor)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_pack(const double* rhs, double* z, int n, int b, int bp, int a,
+                                              int ap) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long N = (long)n * bp + ap;
+  if (e >= N) return;
+  double v = 0.0;
+  if (e < (long)n * bp) {
+    const int t = (int)(e / bp), s = (int)(e % bp);
+    if (s < b) v = rhs[(long)t * b + s];
+  } else {
+    const int j = (int)(e - (long)n * bp);
+    if (j < a) v = rhs[(long)n * b + j];
+  }
+  z[e] = v;
+}
+
+__global__ __launch_bounds__(256) void k_unpack(const double* z, double* rhs, int n, int b, int bp, int a) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long N = (long)n * b + a;
+  if (e >= N) return;
+  if (e < (long)n * b) {
+    const int t = (int)(e / b), s = (int)(e % b);
+    rhs[e] = z[(long)t * bp + s];
+  } else {
+    rhs[e] = z[(long)n * bp + (e - (long)n * b)];
+  }
+}
+
+// y[r] = yin[r] + alpha * sum_c M[r][c] x[c]; one wave per row.
+__global__ __launch_bounds__(256) void k_gemv_row(const double* M, long ld, int nrows, int ncols, const double* x,
+                                                  const double* yin, double* y, double alpha) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= nrows) return;
+  const double2* m2 = reinterpret_cast<const double2*>(M + (long)r * ld);
+  const double2* x2 = reinterpret_cast<const double2*>(x);
+  double s = 0.0;
+  for (int c = lane; c < ncols / 2; c += 64) {
+    double2 a = m2[c], v = x2[c];
+    s += a.x * v.x + a.y * v.y;
+  }
+  s = dh_wave_sum(s);
+  if (lane == 0) y[r] = (yin ? yin[r] : 0.0) + alpha * s;
+}
+
+// Packed inverse factor, row form: y[c] = dinv[c] w[c] + sum_{r>c} D[c][r] w[r]
+// (y = inv(L)^T w).  One wave per row c.
+__global__ __launch_bounds__(256) void k_upk_row(const double* D, const double* dinv, int bp, const double* w,
+                                                 double* y) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= bp) return;
+  const double* row = D + (long)c * bp;
+  double s = 0.0;
+  for (int r = c + 1 + lane; r < bp; r += 64) s += row[r] * w[r];
+  s = dh_wave_sum(s);
+  if (lane == 0) y[c] = dinv[c] * w[c] + s;
+}
+
+#define DH_CCH 64  // rows per chunk of the column-oriented GEMV partials
+
+// part[chunk][c] = sum_{r in chunk} M[r][c] x[r] (only r < c when upk, for
+// the packed strict upper triangle: y = inv(L) v, y[c] = sum_{r<c} D[r][c] v[r]).
+__global__ __launch_bounds__(256) void k_gemv_col_part(const double* M, long ld, int nrows, int ncols,
+                                                       const double* x, double* part, int upk) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const int chunk = blockIdx.y;
+  if (c >= ncols) return;
+  const int r0 = chunk * DH_CCH;
+  int r1 = min(nrows, r0 + DH_CCH);
+  if (upk) r1 = min(r1, c);
+  double s = 0.0;
+  for (int r = r0; r < r1; ++r) s += M[(long)r * ld + c] * x[r];
+  part[(long)chunk * ncols + c] = s;
+}
+
+// y[c] = base(c) + sign * sum_chunks part[chunk][c]; base = dinv[c] v[c] (fwd) or v[c] (bwd)
+__global__ __launch_bounds__(256) void k_fin_col(const double* part, int nch, int ncols, const double* v,
+                                                 const double* dinv, double sign, double* y) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncols) return;
+  double s = 0.0;
+  for (int k = 0; k < nch; ++k) s += part[(long)k * ncols + c];
+  const double base = dinv ? dinv[c] * v[c] : v[c];
+  y[c] = base + sign * s;
+}
+
+// Two-source column partials: rows of L (nrows1) then of R (nrows2).
+__global__ __launch_bounds__(256) void k_gemv_col_part2(const double* M1, int nr1, const double* x1,
+                                                        const double* M2, int nr2, const double* x2, long ld,
+                                                        int ncols, double* part) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const int chunk = blockIdx.y;
+  if (c >= ncols) return;
+  const int nch1 = (nr1 + DH_CCH - 1) / DH_CCH;
+  double s = 0.0;
+  if (chunk < nch1) {
+    const int r0 = chunk * DH_CCH, r1 = min(nr1, r0 + DH_CCH);
+    for (int r = r0; r < r1; ++r) s += M1[(long)r * ld + c] * x1[r];
+  } else {
+    const int r0 = (chunk - nch1) * DH_CCH, r1 = min(nr2, r0 + DH_CCH);
+    for (int r = r0; r < r1; ++r) s += M2[(long)r * ld + c] * x2[r];
+  }
+  part[(long)chunk * ncols + c] = s;
+}
+
+// Arrowhead tip solves on a 64-vector: y = inv(L_aa) x (trans=0) or inv(L_aa)^T x.
+__global__ __launch_bounds__(64) void k_tip_apply(const double* tinv, const double* x, double* y, int trans) {
+  const int j = threadIdx.x;
+  double s = 0.0;
+  for (int k = 0; k < 64; ++k) s += (trans ? tinv[k * 64 + j] : tinv[j * 64 + k]) * x[k];
+  __syncthreads();
+  y[j] = s;
+}
+
+// z_a -= sum_i part[i][:]  (per-block arrow products of the forward sweep)
+__global__ __launch_bounds__(64) void k_arrow_acc(const double* part, int n, double* za) {
+  const int j = threadIdx.x;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += part[i * 64 + j];
+  za[j] -= s;
+}
+
+// ---------------------------------------------------------------------------
+// logdet
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_logdet_part(const double* D, int bp, int b, double* part) {
+  const double* Db = D + (long)blockIdx.x * bp * bp;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < b; k += blockDim.x) s += log(Db[(long)k * bp + k]);
+  s = dh_block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(64) void k_logdet_fin(const double* part, int n, const double* T, int a, int ap,
+                                                   double* out) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0, c = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double y = part[i] - c;
+    double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  if (T)
+    for (int j = 0; j < a; ++j) s += log(T[j * ap + j]);
+  *out = 2.0 * s;
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_scale(const double* x, double s, double* y, long len) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < len) y[e] = s * x[e];
+}
+
+__global__ __launch_bounds__(256) void k_dot_part(const double* x, const double* y, long len, double* part) {
+  double s = 0.0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long)gridDim.x * blockDim.x)
+    s += x[e] * y[e];
+  s = dh_block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// y_prior (spatio-temporal rows) = Qp_ST x; one thread per row (t, s).
+__global__ __launch_bounds__(256) void k_qp_matvec(dh_model m, Coef cf, const double* x, double* y) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nst = (long)m.ns * m.nt;
+  if (r >= nst) return;
+  const int t = (int)(r / m.ns), s = (int)(r % m.ns);
+  const int nt = m.nt;
+  const long nn = m.fem_nnz;
+  double acc = 0.0;
+  for (int dt = -1; dt <= 1; ++dt) {
+    const int t2 = t + dt;
+    if (t2 < 0 || t2 >= nt) continue;
+    double w[4];
+    for (int mm = 0; mm < 4; ++mm) {
+      double a = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        double mk = (dt == 0) ? m.m_diag[k * nt + t] : m.m_off[k * (nt - 1) + (dt < 0 ? t2 : t)];
+        a += cf.c[k * 4 + mm] * mk;
+      }
+      w[mm] = a;
+    }
+    const double* xt = x + (long)t2 * m.ns;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int e = m.fem_rowptr[s]; e < m.fem_rowptr[s + 1]; ++e) {
+      const double xv = xt[m.fem_col[e]];
+      s0 += m.fem_val[e] * xv;
+      s1 += m.fem_val[nn + e] * xv;
+      s2 += m.fem_val[2 * nn + e] * xv;
+      s3 += m.fem_val[3 * nn + e] * xv;
+    }
+    acc += w[0] * s0 + w[1] * s1 + w[2] * s2 + w[3] * s3;
+  }
+  y[r] = acc;
+}
+
+// y_like (spatio-temporal rows) = tau (A^T A x_st + (X^T A_st)^T x_fe)
+__global__ __launch_bounds__(256) void k_ata_matvec(dh_model m, double tau, const double* x, double* y) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nst = (long)m.ns * m.nt;
+  if (r >= nst) return;
+  double acc = 0.0;
+  for (long e = m.ata_rowptr[r]; e < m.ata_rowptr[r + 1]; ++e) acc += m.ata_val[e] * x[m.ata_col[e]];
+  for (int j = 0; j < m.nb; ++j) acc += m.ata_arrow[(long)j * nst + r] * x[nst + j];
+  y[r] = tau * acc;
+}
+
+// fixed-effect rows: y_like[nst+j] = tau (sum_r XA[j][r] x[r] + sum_k XX[j][k] x_fe[k]),
+// y_prior[nst+j] = fe_prec x_fe[j].  One workgroup per j.
+__global__ __launch_bounds__(256) void k_ata_matvec_fe(dh_model m, double tau, const double* x, double* yp,
+                                                       double* yl) {
+  const int j = blockIdx.x;
+  const long nst = (long)m.ns * m.nt;
+  double s = 0.0;
+  for (long r = threadIdx.x; r < nst; r += blockDim.x) s += m.ata_arrow[(long)j * nst + r] * x[r];
+  s = dh_block_sum(s);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < m.nb; ++k) s += m.ata_tip[j * m.nb + k] * x[nst + k];
+    yl[nst + j] = tau * s;
+    yp[nst + j] = m.fe_prec * x[nst + j];
+  }
+}
+
+// <X_block, G_m> partials: block q < n is D_q, q >= n is L_{q-n}; one
+// workgroup per (q, 256 rows).  part[(q*nrb + rb)*4 + m].
+__global__ __launch_bounds__(256) void k_contract_fem(const double* diag, const double* lower, int n, int bp,
+                                                      dh_model m, double* part) {
+  const int q = blockIdx.y;
+  const int rb = blockIdx.x;
+  const int s = rb * 256 + threadIdx.x;
+  const double* X = (q < n) ? diag + (long)q * bp * bp : lower + (long)(q - n) * bp * bp;
+  const long nn = m.fem_nnz;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (s < m.ns) {
+    const double* xr = X + (long)s * bp;
+    for (int e = m.fem_rowptr[s]; e < m.fem_rowptr[s + 1]; ++e) {
+      const double xv = xr[m.fem_col[e]];
+      a0 += xv * m.fem_val[e];
+      a1 += xv * m.fem_val[nn + e];
+      a2 += xv * m.fem_val[2 * nn + e];
+      a3 += xv * m.fem_val[3 * nn + e];
+    }
+  }
+  a0 = dh_block_sum(a0);
+  a1 = dh_block_sum(a1);
+  a2 = dh_block_sum(a2);
+  a3 = dh_block_sum(a3);
+  if (threadIdx.x == 0) {
+    double* o = part + ((long)q * gridDim.x + rb) * 4;
+    o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3;
+  }
+}
+
+// x_t^T G_m x_t' partials (latent-ordered x): q < n -> (t,t) = (q,q);
+// q >= n -> rows at t+1, columns at t with t = q - n.
+__global__ __launch_bounds__(256) void k_contract_fem_vec(const double* x, int n, dh_model m, double* part) {
+  const int q = blockIdx.y;
+  const int rb = blockIdx.x;
+  const int s = rb * 256 + threadIdx.x;
+  const int tr = (q < n) ? q : q - n + 1;
+  const int tc = (q < n) ? q : q - n;
+  const long nn = m.fem_nnz;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (s < m.ns) {
+    const double xs = x[(long)tr * m.ns + s];
+    const double* xc = x + (long)tc * m.ns;
+    for (int e = m.fem_rowptr[s]; e < m.fem_rowptr[s + 1]; ++e) {
+      const double xv = xs * xc[m.fem_col[e]];
+      a0 += xv * m.fem_val[e];
+      a1 += xv * m.fem_val[nn + e];
+      a2 += xv * m.fem_val[2 * nn + e];
+      a3 += xv * m.fem_val[3 * nn + e];
+    }
+  }
+  a0 = dh_block_sum(a0);
+  a1 = dh_block_sum(a1);
+  a2 = dh_block_sum(a2);
+  a3 = dh_block_sum(a3);
+  if (threadIdx.x == 0) {
+    double* o = part + ((long)q * gridDim.x + rb) * 4;
+    o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3;
+  }
+}
+
+// <X, A^T A> over the spatio-temporal rows (both triangles), the arrow
+// (twice) and the tip; one thread per spatio-temporal row.
+__global__ __launch_bounds__(256) void k_contract_ata(const double* diag, const double* lower, const double* arrow,
+                                                      const double* tip, int n, int bp, int a, int ap, dh_model m,
+                                                      double* part) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nst = (long)m.ns * m.nt;
+  double acc = 0.0;
+  if (r < nst) {
+    const int t = (int)(r / m.ns), s = (int)(r % m.ns);
+    for (long e = m.ata_rowptr[r]; e < m.ata_rowptr[r + 1]; ++e) {
+      const int c = m.ata_col[e];
+      const int t2 = c / m.ns, s2 = c - t2 * m.ns;
+      double xv;
+      if (t2 == t) xv = diag[(long)t * bp * bp + (long)s * bp + s2];
+      else if (t2 == t - 1) xv = lower[(long)t2 * bp * bp + (long)s * bp + s2];
+      else xv = lower[(long)t * bp * bp + (long)s2 * bp + s];
+      acc += xv * m.ata_val[e];
+    }
+    if (arrow)
+      for (int j = 0; j < a; ++j)
+        acc += 2.0 * arrow[((long)t * ap + j) * bp + s] * m.ata_arrow[(long)j * nst + r];
+  }
+  if (r == 0 && tip)
+    for (int j = 0; j < a; ++j)
+      for (int k = 0; k < a; ++k) acc += tip[j * ap + k] * m.ata_tip[j * a + k];
+  acc = dh_block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// Closed-form prior log-determinant, one thread per spatial eigenvalue j:
+// T_j = sum_k p_k M_k (tridiagonal), p = (e^{2gst} lt^3, e^{2gst+gt} lt^2,
+// e^{2gst+2gt} lt), lt = e^{2gs} + lam_j.  Outputs per workgroup partial sums
+// of (logdet T_j, d/dgs, d/dgt, d/dgst).
+__global__ __launch_bounds__(256) void k_prior_eig(dh_model m, const double* lam, double gs, double gt,
+                                                   double gst, double* lbuf, double* dbuf, double* part) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nt = m.nt;
+  double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
+  if (j < m.ns) {
+    const double k2 = exp(2.0 * gs);
+    const double lt = k2 + lam[j];
+    const double c0 = exp(2.0 * gst), c1 = exp(2.0 * gst + gt), c2 = exp(2.0 * gst + 2.0 * gt);
+    const double p0 = c0 * lt * lt * lt, p1 = c1 * lt * lt, p2 = c2 * lt;
+    double* l = lbuf + (long)j * nt;
+    double* d = dbuf + (long)j * nt;
+    // LDL^T
+    double dprev = p0 * m.m_diag[0] + p1 * m.m_diag[nt] + p2 * m.m_diag[2 * nt];
+    d[0] = dprev;
+    double ld = log(dprev);
+    for (int i = 0; i + 1 < nt; ++i) {
+      const double off = p0 * m.m_off[i] + p1 * m.m_off[(nt - 1) + i] + p2 * m.m_off[2 * (nt - 1) + i];
+      const double li = off / dprev;
+      const double dg = p0 * m.m_diag[i + 1] + p1 * m.m_diag[nt + i + 1] + p2 * m.m_diag[2 * nt + i + 1];
+      dprev = dg - li * off;
+      l[i] = li;
+      d[i + 1] = dprev;
+      ld += log(dprev);
+    }
+    // selected inverse, traces tr(T^-1 M_k)
+    double zn = 1.0 / d[nt - 1];
+    double tr0 = zn * m.m_diag[nt - 1], tr1 = zn * m.m_diag[nt + nt - 1], tr2 = zn * m.m_diag[2 * nt + nt - 1];
+    for (int i = nt - 2; i >= 0; --i) {
+      const double zoff = -l[i] * zn;
+      const double zii = 1.0 / d[i] - l[i] * zoff;
+      tr0 += zii * m.m_diag[i] + 2.0 * zoff * m.m_off[i];
+      tr1 += zii * m.m_diag[nt + i] + 2.0 * zoff * m.m_off[(nt - 1) + i];
+      tr2 += zii * m.m_diag[2 * nt + i] + 2.0 * zoff * m.m_off[2 * (nt - 1) + i];
+      zn = zii;
+    }
+    o0 = ld;
+    o1 = tr0 * c0 * 3.0 * lt * lt * 2.0 * k2 + tr1 * c1 * 2.0 * lt * 2.0 * k2 + tr2 * c2 * 2.0 * k2;
+    o2 = tr1 * p1 + 2.0 * tr2 * p2;
+    o3 = 2.0 * (tr0 * p0 + tr1 * p1 + tr2 * p2);
+  }
+  o0 = dh_block_sum(o0);
+  o1 = dh_block_sum(o1);
+  o2 = dh_block_sum(o2);
+  o3 = dh_block_sum(o3);
+  if (threadIdx.x == 0) {
+    double* o = part + blockIdx.x * 4;
+    o[0] = o0; o[1] = o1; o[2] = o2; o[3] = o3;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+
+struct dh_handle {
+  int n, b, a, bp, ap, nbt;
+  hipStream_t stream;
+  double *diag, *lower, *arrow, *tip, *dinv;
+  double* pinv;     // 64 x 64
+  double* tipinv;   // 64 x 64 dense inverse of the tip factor
+  double* tipdinv;  // 64
+  double* W;        // G x bp x bp scratch (inverse, pobtasi W)
+  double* S;        // bp x bp staging of X_ii
+  int G;            // blocks per batched inverse launch
+  double* Wa;       // ap x bp
+  double* z;        // n*bp + ap padded vector
+  double* v;        // bp temp
+  double* part;     // partial sums
+  long part_len;
+  double* arrpart;  // n x 64
+  double* eigbuf;   // 2 x n x bp
+  int* info;        // device
+  int factored;     // sparsity of the current factor, -1 none
+  int inverted;     // packed inverse of the diagonal factors present
+  std::vector<NodeList> levels;  // inverse recursion, deepest last
+  long long bytes;
+};
+
+static void build_nodes(int o, int s, int d, std::vector<NodeList>& lv) {
+  if (s <= 1) return;
+  if ((int)lv.size() <= d) {
+    NodeList nl;
+    memset(&nl, 0, sizeof(nl));
+    lv.push_back(nl);
+  }
+  const int s1 = s / 2;
+  NodeList& nl = lv[d];
+  nl.o[nl.cnt] = o;
+  nl.s1[nl.cnt] = s1;
+  nl.s[nl.cnt] = s;
+  nl.cnt++;
+  build_nodes(o, s1, d + 1, lv);
+  build_nodes(o + s1, s - s1, d + 1, lv);
+}
+
+static int dalloc(dh_handle* h, double** p, long long count) {
+  if (count <= 0) {
+    *p = nullptr;
+    return DH_OK;
+  }
+  if (hipMalloc((void**)p, (size_t)count * sizeof(double)) != hipSuccess) return DH_EHIP;
+  h->bytes += count * (long long)sizeof(double);
+  return DH_OK;
+}
+
+extern "C" {
+
+int dh_version(void) { return 1; }
+
+int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
+  if (!out || n < 1 || b < 1 || a < 0 || a > 64) return DH_EARG;
+  dh_handle* h = new dh_handle();
+  h->n = n;
+  h->b = b;
+  h->a = a;
+  h->bp = (b + DH_TILE - 1) / DH_TILE * DH_TILE;
+  h->ap = a > 0 ? DH_TILE : 0;
+  h->nbt = h->bp / DH_TILE;
+  if (h->nbt > 2 * DH_MAXNODES) {
+    delete h;
+    return DH_EARG;
+  }
+  h->stream = (hipStream_t)stream;
+  h->bytes = 0;
+  h->factored = -1;
+  h->inverted = 0;
+  const long long bb = (long long)h->bp * h->bp;
+  h->G = n < 8 ? n : 8;
+  int rc = DH_OK;
+  rc |= dalloc(h, &h->diag, n * bb);
+  rc |= dalloc(h, &h->lower, (n - 1) * bb);
+  rc |= dalloc(h, &h->arrow, (long long)n * h->ap * h->bp);
+  rc |= dalloc(h, &h->tip, (long long)h->ap * h->ap);
+  rc |= dalloc(h, &h->dinv, (long long)n * h->bp);
+  rc |= dalloc(h, &h->pinv, 64 * 64);
+  rc |= dalloc(h, &h->tipinv, 64 * 64);
+  rc |= dalloc(h, &h->tipdinv, 64);
+  rc |= dalloc(h, &h->W, h->G * bb);
+  rc |= dalloc(h, &h->S, bb);
+  rc |= dalloc(h, &h->Wa, (long long)64 * h->bp);
+  rc |= dalloc(h, &h->z, (long long)n * h->bp + 64);
+  rc |= dalloc(h, &h->v, h->bp);
+  h->part_len = (long long)(2 * n) * ((h->bp + 255) / 256 + 1) * 4 + 4096 +
+                (long long)((h->bp + 64) / DH_CCH + 2) * h->bp + (long long)n * h->bp / 256 * 4;
+  rc |= dalloc(h, &h->part, h->part_len);
+  rc |= dalloc(h, &h->arrpart, (long long)n * 64);
+  rc |= dalloc(h, &h->eigbuf, 2LL * n * h->bp);
+  if (hipMalloc((void**)&h->info, sizeof(int)) != hipSuccess) rc = DH_EHIP;
+  if (rc != DH_OK) {
+    dh_destroy(h);
+    return DH_EHIP;
+  }
+  build_nodes(0, h->nbt, 0, h->levels);
+  *out = h;
+  return DH_OK;
+}
+
+int dh_destroy(dh_handle* h) {
+  if (!h) return DH_EARG;
+  double* ps[] = {h->diag, h->lower, h->arrow, h->tip, h->dinv, h->pinv, h->tipinv, h->tipdinv,
+                  h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->eigbuf};
+  for (double* p : ps)
+    if (p) (void)hipFree(p);
+  if (h->info) (void)hipFree(h->info);
+  delete h;
+  return DH_OK;
+}
+
+int dh_get_layout(const dh_handle* h, dh_layout* out) {
+  if (!h || !out) return DH_EARG;
+  out->n = h->n;
+  out->b = h->b;
+  out->a = h->a;
+  out->bp = h->bp;
+  out->ap = h->ap;
+  out->diag = h->diag;
+  out->lower = h->lower;
+  out->arrow = h->arrow;
+  out->tip = h->tip;
+  out->dinv = h->dinv;
+  return DH_OK;
+}
+
+int dh_set_stream(dh_handle* h, void* stream) {
+  if (!h) return DH_EARG;
+  h->stream = (hipStream_t)stream;
+  return DH_OK;
+}
+
+long long dh_memory_bytes(const dh_handle* h) { return h ? h->bytes : -1; }
+
+static bool model_ok(const dh_handle* h, const dh_model* m) {
+  return m && m->nt == h->n && m->ns == h->b && m->nt >= 2 && m->nb >= 0 && m->nb <= 64;
+}
+
+int dh_assemble(dh_handle* h, const dh_model* m, const double* coef, double tau, int sparsity, int with_like) {
+  if (!h || !coef || !model_ok(h, m)) return DH_EARG;
+  if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
+  if (sparsity == DH_SPARSITY_BTA && m->nb != h->a) return DH_EARG;
+  Coef cf;
+  memcpy(cf.c, coef, sizeof(cf.c));
+  const long rows = (long)(2 * h->n - 1) * h->bp;
+  hipLaunchKernelGGL(k_assemble_rows, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double), h->stream,
+                     h->diag, h->lower, h->n, h->b, h->bp, *m, cf, tau, with_like);
+  HIPCHK(hipGetLastError());
+  if (h->a > 0) {
+    const int wl = (sparsity == DH_SPARSITY_BTA) ? with_like : 0;
+    hipLaunchKernelGGL(k_assemble_arrow, dim3(h->n * h->ap), dim3(256), 0, h->stream, h->arrow, h->n, h->b, h->a,
+                       h->bp, h->ap, *m, tau, wl);
+    hipLaunchKernelGGL(k_assemble_tip, dim3(1), dim3(256), 0, h->stream, h->tip, h->a, h->ap, *m, tau, wl);
+    HIPCHK(hipGetLastError());
+  }
+  h->factored = -1;
+  h->inverted = 0;
+  return DH_OK;
+}
+
+int dh_scatter_coo(dh_handle* h, long long nnz, const long long* rows, const long long* cols, const double* vals,
+                   int sparsity) {
+  if (!h || nnz < 0 || (nnz > 0 && (!rows || !cols || !vals))) return DH_EARG;
+  if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
+  hipLaunchKernelGGL(k_fill_bta, dim3(4096), dim3(256), 0, h->stream, h->diag, h->lower, h->arrow, h->tip, h->n,
+                     h->b, h->bp, h->a, h->ap);
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_scatter_coo, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, h->stream, h->diag,
+                       h->lower, h->arrow, h->tip, h->n, h->b, h->bp, h->a, h->ap, (long)nnz, rows, cols, vals,
+                       (sparsity == DH_SPARSITY_BTA && h->a > 0) ? 1 : 0);
+  HIPCHK(hipGetLastError());
+  h->factored = -1;
+  h->inverted = 0;
+  return DH_OK;
+}
+
+int dh_cholesky(dh_handle* h, int sparsity) {
+  if (!h) return DH_EARG;
+  if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
+  const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
+  const int n = h->n, bp = h->bp, nbt = h->nbt;
+  const long bb = (long)bp * bp;
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->info, INT_MAX);
+  hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, h->stream, h->diag, (long)bp, h->dinv, h->pinv, h->info, 0);
+  for (int i = 0; i < n; ++i) {
+    BtaPtrs P;
+    P.D = h->diag + i * bb;
+    P.L = (i < n - 1) ? h->lower + i * bb : nullptr;
+    P.R = arrow ? h->arrow + (long)i * h->ap * bp : nullptr;
+    P.Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
+    P.Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * h->ap * bp : nullptr;
+    P.T = arrow ? h->tip : nullptr;
+    P.pinv = h->pinv;
+    P.dinv = h->dinv + (long)i * bp;
+    P.dinvn = (i < n - 1) ? h->dinv + (long)(i + 1) * bp : nullptr;
+    P.info = h->info;
+    P.bp = bp;
+    P.nbt = nbt;
+    P.row_base = i * h->b;  // logical row offset (padding rows never fail)
+    P.row_base_n = (i + 1) * h->b;
+    for (int p = 0; p < nbt; ++p) {
+      const int rows = (nbt - 1 - p) + (P.L ? nbt : 0) + (P.R ? 1 : 0);
+      if (rows > 0) hipLaunchKernelGGL(k_panel, dim3(rows), dim3(256), 0, h->stream, P, p);
+      if (p < nbt - 1) {
+        hipLaunchKernelGGL(k_update, dim3(rows, nbt - 1 - p), dim3(256), 0, h->stream, P, p);
+      }
+    }
+    const int ntri = P.L ? nbt * (nbt + 1) / 2 : 0;
+    const int narr = (P.L && P.R) ? nbt : 0;
+    const int nt = ntri + narr + (P.R ? 1 : 0);
+    if (nt > 0) hipLaunchKernelGGL(k_syrk, dim3(nt), dim3(256), 0, h->stream, P);
+  }
+  if (arrow) {
+    hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, h->stream, h->tip, (long)h->ap, h->tipdinv, h->tipinv,
+                       h->info, n * h->b);
+  }
+  HIPCHK(hipGetLastError());
+  h->factored = sparsity;
+  h->inverted = 0;
+  return DH_OK;
+}
+
+// Packed inverse of every diagonal factor, batched G blocks per launch.
+static int ensure_inverse(dh_handle* h) {
+  if (h->inverted) return DH_OK;
+  const int bp = h->bp;
+  for (int g0 = 0; g0 < h->n; g0 += h->G) {
+    const int gc = (h->n - g0 < h->G) ? h->n - g0 : h->G;
+    for (int d = (int)h->levels.size() - 1; d >= 0; --d) {
+      const NodeList& nl = h->levels[d];
+      int maxt = 0;
+      for (int k = 0; k < nl.cnt; ++k) {
+        int t = (nl.s[k] - nl.s1[k]) * nl.s1[k];
+        if (t > maxt) maxt = t;
+      }
+      dim3 grid(maxt, nl.cnt, gc);
+      hipLaunchKernelGGL(k_trtri_t, grid, dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
+      hipLaunchKernelGGL(k_trtri_inv, grid, dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  h->inverted = 1;
+  return DH_OK;
+}
+
+int dh_solve(dh_handle* h, double* rhs, int sparsity) {
+  if (!h || !rhs) return DH_EARG;
+  if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
+  if (h->factored != sparsity) return DH_ESTATE;
+  int rc = ensure_inverse(h);
+  if (rc) return rc;
+  const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
+  const int n = h->n, b = h->b, bp = h->bp, ap = arrow ? h->ap : 0, a = arrow ? h->a : 0;
+  const long bb = (long)bp * bp;
+  const long N = (long)n * bp + 64;
+  hipLaunchKernelGGL(k_pack, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, rhs, h->z, n, b, bp, a,
+                     64);
+  const int ncb = (bp + 255) / 256;
+  const int nch = (bp + DH_CCH - 1) / DH_CCH;
+  double* za = h->z + (long)n * bp;
+  // forward: y_i = inv(L_ii) (b_i - L_{i-1} y_{i-1}); arrow rows accumulate R_i y_i
+  for (int i = 0; i < n; ++i) {
+    double* zi = h->z + (long)i * bp;
+    const double* src = zi;
+    if (i > 0) {
+      hipLaunchKernelGGL(k_gemv_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, h->lower + (i - 1) * bb, (long)bp,
+                         bp, bp, zi - bp, zi, h->v, -1.0);
+      src = h->v;
+    } else {
+      HIPCHK(hipMemcpyAsync(h->v, zi, bp * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      src = h->v;
+    }
+    hipLaunchKernelGGL(k_gemv_col_part, dim3(ncb, nch), dim3(256), 0, h->stream, h->diag + i * bb, (long)bp, bp,
+                       bp, src, h->part, 1);
+    hipLaunchKernelGGL(k_fin_col, dim3(ncb), dim3(256), 0, h->stream, h->part, nch, bp, src,
+                       h->dinv + (long)i * bp, 1.0, zi);
+    if (arrow)
+      hipLaunchKernelGGL(k_gemv_row, dim3(16), dim3(256), 0, h->stream, h->arrow + (long)i * ap * bp, (long)bp, 64,
+                         bp, zi, (const double*)nullptr, h->arrpart + i * 64, 1.0);
+  }
+  if (arrow) {
+    hipLaunchKernelGGL(k_arrow_acc, dim3(1), dim3(64), 0, h->stream, h->arrpart, n, za);
+    hipLaunchKernelGGL(k_tip_apply, dim3(1), dim3(64), 0, h->stream, h->tipinv, za, h->v, 0);
+    hipLaunchKernelGGL(k_tip_apply, dim3(1), dim3(64), 0, h->stream, h->tipinv, h->v, za, 1);
+  }
+  // backward: x_i = inv(L_ii)^T (y_i - L_i^T x_{i+1} - R_i^T x_a)
+  for (int i = n - 1; i >= 0; --i) {
+    double* zi = h->z + (long)i * bp;
+    const bool hasL = i < n - 1;
+    if (hasL || arrow) {
+      const int nch1 = hasL ? nch : 0;
+      const int nch2 = arrow ? 1 : 0;
+      hipLaunchKernelGGL(k_gemv_col_part2, dim3(ncb, nch1 + nch2), dim3(256), 0, h->stream,
+                         hasL ? h->lower + i * bb : h->diag, hasL ? bp : 0, zi + bp,
+                         arrow ? h->arrow + (long)i * ap * bp : h->diag, arrow ? 64 : 0, za, (long)bp, bp, h->part);
+      hipLaunchKernelGGL(k_fin_col, dim3(ncb), dim3(256), 0, h->stream, h->part, nch1 + nch2, bp, zi,
+                         (const double*)nullptr, -1.0, h->v);
+    } else {
+      HIPCHK(hipMemcpyAsync(h->v, zi, bp * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    }
+    hipLaunchKernelGGL(k_upk_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, h->diag + i * bb,
+                       h->dinv + (long)i * bp, bp, h->v, zi);
+  }
+  const long Nl = (long)n * b + a;
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)((Nl + 255) / 256)), dim3(256), 0, h->stream, h->z, rhs, n, b, bp, a);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_logdet(dh_handle* h, int sparsity, double* out_dev) {
+  if (!h || !out_dev) return DH_EARG;
+  if (h->factored < 0) return DH_ESTATE;
+  const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
+  hipLaunchKernelGGL(k_logdet_part, dim3(h->n), dim3(256), 0, h->stream, h->diag, h->bp, h->b, h->part);
+  hipLaunchKernelGGL(k_logdet_fin, dim3(1), dim3(64), 0, h->stream, h->part, h->n, arrow ? h->tip : nullptr,
+                     h->a, h->ap, out_dev);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+static GTerm term(Opnd A, Opnd B, int K, int klo, int khi, double alpha) {
+  GTerm t;
+  t.A = A;
+  t.B = B;
+  t.K = K;
+  t.klo = klo;
+  t.khi = khi;
+  t.alpha = alpha;
+  return t;
+}
+
+static void launch_gemm(dh_handle* h, GArgs& g) {
+  const int tiles = g.lower ? g.mt * (g.mt + 1) / 2 : g.mt * g.ntl;
+  hipLaunchKernelGGL(k_gemm, dim3(tiles), dim3(256), 0, h->stream, g);
+}
+
+int dh_selected_inversion(dh_handle* h, int sparsity) {
+  if (!h) return DH_EARG;
+  if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
+  if (h->factored != sparsity) return DH_ESTATE;
+  int rc = ensure_inverse(h);
+  if (rc) return rc;
+  const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
+  const int n = h->n, bp = h->bp, nbt = h->nbt;
+  const long bb = (long)bp * bp;
+  double* W = h->W;
+  double* S = h->S;
+  if (arrow) {
+    // X_aa = inv(L_aa)^T inv(L_aa)
+    GArgs g;
+    memset(&g, 0, sizeof(g));
+    g.t[0] = term(Opnd{h->tipinv, 64, M_COL, nullptr}, Opnd{h->tipinv, 64, M_COL, nullptr}, 64, KLO_0, KHI_K, 1.0);
+    g.nterm = 1;
+    g.C = h->tip;
+    g.ldc = 64;
+    g.beta = 0.0;
+    g.mt = 1;
+    g.ntl = 1;
+    launch_gemm(h, g);
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double* D = h->diag + i * bb;
+    double* Lb = (i < n - 1) ? h->lower + i * bb : nullptr;
+    double* R = arrow ? h->arrow + (long)i * 64 * bp : nullptr;
+    double* Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
+    double* Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * 64 * bp : nullptr;
+    const double* dv = h->dinv + (long)i * bp;
+    const Opnd Linv_row{D, bp, M_UPK_ROW, dv};  // opB = Linv  /  opA = Linv^T
+    // 1. W = L_{i+1,i} Linv, Wa = R_i Linv
+    if (Lb) {
+      GArgs g;
+      memset(&g, 0, sizeof(g));
+      g.t[0] = term(Opnd{Lb, bp, M_ROW, nullptr}, Linv_row, bp, KLO_J, KHI_K, 1.0);
+      g.nterm = 1;
+      g.C = W;
+      g.ldc = bp;
+      g.mt = nbt;
+      g.ntl = nbt;
+      launch_gemm(h, g);
+    }
+    if (R) {
+      GArgs g;
+      memset(&g, 0, sizeof(g));
+      g.t[0] = term(Opnd{R, bp, M_ROW, nullptr}, Linv_row, bp, KLO_J, KHI_K, 1.0);
+      g.nterm = 1;
+      g.C = h->Wa;
+      g.ldc = bp;
+      g.mt = 1;
+      g.ntl = nbt;
+      launch_gemm(h, g);
+    }
+    // 2. X_{i+1,i} = -X_{i+1,i+1} W - X_{a,i+1}^T Wa ; X_{a,i} = -X_{a,i+1} W - X_aa Wa
+    if (Lb) {
+      GArgs g;
+      memset(&g, 0, sizeof(g));
+      g.t[0] = term(Opnd{Dn, bp, M_ROW, nullptr}, Opnd{W, bp, M_COL, nullptr}, bp, KLO_0, KHI_K, -1.0);
+      g.nterm = 1;
+      if (Rn) {
+        g.t[1] = term(Opnd{Rn, bp, M_COL, nullptr}, Opnd{h->Wa, bp, M_COL, nullptr}, 64, KLO_0, KHI_K, -1.0);
+        g.nterm = 2;
+      }
+      g.C = Lb;
+      g.ldc = bp;
+      g.mt = nbt;
+      g.ntl = nbt;
+      launch_gemm(h, g);
+    }
+    if (R) {
+      GArgs g;
+      memset(&g, 0, sizeof(g));
+      int k = 0;
+      if (Rn) g.t[k++] = term(Opnd{Rn, bp, M_ROW, nullptr}, Opnd{W, bp, M_COL, nullptr}, bp, KLO_0, KHI_K, -1.0);
+      g.t[k++] = term(Opnd{h->tip, 64, M_ROW, nullptr}, Opnd{h->Wa, bp, M_COL, nullptr}, 64, KLO_0, KHI_K, -1.0);
+      g.nterm = k;
+      g.C = R;
+      g.ldc = bp;
+      g.mt = 1;
+      g.ntl = nbt;
+      launch_gemm(h, g);
+    }
+    // 3. X_ii = Linv^T Linv - W^T X_{i+1,i} - Wa^T X_{a,i}  (lower tiles, into S)
+    {
+      GArgs g;
+      memset(&g, 0, sizeof(g));
+      int k = 0;
+      g.t[k++] = term(Linv_row, Linv_row, bp, KLO_MAXIJ, KHI_K, 1.0);
+      if (Lb) g.t[k++] = term(Opnd{W, bp, M_COL, nullptr}, Opnd{Lb, bp, M_COL, nullptr}, bp, KLO_0, KHI_K, -1.0);
+      if (R) g.t[k++] = term(Opnd{h->Wa, bp, M_COL, nullptr}, Opnd{R, bp, M_COL, nullptr}, 64, KLO_0, KHI_K, -1.0);
+      g.nterm = k;
+      g.C = S;
+      g.ldc = bp;
+      g.mt = nbt;
+      g.ntl = nbt;
+      g.lower = 1;
+      launch_gemm(h, g);
+    }
+    // 4. D_i = symmetric X_ii
+    hipLaunchKernelGGL(k_symm_copy, dim3((unsigned)((bb + 255) / 256)), dim3(256), 0, h->stream, D, S, bp);
+  }
+  HIPCHK(hipGetLastError());
+  h->factored = -1;
+  h->inverted = 0;
+  return DH_OK;
+}
+
+int dh_get_info(dh_handle* h, int* info) {
+  if (!h || !info) return DH_EARG;
+  int v = 0;
+  HIPCHK(hipMemcpyAsync(&v, h->info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  *info = (v == INT_MAX) ? 0 : v;
+  return DH_OK;
+}
+
+int dh_qc_matvec(dh_handle* h, const dh_model* m, const double* coef, double tau, const double* x, double* yp,
+                 double* yl) {
+  if (!h || !coef || !x || !yp || !yl || !model_ok(h, m)) return DH_EARG;
+  Coef cf;
+  memcpy(cf.c, coef, sizeof(cf.c));
+  const long nst = (long)m->ns * m->nt;
+  const unsigned g = (unsigned)((nst + 255) / 256);
+  hipLaunchKernelGGL(k_qp_matvec, dim3(g), dim3(256), 0, h->stream, *m, cf, x, yp);
+  hipLaunchKernelGGL(k_ata_matvec, dim3(g), dim3(256), 0, h->stream, *m, tau, x, yl);
+  if (m->nb > 0) hipLaunchKernelGGL(k_ata_matvec_fe, dim3(m->nb), dim3(256), 0, h->stream, *m, tau, x, yp, yl);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_scale(dh_handle* h, const double* x, double s, double* y, long long len) {
+  if (!h || !x || !y || len < 0) return DH_EARG;
+  if (len == 0) return DH_OK;
+  hipLaunchKernelGGL(k_scale, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, h->stream, x, s, y, (long)len);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_dot(dh_handle* h, const double* x, const double* y, long long len, double* out_dev) {
+  if (!h || !x || !y || !out_dev || len < 0) return DH_EARG;
+  long nb = (len + 255) / 256;
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_dot_part, dim3((unsigned)nb), dim3(256), 0, h->stream, x, y, (long)len, h->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, h->stream, h->part, (int)nb, 1, 1, out_dev);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_contract_fem(dh_handle* h, const dh_model* m, double* out_dev) {
+  if (!h || !out_dev || !model_ok(h, m)) return DH_EARG;
+  const int nq = 2 * h->n - 1;
+  const int nrb = (m->ns + 255) / 256;
+  hipLaunchKernelGGL(k_contract_fem, dim3(nrb, nq), dim3(256), 0, h->stream, h->diag, h->lower, h->n, h->bp, *m,
+                     h->part);
+  hipLaunchKernelGGL(k_sum_seg, dim3((nq * 4 + 255) / 256), dim3(256), 0, h->stream, h->part, nq, nrb, 4,
+                     out_dev);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_contract_fem_vec(dh_handle* h, const dh_model* m, const double* x, double* out_dev) {
+  if (!h || !x || !out_dev || !model_ok(h, m)) return DH_EARG;
+  const int nq = 2 * h->n - 1;
+  const int nrb = (m->ns + 255) / 256;
+  hipLaunchKernelGGL(k_contract_fem_vec, dim3(nrb, nq), dim3(256), 0, h->stream, x, h->n, *m, h->part);
+  hipLaunchKernelGGL(k_sum_seg, dim3((nq * 4 + 255) / 256), dim3(256), 0, h->stream, h->part, nq, nrb, 4,
+                     out_dev);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_contract_ata(dh_handle* h, const dh_model* m, double* out_dev) {
+  if (!h || !out_dev || !model_ok(h, m)) return DH_EARG;
+  const long nst = (long)m->ns * m->nt;
+  const unsigned g = (unsigned)((nst + 255) / 256);
+  const bool arrow = h->a > 0 && m->nb == h->a;
+  hipLaunchKernelGGL(k_contract_ata, dim3(g), dim3(256), 0, h->stream, h->diag, h->lower,
+                     arrow ? h->arrow : nullptr, arrow ? h->tip : nullptr, h->n, h->bp, h->a, h->ap, *m, h->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, h->stream, h->part, (int)g, 1, 1, out_dev);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const double* gam, double* out_dev) {
+  if (!h || !lam || !gam || !out_dev || !model_ok(h, m)) return DH_EARG;
+  const unsigned g = (unsigned)((m->ns + 255) / 256);
+  double* lbuf = h->eigbuf;
+  double* dbuf = h->eigbuf + (long)h->n * h->bp;
+  hipLaunchKernelGGL(k_prior_eig, dim3(g), dim3(256), 0, h->stream, *m, lam, gam[0], gam[1], gam[2], lbuf, dbuf,
+                     h->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, h->stream, h->part, (int)g, 4, 4, out_dev);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,239 @@
+"""One f(theta) evaluation of DALIA's Gaussian spatio-temporal model on the GPU,
+its finite-difference gradient and its adjoint gradient.
+
+`STInla.f` follows `DALIA._evaluate_f` (`src/dalia/core/dalia.py:603-745`,
+Gaussian branch) term by term:
+
+    f = [1/2 logdet Qc - 1/2 mu^T Qc mu]                      (:1110-1168)
+        - [log pi(theta) + lik(eta=0) + 1/2 logdet Qp|BT]     (:679-683, :1074-1108)
+
+with Qc = Qp + e^tau A^T A assembled on the device (`dh_assemble`),
+mu = Qc^-1 (e^tau A^T y) by pobtaf + pobtas, the quadratic form by an SpMV
+with Qc (`dh_qc_matvec`, as `:1161`), and the prior log-determinant by a BT
+factorisation of Qp that drops the arrowhead (`:1100-1103`), reusing the same
+buffers in the same order (Qc factor first, then Qp).
+
+`STInla.fd_gradient` is the reference's 2d+1 central-difference stencil
+(`core/dalia.py:545-581`).  `STInla.value_and_grad` replaces it by the adjoint
+formula
+
+    df/dtheta_j = 1/2 tr(Qc^-1 dQc) - db^T mu + 1/2 mu^T dQc mu
+                  - 1/2 d logdet Qp|BT - d log pi - d lik,
+
+where the trace is a contraction of the selected-inverse blocks (pobtasi)
+against the sparse Kronecker blocks of dQ (`dh_contract_fem`) and against
+A^T A (`dh_contract_ata`), and the prior log-determinant and its derivatives
+come from the Kronecker-eigen closed form (`dh_prior_eig`) when the FEM
+matrices satisfy it, or else from a BT factorisation plus selected inversion
+of Qp.  `FTheta` wraps it as a `torch.autograd.Function`: forward = factor,
+solve, log-determinants; backward = selected inversion + contractions.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from dalia_amd import _lib, hyper
+from dalia_amd.model import DeviceSTModel
+from dalia_amd.solver import HipBTASolver
+
+
+class STInla:
+    def __init__(self, data, device="cuda", closed_form_prior: bool = True):
+        self.data = data
+        self.dm = DeviceSTModel(data, device=device, eigen=closed_form_prior)
+        self.solver = HipBTASolver(diagonal_blocksize=data.ns, n_diag_blocks=data.nt,
+                                   arrowhead_blocksize=data.nb)
+        self.lib = self.solver.lib
+        self.h = self.solver.h
+        N = self.dm.n_latent
+        self.mu = torch.zeros(N, dtype=torch.float64, device="cuda")
+        self.yp = torch.zeros(N, dtype=torch.float64, device="cuda")
+        self.yl = torch.zeros(N, dtype=torch.float64, device="cuda")
+        self.scal = torch.zeros(16, dtype=torch.float64, device="cuda")
+        nq = 2 * data.nt - 1
+        self.cx = torch.zeros(nq * 4, dtype=torch.float64, device="cuda")
+        self.cmu = torch.zeros(nq * 4, dtype=torch.float64, device="cuda")
+        self.eig = torch.zeros(4, dtype=torch.float64, device="cuda")
+        self._state = None
+        self.n_f_evals = 0
+
+    # ------------------------------------------------------------------
+    def _assemble(self, theta, sparsity: str, with_like: bool):
+        coef = DeviceSTModel.coef_array(theta)
+        cptr = coef.ctypes.data_as(_lib.c_double_p)
+        _lib.check(self.lib.dh_assemble(self.h, self.dm.ref(), cptr, self.dm.tau_scale(theta),
+                                        _lib.SPARSITY[sparsity], int(with_like)), "dh_assemble")
+
+    def _conditional(self, theta):
+        """Assemble Qc, factor, solve for mu; queue logdet and the quadratic-form pieces."""
+        L = self.lib
+        self._assemble(theta, "bta", True)
+        _lib.check(L.dh_cholesky(self.h, 1), "dh_cholesky")
+        et = self.dm.tau_scale(theta)
+        N = self.dm.n_latent
+        _lib.check(L.dh_scale(self.h, self.dm.aty.data_ptr(), et, self.mu.data_ptr(), N), "dh_scale")
+        _lib.check(L.dh_solve(self.h, self.mu.data_ptr(), 1), "dh_solve")
+        _lib.check(L.dh_logdet(self.h, 1, self.scal[0:1].data_ptr()), "dh_logdet")
+        coef = DeviceSTModel.coef_array(theta)
+        _lib.check(L.dh_qc_matvec(self.h, self.dm.ref(), coef.ctypes.data_as(_lib.c_double_p), et,
+                                  self.mu.data_ptr(), self.yp.data_ptr(), self.yl.data_ptr()), "dh_qc_matvec")
+        _lib.check(L.dh_dot(self.h, self.mu.data_ptr(), self.yp.data_ptr(), N, self.scal[1:2].data_ptr()), "dh_dot")
+        _lib.check(L.dh_dot(self.h, self.mu.data_ptr(), self.yl.data_ptr(), N, self.scal[2:3].data_ptr()), "dh_dot")
+        _lib.check(L.dh_dot(self.h, self.mu.data_ptr(), self.dm.aty.data_ptr(), N,
+                            self.scal[3:4].data_ptr()), "dh_dot")
+
+    def _scalars(self, theta):
+        lp, glp = hyper.log_prior_and_grad(theta, self.data.priors)
+        lik = hyper.likelihood_eta0(float(theta[-1]), self.dm.n_obs, self.dm.yty)
+        return lp, glp, lik
+
+    # ------------------------------------------------------------------
+    def f(self, theta) -> float:
+        """Plain drop-in f(theta): two block factorisations as in the reference."""
+        theta = np.asarray(theta, dtype=np.float64)
+        self._conditional(theta)
+        self._assemble(theta, "bt", False)
+        _lib.check(self.lib.dh_cholesky(self.h, 0), "dh_cholesky")
+        _lib.check(self.lib.dh_logdet(self.h, 0, self.scal[4:5].data_ptr()), "dh_logdet")
+        s = self.scal[:5].cpu().numpy()
+        ld_c, quad_p, quad_l, _, ld_p = s
+        if np.isnan(ld_c) or np.isnan(ld_p):
+            raise ValueError("Logdet is NaN. Check the input matrix.")
+        lp, _, lik = self._scalars(theta)
+        f = 0.5 * ld_c - 0.5 * (quad_p + quad_l) - (lp + lik + 0.5 * ld_p)
+        self.n_f_evals += 1
+        self._state = None
+        if np.isnan(f):
+            raise ValueError("f(theta) is NaN")
+        return float(f)
+
+    def fd_gradient(self, theta, h: float = 1e-4):
+        """Reference central differences: 2d+1 plain evaluations, reverse order."""
+        theta = np.asarray(theta, dtype=np.float64)
+        d = theta.size
+        n_eval = 2 * d + 1
+        tm = np.repeat(theta.reshape(-1, 1), n_eval, axis=1)
+        tm[:, 1:1 + d] += h * np.eye(d)
+        tm[:, d + 1:] -= h * np.eye(d)
+        fv = np.zeros(n_eval)
+        for k in range(n_eval - 1, -1, -1):
+            fv[k] = self.f(tm[:, k])
+        return fv[0], (fv[1:1 + d] - fv[d + 1:]) / (2 * h)
+
+    # ------------------------------------------------------------------
+    def _prior_logdet_terms(self, theta):
+        """(logdet Qp|BT, d/dtheta_{0..2}) via the Kronecker-eigen closed form."""
+        gs, gt, gst = hyper.interpretable2compute(float(theta[0]), float(theta[1]), float(theta[2]))
+        gam = np.array([gs, gt, gst], dtype=np.float64)
+        _lib.check(self.lib.dh_prior_eig(self.h, self.dm.ref(), self.dm.lam.data_ptr(),
+                                         gam.ctypes.data_as(_lib.c_double_p), self.eig.data_ptr()), "dh_prior_eig")
+
+    def forward(self, theta):
+        """Forward half of the adjoint: f(theta) with the closed-form prior term."""
+        theta = np.asarray(theta, dtype=np.float64)
+        self._conditional(theta)
+        if self.dm.closed_form:
+            self._prior_logdet_terms(theta)
+            s = self.scal[:4].cpu().numpy()
+            e = self.eig.cpu().numpy()
+            ld_p = self.dm.nt * self.dm.logdet_c0 + e[0]
+            dld_p = hyper.DGAMMA.T @ e[1:4]
+        else:
+            s = self.scal[:4].cpu().numpy()
+            ld_p, dld_p = None, None
+        ld_c, quad_p, quad_l, bmu = s
+        if np.isnan(ld_c):
+            raise ValueError("Logdet is NaN. Check the input matrix.")
+        lp, glp, lik = self._scalars(theta)
+        self._state = dict(theta=theta.copy(), ld_c=ld_c, quad_p=quad_p, quad_l=quad_l, bmu=bmu,
+                           ld_p=ld_p, dld_p=dld_p, lp=lp, glp=glp, lik=lik)
+        if ld_p is None:
+            return None
+        return float(0.5 * ld_c - 0.5 * (quad_p + quad_l) - (lp + lik + 0.5 * ld_p))
+
+    def backward(self):
+        """Gradient of the last `forward` (selected inversion + contractions)."""
+        st = self._state
+        if st is None:
+            raise RuntimeError("backward() needs a preceding forward() on this engine")
+        theta = st["theta"]
+        L = self.lib
+        et = self.dm.tau_scale(theta)
+        # mu^T G_m mu blocks (before the buffers are overwritten)
+        _lib.check(L.dh_contract_fem_vec(self.h, self.dm.ref(), self.mu.data_ptr(), self.cmu.data_ptr()),
+                   "dh_contract_fem_vec")
+        _lib.check(L.dh_selected_inversion(self.h, 1), "dh_selected_inversion")
+        _lib.check(L.dh_contract_fem(self.h, self.dm.ref(), self.cx.data_ptr()), "dh_contract_fem")
+        _lib.check(L.dh_contract_ata(self.h, self.dm.ref(), self.scal[5:6].data_ptr()), "dh_contract_ata")
+        nq = 2 * self.dm.nt - 1
+        cx = self.cx.cpu().numpy().reshape(nq, 4)
+        cmu = self.cmu.cpu().numpy().reshape(nq, 4)
+        tr_ata = float(self.scal[5].item())
+        ld_p, dld_p = st["ld_p"], st["dld_p"]
+        if ld_p is None:
+            ld_p, dld_p = self._prior_by_selected_inversion(theta)
+        dc = hyper.kron_coefficients_grad(theta)
+        nt = self.dm.nt
+        mult = np.concatenate([np.ones(nt), 2.0 * np.ones(nt - 1)])[:, None]
+        g = np.zeros(theta.size)
+        for j in range(3):
+            w = self.dm.block_weights(dc[j]) * mult
+            g[j] = 0.5 * float(np.sum(w * cx)) + 0.5 * float(np.sum(w * cmu)) - 0.5 * dld_p[j]
+        g[3] = 0.5 * et * tr_ata - et * st["bmu"] + 0.5 * st["quad_l"]
+        g[3] -= 0.5 * self.dm.n_obs - 0.5 * et * self.dm.yty
+        g -= st["glp"]
+        f = 0.5 * st["ld_c"] - 0.5 * (st["quad_p"] + st["quad_l"]) - (st["lp"] + st["lik"] + 0.5 * ld_p)
+        self._state = None
+        return float(f), g
+
+    def _prior_by_selected_inversion(self, theta):
+        """logdet Qp|BT and its theta-gradient by pobtf + pobtsi (general FEM inputs)."""
+        L = self.lib
+        self._assemble(theta, "bt", False)
+        _lib.check(L.dh_cholesky(self.h, 0), "dh_cholesky")
+        _lib.check(L.dh_logdet(self.h, 0, self.scal[6:7].data_ptr()), "dh_logdet")
+        _lib.check(L.dh_selected_inversion(self.h, 0), "dh_selected_inversion")
+        cp = torch.zeros_like(self.cx)
+        _lib.check(L.dh_contract_fem(self.h, self.dm.ref(), cp.data_ptr()), "dh_contract_fem")
+        nq = 2 * self.dm.nt - 1
+        cp = cp.cpu().numpy().reshape(nq, 4)
+        dc = hyper.kron_coefficients_grad(theta)
+        mult = np.concatenate([np.ones(self.dm.nt), 2.0 * np.ones(self.dm.nt - 1)])[:, None]
+        d = np.array([float(np.sum(self.dm.block_weights(dc[j]) * mult * cp)) for j in range(3)])
+        return float(self.scal[6].item()), d
+
+    def value_and_grad(self, theta):
+        self.forward(theta)
+        return self.backward()
+
+
+class FTheta(torch.autograd.Function):
+    """f(theta) as an autograd node; backward runs the adjoint on the device.
+
+    Forward factors Qc, solves for mu and evaluates the log-determinants;
+    backward overwrites the factor with the selected inverse and contracts it.
+    """
+
+    @staticmethod
+    def forward(ctx, theta: torch.Tensor, engine: STInla):
+        th = theta.detach().cpu().double().numpy()
+        f_val = engine.forward(th)
+        if f_val is None:  # no closed form: value needs the BT factorisation of Qp
+            f_val = engine.f(th)
+            engine.forward(th)
+        ctx.engine = engine
+        ctx.theta = th
+        return torch.tensor(f_val, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        st = ctx.engine._state
+        if st is None or not np.array_equal(st["theta"], ctx.theta):
+            ctx.engine.forward(ctx.theta)
+        _, g = ctx.engine.backward()
+        return grad_out * torch.as_tensor(g, dtype=torch.float64), None

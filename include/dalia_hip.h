@@ -22,6 +22,9 @@
  *                            src/dalia/core/model.py:306-423,
  *                            src/dalia/submodels/spatio_temporal.py:114-149,
  *                            src/dalia/solvers/structured_solver.py:192-468
+ *   dh_scatter_coo           SerinvSolver._spmatrix_to_bta / _spmatrix_to_bt + the block
+ *                            mapping kernel   structured_solver.py:252-468,
+ *                            src/dalia/kernels/blockmapping.py:12-169
  *   dh_cholesky              SerinvSolver.cholesky (pobtaf / pobtf)   structured_solver.py:72-100
  *   dh_solve                 SerinvSolver.solve (pobtas N then C)     structured_solver.py:102-149
  *   dh_logdet                SerinvSolver.logdet                      structured_solver.py:151-166
@@ -91,6 +94,14 @@ long long dh_memory_bytes(const dh_handle* h);
  * coef: [3][4] Kronecker coefficients.  sparsity BT drops the arrowhead. */
 int dh_assemble(dh_handle* h, const dh_model* m, const double* coef, double tau_scale,
                 int sparsity, int with_likelihood);
+
+/* Fill the blocks from a symmetric sparse matrix of order n*b (+ a) given as
+ * COO with global 0-based indices (device arrays, no duplicates): diagonal
+ * blocks take both triangles, then the sub-diagonal blocks, the arrow rows
+ * and the tip; entries outside the BTA pattern are ignored, as are the
+ * arrowhead rows for BT. */
+int dh_scatter_coo(dh_handle* h, long long nnz, const long long* rows, const long long* cols,
+                   const double* vals, int sparsity);
 
 /* In-place BTA (pobtaf) or BT (pobtf) Cholesky.  A failing pivot shows up as
  * NaN in dh_logdet and as a positive value from dh_get_info. */

@@ -1,0 +1,90 @@
+"""GPU parity of assembly, f(theta) and the adjoint gradient against the CPU oracle."""
+
+import numpy as np
+import pytest
+import torch
+
+from dalia_amd import synth
+from oracle import inla as oinla
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module", params=[(6, 5, 3, 0), (9, 4, 2, 3)])
+def small(request):
+    m, nt, nb, seed = request.param
+    return synth.make_st_model(m, nt, nb, seed)
+
+
+def test_assembly_matches_oracle(small):
+    from dalia_amd.inla import STInla
+    d = small
+    print(f"case assembly ns={d.ns} nt={d.nt} nb={d.nb}", flush=True)
+    eng = STInla(d)
+    th = d.theta
+    Qc = oinla.qcond_sparse(d, th)
+    D, L, R, T = oinla.sparse_to_bta(Qc, d.nt, d.ns, d.nb, "bta")
+    eng._assemble(th, "bta", True)
+    s = eng.solver
+    assert rel(s.A_diagonal_blocks.cpu().numpy(), D) < 1e-14
+    assert rel(s.A_lower_diagonal_blocks.cpu().numpy(), L) < 1e-14
+    assert rel(s.A_arrow_bottom_blocks.cpu().numpy(), R) < 1e-14
+    assert rel(s.A_arrow_tip_block.cpu().numpy(), T) < 1e-14
+    Qp = oinla.qprior_sparse(d, th)
+    D, L, _, _ = oinla.sparse_to_bta(Qp, d.nt, d.ns, d.nb, "bt")
+    eng._assemble(th, "bt", False)
+    assert rel(s.A_diagonal_blocks.cpu().numpy(), D) < 1e-14
+    assert rel(s.A_lower_diagonal_blocks.cpu().numpy(), L) < 1e-14
+
+
+def test_f_theta_matches_oracle(small):
+    from dalia_amd.inla import STInla
+    d = small
+    print(f"case f ns={d.ns} nt={d.nt}", flush=True)
+    eng = STInla(d)
+    for th in (d.theta, d.theta + np.array([0.1, -0.2, 0.05, 0.3])):
+        f_ref, parts = oinla.f_theta(d, th, return_parts=True)
+        f = eng.f(th)
+        assert abs(f - f_ref) <= 1e-10 * abs(f_ref)
+        assert rel(eng.mu.cpu().numpy(), parts["mu"]) < 1e-9
+
+
+def test_adjoint_gradient(small):
+    from dalia_amd.inla import STInla
+    d = small
+    print(f"case adjoint ns={d.ns} nt={d.nt}", flush=True)
+    eng = STInla(d)
+    th = d.theta
+    f_ref, g_ref = oinla.adjoint_gradient(d, th)
+    f, g = eng.value_and_grad(th)
+    assert abs(f - f_ref) <= 1e-10 * abs(f_ref)
+    assert rel(g, g_ref) < 1e-8
+    f0, gfd = eng.fd_gradient(th, h=1e-4)
+    assert rel(g, gfd) < 1e-5
+
+
+def test_adjoint_without_closed_form(small):
+    """General FEM inputs: prior term by pobtf + pobtsi instead of the eigen form."""
+    from dalia_amd.inla import STInla
+    d = small
+    eng = STInla(d, closed_form_prior=False)
+    f_ref, g_ref = oinla.adjoint_gradient(d, d.theta)
+    f, g = eng.value_and_grad(d.theta)
+    assert abs(f - f_ref) <= 1e-10 * abs(f_ref)
+    assert rel(g, g_ref) < 1e-8
+
+
+def test_autograd_function(small):
+    from dalia_amd.inla import FTheta, STInla
+    d = small
+    eng = STInla(d)
+    th = torch.tensor(d.theta, dtype=torch.float64, requires_grad=True)
+    f = FTheta.apply(th, eng)
+    f.backward()
+    f_ref, g_ref = oinla.adjoint_gradient(d, d.theta)
+    assert abs(float(f) - f_ref) <= 1e-10 * abs(f_ref)
+    assert rel(th.grad.numpy(), g_ref) < 1e-8
