@@ -107,12 +107,13 @@ __device__ __forceinline__ void dh_mma(dh_d4 acc[2][2], const Opnd& A, int i0, c
 }
 
 // C[i0.., j0..] = alpha*acc + beta*C (trans: C[j][i] instead of C[i][j]).
-// Lane l, item r of a 16x16 MFMA block holds row 4*(l/16)+r, column l%16.
+// Lane l, item r of the 16x16 f64 MFMA result holds row 4*r + l/16 and
+// column l%16 (measured on gfx950 with one-hot operands).
 __device__ __forceinline__ void dh_store(double* C, long ldc, int i0, int j0, const dh_d4 acc[2][2],
                                          double alpha, double beta, bool trans) {
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const int rbase = i0 + (w >> 1) * 32 + 4 * (lane >> 4);
+  const int rbase = i0 + (w >> 1) * 32 + (lane >> 4);
   const int cbase = j0 + (w & 1) * 32 + (lane & 15);
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
@@ -120,7 +121,7 @@ __device__ __forceinline__ void dh_store(double* C, long ldc, int i0, int j0, co
     for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = rbase + bi * 16 + r;
+        const int i = rbase + bi * 16 + 4 * r;
         const int j = cbase + bj * 16;
         double* p = trans ? (C + (long)j * ldc + i) : (C + (long)i * ldc + j);
         double v = alpha * acc[bi][bj][r];
