@@ -1,0 +1,238 @@
+"""Benchmark: adjoint gradient evaluations of f(theta) per second on one MI355X.
+
+Workload (default): the spatio-temporal Gaussian model on a 45x45 grid
+(ns = 2025), nt = 100, nb = 6, 4*ns*nt observations, seed 1, FP64.  One
+"step" is one adjoint gradient evaluation: assembly of Qc, pobtaf, pobtas,
+logdet, quadratic form, Kronecker-eigen prior term, pobtasi and the
+selected-inverse contractions.  Inputs are resident in HBM before timing.
+
+Also reported on the same line: the plain GPU central-difference gradient
+(2d+1 = 9 plain f(theta) evaluations, each with a BTA and a BT
+factorisation), pobtaf TFLOP/s, the roofline of the dominant kernel (HIP
+events on the library's stream) and a CPU baseline (the oracle on the host
+cores, on a bounded sample, scaled to the full workload).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("∇f(θ) evals/s adjoint vs central-diff, ns=2025 nt=100 nb=6 FP64; pobtaf TFLOP/s")
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def f_pobtaf(n, b, a):
+    return n * b ** 3 / 3 + 2 * (n - 1) * b ** 3 + n * a * b ** 2 + 2 * (n - 1) * a * b ** 2 + n * a * a * b + a ** 3 / 3
+
+
+def f_pobtasi(n, b, a):
+    return (n - 1) * (13.0 / 3.0) * b ** 3 + (2.0 / 3.0) * b ** 3 + (n - 1) * (6 * a * b * b + 4 * a * a * b)
+
+
+def syrk_flops_per_launch(n, b, a):
+    """Algorithmic flops of the n launches of k_syrk (SYRK counted once)."""
+    per = [(b ** 3 + 2 * a * b * b + a * a * b) if i < n - 1 else a * a * b for i in range(n)]
+    return float(np.mean(per))
+
+
+def update_flops_per_launch(n, b, a):
+    """Algorithmic flops of the k_update launches (right-looking rank-64 updates)."""
+    tot, cnt = 0.0, 0
+    nbt = (b + 63) // 64
+    for i in range(n):
+        lower = b if i < n - 1 else 0
+        for p in range(nbt - 1):
+            kp = min(64, b - 64 * p)
+            m = b - 64 * (p + 1)
+            if m <= 0:
+                continue
+            tot += 2.0 * kp * (m * (m + 1) / 2 + lower * m + a * m)
+            cnt += 1
+    return tot / max(cnt, 1), cnt
+
+
+def cpu_baseline(workload_args, n_full: int, threads: int):
+    """Oracle f(theta) on a bounded sample (same ns, nb; nt_sample time steps), scaled to n_full.
+
+    The CPU gradient is the reference's: 2d+1 = 9 plain f evaluations.
+    """
+    from threadpoolctl import threadpool_limits
+
+    from dalia_amd import synth
+    from oracle import inla as oinla
+
+    m, nt, nb, seed = workload_args
+    nt_s = 4
+    d = synth.make_st_model(m, nt_s, nb, seed)
+    with threadpool_limits(limits=threads):
+        oinla.f_theta(d, d.theta)  # warm-up
+        reps = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            oinla.f_theta(d, d.theta)
+            reps.append(time.perf_counter() - t0)
+    t_f = float(np.median(reps)) * n_full / nt_s
+    return {
+        "value": 1.0 / (9.0 * t_f),
+        "unit": "grad_evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle f(theta) (scipy/LAPACK, BTA + BT factorisations) on ns={m * m}, nt={nt_s}, nb={nb}, "
+                   f"median of {len(reps)} after 1 warm-up = {np.median(reps):.3f} s, scaled linearly in nt to "
+                   f"nt={n_full} and by 9 evaluations per central-difference gradient"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="st_ns2025_nt100_nb6")
+    ap.add_argument("--fd", type=int, default=1, help="central-difference gradients to time (0: skip)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-kernel", default="k_syrk")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(one f(theta) evaluation per GPU; see the sharding design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from dalia_amd import _lib, synth
+    from dalia_amd.inla import STInla
+
+    wl = synth.WORKLOADS[args.workload]
+    m, nt, nb, seed = wl
+    ns = m * m
+    log(f"building synthetic model {args.workload} (ns={ns}, nt={nt}, nb={nb}, seed={seed})")
+    t0 = time.perf_counter()
+    data = synth.make_st_model(m, nt, nb, seed)
+    eng = STInla(data)
+    torch.cuda.synchronize()
+    log(f"set-up {time.perf_counter() - t0:.1f} s (closed-form prior: {eng.dm.closed_form})")
+    theta = data.theta.copy()
+    lib = eng.lib
+
+    for w in range(args.warmup):
+        f, g = eng.value_and_grad(theta)
+        log(f"warm-up {w}: f={f:.10g}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        f, g = eng.value_and_grad(theta)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / args.steps
+    log(f"adjoint gradient: {t_adj * 1e3:.1f} ms/eval, f={f:.12g}, grad={g}")
+
+    # plain GPU central differences (reference method, h = 1e-4)
+    fd_info = {}
+    if args.fd > 0:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.fd):
+            f_fd, g_fd = eng.fd_gradient(theta, h=1e-4)
+        torch.cuda.synchronize()
+        t_fd = (time.perf_counter() - t0) / args.fd
+        relerr = float(np.linalg.norm(g - g_fd) / np.linalg.norm(g_fd))
+        fd_info = {"fd_grad_evals_per_s": 1.0 / t_fd, "fd_ms_per_grad": t_fd * 1e3,
+                   "adjoint_vs_fd_speedup": t_fd / t_adj, "adjoint_vs_fd_relerr": relerr}
+        log(f"central differences: {t_fd * 1e3:.1f} ms/gradient, adjoint speed-up {t_fd / t_adj:.2f}x, "
+            f"rel. difference {relerr:.2e}")
+
+    # pobtaf alone, HIP events on the library's stream (torch's current stream)
+    eng._assemble(theta, "bta", True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for r in range(3):
+        eng._assemble(theta, "bta", True)
+        ev0.record()
+        _lib.check(lib.dh_cholesky(eng.h, 1), "dh_cholesky")
+        ev1.record()
+        torch.cuda.synchronize()
+        times.append(ev0.elapsed_time(ev1))
+    t_pobtaf = float(np.median(times)) * 1e-3
+    F = f_pobtaf(nt, ns, nb)
+    pobtaf_tflops = F / t_pobtaf / 1e12
+    log(f"pobtaf: {t_pobtaf * 1e3:.2f} ms, {pobtaf_tflops:.2f} TFLOP/s ({100 * pobtaf_tflops / FP64_PEAK_TFLOPS:.1f}% "
+        "of FP64 matrix peak)")
+
+    # roofline of the dominant kernel, per launch, events around each launch
+    kname = args.roofline_kernel
+    kind = _lib.TIMED_KERNELS[kname]
+    eng._assemble(theta, "bta", True)
+    _lib.check(lib.dh_timing_enable(eng.h, kind), "dh_timing_enable")
+    if kname == "k_gemm":
+        _lib.check(lib.dh_cholesky(eng.h, 1), "dh_cholesky")
+        _lib.check(lib.dh_selected_inversion(eng.h, 1), "dh_selected_inversion")
+    else:
+        _lib.check(lib.dh_cholesky(eng.h, 1), "dh_cholesky")
+    tot = ctypes.c_double()
+    cnt = ctypes.c_longlong()
+    _lib.check(lib.dh_timing_read(eng.h, ctypes.byref(tot), ctypes.byref(cnt)), "dh_timing_read")
+    _lib.check(lib.dh_timing_enable(eng.h, -1), "dh_timing_enable")
+    avg_s = tot.value * 1e-3 / max(cnt.value, 1)
+    if kname == "k_syrk":
+        per = syrk_flops_per_launch(nt, ns, nb)
+    elif kname == "k_update":
+        per, _ = update_flops_per_launch(nt, ns, nb)
+    else:
+        per = float("nan")
+    achieved = per / avg_s / 1e12
+    roofline = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "launches": int(cnt.value), "avg_launch_us": avg_s * 1e6, "flops_per_launch": per}
+    log(f"roofline {kname}: {cnt.value} launches, avg {avg_s * 1e6:.1f} us, {achieved:.2f} TFLOP/s")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        log(f"CPU baseline on {threads} threads (bounded sample)")
+        cpu = cpu_baseline(wl, nt, threads)
+        log(f"CPU baseline {cpu['value']:.3g} grad evals/s")
+
+    out = {
+        "metric": METRIC,
+        "value": 1.0 / t_adj,
+        "unit": "grad_evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_adj * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "ns": ns, "nt": nt, "nb": nb, "seed": seed, "theta_dim": 4,
+                   "n_obs": data.n_obs},
+        "pobtaf_ms": t_pobtaf * 1e3,
+        "pobtaf_tflops": pobtaf_tflops,
+        "pobtaf_frac_of_peak": pobtaf_tflops / FP64_PEAK_TFLOPS,
+        **fd_info,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -67,7 +67,12 @@ SIGNATURES = {
     "dh_contract_fem_vec": (_I, [_P, ctypes.POINTER(DhModel), _P, _P]),
     "dh_contract_ata": (_I, [_P, ctypes.POINTER(DhModel), _P]),
     "dh_prior_eig": (_I, [_P, ctypes.POINTER(DhModel), _P, c_double_p, _P]),
+    "dh_timing_enable": (_I, [_P, _I]),
+    "dh_timing_read": (_I, [_P, c_double_p, ctypes.POINTER(_LL)]),
 }
+
+#: kernel kinds of dh_timing_enable
+TIMED_KERNELS = {"k_syrk": 0, "k_update": 1, "k_gemm": 2, "k_panel": 3, "k_leaf": 4, "k_assemble_rows": 5}
 
 _lib = None
 

@@ -31,12 +31,13 @@ struct Opnd {
   const double* dinv;
 };
 
+template <int MODE>
 __device__ __forceinline__ void dh_load4(const Opnd& o, int row, int k, double v[4]) {
-  if (o.mode == M_ROW || o.mode == M_UPK_ROW) {
+  if (MODE == M_ROW || MODE == M_UPK_ROW) {
     const double2* q = reinterpret_cast<const double2*>(o.p + (long)row * o.ld + k);
     double2 x0 = q[0], x1 = q[1];
     v[0] = x0.x; v[1] = x0.y; v[2] = x1.x; v[3] = x1.y;
-    if (o.mode == M_UPK_ROW) {
+    if (MODE == M_UPK_ROW) {
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         int kk = k + m;
@@ -46,7 +47,7 @@ __device__ __forceinline__ void dh_load4(const Opnd& o, int row, int k, double v
   } else {
     const double* q = o.p + (long)k * o.ld + row;
     v[0] = q[0]; v[1] = q[o.ld]; v[2] = q[2 * o.ld]; v[3] = q[3 * o.ld];
-    if (o.mode == M_UPK_COL) {
+    if (MODE == M_UPK_COL) {
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         int kk = k + m;
@@ -68,6 +69,21 @@ __device__ __forceinline__ void dh_zero(dh_d4 acc[2][2]) {
 }
 
 // acc += opA[i0:i0+64, klo:khi] * opB[klo:khi, j0:j0+64] for this wave's quadrant.
+// k advances in chunks of 32 (klo, khi multiples of 32); the loads of the
+// next chunk are issued before the 32 MFMAs of the current one.
+template <int MA, int MB>
+__device__ __forceinline__ void dh_load_chunk(const Opnd& A, const Opnd& B, int ra, int rb, int k,
+                                              double a[2][2][4], double b[2][2][4]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    dh_load4<MA>(A, ra, k + 16 * h, a[h][0]);
+    dh_load4<MA>(A, ra + 16, k + 16 * h, a[h][1]);
+    dh_load4<MB>(B, rb, k + 16 * h, b[h][0]);
+    dh_load4<MB>(B, rb + 16, k + 16 * h, b[h][1]);
+  }
+}
+
+template <int MA, int MB>
 __device__ __forceinline__ void dh_mma(dh_d4 acc[2][2], const Opnd& A, int i0, const Opnd& B, int j0,
                                        int klo, int khi) {
   if (klo >= khi) return;
@@ -76,32 +92,31 @@ __device__ __forceinline__ void dh_mma(dh_d4 acc[2][2], const Opnd& A, int i0, c
   const int ra = i0 + (w >> 1) * 32 + (lane & 15);
   const int rb = j0 + (w & 1) * 32 + (lane & 15);
   const int kq = (lane >> 4) * 4;
-  double a0[4], a1[4], b0[4], b1[4];
-  dh_load4(A, ra, klo + kq, a0);
-  dh_load4(A, ra + 16, klo + kq, a1);
-  dh_load4(B, rb, klo + kq, b0);
-  dh_load4(B, rb + 16, klo + kq, b1);
-  for (int k = klo; k < khi; k += 16) {
-    double na0[4], na1[4], nb0[4], nb1[4];
-    const int kn = k + 16;
-    if (kn < khi) {
-      dh_load4(A, ra, kn + kq, na0);
-      dh_load4(A, ra + 16, kn + kq, na1);
-      dh_load4(B, rb, kn + kq, nb0);
-      dh_load4(B, rb + 16, kn + kq, nb1);
-    }
+  double a[2][2][4], b[2][2][4];
+  dh_load_chunk<MA, MB>(A, B, ra, rb, klo + kq, a, b);
+  for (int k = klo; k < khi; k += 32) {
+    double na[2][2][4], nb[2][2][4];
+    const bool more = k + 32 < khi;
+    if (more) dh_load_chunk<MA, MB>(A, B, ra, rb, k + 32 + kq, na, nb);
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      acc[0][0] = dh_mfma(a0[m], b0[m], acc[0][0]);
-      acc[0][1] = dh_mfma(a0[m], b1[m], acc[0][1]);
-      acc[1][0] = dh_mfma(a1[m], b0[m], acc[1][0]);
-      acc[1][1] = dh_mfma(a1[m], b1[m], acc[1][1]);
-    }
-    if (kn < khi) {
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        a0[m] = na0[m]; a1[m] = na1[m]; b0[m] = nb0[m]; b1[m] = nb1[m];
+        acc[0][0] = dh_mfma(a[h][0][m], b[h][0][m], acc[0][0]);
+        acc[0][1] = dh_mfma(a[h][0][m], b[h][1][m], acc[0][1]);
+        acc[1][0] = dh_mfma(a[h][1][m], b[h][0][m], acc[1][0]);
+        acc[1][1] = dh_mfma(a[h][1][m], b[h][1][m], acc[1][1]);
       }
+    if (more) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            a[h][x][m] = na[h][x][m];
+            b[h][x][m] = nb[h][x][m];
+          }
     }
   }
 }
@@ -130,67 +145,110 @@ __device__ __forceinline__ void dh_store(double* C, long ldc, int i0, int j0, co
       }
 }
 
-// Unblocked Cholesky of a 64x64 tile plus the inverse of its factor.
+// Cholesky of a 64x64 tile fused with the inverse of its factor.
 // On exit the tile holds L in its lower triangle (diagonal included) and
 // inv(L)^T in its strict upper triangle; dinv[k] = 1/L[k][k]; when inv_out
 // is given the dense inv(L) (zeros above the diagonal, ld 64) is written
 // there too.  A pivot that is not > 0 records 1 + row_base + k in *info
-// (atomic min) and lets NaN propagate.  LDS: one 64 x 65 tile whose strict
-// upper triangle receives inv(L)^T while the lower triangle holds L.
+// (atomic min) and lets NaN propagate.
+//
+// Thread (ty, tx) = (tid/16, tid%16) keeps the 16 entries (ty+16u, tx+16v)
+// in registers.  Column step k publishes column k of the trailing matrix and
+// row k of the working inverse W (L W = I is solved alongside) through a
+// double-buffered 2 x 64 LDS strip, so each step costs one barrier, and then
+// applies two masked FMAs per entry:
+//   S[i][j] -= S[i][k] S[j][k] / S[k][k]   (i > k, j > k: elimination)
+//   W[i][j] -= L[i][k] V[k][j]             (i > k >= j: V = row k of inv(L))
+// An entry switches from S to W when its column is eliminated; entries above
+// the diagonal carry unused values.  No global store is issued inside the
+// loop (a barrier waits for outstanding stores): L goes to LDS, inv(L) stays
+// in registers, and both are written once at the end.
 __device__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_out, int* info,
                               int row_base) {
-  __shared__ double S[64][65];
-  __shared__ double dg[64];
-  __shared__ double vd[64];
+  __shared__ double colb[2][64];
+  __shared__ double rowb[2][64];
+  __shared__ double pv[2][2];  // (1/sqrt(p), sqrt(p)) of the current pivot
+  __shared__ double Ls[64][65];
   const int tid = threadIdx.x;
-  for (int e = tid; e < 4096; e += DH_NT) {
-    int r = e >> 6, c = e & 63;
-    S[r][c] = tile[(long)r * ld + c];
-  }
-  __syncthreads();
-  // right-looking elimination with deferred column scaling: one barrier per column
-  for (int k = 0; k < 63; ++k) {
-    const double rp = 1.0 / S[k][k];
-    const int m = 63 - k;  // trailing order
-    for (int e = tid; e < m * m; e += DH_NT) {
-      const int i = k + 1 + e / m;
-      const int j = k + 1 + e % m;
-      if (j <= i) S[i][j] -= S[i][k] * S[j][k] * rp;
+  const int ty = tid >> 4, tx = tid & 15;
+  double R[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = ty + 16 * u, j = tx + 16 * v;
+      R[u][v] = (j <= i) ? tile[(long)i * ld + j] : 0.0;
     }
-    __syncthreads();
-  }
-  if (tid < 64) {
-    const double p = S[tid][tid];
-    if (!(p > 0.0)) atomicMin(info, 1 + row_base + tid);
-    dg[tid] = sqrt(p);
-  }
-  __syncthreads();
-  for (int e = tid; e < 4096; e += DH_NT) {
-    int i = e >> 6, k = e & 63;
-    if (i > k) S[i][k] = S[i][k] / dg[k];
-    else if (i == k) S[i][k] = dg[k];
-  }
-  __syncthreads();
-  // inverse by forward substitution, column j by 4 lanes of one wave;
-  // V[i][j] (i > j) is kept at S[j][i].  Every lane of the group writes the
-  // same value, so each lane only reads entries it wrote itself.
-  {
-    const int j = tid >> 2, sub = tid & 3;
-    const double vjj = 1.0 / S[j][j];
-    if (sub == 0) vd[j] = vjj;
-    for (int i = j + 1; i < 64; ++i) {
-      double part = (sub == 0) ? S[i][j] * vjj : 0.0;
-      for (int k = j + 1 + sub; k < i; k += 4) part += S[i][k] * S[j][k];
-      part += __shfl_xor(part, 1);
-      part += __shfl_xor(part, 2);
-      S[j][i] = -part / S[i][i];
+  // k = 16*KV + kr with KV unrolled, so the register column/row of step k
+  // is selected at compile time
+#pragma unroll
+  for (int KV = 0; KV < 4; ++KV) {
+#pragma unroll 1
+    for (int kr = 0; kr < 16; ++kr) {
+      const int k = 16 * KV + kr;
+      const int bsel = k & 1;
+      if (tx == kr) {  // publish column k, then clear it below the diagonal (it becomes W[., k])
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          colb[bsel][ty + 16 * u] = R[u][KV];
+          if (ty + 16 * u > k) R[u][KV] = 0.0;
+        }
+        if (ty == kr) {  // owner of the pivot: rsq + one Newton step
+          const double p = R[KV][KV];
+          double r = __builtin_amdgcn_rsq(p);
+          r = r * (1.5 - 0.5 * p * r * r);
+          r = r * (1.5 - 0.5 * p * r * r);
+          pv[bsel][0] = r;
+          pv[bsel][1] = p * r;
+          if (!(p > 0.0)) atomicMin(info, 1 + row_base + k);
+        }
+      }
+      if (ty == kr) {  // publish row k of W
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rowb[bsel][tx + 16 * v] = R[KV][v];
+      }
+      __syncthreads();
+      const double rd = pv[bsel][0];
+      const double d = pv[bsel][1];
+      const double rp = rd * rd;
+      if (tid < 64) Ls[tid][k] = (tid > k) ? colb[bsel][tid] * rd : (tid == k ? d : 0.0);
+      double cim[4], lim[4], cjr[4], vjm[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = ty + 16 * u;
+        cim[u] = (i > k) ? colb[bsel][i] : 0.0;
+        lim[u] = cim[u] * rd;
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = tx + 16 * v;
+        cjr[v] = (j > k) ? colb[bsel][j] * rp : 0.0;
+        vjm[v] = (j < k) ? rowb[bsel][j] * rd : (j == k ? rd : 0.0);  // V[k][j]
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          R[u][v] = fma(-cim[u], cjr[v], R[u][v]);
+          R[u][v] = fma(-lim[u], vjm[v], R[u][v]);
+        }
+      if (ty == kr) {  // row k of inv(L) is final
+#pragma unroll
+        for (int v = 0; v < 4; ++v) R[KV][v] = vjm[v];
+      }
     }
   }
   __syncthreads();
-  for (int e = tid; e < 4096; e += DH_NT) {
-    int r = e >> 6, c = e & 63;
-    tile[(long)r * ld + c] = S[r][c];
-    if (inv_out) inv_out[r * 64 + c] = (r > c) ? S[c][r] : (r == c ? vd[r] : 0.0);
-  }
-  if (tid < 64) dinv[tid] = vd[tid];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = ty + 16 * u, j = tx + 16 * v;
+      if (j <= i) {
+        tile[(long)i * ld + j] = Ls[i][j];
+        if (j < i) tile[(long)j * ld + i] = R[u][v];  // inv(L)^T above the diagonal
+        else dinv[i] = R[u][v];
+      }
+      if (inv_out) inv_out[i * 64 + j] = (j <= i) ? R[u][v] : 0.0;
+    }
 }

@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_panel(BtaPtrs P, int p) {
   dh_zero(acc);
   Opnd A{base, P.bp, M_ROW, nullptr};
   Opnd B{P.pinv, DH_TILE, M_ROW, nullptr};  // opB[k][j] = inv[j][k]
-  dh_mma(acc, A, 0, B, 0, 0, DH_TILE);
+  dh_mma<M_ROW, M_ROW>(acc, A, 0, B, 0, 0, DH_TILE);
   dh_store(base, P.bp, 0, 0, acc, 1.0, 0.0, false);
 }
 
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_update(BtaPtrs P, int p) {
   Opnd B{P.D + (long)J * DH_TILE * P.bp + p * DH_TILE, P.bp, M_ROW, nullptr};
   dh_d4 acc[2][2];
   dh_zero(acc);
-  dh_mma(acc, A, 0, B, 0, 0, DH_TILE);
+  dh_mma<M_ROW, M_ROW>(acc, A, 0, B, 0, 0, DH_TILE);
   double* C = rowp + J * DH_TILE;
   dh_store(C, P.bp, 0, 0, acc, -1.0, 1.0, false);
   if (q == 0 && J == p + 1) {
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_syrk(BtaPtrs P) {
     while ((I + 1) * (I + 2) / 2 <= e) ++I;
     const int J = e - I * (I + 1) / 2;
     Opnd A{P.L, P.bp, M_ROW, nullptr};
-    dh_mma(acc, A, I * DH_TILE, A, J * DH_TILE, 0, K);
+    dh_mma<M_ROW, M_ROW>(acc, A, I * DH_TILE, A, J * DH_TILE, 0, K);
     double* C = P.Dn;
     dh_store(C, P.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 1.0, false);
     if (e == 0) {
@@ -318,13 +318,13 @@ __global__ __launch_bounds__(256) void k_syrk(BtaPtrs P) {
   if (e < narr) {
     Opnd A{P.R, P.bp, M_ROW, nullptr};
     Opnd B{P.L, P.bp, M_ROW, nullptr};
-    dh_mma(acc, A, 0, B, e * DH_TILE, 0, K);
+    dh_mma<M_ROW, M_ROW>(acc, A, 0, B, e * DH_TILE, 0, K);
     dh_store(P.Rn, P.bp, 0, e * DH_TILE, acc, -1.0, 1.0, false);
     return;
   }
   // tip
   Opnd A{P.R, P.bp, M_ROW, nullptr};
-  dh_mma(acc, A, 0, A, 0, 0, K);
+  dh_mma<M_ROW, M_ROW>(acc, A, 0, A, 0, 0, K);
   dh_store(P.T, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
 }
 
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) void k_trtri_t(double* Dbase, double* dinvbase
   Opnd B{D + off * bp + off, bp, M_UPK_ROW, dinv + off};  // opB = inv(L11)
   dh_d4 acc[2][2];
   dh_zero(acc);
-  dh_mma(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, s1 * DH_TILE);
+  dh_mma<M_ROW, M_UPK_ROW>(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, s1 * DH_TILE);
   double* C = W + (off + (long)s1 * DH_TILE) * bp + off;
   dh_store(C, bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
 }
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256) void k_trtri_inv(double* Dbase, double* dinvba
   Opnd B{W + off2 * bp + off, bp, M_COL, nullptr};            // opB = T
   dh_d4 acc[2][2];
   dh_zero(acc);
-  dh_mma(acc, A, I * DH_TILE, B, J * DH_TILE, 0, (I + 1) * DH_TILE);
+  dh_mma<M_UPK_COL, M_COL>(acc, A, I * DH_TILE, B, J * DH_TILE, 0, (I + 1) * DH_TILE);
   // C[i][j] = inv21[i][j] lives at D[off + j][off2 + i]
   double* C = D + off * bp + off2;
   dh_store(C, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
@@ -432,7 +432,16 @@ __global__ __launch_bounds__(256) void k_gemm(GArgs g) {
     if (t.khi == KHI_I) khi = min(khi, (I + 1) * DH_TILE);
     dh_d4 acc[2][2];
     dh_zero(acc);
-    dh_mma(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi);
+    switch (t.A.mode * 4 + t.B.mode) {
+      case M_ROW * 4 + M_UPK_ROW: dh_mma<M_ROW, M_UPK_ROW>(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi); break;
+      case M_ROW * 4 + M_COL: dh_mma<M_ROW, M_COL>(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi); break;
+      case M_COL * 4 + M_COL: dh_mma<M_COL, M_COL>(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi); break;
+      case M_UPK_ROW * 4 + M_UPK_ROW:
+        dh_mma<M_UPK_ROW, M_UPK_ROW>(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi);
+        break;
+      case M_ROW * 4 + M_ROW: dh_mma<M_ROW, M_ROW>(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi); break;
+      default: __builtin_trap();
+    }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -854,7 +863,24 @@ struct dh_handle {
   int inverted;     // packed inverse of the diagonal factors present
   std::vector<NodeList> levels;  // inverse recursion, deepest last
   long long bytes;
+  int tkind;                     // timed kernel kind, -1 none
+  std::vector<hipEvent_t> ev;    // event pairs
+  long long nrec;
 };
+
+#define DH_MAXREC 16384
+
+// Launch `kern` and, when its kind is being timed, bracket it with events.
+#define DH_LAUNCH(h, kind, kern, grid, block, shm, ...)                                       \
+  do {                                                                                        \
+    const bool _t = (h)->tkind == (kind) && (h)->nrec < DH_MAXREC;                            \
+    if (_t) (void)hipEventRecord((h)->ev[2 * (h)->nrec], (h)->stream);                        \
+    hipLaunchKernelGGL(kern, grid, block, shm, (h)->stream, __VA_ARGS__);                     \
+    if (_t) {                                                                                 \
+      (void)hipEventRecord((h)->ev[2 * (h)->nrec + 1], (h)->stream);                          \
+      (h)->nrec++;                                                                            \
+    }                                                                                         \
+  } while (0)
 
 static void build_nodes(int o, int s, int d, std::vector<NodeList>& lv) {
   if (s <= 1) return;
@@ -904,6 +930,8 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   h->bytes = 0;
   h->factored = -1;
   h->inverted = 0;
+  h->tkind = -1;
+  h->nrec = 0;
   const long long bb = (long long)h->bp * h->bp;
   h->G = n < 8 ? n : 8;
   int rc = DH_OK;
@@ -942,6 +970,7 @@ int dh_destroy(dh_handle* h) {
   for (double* p : ps)
     if (p) (void)hipFree(p);
   if (h->info) (void)hipFree(h->info);
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   delete h;
   return DH_OK;
 }
@@ -980,8 +1009,8 @@ int dh_assemble(dh_handle* h, const dh_model* m, const double* coef, double tau,
   Coef cf;
   memcpy(cf.c, coef, sizeof(cf.c));
   const long rows = (long)(2 * h->n - 1) * h->bp;
-  hipLaunchKernelGGL(k_assemble_rows, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double), h->stream,
-                     h->diag, h->lower, h->n, h->b, h->bp, *m, cf, tau, with_like);
+  DH_LAUNCH(h, 5, k_assemble_rows, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double), h->diag,
+            h->lower, h->n, h->b, h->bp, *m, cf, tau, with_like);
   HIPCHK(hipGetLastError());
   if (h->a > 0) {
     const int wl = (sparsity == DH_SPARSITY_BTA) ? with_like : 0;
@@ -1018,7 +1047,7 @@ int dh_cholesky(dh_handle* h, int sparsity) {
   const int n = h->n, bp = h->bp, nbt = h->nbt;
   const long bb = (long)bp * bp;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->info, INT_MAX);
-  hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, h->stream, h->diag, (long)bp, h->dinv, h->pinv, h->info, 0);
+  DH_LAUNCH(h, 4, k_leaf, dim3(1), dim3(256), 0, h->diag, (long)bp, h->dinv, h->pinv, h->info, 0);
   for (int i = 0; i < n; ++i) {
     BtaPtrs P;
     P.D = h->diag + i * bb;
@@ -1037,15 +1066,13 @@ int dh_cholesky(dh_handle* h, int sparsity) {
     P.row_base_n = (i + 1) * h->b;
     for (int p = 0; p < nbt; ++p) {
       const int rows = (nbt - 1 - p) + (P.L ? nbt : 0) + (P.R ? 1 : 0);
-      if (rows > 0) hipLaunchKernelGGL(k_panel, dim3(rows), dim3(256), 0, h->stream, P, p);
-      if (p < nbt - 1) {
-        hipLaunchKernelGGL(k_update, dim3(rows, nbt - 1 - p), dim3(256), 0, h->stream, P, p);
-      }
+      if (rows > 0) DH_LAUNCH(h, 3, k_panel, dim3(rows), dim3(256), 0, P, p);
+      if (p < nbt - 1) DH_LAUNCH(h, 1, k_update, dim3(rows, nbt - 1 - p), dim3(256), 0, P, p);
     }
     const int ntri = P.L ? nbt * (nbt + 1) / 2 : 0;
     const int narr = (P.L && P.R) ? nbt : 0;
     const int nt = ntri + narr + (P.R ? 1 : 0);
-    if (nt > 0) hipLaunchKernelGGL(k_syrk, dim3(nt), dim3(256), 0, h->stream, P);
+    if (nt > 0) DH_LAUNCH(h, 0, k_syrk, dim3(nt), dim3(256), 0, P);
   }
   if (arrow) {
     hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, h->stream, h->tip, (long)h->ap, h->tipdinv, h->tipinv,
@@ -1168,7 +1195,7 @@ static GTerm term(Opnd A, Opnd B, int K, int klo, int khi, double alpha) {
 
 static void launch_gemm(dh_handle* h, GArgs& g) {
   const int tiles = g.lower ? g.mt * (g.mt + 1) / 2 : g.mt * g.ntl;
-  hipLaunchKernelGGL(k_gemm, dim3(tiles), dim3(256), 0, h->stream, g);
+  DH_LAUNCH(h, 2, k_gemm, dim3(tiles), dim3(256), 0, g);
 }
 
 int dh_selected_inversion(dh_handle* h, int sparsity) {
@@ -1366,6 +1393,32 @@ int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const doubl
                      h->part);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, h->stream, h->part, (int)g, 4, 4, out_dev);
   HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_timing_enable(dh_handle* h, int kind) {
+  if (!h) return DH_EARG;
+  if (kind >= 0 && h->ev.empty()) {
+    h->ev.resize(2 * DH_MAXREC);
+    for (auto& e : h->ev)
+      if (hipEventCreate(&e) != hipSuccess) return DH_EHIP;
+  }
+  h->tkind = kind;
+  h->nrec = 0;
+  return DH_OK;
+}
+
+int dh_timing_read(dh_handle* h, double* total_ms, long long* count) {
+  if (!h || !total_ms || !count) return DH_EARG;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double t = 0.0;
+  for (long long i = 0; i < h->nrec; ++i) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]));
+    t += ms;
+  }
+  *total_ms = t;
+  *count = h->nrec;
   return DH_OK;
 }
 

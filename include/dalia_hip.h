@@ -143,6 +143,18 @@ int dh_contract_ata(dh_handle* h, const dh_model* m, double* out_dev);
 int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const double* gam,
                  double* out_dev);
 
+/* Per-kernel timing with HIP events on the handle's stream, for the roofline
+ * of one kernel.  kind: 0 k_syrk (Schur update of the next diagonal block in
+ * pobtaf), 1 k_update (panel trailing update), 2 k_gemm (pobtasi tiles),
+ * 3 k_panel, 4 k_leaf, 5 k_assemble_rows; kind < 0 disables.  While enabled,
+ * each launch of that kind is bracketed by an event pair (at most 16384
+ * launches are recorded).  dh_timing_read synchronises the stream and returns
+ * the summed kernel time and the number of recorded launches.  No
+ * counterpart in the reference (its timers are host wall clock,
+ * structured_solver.py:80-100). */
+int dh_timing_enable(dh_handle* h, int kind);
+int dh_timing_read(dh_handle* h, double* total_ms, long long* count);
+
 #ifdef __cplusplus
 }
 #endif
