@@ -131,6 +131,9 @@ def main():
     log(f"set-up {time.perf_counter() - t0:.1f} s (closed-form prior: {eng.dm.closed_form})")
     theta = data.theta.copy()
     lib = eng.lib
+    from dalia_amd import autotune
+    choice = autotune.tune(eng)
+    log(f"autotuned kernel variants: {choice}")
 
     for w in range(args.warmup):
         f, g = eng.value_and_grad(theta)
@@ -190,7 +193,10 @@ def main():
     _lib.check(lib.dh_timing_read(eng.h, ctypes.byref(tot), ctypes.byref(cnt)), "dh_timing_read")
     _lib.check(lib.dh_timing_enable(eng.h, -1), "dh_timing_enable")
     avg_s = tot.value * 1e-3 / max(cnt.value, 1)
-    if kname == "k_syrk":
+    if kname == "k_syrk" and choice["syrk"] == 1:
+        per = float(ns) ** 3  # k_syrk128: the SYRK alone (arrow and tip run in k_syrk_tail)
+        kname = "k_syrk128"
+    elif kname == "k_syrk":
         per = syrk_flops_per_launch(nt, ns, nb)
     elif kname == "k_update":
         per, _ = update_flops_per_launch(nt, ns, nb)
@@ -223,7 +229,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "ns": ns, "nt": nt, "nb": nb, "seed": seed, "theta_dim": 4,
-                   "n_obs": data.n_obs},
+                   "n_obs": data.n_obs, "kernel_variants": {"syrk": choice["syrk"], "gemm": choice["gemm"]}},
         "pobtaf_ms": t_pobtaf * 1e3,
         "pobtaf_tflops": pobtaf_tflops,
         "pobtaf_frac_of_peak": pobtaf_tflops / FP64_PEAK_TFLOPS,

@@ -53,26 +53,29 @@ SIGNATURES = {
     "dh_get_layout": (_I, [_P, ctypes.POINTER(DhLayout)]),
     "dh_set_stream": (_I, [_P, _P]),
     "dh_memory_bytes": (_LL, [_P]),
-    "dh_assemble": (_I, [_P, ctypes.POINTER(DhModel), c_double_p, _D, _I, _I]),
+    "dh_assemble": (_I, [_P, ctypes.POINTER(DhModel), _P, _I, _I]),
     "dh_scatter_coo": (_I, [_P, _LL, _P, _P, _P, _I]),
     "dh_cholesky": (_I, [_P, _I]),
     "dh_solve": (_I, [_P, _P, _I]),
     "dh_logdet": (_I, [_P, _I, _P]),
     "dh_selected_inversion": (_I, [_P, _I]),
     "dh_get_info": (_I, [_P, ctypes.POINTER(_I)]),
-    "dh_qc_matvec": (_I, [_P, ctypes.POINTER(DhModel), c_double_p, _D, _P, _P, _P]),
+    "dh_qc_matvec": (_I, [_P, ctypes.POINTER(DhModel), _P, _P, _P, _P]),
     "dh_scale": (_I, [_P, _P, _D, _P, _LL]),
+    "dh_scale_dev": (_I, [_P, _P, _P, _P, _LL]),
     "dh_dot": (_I, [_P, _P, _P, _LL, _P]),
     "dh_contract_fem": (_I, [_P, ctypes.POINTER(DhModel), _P]),
     "dh_contract_fem_vec": (_I, [_P, ctypes.POINTER(DhModel), _P, _P]),
     "dh_contract_ata": (_I, [_P, ctypes.POINTER(DhModel), _P]),
     "dh_prior_eig": (_I, [_P, ctypes.POINTER(DhModel), _P, c_double_p, _P]),
+    "dh_set_tuning": (_I, [_P, _I, _I]),
     "dh_timing_enable": (_I, [_P, _I]),
     "dh_timing_read": (_I, [_P, c_double_p, ctypes.POINTER(_LL)]),
 }
 
 #: kernel kinds of dh_timing_enable
-TIMED_KERNELS = {"k_syrk": 0, "k_update": 1, "k_gemm": 2, "k_panel": 3, "k_leaf": 4, "k_assemble_rows": 5}
+TIMED_KERNELS = {"k_syrk": 0, "k_update": 1, "k_gemm": 2, "k_panel": 3, "k_leaf": 4, "k_assemble_rows": 5,
+                 "k_syrk_tail": 6}
 
 _lib = None
 

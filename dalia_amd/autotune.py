@@ -1,0 +1,98 @@
+"""Per-shape choice of kernel variants, timed on the device and persisted.
+
+Knobs (`dh_set_tuning`): key 0 = the Schur-update kernel of pobtaf (64x64
+tiles, or 128x128 tiles with the k range split in two), key 1 = the dense
+products of pobtasi (64x64 or 128x128 tiles).  `tune(engine)` times every
+value of every knob on the engine's own matrix with HIP events, applies the
+fastest and records it in a JSON file keyed by (padded block size, number of
+blocks, arrowhead size, device name); later calls read the file.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import torch
+
+from dalia_amd import _lib
+
+DEFAULT_PATH = os.environ.get("DALIA_AMD_TUNING",
+                              os.path.join(os.path.expanduser("~"), ".cache", "dalia_amd", "tuning.json"))
+
+
+def _key(engine) -> str:
+    lay = engine.solver.layout
+    return f"bp{lay.bp}_n{lay.n}_a{lay.a}_{torch.cuda.get_device_name(0)}"
+
+
+def _load(path):
+    try:
+        with open(path) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def apply(engine, choice: dict) -> None:
+    lib, h = engine.lib, engine.h
+    _lib.check(lib.dh_set_tuning(h, 0, int(choice["syrk"])), "dh_set_tuning")
+    _lib.check(lib.dh_set_tuning(h, 1, int(choice["gemm"])), "dh_set_tuning")
+
+
+def _time(fn, reps: int) -> float:
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ts = []
+    for _ in range(reps):
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def tune(engine, path: str = DEFAULT_PATH, reps: int = 2, force: bool = False) -> dict:
+    cache = _load(path)
+    key = _key(engine)
+    if key in cache and not force:
+        apply(engine, cache[key])
+        return cache[key]
+    lib, h = engine.lib, engine.h
+    theta = engine.data.theta
+    engine._set_params(theta)
+    times = {}
+
+    def factor():
+        engine._assemble(None, "bta", True)
+        _lib.check(lib.dh_cholesky(h, 1), "dh_cholesky")
+
+    for v in (0, 1):
+        _lib.check(lib.dh_set_tuning(h, 0, v), "dh_set_tuning")
+        factor()
+        times[f"syrk{v}"] = _time(factor, reps)
+    best_syrk = int(times["syrk1"] < times["syrk0"])
+    _lib.check(lib.dh_set_tuning(h, 0, best_syrk), "dh_set_tuning")
+
+    def selinv():
+        _lib.check(lib.dh_selected_inversion(h, 1), "dh_selected_inversion")
+
+    for v in (0, 1):
+        _lib.check(lib.dh_set_tuning(h, 1, v), "dh_set_tuning")
+        tt = []
+        for _ in range(reps):
+            factor()
+            tt.append(_time(selinv, 1))
+        times[f"gemm{v}"] = float(np.median(tt))
+    best_gemm = int(times["gemm1"] < times["gemm0"])
+    choice = {"syrk": best_syrk, "gemm": best_gemm, "ms": times}
+    apply(engine, choice)
+    cache[key] = choice
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as fh:
+            json.dump(cache, fh, indent=1)
+    except OSError:
+        pass
+    return choice

@@ -1,0 +1,91 @@
+"""Many f(theta) evaluations over model instances of uneven size in one HIP graph.
+
+The evaluations left after the adjoint gradient (line-search points, the FD
+Hessian stencil at the mode: `core/dalia.py:767-910`) are independent.  The
+reference spreads them over MPI ranks (F-level groups, `core/dalia.py:541-573`);
+on one GPU this module captures all of them in one graph:
+
+* every model instance (its own ns, nt, nb) has its own engine, buffers and
+  stream;
+* the evaluations of one instance run back to back on its stream (they share
+  its buffers); instances run concurrently (fork/join inside the capture);
+* each evaluation first copies its 13 theta-dependent parameters from a row
+  of a pinned host table (a captured host-to-device copy), then runs the
+  plain f(theta) kernels and writes its 5 scalars to its own device slot.
+
+The graph is captured once for a given assignment of evaluations to
+instances and replayed for every new set of theta values; the host only
+fills the pinned table and reads the scalar slots back.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from dalia_amd.inla import STInla, hessian_from_values, hessian_stencil
+
+
+class BatchedF:
+    def __init__(self, engines: list[STInla], assignment: list[int]):
+        """engines: one STInla per model instance; assignment[e] = instance of evaluation e."""
+        self.engines = engines
+        self.assignment = list(assignment)
+        E = len(self.assignment)
+        self.table = torch.zeros((E, 13), dtype=torch.float64).pin_memory()
+        self.slots = torch.zeros((E, 5), dtype=torch.float64, device="cuda")
+        self.streams = [torch.cuda.Stream() for _ in engines]
+        self.graph = None
+        self.thetas = [None] * E
+
+    def _enqueue_all(self, capture_stream):
+        for k, eng in enumerate(self.engines):
+            s = self.streams[k]
+            s.wait_stream(capture_stream)
+            with torch.cuda.stream(s):
+                eng.set_stream(s)
+                for e, inst in enumerate(self.assignment):
+                    if inst != k:
+                        continue
+                    eng.params.copy_(self.table[e], non_blocking=True)
+                    eng._enqueue_f(None, out=self.slots[e])
+            capture_stream.wait_stream(s)
+
+    def capture(self):
+        for e, inst in enumerate(self.assignment):  # valid parameters for the warm-up pass
+            self.table[e].copy_(torch.from_numpy(self.engines[inst].param_row(self.engines[inst].data.theta)))
+        cur = torch.cuda.current_stream()
+        self._enqueue_all(cur)  # eager warm-up (allocator, kernels loaded)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._enqueue_all(torch.cuda.current_stream())
+        for eng in self.engines:
+            eng.set_stream(torch.cuda.current_stream())
+        self.graph = g
+        return g
+
+    def __call__(self, thetas) -> np.ndarray:
+        """f at thetas[e] on instance assignment[e]; one graph launch for all."""
+        if self.graph is None:
+            self.capture()
+        assert len(thetas) == len(self.assignment)
+        for e, inst in enumerate(self.assignment):
+            self.table[e].copy_(torch.from_numpy(self.engines[inst].param_row(thetas[e])))
+        self.graph.replay()
+        s = self.slots.cpu().numpy()
+        return np.array([self.engines[inst]._f_from_scalars(np.asarray(thetas[e]), s[e])
+                         for e, inst in enumerate(self.assignment)])
+
+
+def batched_hessians(engines: list[STInla], thetas, h: float = 5e-3, offdiag_points: int = 2):
+    """FD Hessians of every instance at its own theta, all stencil points in one graph launch."""
+    stencils = [hessian_stencil(t, h, offdiag_points) for t in thetas]
+    assignment = [k for k, st in enumerate(stencils) for _ in range(len(st))]
+    bf = BatchedF(engines, assignment)
+    fv = bf([p for st in stencils for p in st])
+    out, k0 = [], 0
+    for st, t in zip(stencils, thetas):
+        out.append(hessian_from_values(fv[k0:k0 + len(st)], len(t), h, offdiag_points))
+        k0 += len(st)
+    return out, bf

@@ -121,6 +121,70 @@ __device__ __forceinline__ void dh_mma(dh_d4 acc[2][2], const Opnd& A, int i0, c
   }
 }
 
+// 128x128 workgroup tile: wave w computes the 64x64 quadrant (w>>1, w&1) as
+// 4x4 MFMA blocks; one 16-wide k chunk is prefetched ahead of 64 MFMAs.
+template <int MA, int MB>
+__device__ __forceinline__ void dh_mma128(dh_d4 acc[4][4], const Opnd& A, int i0, const Opnd& B, int j0,
+                                          int klo, int khi) {
+  if (klo >= khi) return;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int ra = i0 + (w >> 1) * 64 + (lane & 15);
+  const int rb = j0 + (w & 1) * 64 + (lane & 15);
+  const int kq = (lane >> 4) * 4;
+  double a[4][4], b[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    dh_load4<MA>(A, ra + 16 * x, klo + kq, a[x]);
+    dh_load4<MB>(B, rb + 16 * x, klo + kq, b[x]);
+  }
+  for (int k = klo; k < khi; k += 16) {
+    double na[4][4], nb[4][4];
+    const bool more = k + 16 < khi;
+    if (more) {
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        dh_load4<MA>(A, ra + 16 * x, k + 16 + kq, na[x]);
+        dh_load4<MB>(B, rb + 16 * x, k + 16 + kq, nb[x]);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = dh_mfma(a[x][m], b[y][m], acc[x][y]);
+    if (more) {
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          a[x][m] = na[x][m];
+          b[x][m] = nb[x][m];
+        }
+    }
+  }
+}
+
+__device__ __forceinline__ void dh_store128(double* C, long ldc, int i0, int j0, const dh_d4 acc[4][4],
+                                            double alpha, double beta) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int rbase = i0 + (w >> 1) * 64 + (lane >> 4);
+  const int cbase = j0 + (w & 1) * 64 + (lane & 15);
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double* p = C + (long)(rbase + 16 * x + 4 * r) * ldc + cbase + 16 * y;
+        double v = alpha * acc[x][y][r];
+        if (beta != 0.0) v += beta * (*p);
+        *p = v;
+      }
+}
+
 // C[i0.., j0..] = alpha*acc + beta*C (trans: C[j][i] instead of C[i][j]).
 // Lane l, item r of the 16x16 f64 MFMA result holds row 4*r + l/16 and
 // column l%16 (measured on gfx950 with one-hot operands).
@@ -155,20 +219,91 @@ __device__ __forceinline__ void dh_store(double* C, long ldc, int i0, int j0, co
 // Thread (ty, tx) = (tid/16, tid%16) keeps the 16 entries (ty+16u, tx+16v)
 // in registers.  Column step k publishes column k of the trailing matrix and
 // row k of the working inverse W (L W = I is solved alongside) through a
-// double-buffered 2 x 64 LDS strip, so each step costs one barrier, and then
-// applies two masked FMAs per entry:
-//   S[i][j] -= S[i][k] S[j][k] / S[k][k]   (i > k, j > k: elimination)
-//   W[i][j] -= L[i][k] V[k][j]             (i > k >= j: V = row k of inv(L))
-// An entry switches from S to W when its column is eliminated; entries above
-// the diagonal carry unused values.  No global store is issued inside the
+// double-buffered LDS strip, so each step costs one barrier, then applies
+//   S[i][j] -= S[i][k] S[j][k] / S[k][k]   (i >= j > k, elimination)
+//   W[i][j] -= L[i][k] V[k][j]             (i > k >= j, V = row k of inv(L))
+// An entry switches from S to W when its column is eliminated.  Steps are
+// specialised on KV = k/16 at compile time, so finished rows, the upper
+// triangle and most masks drop out.  No global store is issued inside the
 // loop (a barrier waits for outstanding stores): L goes to LDS, inv(L) stays
 // in registers, and both are written once at the end.
+struct LeafShm {
+  double colb[2][64];
+  double rowb[2][64];
+  double pv[2][2];  // (1/sqrt(p), sqrt(p)) of the current pivot
+  double Ls[64][65];
+};
+
+template <int KV>
+__device__ __forceinline__ void dh_leaf_steps(double R[4][4], LeafShm& sh, int ty, int tx, int tid, int* info,
+                                              int row_base) {
+#pragma unroll 1
+  for (int kr = 0; kr < 16; ++kr) {
+    const int k = 16 * KV + kr;
+    const int bsel = k & 1;
+    if (tx == kr) {  // publish column k (rows >= k), then clear it below the diagonal
+#pragma unroll
+      for (int u = KV; u < 4; ++u) {
+        sh.colb[bsel][ty + 16 * u] = R[u][KV];
+        if (u > KV || ty > kr) R[u][KV] = 0.0;
+      }
+      if (ty == kr) {  // pivot owner: rsq + two Newton steps
+        const double p = R[KV][KV];
+        double r = __builtin_amdgcn_rsq(p);
+        r = r * (1.5 - 0.5 * p * r * r);
+        r = r * (1.5 - 0.5 * p * r * r);
+        sh.pv[bsel][0] = r;
+        sh.pv[bsel][1] = p * r;
+        if (!(p > 0.0)) atomicMin(info, 1 + row_base + k);
+      }
+    }
+    if (ty == kr) {  // publish row k of W (columns < k)
+#pragma unroll
+      for (int v = 0; v <= KV; ++v) sh.rowb[bsel][tx + 16 * v] = R[KV][v];
+    }
+    __syncthreads();
+    const double rd = sh.pv[bsel][0];
+    const double d = sh.pv[bsel][1];
+    const double rp = rd * rd;
+    if (tid < 64) sh.Ls[tid][k] = (tid > k) ? sh.colb[bsel][tid] * rd : (tid == k ? d : 0.0);
+    double cim[4], lim[4], cjr[4], vjm[4];
+#pragma unroll
+    for (int u = KV; u < 4; ++u) {
+      const double c = sh.colb[bsel][ty + 16 * u];
+      cim[u] = (u > KV || ty > kr) ? c : 0.0;
+      lim[u] = cim[u] * rd;
+    }
+#pragma unroll
+    for (int v = KV; v < 4; ++v) {
+      const double c = sh.colb[bsel][tx + 16 * v] * rp;
+      cjr[v] = (v > KV || tx > kr) ? c : 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v <= KV; ++v) {
+      if (v < KV) {
+        vjm[v] = sh.rowb[bsel][tx + 16 * v] * rd;
+      } else {
+        const double w = sh.rowb[bsel][tx + 16 * v] * rd;
+        vjm[v] = (tx < kr) ? w : (tx == kr ? rd : 0.0);
+      }
+    }
+#pragma unroll
+    for (int u = KV; u < 4; ++u) {
+#pragma unroll
+      for (int v = KV; v <= u; ++v) R[u][v] = fma(-cim[u], cjr[v], R[u][v]);
+#pragma unroll
+      for (int v = 0; v <= KV; ++v) R[u][v] = fma(-lim[u], vjm[v], R[u][v]);
+    }
+    if (ty == kr) {  // row k of inv(L) is final
+#pragma unroll
+      for (int v = 0; v <= KV; ++v) R[KV][v] = vjm[v];
+    }
+  }
+}
+
 __device__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_out, int* info,
                               int row_base) {
-  __shared__ double colb[2][64];
-  __shared__ double rowb[2][64];
-  __shared__ double pv[2][2];  // (1/sqrt(p), sqrt(p)) of the current pivot
-  __shared__ double Ls[64][65];
+  __shared__ LeafShm sh;
   const int tid = threadIdx.x;
   const int ty = tid >> 4, tx = tid & 15;
   double R[4][4];
@@ -179,65 +314,10 @@ __device__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_o
       const int i = ty + 16 * u, j = tx + 16 * v;
       R[u][v] = (j <= i) ? tile[(long)i * ld + j] : 0.0;
     }
-  // k = 16*KV + kr with KV unrolled, so the register column/row of step k
-  // is selected at compile time
-#pragma unroll
-  for (int KV = 0; KV < 4; ++KV) {
-#pragma unroll 1
-    for (int kr = 0; kr < 16; ++kr) {
-      const int k = 16 * KV + kr;
-      const int bsel = k & 1;
-      if (tx == kr) {  // publish column k, then clear it below the diagonal (it becomes W[., k])
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          colb[bsel][ty + 16 * u] = R[u][KV];
-          if (ty + 16 * u > k) R[u][KV] = 0.0;
-        }
-        if (ty == kr) {  // owner of the pivot: rsq + one Newton step
-          const double p = R[KV][KV];
-          double r = __builtin_amdgcn_rsq(p);
-          r = r * (1.5 - 0.5 * p * r * r);
-          r = r * (1.5 - 0.5 * p * r * r);
-          pv[bsel][0] = r;
-          pv[bsel][1] = p * r;
-          if (!(p > 0.0)) atomicMin(info, 1 + row_base + k);
-        }
-      }
-      if (ty == kr) {  // publish row k of W
-#pragma unroll
-        for (int v = 0; v < 4; ++v) rowb[bsel][tx + 16 * v] = R[KV][v];
-      }
-      __syncthreads();
-      const double rd = pv[bsel][0];
-      const double d = pv[bsel][1];
-      const double rp = rd * rd;
-      if (tid < 64) Ls[tid][k] = (tid > k) ? colb[bsel][tid] * rd : (tid == k ? d : 0.0);
-      double cim[4], lim[4], cjr[4], vjm[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = ty + 16 * u;
-        cim[u] = (i > k) ? colb[bsel][i] : 0.0;
-        lim[u] = cim[u] * rd;
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int j = tx + 16 * v;
-        cjr[v] = (j > k) ? colb[bsel][j] * rp : 0.0;
-        vjm[v] = (j < k) ? rowb[bsel][j] * rd : (j == k ? rd : 0.0);  // V[k][j]
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          R[u][v] = fma(-cim[u], cjr[v], R[u][v]);
-          R[u][v] = fma(-lim[u], vjm[v], R[u][v]);
-        }
-      if (ty == kr) {  // row k of inv(L) is final
-#pragma unroll
-        for (int v = 0; v < 4; ++v) R[KV][v] = vjm[v];
-      }
-    }
-  }
+  dh_leaf_steps<0>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_steps<1>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_steps<2>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_steps<3>(R, sh, ty, tx, tid, info, row_base);
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < 4; ++u)
@@ -245,7 +325,7 @@ __device__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_o
     for (int v = 0; v < 4; ++v) {
       const int i = ty + 16 * u, j = tx + 16 * v;
       if (j <= i) {
-        tile[(long)i * ld + j] = Ls[i][j];
+        tile[(long)i * ld + j] = sh.Ls[i][j];
         if (j < i) tile[(long)j * ld + i] = R[u][v];  // inv(L)^T above the diagonal
         else dinv[i] = R[u][v];
       }

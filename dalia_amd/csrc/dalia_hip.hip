@@ -79,18 +79,24 @@ __global__ __launch_bounds__(256) void k_sum_seg(const double* part, int nseg, i
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
+__global__ __launch_bounds__(256) void k_copy(const double* x, double* y, int len) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < len) y[e] = x[e];
+}
+
 // ---------------------------------------------------------------------------
 // assembly
 // ---------------------------------------------------------------------------
 
-struct Coef {
-  double c[12];
-};
+// theta-dependent parameters live in device memory (so a captured graph can
+// be replayed for a new theta): params[0..11] = Kronecker coefficients c[k][m],
+// params[12] = exp(theta_obs).
+#define DH_NPARAMS 13
 
 // One workgroup per row of a diagonal or sub-diagonal block; the row is built
 // in LDS (zero, FEM Kronecker entries, tau A^T A entries) and streamed out.
 __global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* lower, int n, int b, int bp,
-                                                       dh_model m, Coef cf, double tau, int with_like) {
+                                                       dh_model m, const double* params, int with_like) {
   extern __shared__ double row[];
   const long r = blockIdx.x;
   const long ndiag = (long)n * bp;
@@ -123,10 +129,11 @@ __global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* low
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const double mk = isdiag ? m.m_diag[k * nt + t] : m.m_off[k * (nt - 1) + t];
-        acc += cf.c[k * 4 + mm] * mk;
+        acc += params[k * 4 + mm] * mk;
       }
       w[mm] = acc;
     }
+    const double tau = params[12];
     const int e0 = m.fem_rowptr[s], e1 = m.fem_rowptr[s + 1];
     for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
       const long nn = m.fem_nnz;
@@ -152,7 +159,8 @@ __global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* low
 }
 
 __global__ __launch_bounds__(256) void k_assemble_arrow(double* arrow, int n, int b, int a, int bp, int ap,
-                                                        dh_model m, double tau, int with_like) {
+                                                        dh_model m, const double* params, int with_like) {
+  const double tau = params[12];
   const long r = blockIdx.x;  // t*ap + j
   const int t = (int)(r / ap), j = (int)(r % ap);
   double* out = arrow + r * bp;
@@ -164,8 +172,9 @@ __global__ __launch_bounds__(256) void k_assemble_arrow(double* arrow, int n, in
   }
 }
 
-__global__ __launch_bounds__(256) void k_assemble_tip(double* tip, int a, int ap, dh_model m, double tau,
+__global__ __launch_bounds__(256) void k_assemble_tip(double* tip, int a, int ap, dh_model m, const double* params,
                                                       int with_like) {
+  const double tau = params[12];
   for (int e = threadIdx.x; e < ap * ap; e += blockDim.x) {
     const int j = e / ap, k = e % ap;
     double v;
@@ -328,6 +337,64 @@ __global__ __launch_bounds__(256) void k_syrk(BtaPtrs P) {
   dh_store(P.T, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
 }
 
+// Schur update with 128x128 tiles and the k range split in two: half 0
+// subtracts its product from D_{i+1} in place, half 1 stores its product in
+// the scratch P (lower 128-tiles only); k_syrk_tail then subtracts P.
+__global__ __launch_bounds__(256) void k_syrk128(BtaPtrs P, double* Pbuf) {
+  const int e = blockIdx.x;
+  int I = 0;
+  while ((I + 1) * (I + 2) / 2 <= e) ++I;
+  const int J = e - I * (I + 1) / 2;
+  const int half = blockIdx.y;
+  const int K2 = P.bp / 2;
+  dh_d4 acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = dh_d4{0.0, 0.0, 0.0, 0.0};
+  Opnd A{P.L, P.bp, M_ROW, nullptr};
+  dh_mma128<M_ROW, M_ROW>(acc, A, I * 128, A, J * 128, half * K2, (half + 1) * K2);
+  if (half == 0) dh_store128(P.Dn, P.bp, I * 128, J * 128, acc, -1.0, 1.0);
+  else dh_store128(Pbuf, P.bp, I * 128, J * 128, acc, 1.0, 0.0);
+}
+
+// D_{i+1} -= P on the lower 128-tiles, the arrow and tip updates (64 tiles),
+// then the first leaf of block i+1 (by the workgroup of tile (0,0)).
+__global__ __launch_bounds__(256) void k_syrk_tail(BtaPtrs P, const double* Pbuf) {
+  int e = blockIdx.x;
+  const int nt2 = P.nbt / 2;
+  const int ntri2 = P.L ? nt2 * (nt2 + 1) / 2 : 0;
+  const int narr = (P.L && P.R) ? P.nbt : 0;
+  if (e < ntri2) {
+    int I = 0;
+    while ((I + 1) * (I + 2) / 2 <= e) ++I;
+    const int J = e - I * (I + 1) / 2;
+    for (int x = threadIdx.x; x < 128 * 128; x += blockDim.x) {
+      const long off = (long)(I * 128 + x / 128) * P.bp + J * 128 + (x % 128);
+      P.Dn[off] -= Pbuf[off];
+    }
+    if (e == 0) {
+      __syncthreads();
+      __threadfence();
+      dh_leaf_potrf(P.Dn, P.bp, P.dinvn, P.pinv, P.info, P.row_base_n);
+    }
+    return;
+  }
+  e -= ntri2;
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  if (e < narr) {
+    Opnd A{P.R, P.bp, M_ROW, nullptr};
+    Opnd B{P.L, P.bp, M_ROW, nullptr};
+    dh_mma<M_ROW, M_ROW>(acc, A, 0, B, e * DH_TILE, 0, P.bp);
+    dh_store(P.Rn, P.bp, 0, e * DH_TILE, acc, -1.0, 1.0, false);
+    return;
+  }
+  Opnd A{P.R, P.bp, M_ROW, nullptr};
+  dh_mma<M_ROW, M_ROW>(acc, A, 0, A, 0, 0, P.bp);
+  dh_store(P.T, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
+}
+
 // ---------------------------------------------------------------------------
 // batched inverse of the diagonal factors (packed into the strict upper part)
 // ---------------------------------------------------------------------------
@@ -389,65 +456,69 @@ __global__ __launch_bounds__(256) void k_trtri_inv(double* Dbase, double* dinvba
 }
 
 // ---------------------------------------------------------------------------
-// generic tile GEMM: C = sum_t alpha_t opA_t opB_t + beta C
+// tile GEMM: C = alpha opA opB + beta C, one term per launch, operand access
+// modes as template parameters; 64x64 tiles (optionally only I >= J) or
+// 128x128 tiles
 // ---------------------------------------------------------------------------
 
 enum { KLO_0 = 0, KLO_I = 1, KLO_J = 2, KLO_MAXIJ = 3 };
 enum { KHI_K = 0, KHI_I = 1 };
 
-struct GTerm {
+struct GArgs {
   Opnd A, B;
   int K, klo, khi;
-  double alpha;
-};
-struct GArgs {
-  GTerm t[3];
-  int nterm;
+  double alpha, beta;
   double* C;
   long ldc;
-  double beta;
-  int mt, ntl;
-  int lower;
+  int mt, ntl;  // tile grid
+  int lower;    // only tiles I >= J (64 only)
+  int jmajor;   // tile order: J-major (largest triangular k range first when klo grows with J)
 };
 
-__global__ __launch_bounds__(256) void k_gemm(GArgs g) {
+__device__ __forceinline__ void dh_krange(const GArgs& g, int I, int J, int T, int& klo, int& khi) {
+  klo = 0;
+  khi = g.K;
+  if (g.klo == KLO_I) klo = I * T;
+  else if (g.klo == KLO_J) klo = J * T;
+  else if (g.klo == KLO_MAXIJ) klo = (I > J ? I : J) * T;
+  if (g.khi == KHI_I) khi = min(khi, (I + 1) * T);
+}
+
+template <int MA, int MB>
+__global__ __launch_bounds__(256) void k_gemm64(GArgs g) {
   int I, J;
-  int e = blockIdx.x;
+  const int e = blockIdx.x;
   if (g.lower) {
     I = 0;
     while ((I + 1) * (I + 2) / 2 <= e) ++I;
     J = e - I * (I + 1) / 2;
+  } else if (g.jmajor) {
+    J = e / g.mt;
+    I = e % g.mt;
   } else {
     I = e / g.ntl;
     J = e % g.ntl;
   }
-  dh_d4 tot[2][2];
-  dh_zero(tot);
-  for (int ti = 0; ti < g.nterm; ++ti) {
-    const GTerm& t = g.t[ti];
-    int klo = 0, khi = t.K;
-    if (t.klo == KLO_I) klo = I * DH_TILE;
-    else if (t.klo == KLO_J) klo = J * DH_TILE;
-    else if (t.klo == KLO_MAXIJ) klo = (I > J ? I : J) * DH_TILE;
-    if (t.khi == KHI_I) khi = min(khi, (I + 1) * DH_TILE);
-    dh_d4 acc[2][2];
-    dh_zero(acc);
-    switch (t.A.mode * 4 + t.B.mode) {
-      case M_ROW * 4 + M_UPK_ROW: dh_mma<M_ROW, M_UPK_ROW>(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi); break;
-      case M_ROW * 4 + M_COL: dh_mma<M_ROW, M_COL>(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi); break;
-      case M_COL * 4 + M_COL: dh_mma<M_COL, M_COL>(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi); break;
-      case M_UPK_ROW * 4 + M_UPK_ROW:
-        dh_mma<M_UPK_ROW, M_UPK_ROW>(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi);
-        break;
-      case M_ROW * 4 + M_ROW: dh_mma<M_ROW, M_ROW>(acc, t.A, I * DH_TILE, t.B, J * DH_TILE, klo, khi); break;
-      default: __builtin_trap();
-    }
+  int klo, khi;
+  dh_krange(g, I, J, DH_TILE, klo, khi);
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  dh_mma<MA, MB>(acc, g.A, I * DH_TILE, g.B, J * DH_TILE, klo, khi);
+  dh_store(g.C, g.ldc, I * DH_TILE, J * DH_TILE, acc, g.alpha, g.beta, false);
+}
+
+template <int MA, int MB>
+__global__ __launch_bounds__(256) void k_gemm128(GArgs g) {
+  const int I = blockIdx.x / g.ntl, J = blockIdx.x % g.ntl;
+  int klo, khi;
+  dh_krange(g, I, J, 2 * DH_TILE, klo, khi);
+  dh_d4 acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+  for (int x = 0; x < 4; ++x)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) tot[a][b] += t.alpha * acc[a][b];
-  }
-  dh_store(g.C, g.ldc, I * DH_TILE, J * DH_TILE, tot, 1.0, g.beta, false);
+    for (int y = 0; y < 4; ++y) acc[x][y] = dh_d4{0.0, 0.0, 0.0, 0.0};
+  dh_mma128<MA, MB>(acc, g.A, I * 2 * DH_TILE, g.B, J * 2 * DH_TILE, klo, khi);
+  dh_store128(g.C, g.ldc, I * 2 * DH_TILE, J * 2 * DH_TILE, acc, g.alpha, g.beta);
 }
 
 // D[r][c] = S[max(r,c)][min(r,c)] (lower tiles of S mirrored).
@@ -621,6 +692,11 @@ __global__ __launch_bounds__(256) void k_scale(const double* x, double s, double
   if (e < len) y[e] = s * x[e];
 }
 
+__global__ __launch_bounds__(256) void k_scale_dev(const double* x, const double* sdev, double* y, long len) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < len) y[e] = sdev[0] * x[e];
+}
+
 __global__ __launch_bounds__(256) void k_dot_part(const double* x, const double* y, long len, double* part) {
   double s = 0.0;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long)gridDim.x * blockDim.x)
@@ -630,7 +706,7 @@ __global__ __launch_bounds__(256) void k_dot_part(const double* x, const double*
 }
 
 // y_prior (spatio-temporal rows) = Qp_ST x; one thread per row (t, s).
-__global__ __launch_bounds__(256) void k_qp_matvec(dh_model m, Coef cf, const double* x, double* y) {
+__global__ __launch_bounds__(256) void k_qp_matvec(dh_model m, const double* params, const double* x, double* y) {
   const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long nst = (long)m.ns * m.nt;
   if (r >= nst) return;
@@ -646,7 +722,7 @@ __global__ __launch_bounds__(256) void k_qp_matvec(dh_model m, Coef cf, const do
       double a = 0.0;
       for (int k = 0; k < 3; ++k) {
         double mk = (dt == 0) ? m.m_diag[k * nt + t] : m.m_off[k * (nt - 1) + (dt < 0 ? t2 : t)];
-        a += cf.c[k * 4 + mm] * mk;
+        a += params[k * 4 + mm] * mk;
       }
       w[mm] = a;
     }
@@ -665,7 +741,8 @@ __global__ __launch_bounds__(256) void k_qp_matvec(dh_model m, Coef cf, const do
 }
 
 // y_like (spatio-temporal rows) = tau (A^T A x_st + (X^T A_st)^T x_fe)
-__global__ __launch_bounds__(256) void k_ata_matvec(dh_model m, double tau, const double* x, double* y) {
+__global__ __launch_bounds__(256) void k_ata_matvec(dh_model m, const double* params, const double* x, double* y) {
+  const double tau = params[12];
   const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long nst = (long)m.ns * m.nt;
   if (r >= nst) return;
@@ -677,8 +754,9 @@ __global__ __launch_bounds__(256) void k_ata_matvec(dh_model m, double tau, cons
 
 // fixed-effect rows: y_like[nst+j] = tau (sum_r XA[j][r] x[r] + sum_k XX[j][k] x_fe[k]),
 // y_prior[nst+j] = fe_prec x_fe[j].  One workgroup per j.
-__global__ __launch_bounds__(256) void k_ata_matvec_fe(dh_model m, double tau, const double* x, double* yp,
-                                                       double* yl) {
+__global__ __launch_bounds__(256) void k_ata_matvec_fe(dh_model m, const double* params, const double* x,
+                                                       double* yp, double* yl) {
+  const double tau = params[12];
   const int j = blockIdx.x;
   const long nst = (long)m.ns * m.nt;
   double s = 0.0;
@@ -863,6 +941,8 @@ struct dh_handle {
   int inverted;     // packed inverse of the diagonal factors present
   std::vector<NodeList> levels;  // inverse recursion, deepest last
   long long bytes;
+  int syrk_variant;              // 0: 64-tiles, 1: 128-tiles with split k (needs nbt even)
+  int gemm_variant;              // pobtasi dense products: 0: 64-tiles, 1: 128-tiles (nbt even)
   int tkind;                     // timed kernel kind, -1 none
   std::vector<hipEvent_t> ev;    // event pairs
   long long nrec;
@@ -932,6 +1012,8 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   h->inverted = 0;
   h->tkind = -1;
   h->nrec = 0;
+  h->syrk_variant = 0;
+  h->gemm_variant = 1;
   const long long bb = (long long)h->bp * h->bp;
   h->G = n < 8 ? n : 8;
   int rc = DH_OK;
@@ -1002,21 +1084,19 @@ static bool model_ok(const dh_handle* h, const dh_model* m) {
   return m && m->nt == h->n && m->ns == h->b && m->nt >= 2 && m->nb >= 0 && m->nb <= 64;
 }
 
-int dh_assemble(dh_handle* h, const dh_model* m, const double* coef, double tau, int sparsity, int with_like) {
-  if (!h || !coef || !model_ok(h, m)) return DH_EARG;
+int dh_assemble(dh_handle* h, const dh_model* m, const double* params, int sparsity, int with_like) {
+  if (!h || !params || !model_ok(h, m)) return DH_EARG;
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
   if (sparsity == DH_SPARSITY_BTA && m->nb != h->a) return DH_EARG;
-  Coef cf;
-  memcpy(cf.c, coef, sizeof(cf.c));
   const long rows = (long)(2 * h->n - 1) * h->bp;
   DH_LAUNCH(h, 5, k_assemble_rows, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double), h->diag,
-            h->lower, h->n, h->b, h->bp, *m, cf, tau, with_like);
+            h->lower, h->n, h->b, h->bp, *m, params, with_like);
   HIPCHK(hipGetLastError());
   if (h->a > 0) {
     const int wl = (sparsity == DH_SPARSITY_BTA) ? with_like : 0;
     hipLaunchKernelGGL(k_assemble_arrow, dim3(h->n * h->ap), dim3(256), 0, h->stream, h->arrow, h->n, h->b, h->a,
-                       h->bp, h->ap, *m, tau, wl);
-    hipLaunchKernelGGL(k_assemble_tip, dim3(1), dim3(256), 0, h->stream, h->tip, h->a, h->ap, *m, tau, wl);
+                       h->bp, h->ap, *m, params, wl);
+    hipLaunchKernelGGL(k_assemble_tip, dim3(1), dim3(256), 0, h->stream, h->tip, h->a, h->ap, *m, params, wl);
     HIPCHK(hipGetLastError());
   }
   h->factored = -1;
@@ -1071,8 +1151,15 @@ int dh_cholesky(dh_handle* h, int sparsity) {
     }
     const int ntri = P.L ? nbt * (nbt + 1) / 2 : 0;
     const int narr = (P.L && P.R) ? nbt : 0;
-    const int nt = ntri + narr + (P.R ? 1 : 0);
-    if (nt > 0) DH_LAUNCH(h, 0, k_syrk, dim3(nt), dim3(256), 0, P);
+    if (P.L && h->syrk_variant == 1 && (nbt % 2) == 0) {
+      const int nt2 = nbt / 2;
+      const int ntri2 = nt2 * (nt2 + 1) / 2;
+      DH_LAUNCH(h, 0, k_syrk128, dim3(ntri2, 2), dim3(256), 0, P, h->W);
+      DH_LAUNCH(h, 6, k_syrk_tail, dim3(ntri2 + narr + (P.R ? 1 : 0)), dim3(256), 0, P, (const double*)h->W);
+    } else {
+      const int nt = ntri + narr + (P.R ? 1 : 0);
+      if (nt > 0) DH_LAUNCH(h, 0, k_syrk, dim3(nt), dim3(256), 0, P);
+    }
   }
   if (arrow) {
     hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, h->stream, h->tip, (long)h->ap, h->tipdinv, h->tipinv,
@@ -1131,7 +1218,7 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
                          bp, bp, zi - bp, zi, h->v, -1.0);
       src = h->v;
     } else {
-      HIPCHK(hipMemcpyAsync(h->v, zi, bp * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      hipLaunchKernelGGL(k_copy, dim3((bp + 255) / 256), dim3(256), 0, h->stream, zi, h->v, bp);
       src = h->v;
     }
     hipLaunchKernelGGL(k_gemv_col_part, dim3(ncb, nch), dim3(256), 0, h->stream, h->diag + i * bb, (long)bp, bp,
@@ -1160,7 +1247,7 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
       hipLaunchKernelGGL(k_fin_col, dim3(ncb), dim3(256), 0, h->stream, h->part, nch1 + nch2, bp, zi,
                          (const double*)nullptr, -1.0, h->v);
     } else {
-      HIPCHK(hipMemcpyAsync(h->v, zi, bp * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      hipLaunchKernelGGL(k_copy, dim3((bp + 255) / 256), dim3(256), 0, h->stream, zi, h->v, bp);
     }
     hipLaunchKernelGGL(k_upk_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, h->diag + i * bb,
                        h->dinv + (long)i * bp, bp, h->v, zi);
@@ -1182,21 +1269,41 @@ int dh_logdet(dh_handle* h, int sparsity, double* out_dev) {
   return DH_OK;
 }
 
-static GTerm term(Opnd A, Opnd B, int K, int klo, int khi, double alpha) {
-  GTerm t;
-  t.A = A;
-  t.B = B;
-  t.K = K;
-  t.klo = klo;
-  t.khi = khi;
-  t.alpha = alpha;
-  return t;
+}  // extern "C"
+
+static GArgs gargs(Opnd A, Opnd B, int K, int klo, int khi, double alpha, double beta, double* C, long ldc,
+                   int mt, int ntl, int lower) {
+  GArgs g;
+  g.A = A;
+  g.B = B;
+  g.K = K;
+  g.klo = klo;
+  g.khi = khi;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.C = C;
+  g.ldc = ldc;
+  g.mt = mt;
+  g.ntl = ntl;
+  g.lower = lower;
+  g.jmajor = (klo == KLO_J && !lower) ? 1 : 0;
+  return g;
 }
 
-static void launch_gemm(dh_handle* h, GArgs& g) {
+// 64x64 tiles over an (mt x ntl) tile grid (or its lower triangle).
+template <int MA, int MB>
+static void gemm64(dh_handle* h, const GArgs& g) {
   const int tiles = g.lower ? g.mt * (g.mt + 1) / 2 : g.mt * g.ntl;
-  DH_LAUNCH(h, 2, k_gemm, dim3(tiles), dim3(256), 0, g);
+  DH_LAUNCH(h, 2, (k_gemm64<MA, MB>), dim3(tiles), dim3(256), 0, g);
 }
+
+// 128x128 tiles; mt, ntl count 128-tiles (dimensions multiples of 128).
+template <int MA, int MB>
+static void gemm128(dh_handle* h, const GArgs& g) {
+  DH_LAUNCH(h, 2, (k_gemm128<MA, MB>), dim3(g.mt * g.ntl), dim3(256), 0, g);
+}
+
+extern "C" {
 
 int dh_selected_inversion(dh_handle* h, int sparsity) {
   if (!h) return DH_EARG;
@@ -1206,21 +1313,15 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   if (rc) return rc;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
   const int n = h->n, bp = h->bp, nbt = h->nbt;
+  const bool t128 = (nbt % 2) == 0 && h->gemm_variant == 1;
   const long bb = (long)bp * bp;
   double* W = h->W;
   double* S = h->S;
-  if (arrow) {
-    // X_aa = inv(L_aa)^T inv(L_aa)
-    GArgs g;
-    memset(&g, 0, sizeof(g));
-    g.t[0] = term(Opnd{h->tipinv, 64, M_COL, nullptr}, Opnd{h->tipinv, 64, M_COL, nullptr}, 64, KLO_0, KHI_K, 1.0);
-    g.nterm = 1;
-    g.C = h->tip;
-    g.ldc = 64;
-    g.beta = 0.0;
-    g.mt = 1;
-    g.ntl = 1;
-    launch_gemm(h, g);
+  const Opnd none{nullptr, 0, M_ROW, nullptr};
+  (void)none;
+  if (arrow) {  // X_aa = inv(L_aa)^T inv(L_aa)
+    gemm64<M_COL, M_COL>(h, gargs(Opnd{h->tipinv, 64, M_COL, nullptr}, Opnd{h->tipinv, 64, M_COL, nullptr}, 64,
+                                  KLO_0, KHI_K, 1.0, 0.0, h->tip, 64, 1, 1, 0));
   }
   for (int i = n - 1; i >= 0; --i) {
     double* D = h->diag + i * bb;
@@ -1232,72 +1333,39 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
     const Opnd Linv_row{D, bp, M_UPK_ROW, dv};  // opB = Linv  /  opA = Linv^T
     // 1. W = L_{i+1,i} Linv, Wa = R_i Linv
     if (Lb) {
-      GArgs g;
-      memset(&g, 0, sizeof(g));
-      g.t[0] = term(Opnd{Lb, bp, M_ROW, nullptr}, Linv_row, bp, KLO_J, KHI_K, 1.0);
-      g.nterm = 1;
-      g.C = W;
-      g.ldc = bp;
-      g.mt = nbt;
-      g.ntl = nbt;
-      launch_gemm(h, g);
+      const Opnd A{Lb, bp, M_ROW, nullptr};
+      gemm64<M_ROW, M_UPK_ROW>(h, gargs(A, Linv_row, bp, KLO_J, KHI_K, 1.0, 0.0, W, bp, nbt, nbt, 0));
     }
-    if (R) {
-      GArgs g;
-      memset(&g, 0, sizeof(g));
-      g.t[0] = term(Opnd{R, bp, M_ROW, nullptr}, Linv_row, bp, KLO_J, KHI_K, 1.0);
-      g.nterm = 1;
-      g.C = h->Wa;
-      g.ldc = bp;
-      g.mt = 1;
-      g.ntl = nbt;
-      launch_gemm(h, g);
-    }
+    if (R)
+      gemm64<M_ROW, M_UPK_ROW>(h, gargs(Opnd{R, bp, M_ROW, nullptr}, Linv_row, bp, KLO_J, KHI_K, 1.0, 0.0, h->Wa, bp,
+                                        1, nbt, 0));
     // 2. X_{i+1,i} = -X_{i+1,i+1} W - X_{a,i+1}^T Wa ; X_{a,i} = -X_{a,i+1} W - X_aa Wa
     if (Lb) {
-      GArgs g;
-      memset(&g, 0, sizeof(g));
-      g.t[0] = term(Opnd{Dn, bp, M_ROW, nullptr}, Opnd{W, bp, M_COL, nullptr}, bp, KLO_0, KHI_K, -1.0);
-      g.nterm = 1;
-      if (Rn) {
-        g.t[1] = term(Opnd{Rn, bp, M_COL, nullptr}, Opnd{h->Wa, bp, M_COL, nullptr}, 64, KLO_0, KHI_K, -1.0);
-        g.nterm = 2;
-      }
-      g.C = Lb;
-      g.ldc = bp;
-      g.mt = nbt;
-      g.ntl = nbt;
-      launch_gemm(h, g);
+      const Opnd A{Dn, bp, M_ROW, nullptr}, B{W, bp, M_COL, nullptr};
+      if (t128) gemm128<M_ROW, M_COL>(h, gargs(A, B, bp, KLO_0, KHI_K, -1.0, 0.0, Lb, bp, nbt / 2, nbt / 2, 0));
+      else gemm64<M_ROW, M_COL>(h, gargs(A, B, bp, KLO_0, KHI_K, -1.0, 0.0, Lb, bp, nbt, nbt, 0));
+      if (Rn)
+        gemm64<M_COL, M_COL>(h, gargs(Opnd{Rn, bp, M_COL, nullptr}, Opnd{h->Wa, bp, M_COL, nullptr}, 64, KLO_0,
+                                      KHI_K, -1.0, 1.0, Lb, bp, nbt, nbt, 0));
     }
     if (R) {
-      GArgs g;
-      memset(&g, 0, sizeof(g));
-      int k = 0;
-      if (Rn) g.t[k++] = term(Opnd{Rn, bp, M_ROW, nullptr}, Opnd{W, bp, M_COL, nullptr}, bp, KLO_0, KHI_K, -1.0);
-      g.t[k++] = term(Opnd{h->tip, 64, M_ROW, nullptr}, Opnd{h->Wa, bp, M_COL, nullptr}, 64, KLO_0, KHI_K, -1.0);
-      g.nterm = k;
-      g.C = R;
-      g.ldc = bp;
-      g.mt = 1;
-      g.ntl = nbt;
-      launch_gemm(h, g);
+      double beta = 0.0;
+      if (Rn) {
+        gemm64<M_ROW, M_COL>(h, gargs(Opnd{Rn, bp, M_ROW, nullptr}, Opnd{W, bp, M_COL, nullptr}, bp, KLO_0, KHI_K,
+                                      -1.0, 0.0, R, bp, 1, nbt, 0));
+        beta = 1.0;
+      }
+      gemm64<M_ROW, M_COL>(h, gargs(Opnd{h->tip, 64, M_ROW, nullptr}, Opnd{h->Wa, bp, M_COL, nullptr}, 64, KLO_0,
+                                    KHI_K, -1.0, beta, R, bp, 1, nbt, 0));
     }
     // 3. X_ii = Linv^T Linv - W^T X_{i+1,i} - Wa^T X_{a,i}  (lower tiles, into S)
-    {
-      GArgs g;
-      memset(&g, 0, sizeof(g));
-      int k = 0;
-      g.t[k++] = term(Linv_row, Linv_row, bp, KLO_MAXIJ, KHI_K, 1.0);
-      if (Lb) g.t[k++] = term(Opnd{W, bp, M_COL, nullptr}, Opnd{Lb, bp, M_COL, nullptr}, bp, KLO_0, KHI_K, -1.0);
-      if (R) g.t[k++] = term(Opnd{h->Wa, bp, M_COL, nullptr}, Opnd{R, bp, M_COL, nullptr}, 64, KLO_0, KHI_K, -1.0);
-      g.nterm = k;
-      g.C = S;
-      g.ldc = bp;
-      g.mt = nbt;
-      g.ntl = nbt;
-      g.lower = 1;
-      launch_gemm(h, g);
-    }
+    gemm64<M_UPK_ROW, M_UPK_ROW>(h, gargs(Linv_row, Linv_row, bp, KLO_MAXIJ, KHI_K, 1.0, 0.0, S, bp, nbt, nbt, 1));
+    if (Lb)
+      gemm64<M_COL, M_COL>(h, gargs(Opnd{W, bp, M_COL, nullptr}, Opnd{Lb, bp, M_COL, nullptr}, bp, KLO_0, KHI_K,
+                                    -1.0, 1.0, S, bp, nbt, nbt, 1));
+    if (R)
+      gemm64<M_COL, M_COL>(h, gargs(Opnd{h->Wa, bp, M_COL, nullptr}, Opnd{R, bp, M_COL, nullptr}, 64, KLO_0, KHI_K,
+                                    -1.0, 1.0, S, bp, nbt, nbt, 1));
     // 4. D_i = symmetric X_ii
     hipLaunchKernelGGL(k_symm_copy, dim3((unsigned)((bb + 255) / 256)), dim3(256), 0, h->stream, D, S, bp);
   }
@@ -1316,16 +1384,13 @@ int dh_get_info(dh_handle* h, int* info) {
   return DH_OK;
 }
 
-int dh_qc_matvec(dh_handle* h, const dh_model* m, const double* coef, double tau, const double* x, double* yp,
-                 double* yl) {
-  if (!h || !coef || !x || !yp || !yl || !model_ok(h, m)) return DH_EARG;
-  Coef cf;
-  memcpy(cf.c, coef, sizeof(cf.c));
+int dh_qc_matvec(dh_handle* h, const dh_model* m, const double* params, const double* x, double* yp, double* yl) {
+  if (!h || !params || !x || !yp || !yl || !model_ok(h, m)) return DH_EARG;
   const long nst = (long)m->ns * m->nt;
   const unsigned g = (unsigned)((nst + 255) / 256);
-  hipLaunchKernelGGL(k_qp_matvec, dim3(g), dim3(256), 0, h->stream, *m, cf, x, yp);
-  hipLaunchKernelGGL(k_ata_matvec, dim3(g), dim3(256), 0, h->stream, *m, tau, x, yl);
-  if (m->nb > 0) hipLaunchKernelGGL(k_ata_matvec_fe, dim3(m->nb), dim3(256), 0, h->stream, *m, tau, x, yp, yl);
+  hipLaunchKernelGGL(k_qp_matvec, dim3(g), dim3(256), 0, h->stream, *m, params, x, yp);
+  hipLaunchKernelGGL(k_ata_matvec, dim3(g), dim3(256), 0, h->stream, *m, params, x, yl);
+  if (m->nb > 0) hipLaunchKernelGGL(k_ata_matvec_fe, dim3(m->nb), dim3(256), 0, h->stream, *m, params, x, yp, yl);
   HIPCHK(hipGetLastError());
   return DH_OK;
 }
@@ -1334,6 +1399,15 @@ int dh_scale(dh_handle* h, const double* x, double s, double* y, long long len) 
   if (!h || !x || !y || len < 0) return DH_EARG;
   if (len == 0) return DH_OK;
   hipLaunchKernelGGL(k_scale, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, h->stream, x, s, y, (long)len);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_scale_dev(dh_handle* h, const double* x, const double* s_dev, double* y, long long len) {
+  if (!h || !x || !s_dev || !y || len < 0) return DH_EARG;
+  if (len == 0) return DH_OK;
+  hipLaunchKernelGGL(k_scale_dev, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, h->stream, x, s_dev, y,
+                     (long)len);
   HIPCHK(hipGetLastError());
   return DH_OK;
 }
@@ -1393,6 +1467,14 @@ int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const doubl
                      h->part);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, h->stream, h->part, (int)g, 4, 4, out_dev);
   HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_set_tuning(dh_handle* h, int key, int value) {
+  if (!h) return DH_EARG;
+  if (key == 0 && (value == 0 || value == 1)) h->syrk_variant = value;
+  else if (key == 1 && (value == 0 || value == 1)) h->gemm_variant = value;
+  else return DH_EARG;
   return DH_OK;
 }
 

@@ -59,33 +59,49 @@ class STInla:
         self.cx = torch.zeros(nq * 4, dtype=torch.float64, device="cuda")
         self.cmu = torch.zeros(nq * 4, dtype=torch.float64, device="cuda")
         self.eig = torch.zeros(4, dtype=torch.float64, device="cuda")
+        # theta-dependent kernel parameters: 12 Kronecker coefficients + exp(theta_obs)
+        self.params = torch.zeros(13, dtype=torch.float64, device="cuda")
+        self.params_h = torch.zeros(13, dtype=torch.float64).pin_memory()
+        self.ones = torch.ones(1, dtype=torch.float64, device="cuda")
         self._state = None
+        self._graph = None
         self.n_f_evals = 0
 
     # ------------------------------------------------------------------
+    def _set_params(self, theta):
+        """Stage the theta-dependent scalars in pinned memory and copy them to the device."""
+        p = np.concatenate([DeviceSTModel.coef_array(theta), [self.dm.tau_scale(theta)]])
+        self.params_h.copy_(torch.from_numpy(p))
+        self.params.copy_(self.params_h, non_blocking=True)
+
     def _assemble(self, theta, sparsity: str, with_like: bool):
-        coef = DeviceSTModel.coef_array(theta)
-        cptr = coef.ctypes.data_as(_lib.c_double_p)
-        _lib.check(self.lib.dh_assemble(self.h, self.dm.ref(), cptr, self.dm.tau_scale(theta),
+        if theta is not None:
+            self._set_params(theta)
+        _lib.check(self.lib.dh_assemble(self.h, self.dm.ref(), self.params.data_ptr(),
                                         _lib.SPARSITY[sparsity], int(with_like)), "dh_assemble")
 
-    def _conditional(self, theta):
-        """Assemble Qc, factor, solve for mu; queue logdet and the quadratic-form pieces."""
+    def _conditional(self, theta, out=None):
+        """Assemble Qc, factor, solve for mu; queue logdet and the quadratic-form pieces.
+
+        theta None: the parameters already on the device are used (graph replay).
+        out: 5-element device tensor for (logdet Qc, mu^T Qp mu, mu^T tau A^T A mu,
+        mu^T A^T y, logdet Qp) (default: the engine's own slots).
+        """
+        out = self.scal if out is None else out
         L = self.lib
         self._assemble(theta, "bta", True)
         _lib.check(L.dh_cholesky(self.h, 1), "dh_cholesky")
-        et = self.dm.tau_scale(theta)
         N = self.dm.n_latent
-        _lib.check(L.dh_scale(self.h, self.dm.aty.data_ptr(), et, self.mu.data_ptr(), N), "dh_scale")
+        # mu = e^tau A^T y: scale by params[12] on the device (a 1x1 "scale" kernel via dot-free copy)
+        _lib.check(L.dh_scale_dev(self.h, self.dm.aty.data_ptr(), self.params[12:13].data_ptr(),
+                                  self.mu.data_ptr(), N), "dh_scale_dev")
         _lib.check(L.dh_solve(self.h, self.mu.data_ptr(), 1), "dh_solve")
-        _lib.check(L.dh_logdet(self.h, 1, self.scal[0:1].data_ptr()), "dh_logdet")
-        coef = DeviceSTModel.coef_array(theta)
-        _lib.check(L.dh_qc_matvec(self.h, self.dm.ref(), coef.ctypes.data_as(_lib.c_double_p), et,
+        _lib.check(L.dh_logdet(self.h, 1, out[0:1].data_ptr()), "dh_logdet")
+        _lib.check(L.dh_qc_matvec(self.h, self.dm.ref(), self.params.data_ptr(),
                                   self.mu.data_ptr(), self.yp.data_ptr(), self.yl.data_ptr()), "dh_qc_matvec")
-        _lib.check(L.dh_dot(self.h, self.mu.data_ptr(), self.yp.data_ptr(), N, self.scal[1:2].data_ptr()), "dh_dot")
-        _lib.check(L.dh_dot(self.h, self.mu.data_ptr(), self.yl.data_ptr(), N, self.scal[2:3].data_ptr()), "dh_dot")
-        _lib.check(L.dh_dot(self.h, self.mu.data_ptr(), self.dm.aty.data_ptr(), N,
-                            self.scal[3:4].data_ptr()), "dh_dot")
+        _lib.check(L.dh_dot(self.h, self.mu.data_ptr(), self.yp.data_ptr(), N, out[1:2].data_ptr()), "dh_dot")
+        _lib.check(L.dh_dot(self.h, self.mu.data_ptr(), self.yl.data_ptr(), N, out[2:3].data_ptr()), "dh_dot")
+        _lib.check(L.dh_dot(self.h, self.mu.data_ptr(), self.dm.aty.data_ptr(), N, out[3:4].data_ptr()), "dh_dot")
 
     def _scalars(self, theta):
         lp, glp = hyper.log_prior_and_grad(theta, self.data.priors)
@@ -93,13 +109,79 @@ class STInla:
         return lp, glp, lik
 
     # ------------------------------------------------------------------
+    def _enqueue_f(self, theta, out=None):
+        """All device work of one plain f evaluation (no host synchronisation)."""
+        out = self.scal if out is None else out
+        self._conditional(theta, out)
+        self._assemble(None, "bt", False)
+        _lib.check(self.lib.dh_cholesky(self.h, 0), "dh_cholesky")
+        _lib.check(self.lib.dh_logdet(self.h, 0, out[4:5].data_ptr()), "dh_logdet")
+
+    def set_stream(self, stream) -> None:
+        _lib.check(self.lib.dh_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)), "dh_set_stream")
+
+    def param_row(self, theta) -> np.ndarray:
+        """The 13 device parameters of theta (Kronecker coefficients, exp(theta_obs))."""
+        return np.concatenate([DeviceSTModel.coef_array(theta), [self.dm.tau_scale(theta)]])
+
+    def hessian(self, theta, h: float = 5e-3, offdiag_points: int = 4):
+        """FD Hessian of f at theta (`core/dalia.py:767-910`), plain f per stencil point.
+
+        offdiag_points=4: the reference's stencil, 1 + 2d + 4 d(d-1)/2 points;
+        offdiag_points=2: 1 + 2d + 2 d(d-1)/2 points (+,+ and -,- corners).
+        """
+        pts = hessian_stencil(theta, h, offdiag_points)
+        fv = np.array([self.f(p) for p in pts])
+        return hessian_from_values(fv, len(theta), h, offdiag_points)
+
+    def _f_from_scalars(self, theta, s) -> float:
+        ld_c, quad_p, quad_l, _, ld_p = s
+        if np.isnan(ld_c) or np.isnan(ld_p):
+            raise ValueError("Logdet is NaN. Check the input matrix.")
+        lp, _, lik = self._scalars(theta)
+        f = 0.5 * ld_c - 0.5 * (quad_p + quad_l) - (lp + lik + 0.5 * ld_p)
+        if np.isnan(f):
+            raise ValueError("f(theta) is NaN")
+        return float(f)
+
+    def capture_f(self):
+        """Capture one plain f evaluation (parameter upload included) in a HIP graph."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):  # warm-up on the side stream before capture
+            _lib.check(self.lib.dh_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)), "dh_set_stream")
+            self._set_params(self.data.theta)
+            self._enqueue_f(None)
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            cs = torch.cuda.current_stream()
+            _lib.check(self.lib.dh_set_stream(self.h, ctypes.c_void_p(cs.cuda_stream)), "dh_set_stream")
+            self.params.copy_(self.params_h, non_blocking=True)
+            self._enqueue_f(None)
+        _lib.check(self.lib.dh_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                   "dh_set_stream")
+        self._graph = g
+        return g
+
+    def f_graph(self, theta) -> float:
+        """Plain f(theta) by replaying the captured graph."""
+        if self._graph is None:
+            self.capture_f()
+        theta = np.asarray(theta, dtype=np.float64)
+        p = np.concatenate([DeviceSTModel.coef_array(theta), [self.dm.tau_scale(theta)]])
+        self.params_h.copy_(torch.from_numpy(p))
+        self._graph.replay()
+        s = self.scal[:5].cpu().numpy()
+        self.n_f_evals += 1
+        self._state = None
+        return self._f_from_scalars(theta, s)
+
     def f(self, theta) -> float:
         """Plain drop-in f(theta): two block factorisations as in the reference."""
         theta = np.asarray(theta, dtype=np.float64)
-        self._conditional(theta)
-        self._assemble(theta, "bt", False)
-        _lib.check(self.lib.dh_cholesky(self.h, 0), "dh_cholesky")
-        _lib.check(self.lib.dh_logdet(self.h, 0, self.scal[4:5].data_ptr()), "dh_logdet")
+        self._set_params(theta)
+        self._enqueue_f(None)
         s = self.scal[:5].cpu().numpy()
         ld_c, quad_p, quad_l, _, ld_p = s
         if np.isnan(ld_c) or np.isnan(ld_p):
@@ -194,7 +276,7 @@ class STInla:
     def _prior_by_selected_inversion(self, theta):
         """logdet Qp|BT and its theta-gradient by pobtf + pobtsi (general FEM inputs)."""
         L = self.lib
-        self._assemble(theta, "bt", False)
+        self._assemble(None, "bt", False)
         _lib.check(L.dh_cholesky(self.h, 0), "dh_cholesky")
         _lib.check(L.dh_logdet(self.h, 0, self.scal[6:7].data_ptr()), "dh_logdet")
         _lib.check(L.dh_selected_inversion(self.h, 0), "dh_selected_inversion")
@@ -210,6 +292,47 @@ class STInla:
     def value_and_grad(self, theta):
         self.forward(theta)
         return self.backward()
+
+
+def hessian_stencil(theta, h: float, offdiag_points: int = 4) -> np.ndarray:
+    """Stencil points of the FD Hessian (rows), f(theta) first, then +-e_i, then pairs."""
+    theta = np.asarray(theta, dtype=np.float64)
+    d = theta.size
+    E = h * np.eye(d)
+    pts = [theta.copy()]
+    for i in range(d):
+        pts += [theta + E[i], theta - E[i]]
+    for i in range(d):
+        for j in range(i + 1, d):
+            if offdiag_points == 4:
+                pts += [theta + E[i] + E[j], theta + E[i] - E[j], theta - E[i] + E[j], theta - E[i] - E[j]]
+            elif offdiag_points == 2:
+                pts += [theta + E[i] + E[j], theta - E[i] - E[j]]
+            else:
+                raise ValueError("offdiag_points must be 4 or 2")
+    return np.stack(pts)
+
+
+def hessian_from_values(fv: np.ndarray, d: int, h: float, offdiag_points: int = 4) -> np.ndarray:
+    """Assemble H from the stencil values (`core/dalia.py:887-902` for the 4-point form)."""
+    H = np.zeros((d, d))
+    f0 = fv[0]
+    fp, fm = fv[1:1 + 2 * d:2], fv[2:2 + 2 * d:2]
+    for i in range(d):
+        H[i, i] = (fp[i] - 2 * f0 + fm[i]) / h ** 2
+    k = 1 + 2 * d
+    for i in range(d):
+        for j in range(i + 1, d):
+            if offdiag_points == 4:
+                fpp, fpm, fmp, fmm = fv[k:k + 4]
+                k += 4
+                H[i, j] = (fpp - fpm - fmp + fmm) / (4 * h * h)
+            else:
+                fpp, fmm = fv[k:k + 2]
+                k += 2
+                H[i, j] = (fpp + fmm - fp[i] - fm[i] - fp[j] - fm[j] + 2 * f0) / (2 * h * h)
+            H[j, i] = H[i, j]
+    return H
 
 
 class FTheta(torch.autograd.Function):

@@ -89,10 +89,12 @@ int dh_set_stream(dh_handle* h, void* hip_stream);
 /* Bytes of device memory the handle owns. */
 long long dh_memory_bytes(const dh_handle* h);
 
-/* Fill the blocks with Q = sum_k sum_m coef[k*4+m] (M_k kron G_m)
+/* Fill the blocks with Q = sum_k sum_m c[k][m] (M_k kron G_m)
  * (+ fe_prec I on the tip for BTA) (+ tau_scale A^T A when with_likelihood).
- * coef: [3][4] Kronecker coefficients.  sparsity BT drops the arrowhead. */
-int dh_assemble(dh_handle* h, const dh_model* m, const double* coef, double tau_scale,
+ * params: DEVICE array of 13 doubles, c[3][4] row-major then tau_scale =
+ * exp(theta_obs); read by the kernels at run time, so a captured graph can be
+ * replayed after updating them.  sparsity BT drops the arrowhead. */
+int dh_assemble(dh_handle* h, const dh_model* m, const double* params,
                 int sparsity, int with_likelihood);
 
 /* Fill the blocks from a symmetric sparse matrix of order n*b (+ a) given as
@@ -122,12 +124,15 @@ int dh_selected_inversion(dh_handle* h, int sparsity);
 int dh_get_info(dh_handle* h, int* info);
 
 /* y_prior = Q_prior x (spatio-temporal part + fe_prec on the fixed effects),
- * y_like = tau_scale A^T A x; vectors of length ns*nt + nb (device). */
-int dh_qc_matvec(dh_handle* h, const dh_model* m, const double* coef, double tau_scale,
+ * y_like = tau_scale A^T A x; vectors of length ns*nt + nb (device);
+ * params as for dh_assemble (device). */
+int dh_qc_matvec(dh_handle* h, const dh_model* m, const double* params,
                  const double* x, double* y_prior, double* y_like);
 
 /* y = s * x (device vectors of length len). */
 int dh_scale(dh_handle* h, const double* x, double s, double* y, long long len);
+/* y = s_dev[0] * x, the scale read from device memory (graph-replay friendly). */
+int dh_scale_dev(dh_handle* h, const double* x, const double* s_dev, double* y, long long len);
 /* *out_dev = x . y */
 int dh_dot(dh_handle* h, const double* x, const double* y, long long len, double* out_dev);
 
@@ -143,10 +148,19 @@ int dh_contract_ata(dh_handle* h, const dh_model* m, double* out_dev);
 int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const double* gam,
                  double* out_dev);
 
+/* Tuning knobs (chosen per shape by the autotuner in dalia_amd/autotune.py):
+ * key 0: Schur-update kernel of pobtaf, 0 = 64x64 tiles (one launch),
+ *        1 = 128x128 tiles with the k range split in two (+ a tail launch);
+ * key 1: dense products of pobtasi, 0 = 64x64 tiles, 1 = 128x128 tiles.
+ * The 128 variants need an even number of 64-tiles per block; otherwise the
+ * 64 variants run.  No counterpart in the reference. */
+int dh_set_tuning(dh_handle* h, int key, int value);
+
 /* Per-kernel timing with HIP events on the handle's stream, for the roofline
  * of one kernel.  kind: 0 k_syrk (Schur update of the next diagonal block in
- * pobtaf), 1 k_update (panel trailing update), 2 k_gemm (pobtasi tiles),
- * 3 k_panel, 4 k_leaf, 5 k_assemble_rows; kind < 0 disables.  While enabled,
+ * pobtaf; k_syrk128 for tuning value 1), 1 k_update (panel trailing update),
+ * 2 k_gemm64/k_gemm128 (pobtasi tiles), 3 k_panel, 4 k_leaf,
+ * 5 k_assemble_rows, 6 k_syrk_tail; kind < 0 disables.  While enabled,
  * each launch of that kind is bracketed by an event pair (at most 16384
  * launches are recorded).  dh_timing_read synchronises the stream and returns
  * the summed kernel time and the number of recorded launches.  No
