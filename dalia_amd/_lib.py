@@ -104,5 +104,6 @@ class DhError(RuntimeError):
 
 def check(rc: int, what: str) -> None:
     if rc != 0:
-        names = {-1: "bad argument", -2: "HIP error", -3: "wrong state"}
+        names = {-1: "bad argument", -2: "HIP error", -3: "wrong state",
+                 -4: "a fused launch gave up waiting for a factored tile"}
         raise DhError(f"{what} failed: {names.get(rc, rc)}")

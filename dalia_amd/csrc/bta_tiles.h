@@ -216,89 +216,194 @@ __device__ __forceinline__ void dh_store(double* C, long ldc, int i0, int j0, co
 // there too.  A pivot that is not > 0 records 1 + row_base + k in *info
 // (atomic min) and lets NaN propagate.
 //
-// Thread (ty, tx) = (tid/16, tid%16) keeps the 16 entries (ty+16u, tx+16v)
-// in registers.  Column step k publishes column k of the trailing matrix and
-// row k of the working inverse W (L W = I is solved alongside) through a
-// double-buffered LDS strip, so each step costs one barrier, then applies
-//   S[i][j] -= S[i][k] S[j][k] / S[k][k]   (i >= j > k, elimination)
-//   W[i][j] -= L[i][k] V[k][j]             (i > k >= j, V = row k of inv(L))
-// An entry switches from S to W when its column is eliminated.  Steps are
-// specialised on KV = k/16 at compile time, so finished rows, the upper
-// triangle and most masks drop out.  No global store is issued inside the
-// loop (a barrier waits for outstanding stores): L goes to LDS, inv(L) stays
-// in registers, and both are written once at the end.
+// Blocked right-looking elimination, 4 columns per barrier.  Thread
+// (ty, tx) = (tid/16, tid%16) keeps the entries (ty+16u, tx+16v) in
+// registers.  Step c (columns K0 = 4c .. K0+3) publishes those 4 columns of
+// the trailing matrix S and rows K0..K0+3 of the working inverse W (L W = I
+// is solved alongside) through LDS; after one barrier every thread factors
+// the 4x4 pivot block P = L_B L_B^T and inverts it (V_B = inv(L_B))
+// redundantly, then
+//   L[i][K0+b] = sum_m S[i][K0+m] V_B[b][m]            (rows i >= K0+4)
+//   V[K0+a][j] = sum_m V_B[a][m] W[K0+m][j]            (columns j < K0)
+//   S[i][j]   -= sum_b L[i][K0+b] L[j][K0+b]            (i, j >= K0+4)
+//   W[i][j]   -= sum_b L[i][K0+b] V[K0+b][j]            (i >= K0+4, j < K0+4)
+// Entries above the diagonal carry unused values.  No global store is issued
+// inside the loop (a barrier waits for outstanding stores): L goes to LDS,
+// inv(L) stays in registers, and both are written once at the end.
 struct LeafShm {
-  double colb[2][64];
-  double rowb[2][64];
-  double pv[2][2];  // (1/sqrt(p), sqrt(p)) of the current pivot
+  double colb[2][4][64];  // columns K0..K0+3 of S (rows >= K0)
+  double rowb[2][4][64];  // rows K0..K0+3 of W (columns < K0)
   double Ls[64][65];
 };
 
+__device__ __forceinline__ double dh_rsq_nr(double p) {
+  double r = __builtin_amdgcn_rsq(p);
+  r = r * (1.5 - 0.5 * p * r * r);
+  r = r * (1.5 - 0.5 * p * r * r);
+  return r;
+}
+
 template <int KV>
-__device__ __forceinline__ void dh_leaf_steps(double R[4][4], LeafShm& sh, int ty, int tx, int tid, int* info,
-                                              int row_base) {
-#pragma unroll 1
-  for (int kr = 0; kr < 16; ++kr) {
-    const int k = 16 * KV + kr;
-    const int bsel = k & 1;
-    if (tx == kr) {  // publish column k (rows >= k), then clear it below the diagonal
-#pragma unroll
-      for (int u = KV; u < 4; ++u) {
-        sh.colb[bsel][ty + 16 * u] = R[u][KV];
-        if (u > KV || ty > kr) R[u][KV] = 0.0;
-      }
-      if (ty == kr) {  // pivot owner: rsq + two Newton steps
-        const double p = R[KV][KV];
-        double r = __builtin_amdgcn_rsq(p);
-        r = r * (1.5 - 0.5 * p * r * r);
-        r = r * (1.5 - 0.5 * p * r * r);
-        sh.pv[bsel][0] = r;
-        sh.pv[bsel][1] = p * r;
-        if (!(p > 0.0)) atomicMin(info, 1 + row_base + k);
-      }
-    }
-    if (ty == kr) {  // publish row k of W (columns < k)
-#pragma unroll
-      for (int v = 0; v <= KV; ++v) sh.rowb[bsel][tx + 16 * v] = R[KV][v];
-    }
-    __syncthreads();
-    const double rd = sh.pv[bsel][0];
-    const double d = sh.pv[bsel][1];
-    const double rp = rd * rd;
-    if (tid < 64) sh.Ls[tid][k] = (tid > k) ? sh.colb[bsel][tid] * rd : (tid == k ? d : 0.0);
-    double cim[4], lim[4], cjr[4], vjm[4];
+__device__ __forceinline__ void dh_leaf_block(double R[4][4], LeafShm& sh, int ty, int tx, int tid, int* info,
+                                              int row_base, int c4) {
+  const int kq = 4 * c4;
+  const int K0 = 16 * KV + kq;
+  const int bsel = c4 & 1;
+  const bool colown = tx >= kq && tx < kq + 4;
+  const bool rowown = ty >= kq && ty < kq + 4;
+  if (colown) {
 #pragma unroll
     for (int u = KV; u < 4; ++u) {
-      const double c = sh.colb[bsel][ty + 16 * u];
-      cim[u] = (u > KV || ty > kr) ? c : 0.0;
-      lim[u] = cim[u] * rd;
-    }
-#pragma unroll
-    for (int v = KV; v < 4; ++v) {
-      const double c = sh.colb[bsel][tx + 16 * v] * rp;
-      cjr[v] = (v > KV || tx > kr) ? c : 0.0;
-    }
-#pragma unroll
-    for (int v = 0; v <= KV; ++v) {
-      if (v < KV) {
-        vjm[v] = sh.rowb[bsel][tx + 16 * v] * rd;
-      } else {
-        const double w = sh.rowb[bsel][tx + 16 * v] * rd;
-        vjm[v] = (tx < kr) ? w : (tx == kr ? rd : 0.0);
-      }
-    }
-#pragma unroll
-    for (int u = KV; u < 4; ++u) {
-#pragma unroll
-      for (int v = KV; v <= u; ++v) R[u][v] = fma(-cim[u], cjr[v], R[u][v]);
-#pragma unroll
-      for (int v = 0; v <= KV; ++v) R[u][v] = fma(-lim[u], vjm[v], R[u][v]);
-    }
-    if (ty == kr) {  // row k of inv(L) is final
-#pragma unroll
-      for (int v = 0; v <= KV; ++v) R[KV][v] = vjm[v];
+      sh.colb[bsel][tx - kq][ty + 16 * u] = R[u][KV];
+      if (u > KV || ty >= kq + 4) R[u][KV] = 0.0;  // becomes W[i][K0+jj] below the block
     }
   }
+  if (rowown) {
+#pragma unroll
+    for (int v = 0; v <= KV; ++v)
+      if (v < KV || tx < kq) sh.rowb[bsel][ty - kq][tx + 16 * v] = R[KV][v];
+  }
+  __syncthreads();
+  // 4x4 pivot block: Cholesky and inverse (every thread)
+  double P[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b <= a; ++b) P[a][b] = sh.colb[bsel][b][K0 + a];
+  double Lb[4][4], Vb[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) Lb[a][b] = Vb[a][b] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double piv = P[k][k];
+#pragma unroll
+    for (int m = 0; m < k; ++m) piv -= Lb[k][m] * Lb[k][m];
+    const double r = dh_rsq_nr(piv);
+    if (tid == 0 && !(piv > 0.0)) atomicMin(info, 1 + row_base + K0 + k);
+    Lb[k][k] = piv * r;
+    Vb[k][k] = r;
+#pragma unroll
+    for (int a = k + 1; a < 4; ++a) {
+      double x = P[a][k];
+#pragma unroll
+      for (int m = 0; m < k; ++m) x -= Lb[a][m] * Lb[k][m];
+      Lb[a][k] = x * r;
+    }
+  }
+#pragma unroll
+  for (int a = 1; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < a; ++b) {
+      double x = 0.0;
+#pragma unroll
+      for (int m = b; m < a; ++m) x += Lb[a][m] * Vb[m][b];
+      Vb[a][b] = -Vb[a][a] * x;
+    }
+  // L columns K0..K0+3 into LDS (thread tid < 64 owns row tid)
+  if (tid < 64) {
+    const int i = tid;
+    if (i >= K0 + 4) {
+      double c[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) c[m] = sh.colb[bsel][m][i];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        double x = 0.0;
+#pragma unroll
+        for (int m = 0; m <= b; ++m) x += c[m] * Vb[b][m];
+        sh.Ls[i][K0 + b] = x;
+      }
+    } else if (i >= K0) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        double x = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) x = (a == i - K0) ? Lb[a][b] : x;
+        sh.Ls[i][K0 + b] = x;
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) sh.Ls[i][K0 + b] = 0.0;
+    }
+  }
+  // per column: S-part factors (L[j][K0+b]) or W-part factors (V[K0+b][j])
+  double gj[4][4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int j = tx + 16 * v;
+    if (v > KV || (v == KV && tx >= kq + 4)) {  // S part: j >= K0+4
+      double c[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) c[m] = sh.colb[bsel][m][j];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        double x = 0.0;
+#pragma unroll
+        for (int m = 0; m <= b; ++m) x += c[m] * Vb[b][m];
+        gj[v][b] = x;
+      }
+    } else if (v < KV || tx < kq) {  // W part: j < K0
+      double w[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) w[m] = sh.rowb[bsel][m][j];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        double x = 0.0;
+#pragma unroll
+        for (int m = 0; m <= b; ++m) x += Vb[b][m] * w[m];
+        gj[v][b] = x;
+      }
+    } else {  // in-block column j = K0 + (tx - kq): V[K0+b][j] = Vb[b][tx-kq]
+      const int jj = tx - kq;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        double x = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) x = (m == jj && b >= m) ? Vb[b][m] : x;
+        gj[v][b] = x;
+      }
+    }
+  }
+  // rank-4 updates of the rows below the block (lower and diagonal 16x16 sub-blocks only)
+#pragma unroll
+  for (int u = KV; u < 4; ++u) {
+    const int i = ty + 16 * u;
+    const bool below = (u > KV) || (ty >= kq + 4);
+    double c[4], li[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) c[m] = below ? sh.colb[bsel][m][i] : 0.0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      double x = 0.0;
+#pragma unroll
+      for (int m = 0; m <= b; ++m) x += c[m] * Vb[b][m];
+      li[b] = x;
+    }
+#pragma unroll
+    for (int v = 0; v <= u; ++v)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) R[u][v] = fma(-li[b], gj[v][b], R[u][v]);
+  }
+  // rows of the block are final: row K0+aa of inv(L)
+  if (rowown) {
+    const int aa = ty - kq;
+#pragma unroll
+    for (int v = 0; v <= KV; ++v) {
+      double x = 0.0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) x = (b == aa) ? gj[v][b] : x;
+      R[KV][v] = x;
+    }
+  }
+}
+
+template <int KV>
+__device__ __forceinline__ void dh_leaf_blocks(double R[4][4], LeafShm& sh, int ty, int tx, int tid, int* info,
+                                               int row_base) {
+#pragma unroll 1
+  for (int c4 = 0; c4 < 4; ++c4) dh_leaf_block<KV>(R, sh, ty, tx, tid, info, row_base, c4);
 }
 
 __device__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_out, int* info,
@@ -314,10 +419,10 @@ __device__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_o
       const int i = ty + 16 * u, j = tx + 16 * v;
       R[u][v] = (j <= i) ? tile[(long)i * ld + j] : 0.0;
     }
-  dh_leaf_steps<0>(R, sh, ty, tx, tid, info, row_base);
-  dh_leaf_steps<1>(R, sh, ty, tx, tid, info, row_base);
-  dh_leaf_steps<2>(R, sh, ty, tx, tid, info, row_base);
-  dh_leaf_steps<3>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_blocks<0>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_blocks<1>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_blocks<2>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_blocks<3>(R, sh, ty, tx, tid, info, row_base);
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < 4; ++u)

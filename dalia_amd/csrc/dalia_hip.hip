@@ -26,6 +26,7 @@
 #define DH_EARG -1
 #define DH_EHIP -2
 #define DH_ESTATE -3
+#define DH_ESYNC -4
 
 #define HIPCHK(x)                          \
   do {                                     \
@@ -78,6 +79,10 @@ __global__ __launch_bounds__(256) void k_sum_seg(const double* part, int nseg, i
 }
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
+
+__global__ __launch_bounds__(256) void k_fill_int(int* p, int v, int len) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < len; e += gridDim.x * blockDim.x) p[e] = v;
+}
 
 __global__ __launch_bounds__(256) void k_copy(const double* x, double* y, int len) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -242,9 +247,51 @@ struct BtaPtrs {
   double* pinv; // 64x64 dense inverse of the current diagonal tile
   double* dinv; // block i 1/diag(L)
   double* dinvn;// block i+1 1/diag(L)
+  double* Ln;   // block i+1 sub-diagonal (k_syrk fused panel 0), or null
   int* info;
+  int* flag;    // readiness flag of the diagonal tile factored in this launch (fused TRSM)
+  int* err;     // set when a flag wait gives up
   int bp, nbt, row_base, row_base_n;
+  int fuse;     // 1: workgroups of the next panel column wait for the leaf and apply the TRSM
 };
+
+// Cross-workgroup hand-off of one factored diagonal tile inside a launch.
+// The waiting workgroups are dispatched after the signalling one (it has the
+// lowest linear id), so the wait ends; it is bounded anyway and records a
+// failure in *err instead of spinning forever.
+__device__ __forceinline__ void dh_signal(int* flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__device__ __forceinline__ void dh_wait(int* flag, int* err) {
+  if (threadIdx.x == 0) {
+    long it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++it > (1L << 24)) {
+        atomicOr(err, 1);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// In place: X = X inv(L_pp)^T for one 64x64 tile (pinv holds the dense inv(L_pp)).
+__device__ __forceinline__ void dh_tile_trsm(double* X, long ld, const double* pinv) {
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  Opnd A{X, ld, M_ROW, nullptr};
+  Opnd B{pinv, DH_TILE, M_ROW, nullptr};
+  dh_mma<M_ROW, M_ROW>(acc, A, 0, B, 0, 0, DH_TILE);
+  __syncthreads();
+  dh_store(X, ld, 0, 0, acc, 1.0, 0.0, false);
+}
 
 __global__ __launch_bounds__(256) void k_leaf(double* tile, long ld, double* dinv, double* inv_out, int* info,
                                               int row_base) {
@@ -272,6 +319,7 @@ __global__ __launch_bounds__(256) void k_panel(BtaPtrs P, int p) {
   Opnd A{base, P.bp, M_ROW, nullptr};
   Opnd B{P.pinv, DH_TILE, M_ROW, nullptr};  // opB[k][j] = inv[j][k]
   dh_mma<M_ROW, M_ROW>(acc, A, 0, B, 0, 0, DH_TILE);
+  __syncthreads();  // in place: every wave has read the whole tile before any writes
   dh_store(base, P.bp, 0, 0, acc, 1.0, 0.0, false);
 }
 
@@ -295,6 +343,10 @@ __global__ __launch_bounds__(256) void k_update(BtaPtrs P, int p) {
     __syncthreads();
     __threadfence();
     dh_leaf_potrf(C, P.bp, P.dinv + J * DH_TILE, P.pinv, P.info, P.row_base + J * DH_TILE);
+    if (P.fuse) dh_signal(P.flag);
+  } else if (P.fuse && J == p + 1) {
+    dh_wait(P.flag, P.err);
+    dh_tile_trsm(C, P.bp, P.pinv);
   }
 }
 
@@ -320,6 +372,10 @@ __global__ __launch_bounds__(256) void k_syrk(BtaPtrs P) {
       __syncthreads();
       __threadfence();
       dh_leaf_potrf(C, P.bp, P.dinvn, P.pinv, P.info, P.row_base_n);
+      if (P.fuse) dh_signal(P.flag);
+    } else if (P.fuse && J == 0) {  // first panel column of block i+1
+      dh_wait(P.flag, P.err);
+      dh_tile_trsm(C + (long)I * DH_TILE * P.bp, P.bp, P.pinv);
     }
     return;
   }
@@ -329,12 +385,25 @@ __global__ __launch_bounds__(256) void k_syrk(BtaPtrs P) {
     Opnd B{P.L, P.bp, M_ROW, nullptr};
     dh_mma<M_ROW, M_ROW>(acc, A, 0, B, e * DH_TILE, 0, K);
     dh_store(P.Rn, P.bp, 0, e * DH_TILE, acc, -1.0, 1.0, false);
+    if (P.fuse && e == 0) {
+      dh_wait(P.flag, P.err);
+      dh_tile_trsm(P.Rn, P.bp, P.pinv);
+    }
     return;
   }
-  // tip
-  Opnd A{P.R, P.bp, M_ROW, nullptr};
-  dh_mma<M_ROW, M_ROW>(acc, A, 0, A, 0, 0, K);
-  dh_store(P.T, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
+  e -= narr;
+  if (P.R && e == 0) {  // tip
+    Opnd A{P.R, P.bp, M_ROW, nullptr};
+    dh_mma<M_ROW, M_ROW>(acc, A, 0, A, 0, 0, K);
+    dh_store(P.T, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
+    return;
+  }
+  if (P.R) e -= 1;
+  // remaining workgroups: first panel column of block i+1's sub-diagonal block
+  if (P.fuse && P.Ln && e < nbt) {
+    dh_wait(P.flag, P.err);
+    dh_tile_trsm(P.Ln + (long)e * DH_TILE * P.bp, P.bp, P.pinv);
+  }
 }
 
 // Schur update with 128x128 tiles and the k range split in two: half 0
@@ -937,6 +1006,8 @@ struct dh_handle {
   double* arrpart;  // n x 64
   double* eigbuf;   // 2 x n x bp
   int* info;        // device
+  int* flags;       // n * nbt readiness flags of fused launches
+  int* err;         // flag-wait failures
   int factored;     // sparsity of the current factor, -1 none
   int inverted;     // packed inverse of the diagonal factors present
   std::vector<NodeList> levels;  // inverse recursion, deepest last
@@ -1036,6 +1107,8 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   rc |= dalloc(h, &h->arrpart, (long long)n * 64);
   rc |= dalloc(h, &h->eigbuf, 2LL * n * h->bp);
   if (hipMalloc((void**)&h->info, sizeof(int)) != hipSuccess) rc = DH_EHIP;
+  if (hipMalloc((void**)&h->flags, sizeof(int) * ((size_t)n * h->nbt + 1)) != hipSuccess) rc = DH_EHIP;
+  if (hipMalloc((void**)&h->err, sizeof(int)) != hipSuccess) rc = DH_EHIP;
   if (rc != DH_OK) {
     dh_destroy(h);
     return DH_EHIP;
@@ -1052,6 +1125,8 @@ int dh_destroy(dh_handle* h) {
   for (double* p : ps)
     if (p) (void)hipFree(p);
   if (h->info) (void)hipFree(h->info);
+  if (h->flags) (void)hipFree(h->flags);
+  if (h->err) (void)hipFree(h->err);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   delete h;
   return DH_OK;
@@ -1127,7 +1202,10 @@ int dh_cholesky(dh_handle* h, int sparsity) {
   const int n = h->n, bp = h->bp, nbt = h->nbt;
   const long bb = (long)bp * bp;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->info, INT_MAX);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->err, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, h->stream, h->flags, 0, n * nbt);
   DH_LAUNCH(h, 4, k_leaf, dim3(1), dim3(256), 0, h->diag, (long)bp, h->dinv, h->pinv, h->info, 0);
+  bool first_done = false;  // panel 0 of block i already solved by the previous launch
   for (int i = 0; i < n; ++i) {
     BtaPtrs P;
     P.D = h->diag + i * bb;
@@ -1135,19 +1213,28 @@ int dh_cholesky(dh_handle* h, int sparsity) {
     P.R = arrow ? h->arrow + (long)i * h->ap * bp : nullptr;
     P.Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
     P.Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * h->ap * bp : nullptr;
+    P.Ln = (i < n - 2) ? h->lower + (i + 1) * bb : nullptr;
     P.T = arrow ? h->tip : nullptr;
     P.pinv = h->pinv;
     P.dinv = h->dinv + (long)i * bp;
     P.dinvn = (i < n - 1) ? h->dinv + (long)(i + 1) * bp : nullptr;
     P.info = h->info;
+    P.err = h->err;
     P.bp = bp;
     P.nbt = nbt;
     P.row_base = i * h->b;  // logical row offset (padding rows never fail)
     P.row_base_n = (i + 1) * h->b;
-    for (int p = 0; p < nbt; ++p) {
+    if (!first_done) {
+      const int rows0 = (nbt - 1) + (P.L ? nbt : 0) + (P.R ? 1 : 0);
+      P.fuse = 0;
+      P.flag = h->flags;
+      if (rows0 > 0) DH_LAUNCH(h, 3, k_panel, dim3(rows0), dim3(256), 0, P, 0);
+    }
+    for (int p = 0; p < nbt - 1; ++p) {
       const int rows = (nbt - 1 - p) + (P.L ? nbt : 0) + (P.R ? 1 : 0);
-      if (rows > 0) DH_LAUNCH(h, 3, k_panel, dim3(rows), dim3(256), 0, P, p);
-      if (p < nbt - 1) DH_LAUNCH(h, 1, k_update, dim3(rows, nbt - 1 - p), dim3(256), 0, P, p);
+      P.fuse = 1;
+      P.flag = h->flags + (long)i * nbt + p + 1;
+      DH_LAUNCH(h, 1, k_update, dim3(rows, nbt - 1 - p), dim3(256), 0, P, p);
     }
     const int ntri = P.L ? nbt * (nbt + 1) / 2 : 0;
     const int narr = (P.L && P.R) ? nbt : 0;
@@ -1156,9 +1243,14 @@ int dh_cholesky(dh_handle* h, int sparsity) {
       const int ntri2 = nt2 * (nt2 + 1) / 2;
       DH_LAUNCH(h, 0, k_syrk128, dim3(ntri2, 2), dim3(256), 0, P, h->W);
       DH_LAUNCH(h, 6, k_syrk_tail, dim3(ntri2 + narr + (P.R ? 1 : 0)), dim3(256), 0, P, (const double*)h->W);
+      first_done = false;
     } else {
-      const int nt = ntri + narr + (P.R ? 1 : 0);
+      P.fuse = P.L ? 1 : 0;
+      P.flag = h->flags + (long)(i + 1) * nbt;
+      const int extra = (P.fuse && P.Ln) ? nbt : 0;
+      const int nt = ntri + narr + (P.R ? 1 : 0) + extra;
       if (nt > 0) DH_LAUNCH(h, 0, k_syrk, dim3(nt), dim3(256), 0, P);
+      first_done = P.fuse != 0;
     }
   }
   if (arrow) {
@@ -1377,9 +1469,11 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
 
 int dh_get_info(dh_handle* h, int* info) {
   if (!h || !info) return DH_EARG;
-  int v = 0;
+  int v = 0, e = 0;
   HIPCHK(hipMemcpyAsync(&v, h->info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(&e, h->err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
+  if (e) return DH_ESYNC;
   *info = (v == INT_MAX) ? 0 : v;
   return DH_OK;
 }

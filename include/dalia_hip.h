@@ -120,7 +120,9 @@ int dh_logdet(dh_handle* h, int sparsity, double* out_dev);
  * diagonal blocks and the tip are stored full symmetric. */
 int dh_selected_inversion(dh_handle* h, int sparsity);
 
-/* Synchronises the stream; *info = 0 or 1 + global row of the first failing pivot. */
+/* Synchronises the stream; *info = 0 or 1 + global row of the first failing pivot.
+ * Returns -4 if a workgroup gave up waiting for a factored diagonal tile
+ * inside a fused launch (results are then invalid). */
 int dh_get_info(dh_handle* h, int* info);
 
 /* y_prior = Q_prior x (spatio-temporal part + fe_prec on the fixed effects),
