@@ -180,20 +180,22 @@ def main():
 
     # roofline of the dominant kernel, per launch, events around each launch
     kname = args.roofline_kernel
+    if kname == "k_syrk" and choice["alg"] == 1:
+        kname = "k_pobtaf_dag"  # the persistent schedule has no separate Schur-update launch
     kind = _lib.TIMED_KERNELS[kname]
     eng._assemble(theta, "bta", True)
     _lib.check(lib.dh_timing_enable(eng.h, kind), "dh_timing_enable")
+    _lib.check(lib.dh_cholesky(eng.h, 1), "dh_cholesky")
     if kname == "k_gemm":
-        _lib.check(lib.dh_cholesky(eng.h, 1), "dh_cholesky")
         _lib.check(lib.dh_selected_inversion(eng.h, 1), "dh_selected_inversion")
-    else:
-        _lib.check(lib.dh_cholesky(eng.h, 1), "dh_cholesky")
     tot = ctypes.c_double()
     cnt = ctypes.c_longlong()
     _lib.check(lib.dh_timing_read(eng.h, ctypes.byref(tot), ctypes.byref(cnt)), "dh_timing_read")
     _lib.check(lib.dh_timing_enable(eng.h, -1), "dh_timing_enable")
     avg_s = tot.value * 1e-3 / max(cnt.value, 1)
-    if kname == "k_syrk" and choice["syrk"] == 1:
+    if kname == "k_pobtaf_dag":
+        per = f_pobtaf(nt, ns, nb) - nb ** 3 / 3  # the whole factorization except the tip's own Cholesky
+    elif kname == "k_syrk" and choice["syrk"] == 1:
         per = float(ns) ** 3  # k_syrk128: the SYRK alone (arrow and tip run in k_syrk_tail)
         kname = "k_syrk128"
     elif kname == "k_syrk":
@@ -229,7 +231,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "ns": ns, "nt": nt, "nb": nb, "seed": seed, "theta_dim": 4,
-                   "n_obs": data.n_obs, "kernel_variants": {"syrk": choice["syrk"], "gemm": choice["gemm"]}},
+                   "n_obs": data.n_obs, "kernel_variants": {"pobtaf_schedule": "tile-task" if choice["alg"] == 1 else "per-panel",
+                                                        "syrk": choice["syrk"], "gemm": choice["gemm"]}},
         "pobtaf_ms": t_pobtaf * 1e3,
         "pobtaf_tflops": pobtaf_tflops,
         "pobtaf_frac_of_peak": pobtaf_tflops / FP64_PEAK_TFLOPS,

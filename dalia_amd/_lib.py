@@ -75,7 +75,7 @@ SIGNATURES = {
 
 #: kernel kinds of dh_timing_enable
 TIMED_KERNELS = {"k_syrk": 0, "k_update": 1, "k_gemm": 2, "k_panel": 3, "k_leaf": 4, "k_assemble_rows": 5,
-                 "k_syrk_tail": 6}
+                 "k_syrk_tail": 6, "k_pobtaf_dag": 7}
 
 _lib = None
 

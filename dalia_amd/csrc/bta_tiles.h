@@ -185,6 +185,65 @@ __device__ __forceinline__ void dh_store128(double* C, long ldc, int i0, int j0,
       }
 }
 
+// Operand registers of opA[i0:i0+64, 0:64] for dh_mma_a64 (loaded ahead of
+// the rest of the product).
+template <int MA>
+__device__ __forceinline__ void dh_load_a64(const Opnd& A, int i0, double a[2][2][2][4]) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int ra = i0 + (w >> 1) * 32 + (lane & 15);
+  const int kq = (lane >> 4) * 4;
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      dh_load4<MA>(A, ra, 32 * kc + 16 * h + kq, a[kc][h][0]);
+      dh_load4<MA>(A, ra + 16, 32 * kc + 16 * h + kq, a[kc][h][1]);
+    }
+}
+
+// acc += opA[i0:i0+64, 0:64] opB[j0:j0+64, 0:64]^T with opA from dh_load_a64.
+template <int MB>
+__device__ __forceinline__ void dh_mma_a64(dh_d4 acc[2][2], const double a[2][2][2][4], const Opnd& B, int j0) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int rb = j0 + (w & 1) * 32 + (lane & 15);
+  const int kq = (lane >> 4) * 4;
+  double b[2][2][2][4];
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      dh_load4<MB>(B, rb, 32 * kc + 16 * h + kq, b[kc][h][0]);
+      dh_load4<MB>(B, rb + 16, 32 * kc + 16 * h + kq, b[kc][h][1]);
+    }
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        acc[0][0] = dh_mfma(a[kc][h][0][m], b[kc][h][0][m], acc[0][0]);
+        acc[0][1] = dh_mfma(a[kc][h][0][m], b[kc][h][1][m], acc[0][1]);
+        acc[1][0] = dh_mfma(a[kc][h][1][m], b[kc][h][0][m], acc[1][0]);
+        acc[1][1] = dh_mfma(a[kc][h][1][m], b[kc][h][1][m], acc[1][1]);
+      }
+}
+
+// The inverse of dh_store's mapping: v = C[i0.., j0..] in accumulator layout.
+__device__ __forceinline__ void dh_load_acc(const double* C, long ldc, int i0, int j0, dh_d4 v[2][2]) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int rbase = i0 + (w >> 1) * 32 + (lane >> 4);
+  const int cbase = j0 + (w & 1) * 32 + (lane & 15);
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[bi][bj][r] = C[(long)(rbase + bi * 16 + 4 * r) * ldc + cbase + bj * 16];
+}
+
 // C[i0.., j0..] = alpha*acc + beta*C (trans: C[j][i] instead of C[i][j]).
 // Lane l, item r of the 16x16 f64 MFMA result holds row 4*r + l/16 and
 // column l%16 (measured on gfx950 with one-hot operands).
@@ -406,8 +465,10 @@ __device__ __forceinline__ void dh_leaf_blocks(double R[4][4], LeafShm& sh, int 
   for (int c4 = 0; c4 < 4; ++c4) dh_leaf_block<KV>(R, sh, ty, tx, tid, info, row_base, c4);
 }
 
-__device__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_out, int* info,
-                              int row_base) {
+// Factor the tile read from src (leading dimension lds; may be LDS) into
+// tile (ld); src == tile factors in place.
+__device__ void dh_leaf_potrf2(const double* src, long lds, double* tile, long ld, double* dinv, double* inv_out,
+                               int* info, int row_base) {
   __shared__ LeafShm sh;
   const int tid = threadIdx.x;
   const int ty = tid >> 4, tx = tid & 15;
@@ -417,7 +478,7 @@ __device__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_o
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int i = ty + 16 * u, j = tx + 16 * v;
-      R[u][v] = (j <= i) ? tile[(long)i * ld + j] : 0.0;
+      R[u][v] = (j <= i) ? src[(long)i * lds + j] : 0.0;
     }
   dh_leaf_blocks<0>(R, sh, ty, tx, tid, info, row_base);
   dh_leaf_blocks<1>(R, sh, ty, tx, tid, info, row_base);
@@ -436,4 +497,9 @@ __device__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_o
       }
       if (inv_out) inv_out[i * 64 + j] = (j <= i) ? R[u][v] : 0.0;
     }
+}
+
+__device__ __forceinline__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_out, int* info,
+                                              int row_base) {
+  dh_leaf_potrf2(tile, ld, tile, ld, dinv, inv_out, info, row_base);
 }

@@ -12,6 +12,7 @@
 //   adjoint       k_qp_matvec, k_ata_matvec*, k_contract_*, k_prior_eig
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -462,6 +463,294 @@ __global__ __launch_bounds__(256) void k_syrk_tail(BtaPtrs P, const double* Pbuf
   Opnd A{P.R, P.bp, M_ROW, nullptr};
   dh_mma<M_ROW, M_ROW>(acc, A, 0, A, 0, 0, P.bp);
   dh_store(P.T, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
+}
+
+// ---------------------------------------------------------------------------
+// pobtaf as one persistent launch of left-looking tile tasks
+// ---------------------------------------------------------------------------
+//
+// Every 64x64 tile of the factor is written once by the task that owns it,
+// after accumulating its whole Schur complement in registers:
+//   diagonal block i, tile (r, c), r >= c:
+//     A - sum_k L_{i,i-1}(r,k) L_{i,i-1}(c,k)^T - sum_{k<c} L_ii(r,k) L_ii(c,k)^T,
+//     then Cholesky (r == c) or times inv(L_ii(c,c))^T;
+//   sub-diagonal block L_{i+1,i}, tile (r, c): A - sum_{k<c} L(r,k) L_ii(c,k)^T, then the same solve;
+//   arrow tile c of block i: A - sum_k L_{a,i-1}(k) L_{i,i-1}(c,k)^T - sum_{k<c} L_{a,i}(k) L_ii(c,k)^T, solve;
+//   tip partial of block i: T -= sum_k L_{a,i}(k) L_{a,i}(k)^T, applied in block order.
+// The sequential part is the chain of diagonal tiles.  It is carried by one
+// CHAIN task per tile column c: helper tasks leave the partial sums of tiles
+// (c, c) and (c, c-1) without their last term in place, and CHAIN(c) solves
+// tile (c, c-1) against the factor of tile (c-1, c-1), applies that last
+// rank-64 term to tile (c, c) and factors it.  For c = 0 the last term is the
+// previous block's sub-diagonal tile (0, nbt-1).  Each helper is split so
+// that it is never late: OLD_D/OLD_L(c) (the previous block's terms and own
+// terms k < c - look) are queued look columns ahead, PRE_D/PRE_L(c) (own
+// terms c - look <= k < c - 1) one column ahead.
+// Tasks are handed out by an atomic counter in block-major order (a decode
+// table per block shape), and every task only waits for tasks with smaller
+// ids, so the oldest unfinished task can always proceed and the persistent
+// grid drains; the chain overlaps the bulk of the products of later columns
+// and of the next block.
+enum { DK_PRE_D = 0, DK_PRE_L = 1, DK_CHAIN = 2, DK_DIAG = 3, DK_LOWER = 4, DK_ARROW = 5, DK_TIP = 6,
+       DK_OLD_D = 7, DK_OLD_L = 8 };
+
+struct DagArgs {
+  double *diag, *lower, *arrow, *tip, *dinv;
+  double* pinv;    // dense inv(L) of every diagonal tile, [2][nbt][64][64] (block parity, tile column)
+  int* flags;      // per block, see dag_fstride
+  int* counter;    // next task
+  const int* tab;  // decode tables: blocks 0..n-2 (tfull entries), last block
+  int* info;
+  int* err;
+  int n, b, bp, nbt, arrow_on;
+  int tfull;       // tasks of blocks 0..n-2 (the last block has no sub-diagonal tiles)
+  int ntasks;
+  int look;        // helper lookahead in tile columns (>= 1)
+};
+
+#define DH_DAG_SPIN (1 << 20)
+#define DH_DAG_LOOK 6
+#define DH_LDT 66  // leading dimension of the staged tile in LDS (16-byte aligned rows)
+
+// Flags of one block: final D tiles [nbt][nbt], final L tiles [nbt][nbt],
+// final arrow tiles [nbt], tip partial applied, PRE_D, PRE_L, OLD_D, OLD_L done [nbt each].
+__device__ __host__ __forceinline__ int dag_fstride(int nbt) { return 2 * nbt * nbt + 5 * nbt + 1; }
+
+// Number (1..64) of consecutive k in [k, kend) whose tiles are final in both
+// flag rows (fb may be null), after waiting until there is at least one.
+// Workgroup-collective; ends with an agent-scope acquire.  A wait that
+// exceeds its bound records a failure in *err, and once *err is set every
+// wait returns at once, so the launch always drains.  Wave 0 polls; the
+// count is made wave-uniform (readfirstlane) so that every branch around the
+// barriers of the callers is a scalar branch.
+__device__ int dag_ready(const int* fa, const int* fb, int k, int kend, int* err, int* s_n) {
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x) < 64) {
+    const int lane = threadIdx.x;
+    int it = 0;
+    int cnt;
+    for (;;) {
+      const int kk = k + lane;
+      bool ok = true;
+      if (kk < kend) {
+        ok = __hip_atomic_load(fa + kk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        if (fb) ok = ok && __hip_atomic_load(fb + kk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+      }
+      const unsigned long long notready = __ballot(!ok);
+      cnt = notready ? __ffsll(notready) - 1 : 64;
+      cnt = min(cnt, kend - k);
+      if (cnt > 0) break;
+      if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0) {
+        cnt = kend - k;
+        break;
+      }
+      if (++it > DH_DAG_SPIN) {
+        cnt = -1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    if (cnt < 0) {
+      cnt = kend - k;
+      atomicOr(err, 2);
+    }
+    *s_n = cnt;  // every lane of wave 0 writes the same value
+  }
+  __syncthreads();
+  const int cnt = __builtin_amdgcn_readfirstlane(*s_n);
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return cnt;
+}
+
+// acc += opA[ra.., k] opB[rb.., k] over tile columns k in [0, kend), each
+// tile column consumed once its flags are set.
+__device__ __forceinline__ void dag_accumulate(dh_d4 acc[2][2], const double* A, int ra, const double* B, int rb,
+                                               int bp, const int* fa, const int* fb, int kend, int* err,
+                                               int* s_n) {
+  const Opnd oa{A, bp, M_ROW, nullptr};
+  const Opnd ob{B, bp, M_ROW, nullptr};
+  for (int k = 0; k < kend;) {
+    const int cnt = dag_ready(fa, fb, k, kend, err, s_n);
+    dh_mma<M_ROW, M_ROW>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
+    k += cnt;
+  }
+}
+
+// dag_accumulate over tile columns k in [k0, kend).
+__device__ __forceinline__ void dag_accumulate_from(dh_d4 acc[2][2], const double* A, int ra, const double* B,
+                                                    int rb, int bp, const int* fa, const int* fb, int k0, int kend,
+                                                    int* err, int* s_n) {
+  const Opnd oa{A, bp, M_ROW, nullptr};
+  const Opnd ob{B, bp, M_ROW, nullptr};
+  for (int k = k0; k < kend;) {
+    const int cnt = dag_ready(fa, fb, k, kend, err, s_n);
+    dh_mma<M_ROW, M_ROW>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
+    k += cnt;
+  }
+}
+
+// C = C inv(L)^T in place, inv(L) dense row-major (ld 64).
+__device__ __forceinline__ void dag_trsm(double* C, const double* pinv, int bp) {
+  dh_d4 x[2][2];
+  dh_zero(x);
+  dh_mma<M_ROW, M_ROW>(x, Opnd{C, bp, M_ROW, nullptr}, 0, Opnd{pinv, DH_TILE, M_ROW, nullptr}, 0, 0, DH_TILE);
+  __syncthreads();  // every wave has read C
+  dh_store(C, bp, 0, 0, x, 1.0, 0.0, false);
+}
+
+// Publish a finished tile: wave 0 (a wave-uniform branch, so that no lane
+// leaves the task loop on its own) releases the tile at agent scope and sets
+// its flag, every lane storing the same value.
+__device__ __forceinline__ void dag_signal(int* flag) {
+  __syncthreads();
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x) < 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// One task (t wave-uniform); every path ends at the single signal.
+__device__ void dag_task(const DagArgs& g, int t, int* s_n, double* s_tile) {
+  const int nbt = g.nbt, bp = g.bp;
+  const long bb = (long)bp * bp;
+  const int FS = dag_fstride(nbt);
+  const int i = min(t / g.tfull, g.n - 1);
+  const int e = t - i * g.tfull;
+  const int code = g.tab[(i < g.n - 1) ? e : g.tfull + e];
+  const int kind = code >> 24, r = (code >> 12) & 0xfff, c = code & 0xfff;
+  const long T64 = (long)DH_TILE * bp;  // one tile row
+  double* D = g.diag + i * bb;
+  double* Lo = g.lower + i * bb;         // i < n-1
+  double* Ar = g.arrow + (long)i * T64;
+  const double* Lp = g.lower + (i - 1) * bb;       // i > 0
+  const double* Ap = g.arrow + (long)(i - 1) * T64;
+  double* dinv = g.dinv + (long)i * bp;
+  double* pinv = g.pinv + (long)(i & 1) * nbt * DH_TILE * DH_TILE;  // this block's dense tile inverses
+  int* fD = g.flags + (long)i * FS;
+  int* fL = fD + nbt * nbt;
+  int* fA = fL + nbt * nbt;
+  int* fT = fA + nbt;
+  int* fPD = fT + 1;
+  int* fPL = fPD + nbt;
+  int* fOD = fPL + nbt;
+  int* fOL = fOD + nbt;
+  const int kold = max(0, r - g.look);  // helpers of tile row r: own terms below kold are OLD's
+  const int* fLp = fL - FS;  // block i-1
+  const int* fAp = fA - FS;
+  const int* fTp = fT - FS;
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  int* fdone;
+  if (kind == DK_TIP) {
+    dag_accumulate(acc, Ar, 0, Ar, 0, bp, fA, nullptr, nbt, g.err, s_n);
+    if (i > 0) dag_ready(fTp, nullptr, 0, 1, g.err, s_n);
+    dh_store(g.tip, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
+    fdone = fT;
+  } else if (kind == DK_OLD_D) {  // tile (r, r): previous block's terms and own terms k < kold
+    if (i > 0)
+      dag_accumulate(acc, Lp, r * DH_TILE, Lp, r * DH_TILE, bp, fLp + r * nbt, nullptr, r == 0 ? nbt - 1 : nbt,
+                     g.err, s_n);
+    dag_accumulate(acc, D, r * DH_TILE, D, r * DH_TILE, bp, fD + r * nbt, nullptr, kold, g.err, s_n);
+    dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    fdone = fOD + r;
+  } else if (kind == DK_OLD_L) {  // tile (r, r-1): previous block's terms and own terms k < kold
+    if (i > 0)
+      dag_accumulate(acc, Lp, r * DH_TILE, Lp, (r - 1) * DH_TILE, bp, fLp + r * nbt, fLp + (r - 1) * nbt, nbt,
+                     g.err, s_n);
+    dag_accumulate(acc, D, r * DH_TILE, D, (r - 1) * DH_TILE, bp, fD + r * nbt, fD + (r - 1) * nbt, kold, g.err,
+                   s_n);
+    dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    fdone = fOL + r;
+  } else if (kind == DK_PRE_D) {  // tile (r, r), r >= 1: own terms kold <= k < r-1
+    dag_accumulate_from(acc, D, r * DH_TILE, D, r * DH_TILE, bp, fD + r * nbt, nullptr, kold, r - 1, g.err, s_n);
+    dag_ready(fOD + r, nullptr, 0, 1, g.err, s_n);
+    dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    fdone = fPD + r;
+  } else if (kind == DK_PRE_L) {  // tile (r, r-1), r >= 1: own terms kold <= k < r-1
+    dag_accumulate_from(acc, D, r * DH_TILE, D, (r - 1) * DH_TILE, bp, fD + r * nbt, fD + (r - 1) * nbt, kold,
+                        r - 1, g.err, s_n);
+    dag_ready(fOL + r, nullptr, 0, 1, g.err, s_n);
+    dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    fdone = fPL + r;
+  } else if (kind == DK_CHAIN) {  // tile (c, c)
+    double* C = D + c * T64 + c * DH_TILE;
+    double* pv = pinv + c * DH_TILE * DH_TILE;
+    // the inverse slot was last read by block i-2's column c, whose arrow tile is the only reader
+    // that is not already a dependency of this task
+    if (i >= 2 && g.arrow_on) dag_ready(fA - 2 * FS + c, nullptr, 0, 1, g.err, s_n);
+    dag_ready(c == 0 ? fOD : fPD + c, nullptr, 0, 1, g.err, s_n);
+    dh_d4 part[2][2];
+    dh_load_acc(C, bp, 0, 0, part);  // partial sum of tile (c, c) without its last term
+    if (c >= 1) {
+      double* X = D + c * T64 + (c - 1) * DH_TILE;
+      dag_ready(fPL + c, nullptr, 0, 1, g.err, s_n);
+      double xa[2][2][2][4];
+      dh_load_a64<M_ROW>(Opnd{X, bp, M_ROW, nullptr}, 0, xa);
+      dag_ready(fD + (c - 1) * nbt + (c - 1), nullptr, 0, 1, g.err, s_n);  // the chain dependency
+      dh_d4 x[2][2];
+      dh_zero(x);
+      dh_mma_a64<M_ROW>(x, xa, Opnd{pv - DH_TILE * DH_TILE, DH_TILE, M_ROW, nullptr}, 0);
+      dh_store(X, bp, 0, 0, x, 1.0, 0.0, false);  // final L(c, c-1)
+      dh_store(s_tile, DH_LDT, 0, 0, x, 1.0, 0.0, false);
+      dag_signal(fD + c * nbt + (c - 1));
+      const Opnd xs{s_tile, DH_LDT, M_ROW, nullptr};
+      dh_mma<M_ROW, M_ROW>(acc, xs, 0, xs, 0, 0, DH_TILE);
+    } else if (i > 0) {
+      dag_ready(fLp + (nbt - 1), nullptr, 0, 1, g.err, s_n);
+      const Opnd xl{Lp + (nbt - 1) * DH_TILE, bp, M_ROW, nullptr};
+      dh_mma<M_ROW, M_ROW>(acc, xl, 0, xl, 0, 0, DH_TILE);
+    }
+#pragma unroll
+    for (int x0 = 0; x0 < 2; ++x0)
+#pragma unroll
+      for (int x1 = 0; x1 < 2; ++x1) acc[x0][x1] = part[x0][x1] - acc[x0][x1];
+    __syncthreads();  // every wave has read the staged L(c, c-1)
+    dh_store(s_tile, DH_LDT, 0, 0, acc, 1.0, 0.0, false);
+    __syncthreads();
+    dh_leaf_potrf2(s_tile, DH_LDT, C, bp, dinv + c * DH_TILE, pv, g.info, i * g.b + c * DH_TILE);
+    fdone = fD + c * nbt + c;
+  } else {
+    double* C;
+    if (kind == DK_DIAG) {  // tile (r, c), r >= c + 2
+      if (i > 0)
+        dag_accumulate(acc, Lp, r * DH_TILE, Lp, c * DH_TILE, bp, fLp + r * nbt, fLp + c * nbt, nbt, g.err, s_n);
+      dag_accumulate(acc, D, r * DH_TILE, D, c * DH_TILE, bp, fD + r * nbt, fD + c * nbt, c, g.err, s_n);
+      C = D + r * T64 + c * DH_TILE;
+      fdone = fD + r * nbt + c;
+    } else if (kind == DK_LOWER) {  // sub-diagonal block tile (r, c)
+      dag_accumulate(acc, Lo, r * DH_TILE, D, c * DH_TILE, bp, fL + r * nbt, fD + c * nbt, c, g.err, s_n);
+      C = Lo + r * T64 + c * DH_TILE;
+      fdone = fL + r * nbt + c;
+    } else {  // arrow tile c
+      if (i > 0) dag_accumulate(acc, Ap, 0, Lp, c * DH_TILE, bp, fAp, fLp + c * nbt, nbt, g.err, s_n);
+      dag_accumulate(acc, Ar, 0, D, c * DH_TILE, bp, fA, fD + c * nbt, c, g.err, s_n);
+      C = Ar + c * DH_TILE;
+      fdone = fA + c;
+    }
+    dh_store(C, bp, 0, 0, acc, -1.0, 1.0, false);
+    dag_ready(fD + c * nbt + c, nullptr, 0, 1, g.err, s_n);  // factor of tile (c, c); orders the store above
+    dag_trsm(C, pinv + c * DH_TILE * DH_TILE, bp);
+  }
+  dag_signal(fdone);
+}
+
+// Tasks are claimed by wave 0 under a wave-uniform branch: lane 0 adds 1 and
+// the other lanes add 0, and every lane writes lane 0's ticket, so the loop
+// contains no lane-divergent branch around its barriers.
+__global__ __launch_bounds__(256) void k_pobtaf_dag(DagArgs g) {
+  __shared__ int s_task, s_n;
+  __shared__ __align__(16) double s_tile[DH_TILE * DH_LDT];
+  for (;;) {
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x) < 64) {
+      const int old = atomicAdd(g.counter, threadIdx.x == 0 ? 1 : 0);
+      s_task = __builtin_amdgcn_readfirstlane(old);
+    }
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_task);
+    __syncthreads();
+    if (t >= g.ntasks) break;
+    dag_task(g, t, &s_n, s_tile);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1014,6 +1303,13 @@ struct dh_handle {
   long long bytes;
   int syrk_variant;              // 0: 64-tiles, 1: 128-tiles with split k (needs nbt even)
   int gemm_variant;              // pobtasi dense products: 0: 64-tiles, 1: 128-tiles (nbt even)
+  int pobtaf_alg;                // 0: one launch per panel, 1: persistent tile-task launch
+  int* dagflags;                 // n * (2 nbt^2 + nbt + 1) tile readiness flags
+  int* dagcounter;               // task counter of the persistent launch
+  int* dagtab;                   // decode tables of the persistent launch (bta and bt schedules)
+  double* dagpinv;               // dense inverses of the diagonal tiles, [2][nbt][64][64]
+  int dagt[2][2];                // [arrow][full block, last block] task counts
+  int dag_wgs;                   // persistent workgroups
   int tkind;                     // timed kernel kind, -1 none
   std::vector<hipEvent_t> ev;    // event pairs
   long long nrec;
@@ -1048,6 +1344,35 @@ static void build_nodes(int o, int s, int d, std::vector<NodeList>& lv) {
   nl.cnt++;
   build_nodes(o, s1, d + 1, lv);
   build_nodes(o + s1, s - s1, d + 1, lv);
+}
+
+// Task order of one block of the persistent pobtaf (nl sub-diagonal tile
+// rows: nbt, or 0 for the last block): OLD_D(0), OLD_D/OLD_L(1..look), then
+// per tile column c: CHAIN(c), OLD_D/OLD_L(c+look), PRE_D/PRE_L(c+1), the
+// diagonal-block tiles below (c+1, c), the sub-diagonal tiles and the arrow
+// tile of column c; then the tip partial.
+static void dag_table(int nbt, int nl, int arrow, int look, std::vector<int>& tab) {
+  auto enc = [](int kind, int r, int c) { return (kind << 24) | (r << 12) | c; };
+  tab.push_back(enc(DK_OLD_D, 0, 0));
+  for (int r = 1; r <= look && r < nbt; ++r) {
+    tab.push_back(enc(DK_OLD_D, r, r));
+    tab.push_back(enc(DK_OLD_L, r, r - 1));
+  }
+  for (int c = 0; c < nbt; ++c) {
+    tab.push_back(enc(DK_CHAIN, c, c));
+    if (c >= 1 && c + look < nbt) {
+      tab.push_back(enc(DK_OLD_D, c + look, c + look));
+      tab.push_back(enc(DK_OLD_L, c + look, c + look - 1));
+    }
+    if (c + 1 < nbt) {
+      tab.push_back(enc(DK_PRE_D, c + 1, c + 1));
+      tab.push_back(enc(DK_PRE_L, c + 1, c));
+    }
+    for (int r = c + 2; r < nbt; ++r) tab.push_back(enc(DK_DIAG, r, c));
+    for (int r = 0; r < nl; ++r) tab.push_back(enc(DK_LOWER, r, c));
+    if (arrow) tab.push_back(enc(DK_ARROW, 0, c));
+  }
+  if (arrow) tab.push_back(enc(DK_TIP, 0, 0));
 }
 
 static int dalloc(dh_handle* h, double** p, long long count) {
@@ -1085,6 +1410,17 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   h->nrec = 0;
   h->syrk_variant = 0;
   h->gemm_variant = 1;
+  h->pobtaf_alg = 1;
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pobtaf_dag, 256, 0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    h->dag_wgs = per_cu * cus;
+  }
   const long long bb = (long long)h->bp * h->bp;
   h->G = n < 8 ? n : 8;
   int rc = DH_OK;
@@ -1106,9 +1442,25 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   rc |= dalloc(h, &h->part, h->part_len);
   rc |= dalloc(h, &h->arrpart, (long long)n * 64);
   rc |= dalloc(h, &h->eigbuf, 2LL * n * h->bp);
+  rc |= dalloc(h, &h->dagpinv, 2LL * h->nbt * 64 * 64);
   if (hipMalloc((void**)&h->info, sizeof(int)) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->flags, sizeof(int) * ((size_t)n * h->nbt + 1)) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->err, sizeof(int)) != hipSuccess) rc = DH_EHIP;
+  const size_t nflags = (size_t)n * dag_fstride(h->nbt);
+  if (hipMalloc((void**)&h->dagflags, sizeof(int) * nflags) != hipSuccess) rc = DH_EHIP;
+  if (hipMalloc((void**)&h->dagcounter, sizeof(int)) != hipSuccess) rc = DH_EHIP;
+  if (rc == DH_OK) {
+    std::vector<int> tab;
+    for (int ar = 0; ar < 2; ++ar)
+      for (int last = 0; last < 2; ++last) {
+        const size_t before = tab.size();
+        dag_table(h->nbt, last ? 0 : h->nbt, ar, DH_DAG_LOOK, tab);
+        h->dagt[ar][last] = (int)(tab.size() - before);
+      }
+    if (hipMalloc((void**)&h->dagtab, sizeof(int) * tab.size()) != hipSuccess ||
+        hipMemcpy(h->dagtab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+      rc = DH_EHIP;
+  }
   if (rc != DH_OK) {
     dh_destroy(h);
     return DH_EHIP;
@@ -1121,12 +1473,15 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
 int dh_destroy(dh_handle* h) {
   if (!h) return DH_EARG;
   double* ps[] = {h->diag, h->lower, h->arrow, h->tip, h->dinv, h->pinv, h->tipinv, h->tipdinv,
-                  h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->eigbuf};
+                  h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->eigbuf, h->dagpinv};
   for (double* p : ps)
     if (p) (void)hipFree(p);
   if (h->info) (void)hipFree(h->info);
   if (h->flags) (void)hipFree(h->flags);
   if (h->err) (void)hipFree(h->err);
+  if (h->dagflags) (void)hipFree(h->dagflags);
+  if (h->dagcounter) (void)hipFree(h->dagcounter);
+  if (h->dagtab) (void)hipFree(h->dagtab);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   delete h;
   return DH_OK;
@@ -1203,6 +1558,40 @@ int dh_cholesky(dh_handle* h, int sparsity) {
   const long bb = (long)bp * bp;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->info, INT_MAX);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->err, 0);
+  if (h->pobtaf_alg == 1) {
+    hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags, 0, n * dag_fstride(nbt));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
+    const int ar = arrow ? 1 : 0;
+    DagArgs g;
+    g.diag = h->diag;
+    g.lower = h->lower;
+    g.arrow = h->arrow;
+    g.tip = h->tip;
+    g.dinv = h->dinv;
+    g.pinv = h->dagpinv;
+    g.flags = h->dagflags;
+    g.counter = h->dagcounter;
+    g.tab = h->dagtab + (ar ? h->dagt[0][0] + h->dagt[0][1] : 0);
+    g.info = h->info;
+    g.err = h->err;
+    g.n = n;
+    g.b = h->b;
+    g.bp = bp;
+    g.nbt = nbt;
+    g.arrow_on = ar;
+    g.tfull = h->dagt[ar][0];
+    g.ntasks = (n - 1) * h->dagt[ar][0] + h->dagt[ar][1];
+    g.look = DH_DAG_LOOK;
+    const unsigned wgs = (unsigned)std::min<int>(g.ntasks, h->dag_wgs);
+    DH_LAUNCH(h, 7, k_pobtaf_dag, dim3(wgs), dim3(256), 0, g);
+    if (arrow)
+      hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, h->stream, h->tip, (long)h->ap, h->tipdinv, h->tipinv,
+                         h->info, n * h->b);
+    HIPCHK(hipGetLastError());
+    h->factored = sparsity;
+    h->inverted = 0;
+    return DH_OK;
+  }
   hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, h->stream, h->flags, 0, n * nbt);
   DH_LAUNCH(h, 4, k_leaf, dim3(1), dim3(256), 0, h->diag, (long)bp, h->dinv, h->pinv, h->info, 0);
   bool first_done = false;  // panel 0 of block i already solved by the previous launch
@@ -1568,6 +1957,7 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
   if (!h) return DH_EARG;
   if (key == 0 && (value == 0 || value == 1)) h->syrk_variant = value;
   else if (key == 1 && (value == 0 || value == 1)) h->gemm_variant = value;
+  else if (key == 2 && (value == 0 || value == 1)) h->pobtaf_alg = value;
   else return DH_EARG;
   return DH_OK;
 }

@@ -153,7 +153,10 @@ int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const doubl
 /* Tuning knobs (chosen per shape by the autotuner in dalia_amd/autotune.py):
  * key 0: Schur-update kernel of pobtaf, 0 = 64x64 tiles (one launch),
  *        1 = 128x128 tiles with the k range split in two (+ a tail launch);
- * key 1: dense products of pobtasi, 0 = 64x64 tiles, 1 = 128x128 tiles.
+ * key 1: dense products of pobtasi, 0 = 64x64 tiles, 1 = 128x128 tiles;
+ * key 2: pobtaf/pobtf schedule, 0 = one launch per 64-column panel (key 0
+ *        applies), 1 = one persistent launch of left-looking tile tasks
+ *        synchronised by per-tile readiness flags (default).
  * The 128 variants need an even number of 64-tiles per block; otherwise the
  * 64 variants run.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
@@ -162,7 +165,8 @@ int dh_set_tuning(dh_handle* h, int key, int value);
  * of one kernel.  kind: 0 k_syrk (Schur update of the next diagonal block in
  * pobtaf; k_syrk128 for tuning value 1), 1 k_update (panel trailing update),
  * 2 k_gemm64/k_gemm128 (pobtasi tiles), 3 k_panel, 4 k_leaf,
- * 5 k_assemble_rows, 6 k_syrk_tail; kind < 0 disables.  While enabled,
+ * 5 k_assemble_rows, 6 k_syrk_tail, 7 k_pobtaf_dag (the persistent pobtaf
+ * launch); kind < 0 disables.  While enabled,
  * each launch of that kind is bracketed by an event pair (at most 16384
  * launches are recorded).  dh_timing_read synchronises the stream and returns
  * the summed kernel time and the number of recorded launches.  No
