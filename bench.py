@@ -186,7 +186,7 @@ def main():
     eng._assemble(theta, "bta", True)
     _lib.check(lib.dh_timing_enable(eng.h, kind), "dh_timing_enable")
     _lib.check(lib.dh_cholesky(eng.h, 1), "dh_cholesky")
-    if kname == "k_gemm":
+    if kname == "k_sel":
         _lib.check(lib.dh_selected_inversion(eng.h, 1), "dh_selected_inversion")
     tot = ctypes.c_double()
     cnt = ctypes.c_longlong()
@@ -232,7 +232,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "ns": ns, "nt": nt, "nb": nb, "seed": seed, "theta_dim": 4,
                    "n_obs": data.n_obs, "kernel_variants": {"pobtaf_schedule": "tile-task" if choice["alg"] == 1 else "per-panel",
-                                                        "syrk": choice["syrk"], "gemm": choice["gemm"]}},
+                                                        "syrk": choice["syrk"]}},
         "pobtaf_ms": t_pobtaf * 1e3,
         "pobtaf_tflops": pobtaf_tflops,
         "pobtaf_frac_of_peak": pobtaf_tflops / FP64_PEAK_TFLOPS,

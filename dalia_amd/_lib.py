@@ -50,6 +50,8 @@ SIGNATURES = {
     "dh_version": (_I, []),
     "dh_create": (_I, [_I, _I, _I, _P, ctypes.POINTER(_P)]),
     "dh_destroy": (_I, [_P]),
+    "dh_marginal_variances": (_I, [_P, _P]),
+    "dh_obs_variances": (_I, [_P, _LL, _P, _P, _P, _P, _P]),
     "dh_get_layout": (_I, [_P, ctypes.POINTER(DhLayout)]),
     "dh_set_stream": (_I, [_P, _P]),
     "dh_memory_bytes": (_LL, [_P]),
@@ -74,7 +76,7 @@ SIGNATURES = {
 }
 
 #: kernel kinds of dh_timing_enable
-TIMED_KERNELS = {"k_syrk": 0, "k_update": 1, "k_gemm": 2, "k_panel": 3, "k_leaf": 4, "k_assemble_rows": 5,
+TIMED_KERNELS = {"k_syrk": 0, "k_update": 1, "k_sel": 2, "k_panel": 3, "k_leaf": 4, "k_assemble_rows": 5,
                  "k_syrk_tail": 6, "k_pobtaf_dag": 7}
 
 _lib = None

@@ -3,8 +3,7 @@
 Knobs (`dh_set_tuning`): key 2 = the pobtaf schedule (one persistent launch
 of left-looking tile tasks, or one launch per 64-column panel), key 0 = the
 Schur-update kernel of the per-panel schedule (64x64 tiles, or 128x128 tiles
-with the k range split in two), key 1 = the dense products of pobtasi (64x64
-or 128x128 tiles).  `tune(engine)` times every
+with the k range split in two).  `tune(engine)` times every
 value of every knob on the engine's own matrix with HIP events, applies the
 fastest and records it in a JSON file keyed by (padded block size, number of
 blocks, arrowhead size, device name); later calls read the file.
@@ -26,7 +25,7 @@ DEFAULT_PATH = os.environ.get("DALIA_AMD_TUNING",
 
 def _key(engine) -> str:
     lay = engine.solver.layout
-    return f"v2_bp{lay.bp}_n{lay.n}_a{lay.a}_{torch.cuda.get_device_name(0)}"
+    return f"v3_bp{lay.bp}_n{lay.n}_a{lay.a}_{torch.cuda.get_device_name(0)}"
 
 
 def _load(path):
@@ -41,7 +40,6 @@ def apply(engine, choice: dict) -> None:
     lib, h = engine.lib, engine.h
     _lib.check(lib.dh_set_tuning(h, 2, int(choice["alg"])), "dh_set_tuning")
     _lib.check(lib.dh_set_tuning(h, 0, int(choice["syrk"])), "dh_set_tuning")
-    _lib.check(lib.dh_set_tuning(h, 1, int(choice["gemm"])), "dh_set_tuning")
 
 
 def _time(fn, reps: int) -> float:
@@ -82,18 +80,7 @@ def tune(engine, path: str = DEFAULT_PATH, reps: int = 2, force: bool = False) -
     _lib.check(lib.dh_set_tuning(h, 2, best_alg), "dh_set_tuning")
     _lib.check(lib.dh_set_tuning(h, 0, best_syrk), "dh_set_tuning")
 
-    def selinv():
-        _lib.check(lib.dh_selected_inversion(h, 1), "dh_selected_inversion")
-
-    for v in (0, 1):
-        _lib.check(lib.dh_set_tuning(h, 1, v), "dh_set_tuning")
-        tt = []
-        for _ in range(reps):
-            factor()
-            tt.append(_time(selinv, 1))
-        times[f"gemm{v}"] = float(np.median(tt))
-    best_gemm = int(times["gemm1"] < times["gemm0"])
-    choice = {"alg": best_alg, "syrk": best_syrk, "gemm": best_gemm, "ms": times}
+    choice = {"alg": best_alg, "syrk": best_syrk, "ms": times}
     apply(engine, choice)
     cache[key] = choice
     try:

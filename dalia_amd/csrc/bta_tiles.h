@@ -3,14 +3,18 @@
 // A workgroup is 256 threads = 4 wave64s and owns one 64x64 output tile;
 // wave w computes the 32x32 quadrant (w>>1, w&1) as 2x2 blocks of the
 // v_mfma_f64_16x16x4 instruction.  Operands are streamed straight from
-// global memory (L2) in chunks of 16 along k: every lane fetches 4
-// consecutive k of one row, and the 4 MFMAs of a chunk use the k-permuted
-// order {4g+m}, which leaves the contraction unchanged because A and B use
-// the same permutation.
+// global memory (L2) in chunks of 16 along k: lane group g = lane/16 holds
+// the k offsets {2g, 2g+1, 2g+8, 2g+9} of its row, so that each load
+// instruction of a row-major operand reads 64 contiguous bytes per row, and
+// MFMA m of a chunk contracts over offset m of every group.  A and B use
+// the same permutation of k, which leaves the contraction unchanged.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #define DH_TILE 64
+#ifndef DH_LEAF_TS
+#define DH_LEAF_TS(k)  // phase hook of the leaf steps (timing probes define it)
+#endif
 #define DH_NT 256
 
 typedef double dh_d4 __attribute__((ext_vector_type(4)));
@@ -31,26 +35,31 @@ struct Opnd {
   const double* dinv;
 };
 
+// k offset of item m relative to the group base (chunk start + 2g).
+__device__ __forceinline__ constexpr int dh_koff(int m) { return (m & 1) + 8 * (m >> 1); }
+
+// v[m] = opM[row][k + dh_koff(m)], k = chunk start + 2g.
 template <int MODE>
 __device__ __forceinline__ void dh_load4(const Opnd& o, int row, int k, double v[4]) {
   if (MODE == M_ROW || MODE == M_UPK_ROW) {
-    const double2* q = reinterpret_cast<const double2*>(o.p + (long)row * o.ld + k);
-    double2 x0 = q[0], x1 = q[1];
+    const double* q = o.p + (long)row * o.ld + k;
+    const double2 x0 = *reinterpret_cast<const double2*>(q);
+    const double2 x1 = *reinterpret_cast<const double2*>(q + 8);
     v[0] = x0.x; v[1] = x0.y; v[2] = x1.x; v[3] = x1.y;
     if (MODE == M_UPK_ROW) {
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        int kk = k + m;
+        const int kk = k + dh_koff(m);
         v[m] = kk > row ? v[m] : (kk == row ? o.dinv[row] : 0.0);
       }
     }
   } else {
     const double* q = o.p + (long)k * o.ld + row;
-    v[0] = q[0]; v[1] = q[o.ld]; v[2] = q[2 * o.ld]; v[3] = q[3 * o.ld];
+    v[0] = q[0]; v[1] = q[o.ld]; v[2] = q[8 * o.ld]; v[3] = q[9 * o.ld];
     if (MODE == M_UPK_COL) {
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        int kk = k + m;
+        const int kk = k + dh_koff(m);
         v[m] = kk < row ? v[m] : (kk == row ? o.dinv[row] : 0.0);
       }
     }
@@ -91,7 +100,7 @@ __device__ __forceinline__ void dh_mma(dh_d4 acc[2][2], const Opnd& A, int i0, c
   const int w = threadIdx.x >> 6;
   const int ra = i0 + (w >> 1) * 32 + (lane & 15);
   const int rb = j0 + (w & 1) * 32 + (lane & 15);
-  const int kq = (lane >> 4) * 4;
+  const int kq = (lane >> 4) * 2;  // group base offset within a 16-chunk
   double a[2][2][4], b[2][2][4];
   dh_load_chunk<MA, MB>(A, B, ra, rb, klo + kq, a, b);
   for (int k = klo; k < khi; k += 32) {
@@ -131,7 +140,7 @@ __device__ __forceinline__ void dh_mma128(dh_d4 acc[4][4], const Opnd& A, int i0
   const int w = threadIdx.x >> 6;
   const int ra = i0 + (w >> 1) * 64 + (lane & 15);
   const int rb = j0 + (w & 1) * 64 + (lane & 15);
-  const int kq = (lane >> 4) * 4;
+  const int kq = (lane >> 4) * 2;  // group base offset within a 16-chunk
   double a[4][4], b[4][4];
 #pragma unroll
   for (int x = 0; x < 4; ++x) {
@@ -166,6 +175,7 @@ __device__ __forceinline__ void dh_mma128(dh_d4 acc[4][4], const Opnd& A, int i0
   }
 }
 
+// C = alpha*acc + beta*C on a 128x128 tile.
 __device__ __forceinline__ void dh_store128(double* C, long ldc, int i0, int j0, const dh_d4 acc[4][4],
                                             double alpha, double beta) {
   const int lane = threadIdx.x & 63;
@@ -192,7 +202,7 @@ __device__ __forceinline__ void dh_load_a64(const Opnd& A, int i0, double a[2][2
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int ra = i0 + (w >> 1) * 32 + (lane & 15);
-  const int kq = (lane >> 4) * 4;
+  const int kq = (lane >> 4) * 2;  // group base offset within a 16-chunk
 #pragma unroll
   for (int kc = 0; kc < 2; ++kc)
 #pragma unroll
@@ -208,7 +218,7 @@ __device__ __forceinline__ void dh_mma_a64(dh_d4 acc[2][2], const double a[2][2]
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int rb = j0 + (w & 1) * 32 + (lane & 15);
-  const int kq = (lane >> 4) * 4;
+  const int kq = (lane >> 4) * 2;  // group base offset within a 16-chunk
   double b[2][2][2][4];
 #pragma unroll
   for (int kc = 0; kc < 2; ++kc)
@@ -275,60 +285,67 @@ __device__ __forceinline__ void dh_store(double* C, long ldc, int i0, int j0, co
 // there too.  A pivot that is not > 0 records 1 + row_base + k in *info
 // (atomic min) and lets NaN propagate.
 //
-// Blocked right-looking elimination, 4 columns per barrier.  Thread
+// Blocked right-looking elimination, 4 columns per step.  Thread
 // (ty, tx) = (tid/16, tid%16) keeps the entries (ty+16u, tx+16v) in
 // registers.  Step c (columns K0 = 4c .. K0+3) publishes those 4 columns of
 // the trailing matrix S and rows K0..K0+3 of the working inverse W (L W = I
-// is solved alongside) through LDS; after one barrier every thread factors
-// the 4x4 pivot block P = L_B L_B^T and inverts it (V_B = inv(L_B))
-// redundantly, then
-//   L[i][K0+b] = sum_m S[i][K0+m] V_B[b][m]            (rows i >= K0+4)
-//   V[K0+a][j] = sum_m V_B[a][m] W[K0+m][j]            (columns j < K0)
-//   S[i][j]   -= sum_b L[i][K0+b] L[j][K0+b]            (i, j >= K0+4)
-//   W[i][j]   -= sum_b L[i][K0+b] V[K0+b][j]            (i >= K0+4, j < K0+4)
+// is solved alongside) through LDS; after a barrier every thread factors the
+// 4x4 pivot block P = L_B L_B^T and inverts it (V_B = inv(L_B)) redundantly,
+// and thread (j, b) = (tid/4, tid%4) computes one entry of
+//   G[j][b] = L[j][K0+b] = sum_m S[j][K0+m] V_B[b][m]     (j >= K0+4)
+//           = V[K0+b][j] = sum_m V_B[b][m] W[K0+m][j]     (j < K0; V_B for j in the block)
+// into LDS; after a second barrier every thread applies
+//   S[i][j] -= sum_b G[i][b] G[j][b]  (i, j >= K0+4),  W[i][j] -= sum_b G[i][b] G[j][b]  (j < K0+4)
+// to its own entries of the rows below the block, and the owners of rows
+// K0..K0+3 replace them by the final inverse rows V[K0+a][j] = G[j][a].
 // Entries above the diagonal carry unused values.  No global store is issued
 // inside the loop (a barrier waits for outstanding stores): L goes to LDS,
 // inv(L) stays in registers, and both are written once at the end.
-struct LeafShm {
-  double colb[2][4][64];  // columns K0..K0+3 of S (rows >= K0)
-  double rowb[2][4][64];  // rows K0..K0+3 of W (columns < K0)
-  double Ls[64][65];
+template <int NB>  // tile of 16*NB rows
+struct __align__(16) LeafShmT {
+  double colb[4][16 * NB];  // columns K0..K0+3 of S (rows >= K0)
+  double rowb[4][16 * NB];  // rows K0..K0+3 of W (columns < K0)
+  double G[16 * NB][4];     // column factors of the step
+  double Ls[16 * NB][16 * NB + 1];
 };
+typedef LeafShmT<4> LeafShm;
 
 __device__ __forceinline__ double dh_rsq_nr(double p) {
-  double r = __builtin_amdgcn_rsq(p);
-  r = r * (1.5 - 0.5 * p * r * r);
-  r = r * (1.5 - 0.5 * p * r * r);
-  return r;
+  // v_rsq_f64 is accurate to about 2^-24 (relative); one third-order
+  // correction r (1 + e/2 + 3e^2/8), e = 1 - p r^2, brings it to double precision.
+  const double r = __builtin_amdgcn_rsq(p);
+  const double e = fma(-p * r, r, 1.0);
+  return fma(r * e, fma(e, 0.375, 0.5), r);
 }
 
-template <int KV>
-__device__ __forceinline__ void dh_leaf_block(double R[4][4], LeafShm& sh, int ty, int tx, int tid, int* info,
-                                              int row_base, int c4) {
+template <int NB, int KV>
+__device__ __forceinline__ void dh_leaf_block(double R[NB][NB], LeafShmT<NB>& sh, int ty, int tx, int tid,
+                                              int* info, int row_base, int c4) {
   const int kq = 4 * c4;
   const int K0 = 16 * KV + kq;
-  const int bsel = c4 & 1;
   const bool colown = tx >= kq && tx < kq + 4;
   const bool rowown = ty >= kq && ty < kq + 4;
+  DH_LEAF_TS(0);
   if (colown) {
 #pragma unroll
-    for (int u = KV; u < 4; ++u) {
-      sh.colb[bsel][tx - kq][ty + 16 * u] = R[u][KV];
+    for (int u = KV; u < NB; ++u) {
+      sh.colb[tx - kq][ty + 16 * u] = R[u][KV];
       if (u > KV || ty >= kq + 4) R[u][KV] = 0.0;  // becomes W[i][K0+jj] below the block
     }
   }
   if (rowown) {
 #pragma unroll
     for (int v = 0; v <= KV; ++v)
-      if (v < KV || tx < kq) sh.rowb[bsel][ty - kq][tx + 16 * v] = R[KV][v];
+      if (v < KV || tx < kq) sh.rowb[ty - kq][tx + 16 * v] = R[KV][v];
   }
   __syncthreads();
+  DH_LEAF_TS(1);
   // 4x4 pivot block: Cholesky and inverse (every thread)
   double P[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b <= a; ++b) P[a][b] = sh.colb[bsel][b][K0 + a];
+    for (int b = 0; b <= a; ++b) P[a][b] = sh.colb[b][K0 + a];
   double Lb[4][4], Vb[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -360,90 +377,62 @@ __device__ __forceinline__ void dh_leaf_block(double R[4][4], LeafShm& sh, int t
       for (int m = b; m < a; ++m) x += Lb[a][m] * Vb[m][b];
       Vb[a][b] = -Vb[a][a] * x;
     }
-  // L columns K0..K0+3 into LDS (thread tid < 64 owns row tid)
-  if (tid < 64) {
-    const int i = tid;
-    if (i >= K0 + 4) {
-      double c[4];
+  DH_LEAF_TS(2);
+  if (tid < 64 * NB) {  // one column factor per thread
+    const int j = tid >> 2, b = tid & 3;
+    double vb[4];  // row b of V_B
 #pragma unroll
-      for (int m = 0; m < 4; ++m) c[m] = sh.colb[bsel][m][i];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        double x = 0.0;
-#pragma unroll
-        for (int m = 0; m <= b; ++m) x += c[m] * Vb[b][m];
-        sh.Ls[i][K0 + b] = x;
-      }
-    } else if (i >= K0) {
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        double x = 0.0;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) x = (a == i - K0) ? Lb[a][b] : x;
-        sh.Ls[i][K0 + b] = x;
-      }
-    } else {
-#pragma unroll
-      for (int b = 0; b < 4; ++b) sh.Ls[i][K0 + b] = 0.0;
-    }
-  }
-  // per column: S-part factors (L[j][K0+b]) or W-part factors (V[K0+b][j])
-  double gj[4][4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int j = tx + 16 * v;
-    if (v > KV || (v == KV && tx >= kq + 4)) {  // S part: j >= K0+4
-      double c[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) c[m] = sh.colb[bsel][m][j];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        double x = 0.0;
-#pragma unroll
-        for (int m = 0; m <= b; ++m) x += c[m] * Vb[b][m];
-        gj[v][b] = x;
-      }
-    } else if (v < KV || tx < kq) {  // W part: j < K0
-      double w[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) w[m] = sh.rowb[bsel][m][j];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        double x = 0.0;
-#pragma unroll
-        for (int m = 0; m <= b; ++m) x += Vb[b][m] * w[m];
-        gj[v][b] = x;
-      }
-    } else {  // in-block column j = K0 + (tx - kq): V[K0+b][j] = Vb[b][tx-kq]
-      const int jj = tx - kq;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        double x = 0.0;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) x = (m == jj && b >= m) ? Vb[b][m] : x;
-        gj[v][b] = x;
-      }
-    }
-  }
-  // rank-4 updates of the rows below the block (lower and diagonal 16x16 sub-blocks only)
-#pragma unroll
-  for (int u = KV; u < 4; ++u) {
-    const int i = ty + 16 * u;
-    const bool below = (u > KV) || (ty >= kq + 4);
-    double c[4], li[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) c[m] = below ? sh.colb[bsel][m][i] : 0.0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int m = 0; m < 4; ++m) {
       double x = 0.0;
 #pragma unroll
-      for (int m = 0; m <= b; ++m) x += c[m] * Vb[b][m];
-      li[b] = x;
+      for (int a = 0; a < 4; ++a) x = (a == b) ? Vb[a][m] : x;
+      vb[m] = x;
     }
+    double g = 0.0;
+    if (j >= K0 + 4) {
 #pragma unroll
-    for (int v = 0; v <= u; ++v)
+      for (int m = 0; m < 4; ++m) g = fma(sh.colb[m][j], vb[m], g);
+      sh.Ls[j][K0 + b] = g;
+    } else if (j >= K0) {
+      const int jj = j - K0;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) R[u][v] = fma(-li[b], gj[v][b], R[u][v]);
+      for (int m = 0; m < 4; ++m) g = (m == jj) ? vb[m] : g;
+      double lv = 0.0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) lv = (a == jj && m == b) ? Lb[a][m] : lv;
+      sh.Ls[j][K0 + b] = lv;
+    } else {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) g = fma(vb[m], sh.rowb[m][j], g);
+    }
+    sh.G[j][b] = g;
+  }
+  __syncthreads();
+  DH_LEAF_TS(3);
+  double gj[NB][4];
+#pragma unroll
+  for (int v = 0; v < NB; ++v) {
+    const double2* q = reinterpret_cast<const double2*>(&sh.G[tx + 16 * v][0]);
+    const double2 g0 = q[0], g1 = q[1];
+    gj[v][0] = g0.x; gj[v][1] = g0.y; gj[v][2] = g1.x; gj[v][3] = g1.y;
+  }
+  DH_LEAF_TS(4);
+  // rank-4 updates of the rows below the block (lower and diagonal 16x16 sub-blocks only)
+#pragma unroll
+  for (int u = KV; u < NB; ++u) {
+    const int i = ty + 16 * u;
+    const bool below = (u > KV) || (ty >= kq + 4);
+    if (below) {
+      const double2* q = reinterpret_cast<const double2*>(&sh.G[i][0]);
+      const double2 g0 = q[0], g1 = q[1];
+      const double li[4] = {g0.x, g0.y, g1.x, g1.y};
+#pragma unroll
+      for (int v = 0; v <= u; ++v)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) R[u][v] = fma(-li[b], gj[v][b], R[u][v]);
+    }
   }
   // rows of the block are final: row K0+aa of inv(L)
   if (rowown) {
@@ -458,11 +447,11 @@ __device__ __forceinline__ void dh_leaf_block(double R[4][4], LeafShm& sh, int t
   }
 }
 
-template <int KV>
-__device__ __forceinline__ void dh_leaf_blocks(double R[4][4], LeafShm& sh, int ty, int tx, int tid, int* info,
-                                               int row_base) {
+template <int NB, int KV>
+__device__ __forceinline__ void dh_leaf_blocks(double R[NB][NB], LeafShmT<NB>& sh, int ty, int tx, int tid,
+                                               int* info, int row_base) {
 #pragma unroll 1
-  for (int c4 = 0; c4 < 4; ++c4) dh_leaf_block<KV>(R, sh, ty, tx, tid, info, row_base, c4);
+  for (int c4 = 0; c4 < 4; ++c4) dh_leaf_block<NB, KV>(R, sh, ty, tx, tid, info, row_base, c4);
 }
 
 // Factor the tile read from src (leading dimension lds; may be LDS) into
@@ -480,10 +469,10 @@ __device__ void dh_leaf_potrf2(const double* src, long lds, double* tile, long l
       const int i = ty + 16 * u, j = tx + 16 * v;
       R[u][v] = (j <= i) ? src[(long)i * lds + j] : 0.0;
     }
-  dh_leaf_blocks<0>(R, sh, ty, tx, tid, info, row_base);
-  dh_leaf_blocks<1>(R, sh, ty, tx, tid, info, row_base);
-  dh_leaf_blocks<2>(R, sh, ty, tx, tid, info, row_base);
-  dh_leaf_blocks<3>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_blocks<4, 0>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_blocks<4, 1>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_blocks<4, 2>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_blocks<4, 3>(R, sh, ty, tx, tid, info, row_base);
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < 4; ++u)
@@ -502,4 +491,108 @@ __device__ void dh_leaf_potrf2(const double* src, long lds, double* tile, long l
 __device__ __forceinline__ void dh_leaf_potrf(double* tile, long ld, double* dinv, double* inv_out, int* info,
                                               int row_base) {
   dh_leaf_potrf2(tile, ld, tile, ld, dinv, inv_out, info, row_base);
+}
+
+// ---------------------------------------------------------------------------
+// 64x64 Cholesky + inverse from LDS: blocked 32x32 leaves on the two diagonal
+// halves, the off-diagonal half of L and of inv(L) and the Schur update as
+// 32-deep MFMA products, all in LDS.
+//   L21 = A21 inv(L11)^T,  A22 -= L21 L21^T,  L22 = chol(A22),
+//   inv(L)21 = -inv(L22) (L21 inv(L11)).
+// ---------------------------------------------------------------------------
+#define DH_LDS_LD 66  // leading dimension of the LDS tiles (16-byte aligned rows)
+__shared__ __align__(16) double dh_lS[64 * DH_LDS_LD];  // the tile; L on exit (lower)
+__shared__ __align__(16) double dh_lV[64 * DH_LDS_LD];  // inv(L) on exit (lower); scratch above
+
+// One 16x16 block per wave: acc[r] (lane l) = sum_{k<32} A[i][k] B(k, j) at
+// i = 4r + l/16, j = l%16, with A row-major (ld DH_LDS_LD) and B(k, j) =
+// B[j][k] (TB) or B[k][j].
+template <bool TB>
+__device__ __forceinline__ dh_d4 dh_lds_mm16(const double* A, const double* B) {
+  const int l = threadIdx.x & 63;
+  const int i = l & 15, kk = l >> 4;
+  dh_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int k = 4 * s + kk;
+    const double a = A[i * DH_LDS_LD + k];
+    const double b = TB ? B[i * DH_LDS_LD + k] : B[k * DH_LDS_LD + i];
+    acc = dh_mfma(a, b, acc);
+  }
+  return acc;
+}
+
+// 32x32 blocked leaf on S (ld DH_LDS_LD): L into S's lower triangle, inv(L)
+// into V (zeros above the diagonal).
+__device__ __forceinline__ void dh_leaf32_lds(double* S, double* V, LeafShmT<2>& sh, int* info, int row_base) {
+  const int tid = threadIdx.x;
+  const int ty = tid >> 4, tx = tid & 15;
+  double R[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int i = ty + 16 * u, j = tx + 16 * v;
+      R[u][v] = (j <= i) ? S[i * DH_LDS_LD + j] : 0.0;
+    }
+  dh_leaf_blocks<2, 0>(R, sh, ty, tx, tid, info, row_base);
+  dh_leaf_blocks<2, 1>(R, sh, ty, tx, tid, info, row_base);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int i = ty + 16 * u, j = tx + 16 * v;
+      if (j <= i) S[i * DH_LDS_LD + j] = sh.Ls[i][j];
+      V[i * DH_LDS_LD + j] = (j <= i) ? R[u][v] : 0.0;
+    }
+}
+
+// Factor the 64x64 tile held in dh_lS (lower part; clobbered) into tile (ld):
+// L below and on the diagonal, inv(L)^T above it, dinv = 1/diag(L), and the
+// dense inv(L) (ld 64) into inv_out when given.  Same error convention as
+// dh_leaf_potrf.
+__device__ void dh_leaf64_lds(double* tile, long ld, double* dinv, double* inv_out, int* info, int row_base) {
+  __shared__ LeafShmT<2> sh;
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63;
+  const int bi = w >> 1, bj = w & 1;
+  const int orow = 4 * 0 + (l >> 4), ocol = l & 15;  // output row (plus 4r) and column of a 16x16 block
+  double* S = dh_lS;
+  double* V = dh_lV;
+  dh_leaf32_lds(S, V, sh, info, row_base);
+  __syncthreads();
+  // L21 = A21 inv(L11)^T, block (bi, bj) of 16x16 per wave
+  dh_d4 x = dh_lds_mm16<true>(S + (32 + 16 * bi) * DH_LDS_LD, V + (16 * bj) * DH_LDS_LD);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) S[(32 + 16 * bi + 4 * r + orow) * DH_LDS_LD + 16 * bj + ocol] = x[r];
+  __syncthreads();
+  // A22 -= L21 L21^T
+  x = dh_lds_mm16<true>(S + (32 + 16 * bi) * DH_LDS_LD, S + (32 + 16 * bj) * DH_LDS_LD);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) S[(32 + 16 * bi + 4 * r + orow) * DH_LDS_LD + 32 + 16 * bj + ocol] -= x[r];
+  __syncthreads();
+  dh_leaf32_lds(S + 32 * DH_LDS_LD + 32, V + 32 * DH_LDS_LD + 32, sh, info, row_base + 32);
+  __syncthreads();
+  // T = L21 inv(L11), staged in V's upper-right block
+  x = dh_lds_mm16<false>(S + (32 + 16 * bi) * DH_LDS_LD, V + 16 * bj);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) V[(16 * bi + 4 * r + orow) * DH_LDS_LD + 32 + 16 * bj + ocol] = x[r];
+  __syncthreads();
+  // inv(L)21 = -inv(L22) T
+  x = dh_lds_mm16<false>(V + (32 + 16 * bi) * DH_LDS_LD + 32, V + 32 + 16 * bj);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) V[(32 + 16 * bi + 4 * r + orow) * DH_LDS_LD + 16 * bj + ocol] = -x[r];
+  __syncthreads();
+#pragma unroll 4
+  for (int e0 = 0; e0 < 64 * 64; e0 += 256) {  // uniform trip count (256 threads)
+    const int e = e0 + tid;
+    const int i = e >> 6, j = e & 63;
+    const double vij = (j <= i) ? V[i * DH_LDS_LD + j] : 0.0;
+    if (j <= i) tile[(long)i * ld + j] = S[i * DH_LDS_LD + j];
+    if (j < i) tile[(long)j * ld + i] = vij;
+    if (j == i) dinv[i] = vij;
+    if (inv_out) inv_out[e] = vij;
+  }
 }

@@ -510,7 +510,6 @@ struct DagArgs {
 
 #define DH_DAG_SPIN (1 << 20)
 #define DH_DAG_LOOK 6
-#define DH_LDT 66  // leading dimension of the staged tile in LDS (16-byte aligned rows)
 
 // Flags of one block: final D tiles [nbt][nbt], final L tiles [nbt][nbt],
 // final arrow tiles [nbt], tip partial applied, PRE_D, PRE_L, OLD_D, OLD_L done [nbt each].
@@ -610,7 +609,7 @@ __device__ __forceinline__ void dag_signal(int* flag) {
 }
 
 // One task (t wave-uniform); every path ends at the single signal.
-__device__ void dag_task(const DagArgs& g, int t, int* s_n, double* s_tile) {
+__device__ void dag_task(const DagArgs& g, int t, int* s_n) {
   const int nbt = g.nbt, bp = g.bp;
   const long bb = (long)bp * bp;
   const int FS = dag_fstride(nbt);
@@ -691,9 +690,9 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n, double* s_tile) {
       dh_zero(x);
       dh_mma_a64<M_ROW>(x, xa, Opnd{pv - DH_TILE * DH_TILE, DH_TILE, M_ROW, nullptr}, 0);
       dh_store(X, bp, 0, 0, x, 1.0, 0.0, false);  // final L(c, c-1)
-      dh_store(s_tile, DH_LDT, 0, 0, x, 1.0, 0.0, false);
+      dh_store(dh_lV, DH_LDS_LD, 0, 0, x, 1.0, 0.0, false);
       dag_signal(fD + c * nbt + (c - 1));
-      const Opnd xs{s_tile, DH_LDT, M_ROW, nullptr};
+      const Opnd xs{dh_lV, DH_LDS_LD, M_ROW, nullptr};
       dh_mma<M_ROW, M_ROW>(acc, xs, 0, xs, 0, 0, DH_TILE);
     } else if (i > 0) {
       dag_ready(fLp + (nbt - 1), nullptr, 0, 1, g.err, s_n);
@@ -704,10 +703,9 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n, double* s_tile) {
     for (int x0 = 0; x0 < 2; ++x0)
 #pragma unroll
       for (int x1 = 0; x1 < 2; ++x1) acc[x0][x1] = part[x0][x1] - acc[x0][x1];
-    __syncthreads();  // every wave has read the staged L(c, c-1)
-    dh_store(s_tile, DH_LDT, 0, 0, acc, 1.0, 0.0, false);
+    dh_store(dh_lS, DH_LDS_LD, 0, 0, acc, 1.0, 0.0, false);
     __syncthreads();
-    dh_leaf_potrf2(s_tile, DH_LDT, C, bp, dinv + c * DH_TILE, pv, g.info, i * g.b + c * DH_TILE);
+    dh_leaf64_lds(C, bp, dinv + c * DH_TILE, pv, g.info, i * g.b + c * DH_TILE);
     fdone = fD + c * nbt + c;
   } else {
     double* C;
@@ -739,7 +737,6 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n, double* s_tile) {
 // contains no lane-divergent branch around its barriers.
 __global__ __launch_bounds__(256) void k_pobtaf_dag(DagArgs g) {
   __shared__ int s_task, s_n;
-  __shared__ __align__(16) double s_tile[DH_TILE * DH_LDT];
   for (;;) {
     if (__builtin_amdgcn_readfirstlane(threadIdx.x) < 64) {
       const int old = atomicAdd(g.counter, threadIdx.x == 0 ? 1 : 0);
@@ -749,7 +746,7 @@ __global__ __launch_bounds__(256) void k_pobtaf_dag(DagArgs g) {
     const int t = __builtin_amdgcn_readfirstlane(s_task);
     __syncthreads();
     if (t >= g.ntasks) break;
-    dag_task(g, t, &s_n, s_tile);
+    dag_task(g, t, &s_n);
   }
 }
 
@@ -815,8 +812,7 @@ __global__ __launch_bounds__(256) void k_trtri_inv(double* Dbase, double* dinvba
 
 // ---------------------------------------------------------------------------
 // tile GEMM: C = alpha opA opB + beta C, one term per launch, operand access
-// modes as template parameters; 64x64 tiles (optionally only I >= J) or
-// 128x128 tiles
+// modes as template parameters; 64x64 tiles (optionally only I >= J)
 // ---------------------------------------------------------------------------
 
 enum { KLO_0 = 0, KLO_I = 1, KLO_J = 2, KLO_MAXIJ = 3 };
@@ -865,18 +861,95 @@ __global__ __launch_bounds__(256) void k_gemm64(GArgs g) {
   dh_store(g.C, g.ldc, I * DH_TILE, J * DH_TILE, acc, g.alpha, g.beta, false);
 }
 
-template <int MA, int MB>
-__global__ __launch_bounds__(256) void k_gemm128(GArgs g) {
-  const int I = blockIdx.x / g.ntl, J = blockIdx.x % g.ntl;
-  int klo, khi;
-  dh_krange(g, I, J, 2 * DH_TILE, klo, khi);
-  dh_d4 acc[4][4];
+
+// ---------------------------------------------------------------------------
+// pobtasi: one backward step i in three launches (64x64 tiles)
+// ---------------------------------------------------------------------------
+struct SelArgs {
+  const double* D;     // block i: L_ii below the diagonal, inv(L_ii)^T above it (ld bp)
+  const double* dinv;  // 1 / diag(L_ii)
+  double* Lb;          // L_{i+1,i} on entry, X_{i+1,i} on exit; null for the last block
+  double* R;           // L_{a,i} on entry, X_{a,i} on exit; null without the arrowhead
+  const double* Dn;    // X_{i+1,i+1} (symmetric, full)
+  const double* Rn;    // X_{a,i+1}, null for the last block or without the arrowhead
+  const double* tip;   // X_aa (ld 64)
+  double* WT;          // W^T, W = L_{i+1,i} inv(L_ii) (ld wld)
+  double* Wa;          // Wa = L_{a,i} inv(L_ii) (64 x bp)
+  double* S;           // lower tiles of X_ii (ld bp)
+  int bp, nbt, wld;
+};
+
+// WT = inv(L_ii)^T L_{i+1,i}^T (tiles I, J; k >= 64 I) and Wa = L_{a,i} inv(L_ii) (k >= 64 J).
+__global__ __launch_bounds__(256) void k_sel_w(SelArgs a) {
+  const int nw = a.Lb ? a.nbt * a.nbt : 0;
+  const Opnd linv{a.D, a.bp, M_UPK_ROW, a.dinv};  // opM[r][k] = inv(L)[k][r]
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  int e = blockIdx.x;
+  if (e < nw) {
+    const int I = e / a.nbt, J = e % a.nbt;
+    dh_mma<M_UPK_ROW, M_ROW>(acc, linv, I * DH_TILE, Opnd{a.Lb, a.bp, M_ROW, nullptr}, J * DH_TILE, I * DH_TILE, a.bp);
+    dh_store(a.WT, a.wld, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
+    return;
+  }
+  e -= nw;
+  dh_mma<M_ROW, M_UPK_ROW>(acc, Opnd{a.R, a.bp, M_ROW, nullptr}, 0, linv, e * DH_TILE, e * DH_TILE, a.bp);
+  dh_store(a.Wa, a.bp, 0, e * DH_TILE, acc, 1.0, 0.0, false);
+}
+
+// X_{i+1,i} = -X_{i+1,i+1} W - X_{a,i+1}^T Wa  (X_{i+1,i+1} read by columns: it is symmetric) and
+// X_{a,i} = -X_{a,i+1} W - X_aa Wa.
+__global__ __launch_bounds__(256) void k_sel_x(SelArgs a) {
+  const int nx = a.Lb ? a.nbt * a.nbt : 0;
+  dh_d4 acc[2][2], acc2[2][2];
+  dh_zero(acc);
+  dh_zero(acc2);
+  int e = blockIdx.x;
+  if (e < nx) {
+    const int I = e / a.nbt, J = e % a.nbt;
+    dh_mma<M_COL, M_ROW>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.WT, a.wld, M_ROW, nullptr},
+                         J * DH_TILE, 0, a.bp);
+    if (a.Rn)
+      dh_mma<M_COL, M_COL>(acc2, Opnd{a.Rn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.Wa, a.bp, M_COL, nullptr},
+                           J * DH_TILE, 0, DH_TILE);
 #pragma unroll
-  for (int x = 0; x < 4; ++x)
+    for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < 4; ++y) acc[x][y] = dh_d4{0.0, 0.0, 0.0, 0.0};
-  dh_mma128<MA, MB>(acc, g.A, I * 2 * DH_TILE, g.B, J * 2 * DH_TILE, klo, khi);
-  dh_store128(g.C, g.ldc, I * 2 * DH_TILE, J * 2 * DH_TILE, acc, g.alpha, g.beta);
+      for (int y = 0; y < 2; ++y) acc[x][y] += acc2[x][y];
+    dh_store(a.Lb, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
+    return;
+  }
+  e -= nx;
+  if (a.Rn)
+    dh_mma<M_ROW, M_ROW>(acc, Opnd{a.Rn, a.bp, M_ROW, nullptr}, 0, Opnd{a.WT, a.wld, M_ROW, nullptr}, e * DH_TILE,
+                         0, a.bp);
+  dh_mma<M_ROW, M_COL>(acc, Opnd{a.tip, DH_TILE, M_ROW, nullptr}, 0, Opnd{a.Wa, a.bp, M_COL, nullptr}, e * DH_TILE,
+                       0, DH_TILE);
+  dh_store(a.R, a.bp, 0, e * DH_TILE, acc, -1.0, 0.0, false);
+}
+
+// Lower tiles of X_ii = inv(L)^T inv(L) - W^T X_{i+1,i} - Wa^T X_{a,i} into S.
+__global__ __launch_bounds__(256) void k_sel_s(SelArgs a) {
+  const int e = blockIdx.x;
+  int I = 0;
+  while ((I + 1) * (I + 2) / 2 <= e) ++I;
+  const int J = e - I * (I + 1) / 2;
+  const Opnd linv{a.D, a.bp, M_UPK_ROW, a.dinv};
+  dh_d4 acc[2][2], acc2[2][2];
+  dh_zero(acc);
+  dh_zero(acc2);
+  dh_mma<M_UPK_ROW, M_UPK_ROW>(acc, linv, I * DH_TILE, linv, J * DH_TILE, I * DH_TILE, a.bp);
+  if (a.Lb)
+    dh_mma<M_ROW, M_COL>(acc2, Opnd{a.WT, a.wld, M_ROW, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
+                         J * DH_TILE, 0, a.bp);
+  if (a.R)
+    dh_mma<M_COL, M_COL>(acc2, Opnd{a.Wa, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.R, a.bp, M_COL, nullptr},
+                         J * DH_TILE, 0, DH_TILE);
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] -= acc2[x][y];
+  dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
 }
 
 // D[r][c] = S[max(r,c)][min(r,c)] (lower tiles of S mirrored).
@@ -1274,6 +1347,65 @@ __global__ __launch_bounds__(256) void k_prior_eig(dh_model m, const double* lam
 }
 
 // ---------------------------------------------------------------------------
+// marginal variances from the selected inverse
+// ---------------------------------------------------------------------------
+
+// out[i b + s] = X_ii[s][s], out[n b + j] = X_aa[j][j].
+__global__ __launch_bounds__(256) void k_diag_extract(const double* diag, const double* tip, int n, int b, int bp,
+                                                      int a, double* out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nst = (long)n * b;
+  if (e < nst) {
+    const long i = e / b, s = e % b;
+    out[e] = diag[i * (long)bp * bp + s * (bp + 1)];
+  } else if (e < nst + a) {
+    const int j = (int)(e - nst);
+    out[e] = tip[j * (DH_TILE + 1)];
+  }
+}
+
+// Entry (p, q) of the selected inverse on the BTA pattern (latent indices,
+// spatio-temporal block first, then the a fixed effects); 0 outside it.
+__device__ __forceinline__ double dh_x_entry(const double* diag, const double* lower, const double* arrow,
+                                             const double* tip, int n, int b, int bp, long p, long q) {
+  const long nst = (long)n * b;
+  const long bb = (long)bp * bp;
+  if (p >= nst && q >= nst) return tip[(p - nst) * DH_TILE + (q - nst)];
+  if (p >= nst || q >= nst) {
+    const long j = (p >= nst) ? p - nst : q - nst;
+    const long r = (p >= nst) ? q : p;
+    return arrow[((r / b) * DH_TILE + j) * (long)bp + r % b];
+  }
+  const long tp = p / b, tq = q / b, sp = p % b, sq = q % b;
+  if (tp == tq) return diag[tp * bb + sp * bp + sq];
+  if (tp == tq + 1) return lower[tq * bb + sp * bp + sq];
+  if (tq == tp + 1) return lower[tp * bb + sq * bp + sp];
+  return 0.0;
+}
+
+// out[o] = a_o^T X a_o for the observation rows a_o = [A_st row o, xcov row o]
+// (A_st CSR over the n b spatio-temporal latents, xcov dense n_obs x a).
+__global__ __launch_bounds__(256) void k_obs_variances(const double* diag, const double* lower, const double* arrow,
+                                                       const double* tip, int n, int b, int bp, int a, long nobs,
+                                                       const long long* rowptr, const int* col, const double* val,
+                                                       const double* xcov, double* out) {
+  const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nobs) return;
+  const long nst = (long)n * b;
+  const long r0 = rowptr[o], r1 = rowptr[o + 1];
+  double v = 0.0;
+  for (long u = r0; u < r1; ++u) {
+    const long p = col[u];
+    const double ap = val[u];
+    for (long w = r0; w < r1; ++w) v += ap * val[w] * dh_x_entry(diag, lower, arrow, tip, n, b, bp, p, col[w]);
+    for (int j = 0; j < a; ++j) v += 2.0 * ap * xcov[o * a + j] * dh_x_entry(diag, lower, arrow, tip, n, b, bp, nst + j, p);
+  }
+  for (int j = 0; j < a; ++j)
+    for (int k = 0; k < a; ++k) v += xcov[o * a + j] * xcov[o * a + k] * tip[j * DH_TILE + k];
+  out[o] = v;
+}
+
+// ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
 
@@ -1302,7 +1434,6 @@ struct dh_handle {
   std::vector<NodeList> levels;  // inverse recursion, deepest last
   long long bytes;
   int syrk_variant;              // 0: 64-tiles, 1: 128-tiles with split k (needs nbt even)
-  int gemm_variant;              // pobtasi dense products: 0: 64-tiles, 1: 128-tiles (nbt even)
   int pobtaf_alg;                // 0: one launch per panel, 1: persistent tile-task launch
   int* dagflags;                 // n * (2 nbt^2 + nbt + 1) tile readiness flags
   int* dagcounter;               // task counter of the persistent launch
@@ -1409,7 +1540,6 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   h->tkind = -1;
   h->nrec = 0;
   h->syrk_variant = 0;
-  h->gemm_variant = 1;
   h->pobtaf_alg = 1;
   {
     int dev = 0, cus = 0;
@@ -1778,11 +1908,6 @@ static void gemm64(dh_handle* h, const GArgs& g) {
   DH_LAUNCH(h, 2, (k_gemm64<MA, MB>), dim3(tiles), dim3(256), 0, g);
 }
 
-// 128x128 tiles; mt, ntl count 128-tiles (dimensions multiples of 128).
-template <int MA, int MB>
-static void gemm128(dh_handle* h, const GArgs& g) {
-  DH_LAUNCH(h, 2, (k_gemm128<MA, MB>), dim3(g.mt * g.ntl), dim3(256), 0, g);
-}
 
 extern "C" {
 
@@ -1794,65 +1919,64 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   if (rc) return rc;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
   const int n = h->n, bp = h->bp, nbt = h->nbt;
-  const bool t128 = (nbt % 2) == 0 && h->gemm_variant == 1;
   const long bb = (long)bp * bp;
   double* W = h->W;
   double* S = h->S;
-  const Opnd none{nullptr, 0, M_ROW, nullptr};
-  (void)none;
   if (arrow) {  // X_aa = inv(L_aa)^T inv(L_aa)
     gemm64<M_COL, M_COL>(h, gargs(Opnd{h->tipinv, 64, M_COL, nullptr}, Opnd{h->tipinv, 64, M_COL, nullptr}, 64,
                                   KLO_0, KHI_K, 1.0, 0.0, h->tip, 64, 1, 1, 0));
   }
   for (int i = n - 1; i >= 0; --i) {
     double* D = h->diag + i * bb;
-    double* Lb = (i < n - 1) ? h->lower + i * bb : nullptr;
-    double* R = arrow ? h->arrow + (long)i * 64 * bp : nullptr;
-    double* Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
-    double* Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * 64 * bp : nullptr;
-    const double* dv = h->dinv + (long)i * bp;
-    const Opnd Linv_row{D, bp, M_UPK_ROW, dv};  // opB = Linv  /  opA = Linv^T
-    // 1. W = L_{i+1,i} Linv, Wa = R_i Linv
-    if (Lb) {
-      const Opnd A{Lb, bp, M_ROW, nullptr};
-      gemm64<M_ROW, M_UPK_ROW>(h, gargs(A, Linv_row, bp, KLO_J, KHI_K, 1.0, 0.0, W, bp, nbt, nbt, 0));
-    }
-    if (R)
-      gemm64<M_ROW, M_UPK_ROW>(h, gargs(Opnd{R, bp, M_ROW, nullptr}, Linv_row, bp, KLO_J, KHI_K, 1.0, 0.0, h->Wa, bp,
-                                        1, nbt, 0));
-    // 2. X_{i+1,i} = -X_{i+1,i+1} W - X_{a,i+1}^T Wa ; X_{a,i} = -X_{a,i+1} W - X_aa Wa
-    if (Lb) {
-      const Opnd A{Dn, bp, M_ROW, nullptr}, B{W, bp, M_COL, nullptr};
-      if (t128) gemm128<M_ROW, M_COL>(h, gargs(A, B, bp, KLO_0, KHI_K, -1.0, 0.0, Lb, bp, nbt / 2, nbt / 2, 0));
-      else gemm64<M_ROW, M_COL>(h, gargs(A, B, bp, KLO_0, KHI_K, -1.0, 0.0, Lb, bp, nbt, nbt, 0));
-      if (Rn)
-        gemm64<M_COL, M_COL>(h, gargs(Opnd{Rn, bp, M_COL, nullptr}, Opnd{h->Wa, bp, M_COL, nullptr}, 64, KLO_0,
-                                      KHI_K, -1.0, 1.0, Lb, bp, nbt, nbt, 0));
-    }
-    if (R) {
-      double beta = 0.0;
-      if (Rn) {
-        gemm64<M_ROW, M_COL>(h, gargs(Opnd{Rn, bp, M_ROW, nullptr}, Opnd{W, bp, M_COL, nullptr}, bp, KLO_0, KHI_K,
-                                      -1.0, 0.0, R, bp, 1, nbt, 0));
-        beta = 1.0;
-      }
-      gemm64<M_ROW, M_COL>(h, gargs(Opnd{h->tip, 64, M_ROW, nullptr}, Opnd{h->Wa, bp, M_COL, nullptr}, 64, KLO_0,
-                                    KHI_K, -1.0, beta, R, bp, 1, nbt, 0));
-    }
-    // 3. X_ii = Linv^T Linv - W^T X_{i+1,i} - Wa^T X_{a,i}  (lower tiles, into S)
-    gemm64<M_UPK_ROW, M_UPK_ROW>(h, gargs(Linv_row, Linv_row, bp, KLO_MAXIJ, KHI_K, 1.0, 0.0, S, bp, nbt, nbt, 1));
-    if (Lb)
-      gemm64<M_COL, M_COL>(h, gargs(Opnd{W, bp, M_COL, nullptr}, Opnd{Lb, bp, M_COL, nullptr}, bp, KLO_0, KHI_K,
-                                    -1.0, 1.0, S, bp, nbt, nbt, 1));
-    if (R)
-      gemm64<M_COL, M_COL>(h, gargs(Opnd{h->Wa, bp, M_COL, nullptr}, Opnd{R, bp, M_COL, nullptr}, 64, KLO_0, KHI_K,
-                                    -1.0, 1.0, S, bp, nbt, nbt, 1));
+    SelArgs sa;
+    sa.D = D;
+    sa.dinv = h->dinv + (long)i * bp;
+    sa.Lb = (i < n - 1) ? h->lower + i * bb : nullptr;
+    sa.R = arrow ? h->arrow + (long)i * 64 * bp : nullptr;
+    sa.Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
+    sa.Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * 64 * bp : nullptr;
+    sa.tip = h->tip;
+    // W^T with a padded row pitch: rows of a power-of-two pitch read row-wise by one tile share
+    // memory channels (the product reading it by rows runs about 30% slower)
+    sa.WT = W;
+    sa.Wa = h->Wa;
+    sa.S = S;
+    sa.bp = bp;
+    sa.nbt = nbt;
+    sa.wld = bp + 8;
+    const int nw = (sa.Lb ? nbt * nbt : 0) + (sa.R ? nbt : 0);
+    // 1. W = L_{i+1,i} inv(L_ii) (as W^T), Wa = L_{a,i} inv(L_ii)
+    if (nw > 0) DH_LAUNCH(h, 2, k_sel_w, dim3(nw), dim3(256), 0, sa);
+    // 2. X_{i+1,i}, X_{a,i}
+    if (nw > 0) DH_LAUNCH(h, 2, k_sel_x, dim3(nw), dim3(256), 0, sa);
+    // 3. X_ii (lower tiles, into S)
+    DH_LAUNCH(h, 2, k_sel_s, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, sa);
     // 4. D_i = symmetric X_ii
     hipLaunchKernelGGL(k_symm_copy, dim3((unsigned)((bb + 255) / 256)), dim3(256), 0, h->stream, D, S, bp);
   }
   HIPCHK(hipGetLastError());
   h->factored = -1;
   h->inverted = 0;
+  return DH_OK;
+}
+
+int dh_marginal_variances(dh_handle* h, double* out) {
+  if (!h || !out) return DH_EARG;
+  const long len = (long)h->n * h->b + h->a;
+  hipLaunchKernelGGL(k_diag_extract, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, h->stream, h->diag, h->tip,
+                     h->n, h->b, h->bp, h->a, out);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_obs_variances(dh_handle* h, long long n_obs, const long long* rowptr, const int* col, const double* val,
+                     const double* xcov, double* out) {
+  if (!h || n_obs < 0 || (n_obs > 0 && (!rowptr || !col || !val || !out)) || (h->a > 0 && n_obs > 0 && !xcov))
+    return DH_EARG;
+  if (n_obs == 0) return DH_OK;
+  hipLaunchKernelGGL(k_obs_variances, dim3((unsigned)((n_obs + 255) / 256)), dim3(256), 0, h->stream, h->diag,
+                     h->lower, h->arrow, h->tip, h->n, h->b, h->bp, h->a, (long)n_obs, rowptr, col, val, xcov, out);
+  HIPCHK(hipGetLastError());
   return DH_OK;
 }
 
@@ -1956,7 +2080,6 @@ int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const doubl
 int dh_set_tuning(dh_handle* h, int key, int value) {
   if (!h) return DH_EARG;
   if (key == 0 && (value == 0 || value == 1)) h->syrk_variant = value;
-  else if (key == 1 && (value == 0 || value == 1)) h->gemm_variant = value;
   else if (key == 2 && (value == 0 || value == 1)) h->pobtaf_alg = value;
   else return DH_EARG;
   return DH_OK;

@@ -66,10 +66,12 @@ class STInla:
         self._state = None
         self._graph = None
         self.n_f_evals = 0
+        self._theta_last = None  # last theta evaluated (the reference keeps it in model.theta)
 
     # ------------------------------------------------------------------
     def _set_params(self, theta):
         """Stage the theta-dependent scalars in pinned memory and copy them to the device."""
+        self._theta_last = np.array(theta, dtype=np.float64)
         p = np.concatenate([DeviceSTModel.coef_array(theta), [self.dm.tau_scale(theta)]])
         self.params_h.copy_(torch.from_numpy(p))
         self.params.copy_(self.params_h, non_blocking=True)
@@ -292,6 +294,53 @@ class STInla:
     def value_and_grad(self, theta):
         self.forward(theta)
         return self.backward()
+
+    # ------------------------------------------------------------------
+    # marginal variances (the selected inverse of Qc at theta)
+    def _selected_inverse(self, theta, x_star):
+        """Factor Qc(theta) and overwrite the factor with its selected inverse.
+
+        Mirrors DALIA._compute_covariance_latent_parameters (core/dalia.py:912-937).
+        For the Gaussian likelihood Qc does not depend on x, so x_star is only
+        validated, as the reference requires both or neither.
+        """
+        if (theta is None) != (x_star is None):
+            raise ValueError("BOTH or NEITHER theta and x_star must be provided to compute the marginal variances.")
+        if theta is None:
+            if self._theta_last is None:
+                raise ValueError("no stored theta: call f / value_and_grad first or pass theta and x_star")
+            theta = self._theta_last
+        self._assemble(np.asarray(theta, dtype=np.float64), "bta", True)
+        _lib.check(self.lib.dh_cholesky(self.h, 1), "dh_cholesky")
+        _lib.check(self.lib.dh_selected_inversion(self.h, 1), "dh_selected_inversion")
+
+    def marginal_variances_latent(self, theta=None, x_star=None) -> np.ndarray:
+        """diag(Qc(theta)^-1) over all n_latent entries (spatio-temporal, then fixed effects).
+
+        Mirrors DALIA.get_marginal_variances_latent_parameters (core/dalia.py:939-962).
+        """
+        self._selected_inverse(theta, x_star)
+        out = torch.empty(self.dm.n_latent, dtype=torch.float64, device="cuda")
+        _lib.check(self.lib.dh_marginal_variances(self.h, out.data_ptr()), "dh_marginal_variances")
+        return out.cpu().numpy()
+
+    def marginal_variances_observations(self, theta=None, x_star=None) -> np.ndarray:
+        """diag(A Qc(theta)^-1 A^T), one entry per observation.
+
+        Mirrors DALIA.get_marginal_variances_observations (core/dalia.py:964-1016).
+        """
+        if not hasattr(self, "_a_dev"):
+            a = self.data.a_st.tocsr()
+            self._a_dev = (torch.from_numpy(a.indptr.astype(np.int64)).cuda(),
+                           torch.from_numpy(a.indices.astype(np.int32)).cuda(),
+                           torch.from_numpy(a.data.astype(np.float64)).cuda(),
+                           torch.from_numpy(np.ascontiguousarray(self.data.x_cov, dtype=np.float64)).cuda())
+        self._selected_inverse(theta, x_star)
+        rp, col, val, xc = self._a_dev
+        out = torch.empty(self.dm.n_obs, dtype=torch.float64, device="cuda")
+        _lib.check(self.lib.dh_obs_variances(self.h, self.dm.n_obs, rp.data_ptr(), col.data_ptr(), val.data_ptr(),
+                                             xc.data_ptr(), out.data_ptr()), "dh_obs_variances")
+        return out.cpu().numpy()
 
 
 def hessian_stencil(theta, h: float, offdiag_points: int = 4) -> np.ndarray:

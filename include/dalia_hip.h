@@ -150,21 +150,34 @@ int dh_contract_ata(dh_handle* h, const dh_model* m, double* out_dev);
 int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const double* gam,
                  double* out_dev);
 
+/* Marginal variances of the latent field, after dh_selected_inversion (BTA):
+ * out[i b + s] = X_ii[s][s], out[n b + j] = X_aa[j][j]  (length n b + a, device).
+ * Replaces extract_diagonal(_structured_to_spmatrix(eye, "bta")) in
+ * DALIA.get_marginal_variances_latent_parameters (core/dalia.py:939-962). */
+int dh_marginal_variances(dh_handle* h, double* out);
+/* Marginal variances of the observations, after dh_selected_inversion (BTA):
+ * out[o] = a_o^T X a_o with a_o = [A_st row o, xcov row o]; A_st as CSR over
+ * the n b spatio-temporal latents (int64 row pointers, int32 columns), xcov
+ * dense row-major n_obs x a (may be null when a = 0).  All device pointers.
+ * Replaces diag(A X A^T) in DALIA.get_marginal_variances_observations
+ * (core/dalia.py:964-1016). */
+int dh_obs_variances(dh_handle* h, long long n_obs, const long long* rowptr, const int* col, const double* val,
+                     const double* xcov, double* out);
+
 /* Tuning knobs (chosen per shape by the autotuner in dalia_amd/autotune.py):
- * key 0: Schur-update kernel of pobtaf, 0 = 64x64 tiles (one launch),
- *        1 = 128x128 tiles with the k range split in two (+ a tail launch);
- * key 1: dense products of pobtasi, 0 = 64x64 tiles, 1 = 128x128 tiles;
+ * key 0: Schur-update kernel of the per-panel pobtaf schedule, 0 = 64x64
+ *        tiles (one launch), 1 = 128x128 tiles with the k range split in two
+ *        (+ a tail launch; needs an even number of 64-tiles per block);
  * key 2: pobtaf/pobtf schedule, 0 = one launch per 64-column panel (key 0
  *        applies), 1 = one persistent launch of left-looking tile tasks
  *        synchronised by per-tile readiness flags (default).
- * The 128 variants need an even number of 64-tiles per block; otherwise the
- * 64 variants run.  No counterpart in the reference. */
+ * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 
 /* Per-kernel timing with HIP events on the handle's stream, for the roofline
  * of one kernel.  kind: 0 k_syrk (Schur update of the next diagonal block in
  * pobtaf; k_syrk128 for tuning value 1), 1 k_update (panel trailing update),
- * 2 k_gemm64/k_gemm128 (pobtasi tiles), 3 k_panel, 4 k_leaf,
+ * 2 pobtasi tile launches (k_sel_w, k_sel_x, k_sel_s), 3 k_panel, 4 k_leaf,
  * 5 k_assemble_rows, 6 k_syrk_tail, 7 k_pobtaf_dag (the persistent pobtaf
  * launch); kind < 0 disables.  While enabled,
  * each launch of that kind is bracketed by an event pair (at most 16384

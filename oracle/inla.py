@@ -404,6 +404,44 @@ def adjoint_gradient(data, theta):
 
 
 # ---------------------------------------------------------------------------
+# marginal variances (core/dalia.py:912-1016)
+# ---------------------------------------------------------------------------
+
+def selected_inverse_blocks(data, theta):
+    """Selected inverse of Qc(theta) on the BTA pattern (core/dalia.py:912-937)."""
+    n, b, a = data.nt, data.ns, data.nb
+    Qc = qcond_sparse(data, np.asarray(theta, dtype=np.float64))
+    D, L, R, T = sparse_to_bta(Qc, n, b, a, "bta")
+    bta.pobtaf(D, L, R, T)
+    bta.pobtasi(D, L, R, T)
+    return D, L, R, T
+
+
+def marginal_variances_latent(data, theta) -> np.ndarray:
+    """diag(Qc^-1): diagonals of the selected inverse blocks, then the tip
+    (core/dalia.py:939-962, extract_diagonal of _structured_to_spmatrix(eye))."""
+    D, _, _, T = selected_inverse_blocks(data, theta)
+    parts = [np.diag(D[i]) for i in range(data.nt)]
+    if data.nb:
+        parts.append(np.diag(T))
+    return np.concatenate(parts)
+
+
+def marginal_variances_observations(data, theta) -> np.ndarray:
+    """diag(A X A^T), X the selected inverse on Qc's pattern (core/dalia.py:964-1016).
+
+    Every pair of latents that share an observation row is a nonzero of
+    A^T A, hence of Qc, so X on the pattern is enough; the product is formed
+    from the densified pattern (small inputs only).
+    """
+    D, L, R, T = selected_inverse_blocks(data, theta)
+    X = bta.bta_to_dense(D, L, R if data.nb else None, T if data.nb else None)
+    A = data.a_full().tocsr()
+    AX = np.asarray(A @ X)
+    return np.einsum("ij,ij->i", AX, A.toarray())
+
+
+# ---------------------------------------------------------------------------
 # Kronecker-eigen closed form for logdet Qp|BT
 # ---------------------------------------------------------------------------
 

@@ -88,3 +88,24 @@ def test_autograd_function(small):
     f_ref, g_ref = oinla.adjoint_gradient(d, d.theta)
     assert abs(float(f) - f_ref) <= 1e-10 * abs(f_ref)
     assert rel(th.grad.numpy(), g_ref) < 1e-8
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 2, 0), (6, 5, 3, 1), (4, 3, 0, 2)])
+def test_marginal_variances(shape):
+    """diag(Qc^-1) and diag(A Qc^-1 A^T) through the HIP selected inversion vs the oracle."""
+    from dalia_amd.inla import STInla
+    d = synth.make_st_model(*shape)
+    print(f"case marginal variances ns={d.ns} nt={d.nt} nb={d.nb}", flush=True)
+    eng = STInla(d)
+    theta = d.theta + np.array([0.05, -0.1, 0.1, 0.2])
+    mv = eng.marginal_variances_latent(theta, np.zeros(d.ns * d.nt + d.nb))
+    ref = oinla.marginal_variances_latent(d, theta)
+    assert np.abs(mv - ref).max() <= 1e-9 * np.abs(ref).max()
+    ov = eng.marginal_variances_observations(theta, np.zeros(d.ns * d.nt + d.nb))
+    oref = oinla.marginal_variances_observations(d, theta)
+    assert np.abs(ov - oref).max() <= 1e-9 * np.abs(oref).max()
+    eng.f(theta)
+    mv2 = eng.marginal_variances_latent()  # stored theta
+    assert np.abs(mv2 - ref).max() <= 1e-9 * np.abs(ref).max()
+    with pytest.raises(ValueError):
+        eng.marginal_variances_latent(theta, None)

@@ -134,3 +134,19 @@ def test_fd_stencils_exact_on_quadratic():
         assert np.allclose(Hn, H, atol=1e-6)
     assert inla.hessian_stencil(np.zeros(7), 0.1, 4).shape[0] == 1 + 14 + 4 * 21
     assert inla.hessian_stencil(np.zeros(7), 0.1, 2).shape[0] == 57
+
+
+def test_marginal_variances_match_dense_inverse():
+    """Selected-inversion marginal variances equal the diagonal of the dense inverse,
+    and diag(A X A^T) from the BTA pattern equals the dense covariance of the observations."""
+    d = synth.make_st_model(4, 3, 2, 5)
+    theta = d.theta + np.array([0.1, -0.2, 0.05, 0.3])
+    Qc = inla.qcond_sparse(d, theta).toarray()
+    C = np.linalg.inv(Qc)
+    mv = inla.marginal_variances_latent(d, theta)
+    assert mv.shape == (d.ns * d.nt + d.nb,)
+    assert np.abs(mv - np.diag(C)).max() <= 1e-12 * np.abs(np.diag(C)).max()
+    A = d.a_full().toarray()
+    ov = inla.marginal_variances_observations(d, theta)
+    ref = np.einsum("ij,jk,ik->i", A, C, A)
+    assert np.abs(ov - ref).max() <= 1e-11 * np.abs(ref).max()
