@@ -51,6 +51,8 @@ SIGNATURES = {
     "dh_create": (_I, [_I, _I, _I, _P, ctypes.POINTER(_P)]),
     "dh_destroy": (_I, [_P]),
     "dh_marginal_variances": (_I, [_P, _P]),
+    "dh_assemble_coreg": (_I, [_P, ctypes.POINTER(DhModel), _I, _P, _I, _I]),
+    "dh_scale_indexed": (_I, [_P, _P, _P, _P, _P, _LL]),
     "dh_obs_variances": (_I, [_P, _LL, _P, _P, _P, _P, _P]),
     "dh_get_layout": (_I, [_P, ctypes.POINTER(DhLayout)]),
     "dh_set_stream": (_I, [_P, _P]),

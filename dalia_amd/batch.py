@@ -9,9 +9,10 @@ on one GPU this module captures all of them in one graph:
   stream;
 * the evaluations of one instance run back to back on its stream (they share
   its buffers); instances run concurrently (fork/join inside the capture);
-* each evaluation first copies its 13 theta-dependent parameters from a row
-  of a pinned host table (a captured host-to-device copy), then runs the
-  plain f(theta) kernels and writes its 5 scalars to its own device slot.
+* each evaluation first copies its theta-dependent parameters (13 for the
+  univariate model, nv*nv*12 + nv for the coregional one) from a row of a
+  pinned host table (a captured host-to-device copy), then runs the plain
+  f(theta) kernels and writes its 5 scalars to its own device slot.
 
 The graph is captured once for a given assignment of evaluations to
 instances and replayed for every new set of theta values; the host only
@@ -32,7 +33,8 @@ class BatchedF:
         self.engines = engines
         self.assignment = list(assignment)
         E = len(self.assignment)
-        self.table = torch.zeros((E, 13), dtype=torch.float64).pin_memory()
+        width = max(int(eng.params.numel()) for eng in engines)
+        self.table = torch.zeros((E, width), dtype=torch.float64).pin_memory()
         self.slots = torch.zeros((E, 5), dtype=torch.float64, device="cuda")
         self.streams = [torch.cuda.Stream() for _ in engines]
         self.graph = None
@@ -47,13 +49,14 @@ class BatchedF:
                 for e, inst in enumerate(self.assignment):
                     if inst != k:
                         continue
-                    eng.params.copy_(self.table[e], non_blocking=True)
+                    eng.params.copy_(self.table[e, :eng.params.numel()], non_blocking=True)
                     eng._enqueue_f(None, out=self.slots[e])
             capture_stream.wait_stream(s)
 
     def capture(self):
         for e, inst in enumerate(self.assignment):  # valid parameters for the warm-up pass
-            self.table[e].copy_(torch.from_numpy(self.engines[inst].param_row(self.engines[inst].data.theta)))
+            row = self.engines[inst].param_row(self.engines[inst].data.theta)
+            self.table[e, :row.size].copy_(torch.from_numpy(row))
         cur = torch.cuda.current_stream()
         self._enqueue_all(cur)  # eager warm-up (allocator, kernels loaded)
         torch.cuda.synchronize()
@@ -71,7 +74,8 @@ class BatchedF:
             self.capture()
         assert len(thetas) == len(self.assignment)
         for e, inst in enumerate(self.assignment):
-            self.table[e].copy_(torch.from_numpy(self.engines[inst].param_row(thetas[e])))
+            row = self.engines[inst].param_row(thetas[e])
+            self.table[e, :row.size].copy_(torch.from_numpy(row))
         self.graph.replay()
         s = self.slots.cpu().numpy()
         return np.array([self.engines[inst]._f_from_scalars(np.asarray(thetas[e]), s[e])

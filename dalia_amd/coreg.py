@@ -1,0 +1,210 @@
+"""f(theta) of the coregional (multivariate) Gaussian model on the HIP BTA path.
+
+Mirrors `models/coregional_model.py` driven by `core/dalia.py:603-745`:
+nv = 2 or 3 spatio-temporal variates on one mesh, each with its own
+(r_s, r_t) prior field, fixed effects and Gaussian likelihood precision,
+coupled by the coregional sigmas and lambdas.  The BTA has blocks of size
+nv*ns (variate-major inside every time block) and an arrowhead of nv*nb.
+
+Device work per evaluation (all on the handle's stream, no host sync until
+the five scalars are read): coregional assembly of Qc (`dh_assemble_coreg`),
+pobtaf, the information vector A^T D y (`dh_scale_indexed`), pobtas,
+logdet, the quadratic form, then the BT prior (assembly without the
+likelihood, pobtf, logdet).  The quadratic form is mu^T b: with Qc mu = b it
+equals the reference's mu^T Qc mu (`core/dalia.py:1161`) up to the solve's
+rounding.  Hyper-priors must be Gaussian (the coregional examples use
+Gaussian priors throughout, `examples/gst_coreg2_small/run.py:80-200`).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import scipy.sparse as sps
+import torch
+
+from dalia_amd import _lib, hyper
+from dalia_amd.inla import hessian_from_values, hessian_stencil
+from dalia_amd.model import _csr, fem_union, tridiag_parts
+from dalia_amd.solver import HipBTASolver
+
+
+def mixing_weights(nv: int, theta) -> np.ndarray:
+    """w[i, j, k]: q_ij = sum_k w[i,j,k] Qu_k (`coregional_model.py:304-497`)."""
+    s2 = np.array([math.exp(2.0 * theta[3 * nv + k]) for k in range(nv)])
+    lam = np.asarray(theta[4 * nv:], dtype=np.float64)
+    w = np.zeros((nv, nv, nv))
+    if nv == 2:
+        l01 = lam[0]
+        w[0, 0] = [1 / s2[0], l01 ** 2 / s2[1]]
+        w[1, 0] = [0.0, -l01 / s2[1]]
+        w[1, 1] = [0.0, 1 / s2[1]]
+    elif nv == 3:
+        l01, l02, l12 = lam
+        w[0, 0] = [1 / s2[0], l01 ** 2 / s2[1], l12 ** 2 / s2[2]]
+        w[1, 0] = [0.0, -l01 / s2[1], l02 * l12 / s2[2]]
+        w[2, 0] = [0.0, 0.0, -l12 / s2[2]]
+        w[1, 1] = [0.0, 1 / s2[1], l02 ** 2 / s2[2]]
+        w[2, 1] = [0.0, 0.0, -l02 / s2[2]]
+        w[2, 2] = [0.0, 0.0, 1 / s2[2]]
+    else:
+        raise ValueError("nv must be 2 or 3")
+    for i in range(nv):
+        for j in range(i + 1, nv):
+            w[i, j] = w[j, i]
+    return w
+
+
+def coreg_param_row(nv: int, sigma_st: float, theta) -> np.ndarray:
+    """Device parameters: C_ij[3][4] for every (i, j) (row-major), then exp(prec_o_i)."""
+    theta = np.asarray(theta, dtype=np.float64)
+    coef = [hyper.kron_coefficients(np.array([theta[3 * k], theta[3 * k + 1], sigma_st])) for k in range(nv)]
+    w = mixing_weights(nv, theta)
+    C = np.einsum("ijk,kab->ijab", w, np.stack(coef))
+    taus = np.array([math.exp(theta[3 * i + 2]) for i in range(nv)])
+    return np.concatenate([C.ravel(), taus])
+
+
+class DeviceCoregModel:
+    """θ-independent inputs of a coregional model, uploaded once."""
+
+    def __init__(self, data, device="cuda"):
+        self.data = data
+        nv, ns, nt, nb = data.nv, data.ns, data.nt, data.nb
+        self.nv, self.ns, self.nt, self.nb = nv, ns, nt, nb
+        self.n_latent = data.n_latent
+        self.n_obs_v = [int(y.size) for y in data.y]
+        self.yty_v = [float(y @ y) for y in data.y]
+        dev = torch.device(device)
+
+        def up(x, dtype=torch.float64):
+            return torch.as_tensor(np.ascontiguousarray(x)).to(dtype=dtype, device=dev)
+
+        rp, col, val = fem_union(data.c0, data.g1, data.g2, data.g3)
+        self.fem_rowptr, self.fem_col, self.fem_val = up(rp, torch.int32), up(col, torch.int32), up(val)
+        md, mo = tridiag_parts(data.m0, data.m1, data.m2, nt)
+        self.m_diag, self.m_off = up(md), up(mo)
+        self.keep = []
+        structs = []
+        nst = ns * nt
+        aty = np.zeros(self.n_latent)
+        vidx = np.zeros(self.n_latent, dtype=np.int32)
+        for i in range(nv):
+            A = _csr(data.a_st[i])
+            X = np.ascontiguousarray(data.x_cov[i], dtype=np.float64)
+            AtA = _csr(A.T @ A)
+            t = dict(rp=up(AtA.indptr.astype(np.int64), torch.int64), col=up(AtA.indices.astype(np.int32), torch.int32),
+                     val=up(AtA.data), arrow=up(np.ascontiguousarray((A.T @ X).T)), tip=up(X.T @ X))
+            self.keep.append(t)
+            structs.append(_lib.DhModel(
+                ns=ns, nt=nt, nb=nb, fem_nnz=int(col.size), fem_rowptr=self.fem_rowptr.data_ptr(),
+                fem_col=self.fem_col.data_ptr(), fem_val=self.fem_val.data_ptr(), m_diag=self.m_diag.data_ptr(),
+                m_off=self.m_off.data_ptr(), ata_nnz=int(AtA.nnz), ata_rowptr=t["rp"].data_ptr(),
+                ata_col=t["col"].data_ptr(), ata_val=t["val"].data_ptr(), ata_arrow=t["arrow"].data_ptr(),
+                ata_tip=t["tip"].data_ptr(), fe_prec=float(data.fe_prec)))
+            # A_i^T y_i placed at the permuted latent positions (time-block t, variate i, node s)
+            aty_st = (A.T @ data.y[i]).reshape(nt, ns)
+            for tt in range(nt):
+                aty[tt * nv * ns + i * ns: tt * nv * ns + (i + 1) * ns] = aty_st[tt]
+                vidx[tt * nv * ns + i * ns: tt * nv * ns + (i + 1) * ns] = i
+            aty[nv * nst + i * nb: nv * nst + (i + 1) * nb] = X.T @ data.y[i]
+            vidx[nv * nst + i * nb: nv * nst + (i + 1) * nb] = i
+        self.models = (_lib.DhModel * nv)(*structs)
+        self.aty = up(aty)
+        self.vidx = up(vidx, torch.int32)
+
+
+class CoregInla:
+    """Plain f(theta) of the coregional model; the engine interface of `STInla`
+    used by `BatchedF` (params, param_row, _enqueue_f, _f_from_scalars, set_stream)."""
+
+    def __init__(self, data, device="cuda"):
+        self.data = data
+        self.dm = DeviceCoregModel(data, device=device)
+        nv = data.nv
+        self.solver = HipBTASolver(diagonal_blocksize=data.b, n_diag_blocks=data.nt, arrowhead_blocksize=data.a)
+        self.lib = self.solver.lib
+        self.h = self.solver.h
+        N = data.n_latent
+        self.b = torch.zeros(N, dtype=torch.float64, device="cuda")
+        self.mu = torch.zeros(N, dtype=torch.float64, device="cuda")
+        self.scal = torch.zeros(8, dtype=torch.float64, device="cuda")
+        self.np = nv * nv * 12 + nv
+        self.params = torch.zeros(self.np, dtype=torch.float64, device="cuda")
+        self.params_h = torch.zeros(self.np, dtype=torch.float64).pin_memory()
+        self.n_f_evals = 0
+
+    def param_row(self, theta) -> np.ndarray:
+        return coreg_param_row(self.data.nv, self.data.sigma_st, theta)
+
+    def _set_params(self, theta):
+        self.params_h.copy_(torch.from_numpy(self.param_row(theta)))
+        self.params.copy_(self.params_h, non_blocking=True)
+
+    def set_stream(self, stream) -> None:
+        _lib.check(self.lib.dh_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)), "dh_set_stream")
+
+    def _enqueue_f(self, theta, out=None):
+        """Device work of one f evaluation; out[0] = logdet Qc, out[1] = mu^T b, out[4] = logdet Qp|BT."""
+        if theta is not None:
+            self._set_params(theta)
+        out = self.scal if out is None else out
+        L, h, nv = self.lib, self.h, self.data.nv
+        N = self.data.n_latent
+        _lib.check(L.dh_assemble_coreg(h, self.dm.models, nv, self.params.data_ptr(), 1, 1), "dh_assemble_coreg")
+        _lib.check(L.dh_cholesky(h, 1), "dh_cholesky")
+        taus = self.params.data_ptr() + 8 * nv * nv * 12
+        _lib.check(L.dh_scale_indexed(h, self.dm.aty.data_ptr(), self.dm.vidx.data_ptr(), ctypes.c_void_p(taus),
+                                      self.b.data_ptr(), N), "dh_scale_indexed")
+        _lib.check(L.dh_scale(h, self.b.data_ptr(), 1.0, self.mu.data_ptr(), N), "dh_scale")
+        _lib.check(L.dh_solve(h, self.mu.data_ptr(), 1), "dh_solve")
+        _lib.check(L.dh_logdet(h, 1, out[0:1].data_ptr()), "dh_logdet")
+        _lib.check(L.dh_dot(h, self.mu.data_ptr(), self.b.data_ptr(), N, out[1:2].data_ptr()), "dh_dot")
+        _lib.check(L.dh_assemble_coreg(h, self.dm.models, nv, self.params.data_ptr(), 0, 0), "dh_assemble_coreg")
+        _lib.check(L.dh_cholesky(h, 0), "dh_cholesky")
+        _lib.check(L.dh_logdet(h, 0, out[4:5].data_ptr()), "dh_logdet")
+
+    def _scalars(self, theta):
+        theta = np.asarray(theta, dtype=np.float64)
+        lp, _ = hyper.log_prior_and_grad(theta, self.data.priors)
+        lik = sum(hyper.likelihood_eta0(float(theta[3 * i + 2]), self.dm.n_obs_v[i], self.dm.yty_v[i])
+                  for i in range(self.data.nv))
+        return lp, lik
+
+    def _f_from_scalars(self, theta, s) -> float:
+        ld_c, quad, ld_p = float(s[0]), float(s[1]), float(s[4])
+        if np.isnan(ld_c) or np.isnan(ld_p):
+            raise ValueError("Logdet is NaN. Check the input matrix.")
+        lp, lik = self._scalars(theta)
+        f = 0.5 * ld_c - 0.5 * quad - (lp + lik + 0.5 * ld_p)
+        if np.isnan(f):
+            raise ValueError("f(theta) is NaN")
+        return float(f)
+
+    def f(self, theta) -> float:
+        theta = np.asarray(theta, dtype=np.float64)
+        if any(p.kind != "gaussian" for p in self.data.priors):
+            raise ValueError("coregional hyper-priors must be Gaussian")
+        self._enqueue_f(theta)
+        s = self.scal[:5].cpu().numpy()
+        self.n_f_evals += 1
+        return self._f_from_scalars(theta, s)
+
+    def fd_gradient(self, theta, h: float = 1e-4):
+        """Central differences, 2d+1 evaluations (`core/dalia.py:513-601`)."""
+        theta = np.asarray(theta, dtype=np.float64)
+        d = theta.size
+        f0 = self.f(theta)
+        g = np.zeros(d)
+        for j in reversed(range(d)):
+            e = np.zeros(d)
+            e[j] = h
+            g[j] = (self.f(theta + e) - self.f(theta - e)) / (2 * h)
+        return f0, g
+
+    def hessian(self, theta, h: float = 5e-3, offdiag_points: int = 4):
+        pts = hessian_stencil(theta, h, offdiag_points)
+        fv = np.array([self.f(p) for p in pts])
+        return hessian_from_values(fv, len(theta), h, offdiag_points)

@@ -194,6 +194,131 @@ __global__ __launch_bounds__(256) void k_assemble_tip(double* tip, int a, int ap
   }
 }
 
+// ---------------------------------------------------------------------------
+// coregional assembly: nv variates on one mesh, variate-major inside every
+// time block ([variate 0 nodes, variate 1 nodes, ...]), fixed effects of all
+// variates in the arrowhead.  Sub-block (i, j) of every diagonal and
+// sub-diagonal block is the spatio-temporal Kronecker form with its own 12
+// coefficients C_ij (the q_ij mixing of the variates' priors); variate i's
+// likelihood term exp(prec_o_i) A_i^T A_i enters sub-block (i, i) only.
+// params: C_ij[3][4] for (i, j) row-major (nv*nv*12), then exp(prec_o_i).
+// ---------------------------------------------------------------------------
+struct DhCoreg {
+  int nv;
+  dh_model m[3];  // shared FEM / temporal parts are read from m[0]
+};
+
+__global__ __launch_bounds__(256) void k_assemble_rows_coreg(double* diag, double* lower, int n, int bp,
+                                                             DhCoreg cm, const double* params, int with_like) {
+  extern __shared__ double row[];
+  const dh_model& m0 = cm.m[0];
+  const int nv = cm.nv, ns = m0.ns, nt = m0.nt;
+  const long r = blockIdx.x;
+  const long ndiag = (long)n * bp;
+  int t, sb, trow, tcol;
+  bool isdiag;
+  double* out;
+  if (r < ndiag) {
+    isdiag = true;
+    t = (int)(r / bp);
+    sb = (int)(r % bp);
+    trow = tcol = t;
+    out = diag + (long)t * bp * bp + (long)sb * bp;
+  } else {
+    isdiag = false;
+    const long rr = r - ndiag;
+    t = (int)(rr / bp);
+    sb = (int)(rr % bp);
+    trow = t + 1;
+    tcol = t;
+    out = lower + (long)t * bp * bp + (long)sb * bp;
+  }
+  for (int c = threadIdx.x; c < bp; c += blockDim.x) row[c] = 0.0;
+  __syncthreads();
+  if (sb < nv * ns) {
+    const int i = sb / ns, s = sb % ns;
+    double mk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mk[k] = isdiag ? m0.m_diag[k * nt + t] : m0.m_off[k * (nt - 1) + t];
+    const int e0 = m0.fem_rowptr[s], e1 = m0.fem_rowptr[s + 1];
+    const long nn = m0.fem_nnz;
+    for (int j = 0; j < nv; ++j) {
+      const double* C = params + (i * nv + j) * 12;
+      double w[4];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) w[f] = C[f] * mk[0] + C[4 + f] * mk[1] + C[8 + f] * mk[2];
+      for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x)
+        row[j * ns + m0.fem_col[e]] = w[0] * m0.fem_val[e] + w[1] * m0.fem_val[nn + e] +
+                                      w[2] * m0.fem_val[2 * nn + e] + w[3] * m0.fem_val[3 * nn + e];
+    }
+    __syncthreads();
+    if (with_like) {
+      const dh_model& mi = cm.m[i];
+      const double tau = params[nv * nv * 12 + i];
+      const long R = (long)trow * ns + s;
+      const long a0 = mi.ata_rowptr[R], a1 = mi.ata_rowptr[R + 1];
+      for (long e = a0 + threadIdx.x; e < a1; e += blockDim.x) {
+        const int c = mi.ata_col[e];
+        const int ct = c / ns;
+        if (ct == tcol) row[i * ns + c - ct * ns] += tau * mi.ata_val[e];
+      }
+    }
+  } else if (isdiag) {
+    if (threadIdx.x == 0) row[sb] = 1.0;
+  }
+  __syncthreads();
+  double2* o2 = reinterpret_cast<double2*>(out);
+  for (int c = threadIdx.x; c < bp / 2; c += blockDim.x) o2[c] = make_double2(row[2 * c], row[2 * c + 1]);
+}
+
+// Arrow row (t, i nb + f): exp(prec_o_i) (X_i^T A_st_i)[f][t ns + s] in columns i ns + s.
+__global__ __launch_bounds__(256) void k_assemble_arrow_coreg(double* arrow, int n, int bp, int ap, DhCoreg cm,
+                                                              const double* params, int with_like) {
+  const long r = blockIdx.x;  // t*ap + jr
+  const int t = (int)(r / ap), jr = (int)(r % ap);
+  const int nv = cm.nv, ns = cm.m[0].ns, nb = cm.m[0].nb;
+  double* out = arrow + r * bp;
+  const int i = nb > 0 ? jr / nb : 0, f = nb > 0 ? jr % nb : 0;
+  const bool live = with_like && jr < nv * nb;
+  const double tau = live ? params[nv * nv * 12 + i] : 0.0;
+  const long nst = (long)ns * cm.m[0].nt;
+  for (int c = threadIdx.x; c < bp; c += blockDim.x) {
+    double v = 0.0;
+    if (live && c >= i * ns && c < (i + 1) * ns) v = tau * cm.m[i].ata_arrow[(long)f * nst + (long)t * ns + (c - i * ns)];
+    out[c] = v;
+  }
+}
+
+// Tip: block diagonal over variates, fe_prec I + exp(prec_o_i) X_i^T X_i; identity on the padding.
+__global__ __launch_bounds__(256) void k_assemble_tip_coreg(double* tip, int ap, DhCoreg cm, const double* params,
+                                                            int with_like) {
+  const int nv = cm.nv, nb = cm.m[0].nb;
+  for (int e = threadIdx.x; e < ap * ap; e += blockDim.x) {
+    const int j = e / ap, k = e % ap;
+    double v;
+    if (j < nv * nb && k < nv * nb) {
+      const int i = j / nb;
+      if (k / nb != i) {
+        v = 0.0;
+      } else {
+        const int fj = j % nb, fk = k % nb;
+        v = (fj == fk) ? cm.m[i].fe_prec : 0.0;
+        if (with_like) v += params[nv * nv * 12 + i] * cm.m[i].ata_tip[fj * nb + fk];
+      }
+    } else {
+      v = (j == k) ? 1.0 : 0.0;
+    }
+    tip[e] = v;
+  }
+}
+
+// y[p] = s[idx[p]] x[p]
+__global__ __launch_bounds__(256) void k_scale_indexed(const double* x, const int* idx, const double* s, double* y,
+                                                       long len) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < len) y[e] = s[idx[e]] * x[e];
+}
+
 // Zero every block, identity on the padded diagonal of the diagonal blocks
 // and of the tip.
 __global__ __launch_bounds__(256) void k_fill_bta(double* diag, double* lower, double* arrow, double* tip, int n,
@@ -1661,6 +1786,42 @@ int dh_assemble(dh_handle* h, const dh_model* m, const double* params, int spars
   }
   h->factored = -1;
   h->inverted = 0;
+  return DH_OK;
+}
+
+int dh_assemble_coreg(dh_handle* h, const dh_model* m, int nv, const double* params, int sparsity, int with_like) {
+  if (!h || !m || !params || nv < 2 || nv > 3) return DH_EARG;
+  if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
+  DhCoreg cm;
+  memset(&cm, 0, sizeof(cm));
+  cm.nv = nv;
+  for (int i = 0; i < nv; ++i) {
+    if (m[i].ns != m[0].ns || m[i].nt != m[0].nt || m[i].nb != m[0].nb || m[i].nt != h->n || m[i].nt < 2) return DH_EARG;
+    cm.m[i] = m[i];
+  }
+  if (nv * m[0].ns != h->b || nv * m[0].nb != h->a || h->a > 64) return DH_EARG;
+  const long rows = (long)(2 * h->n - 1) * h->bp;
+  DH_LAUNCH(h, 5, k_assemble_rows_coreg, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double), h->diag,
+            h->lower, h->n, h->bp, cm, params, with_like);
+  HIPCHK(hipGetLastError());
+  if (h->a > 0) {
+    const int wl = (sparsity == DH_SPARSITY_BTA) ? with_like : 0;
+    hipLaunchKernelGGL(k_assemble_arrow_coreg, dim3(h->n * h->ap), dim3(256), 0, h->stream, h->arrow, h->n, h->bp,
+                       h->ap, cm, params, wl);
+    hipLaunchKernelGGL(k_assemble_tip_coreg, dim3(1), dim3(256), 0, h->stream, h->tip, h->ap, cm, params, wl);
+    HIPCHK(hipGetLastError());
+  }
+  h->factored = -1;
+  h->inverted = 0;
+  return DH_OK;
+}
+
+int dh_scale_indexed(dh_handle* h, const double* x, const int* idx, const double* s_dev, double* y, long long len) {
+  if (!h || !x || !idx || !s_dev || !y || len < 0) return DH_EARG;
+  if (len == 0) return DH_OK;
+  hipLaunchKernelGGL(k_scale_indexed, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, h->stream, x, idx, s_dev, y,
+                     (long)len);
+  HIPCHK(hipGetLastError());
   return DH_OK;
 }
 

@@ -255,3 +255,151 @@ WORKLOADS = {
 def make_workload(name: str) -> STModelData:
     m, nt, nb, seed = WORKLOADS[name]
     return make_st_model(m, nt, nb, seed)
+
+
+# --------------------------------------------------------------------------
+# coregional (multivariate) model
+# --------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class CoregModelData:
+    """nv spatio-temporal Gaussian models on one mesh, coupled by coregionalisation.
+
+    Mirrors `models/coregional_model.py`: every variate i has its own
+    spatio-temporal field (hyperparameters r_s_i, r_t_i; sigma_st fixed to
+    ``sigma_st``, the per-model sigma being replaced by the coregional
+    sigma_i), ``nb`` fixed effects and a Gaussian likelihood with precision
+    prec_o_i.  theta = [r_s_0, r_t_0, prec_o_0, r_s_1, ..., sigma_0, ...,
+    sigma_{nv-1}, lambda_0_1, (lambda_0_2, lambda_1_2)]
+    (`coregional_model.py:101-150`, `configs/models_config.py:62-71`).
+    The latent vector is permuted so that every time block holds all
+    variates ([variate 0 nodes, variate 1 nodes, ...]) and the fixed effects
+    of all variates form the arrowhead (`coregional_model.py:216-231`,
+    `:849-915`).
+    """
+
+    nv: int
+    ns: int
+    nt: int
+    nb: int  # fixed effects per variate
+    c0: sps.csr_matrix
+    g1: sps.csr_matrix
+    g2: sps.csr_matrix
+    g3: sps.csr_matrix
+    m0: sps.csr_matrix
+    m1: sps.csr_matrix
+    m2: sps.csr_matrix
+    a_st: list  # per variate, n_obs_i x (ns*nt)
+    x_cov: list  # per variate, n_obs_i x nb
+    y: list  # per variate
+    theta: np.ndarray
+    priors: list
+    sigma_st: float = 0.0
+    fe_prec: float = 1e-3
+
+    @property
+    def n_obs(self) -> int:
+        return int(sum(v.shape[0] for v in self.y))
+
+    @property
+    def b(self) -> int:
+        return self.nv * self.ns
+
+    @property
+    def a(self) -> int:
+        return self.nv * self.nb
+
+    @property
+    def n_latent(self) -> int:
+        return self.nv * (self.ns * self.nt + self.nb)
+
+    def theta_keys(self) -> list:
+        keys = []
+        for i in range(self.nv):
+            keys += [f"r_s_{i}", f"r_t_{i}", f"prec_o_{i}"]
+        keys += [f"sigma_{i}" for i in range(self.nv)]
+        keys += [f"lambda_{i}_{j}" for i in range(self.nv) for j in range(i + 1, self.nv)]
+        return keys
+
+    def latent_permutation(self) -> np.ndarray:
+        """Columns of bdiag(A_i) in the permuted order (`coregional_model.py:849-915`)."""
+        nst = self.ns * self.nt
+        per = nst + self.nb
+        blocks = [np.arange(nst).reshape(self.nt, self.ns) + i * per for i in range(self.nv)]
+        st = np.hstack(blocks).ravel()
+        fe = [i * per + nst + np.arange(self.nb) for i in range(self.nv)]
+        return np.concatenate([st] + fe)
+
+    def a_full(self) -> sps.csr_matrix:
+        """Permuted block-diagonal design matrix bdiag([A_st_i | X_i])[:, perm]."""
+        blocks = [sps.hstack([self.a_st[i], sps.csr_matrix(self.x_cov[i])], format="csr") for i in range(self.nv)]
+        A = sps.block_diag(blocks, format="csr")
+        return A[:, self.latent_permutation()].tocsr()
+
+    def obs_precision_index(self) -> np.ndarray:
+        """Variate of every observation (rows of a_full)."""
+        return np.concatenate([np.full(self.y[i].shape[0], i) for i in range(self.nv)])
+
+
+#: theta of the bivariate / trivariate workloads: per variate (r_s, r_t, prec_o),
+#: then log sigmas, then lambdas.
+COREG_THETA = {
+    2: np.array([-1.0, 0.7, 1.5, -0.9, 0.6, 1.2, 0.0, 0.2, 0.5]),
+    3: np.array([-1.0, 0.7, 1.5, -0.9, 0.6, 1.2, -1.1, 0.8, 1.0, 0.0, 0.2, -0.1, 0.5, -0.3, 0.4]),
+}
+
+
+def make_coreg_model(m: int, nt: int, nb: int, nv: int, seed: int, obs_per_node: int = 2,
+                     theta=None) -> CoregModelData:
+    """Seeded nv-variate coregional model on an m x m grid (shared mesh and time axis).
+
+    Hyper-priors are Gaussian with precision 0.5 around the generating theta
+    (lambdas around 0), as in `examples/gst_coreg2_small/run.py:80-200`.
+    """
+    if nv not in (2, 3):
+        raise ValueError("nv must be 2 or 3")
+    rng = np.random.default_rng(seed)
+    theta = np.array(COREG_THETA[nv] if theta is None else theta, dtype=np.float64)
+    base = make_st_model(m, nt, 1, seed, obs_per_node=1)
+    ns = m * m
+    a_st, x_cov, y = [], [], []
+    pts, _ = grid_triangles(m)
+    fields = []
+    for i in range(nv):
+        kx = rng.integers(1, 4, 6)
+        ky = rng.integers(1, 4, 6)
+        kt = rng.uniform(0.0, 6.0 * math.pi / max(nt, 2), 6)
+        ph = rng.uniform(0.0, 2.0 * math.pi, 6)
+        amp = rng.standard_normal(6) / math.sqrt(6)
+        f = np.zeros((nt, ns))
+        tn = np.arange(nt, dtype=np.float64)[:, None]
+        for k in range(6):
+            f += amp[k] * np.cos(math.pi * (kx[k] * pts[:, 0][None, :] + ky[k] * pts[:, 1][None, :]) + kt[k] * tn + ph[k])
+        fields.append(f.ravel())
+    for i in range(nv):
+        n_obs = obs_per_node * ns * nt
+        s_xy = rng.uniform(0.0, 1.0, size=(n_obs, 2))
+        t = rng.uniform(0.0, nt, size=n_obs)
+        A = design_matrix(m, nt, s_xy, t)
+        X = rng.standard_normal((n_obs, nb))
+        beta = rng.standard_normal(nb)
+        x = fields[i] + (0.5 * fields[0] if i > 0 else 0.0)
+        prec = theta[3 * i + 2]
+        yi = A @ x + X @ beta + rng.standard_normal(n_obs) * math.exp(-0.5 * prec)
+        a_st.append(A)
+        x_cov.append(X)
+        y.append(yi)
+    priors = [PriorSpec("gaussian", mean=float(v), precision=0.5) for v in theta]
+    n_lam = nv * (nv - 1) // 2
+    for k in range(n_lam):
+        priors[-1 - k] = PriorSpec("gaussian", mean=0.0, precision=0.5)
+    return CoregModelData(nv=nv, ns=ns, nt=nt, nb=nb, c0=base.c0, g1=base.g1, g2=base.g2, g3=base.g3,
+                          m0=base.m0, m1=base.m1, m2=base.m2, a_st=a_st, x_cov=x_cov, y=y, theta=theta,
+                          priors=priors)
+
+
+#: Coregional workloads: (grid side m, nt, nb per variate, nv, seed).
+COREG_WORKLOADS = {
+    "coreg2_ns576_nt24_nb4": (24, 24, 4, 2, 3),
+    "coreg3_ns1600_nt128_nb3": (40, 128, 3, 3, 4),
+}

@@ -97,6 +97,23 @@ long long dh_memory_bytes(const dh_handle* h);
 int dh_assemble(dh_handle* h, const dh_model* m, const double* params,
                 int sparsity, int with_likelihood);
 
+/* Coregional assembly (nv = 2 or 3 variates on one mesh): the handle holds
+ * b = nv*ns, a = nv*nb.  m[0..nv-1] carry each variate's likelihood parts
+ * (ata_*, fe_prec); the FEM and temporal parts are shared and read from m[0].
+ * Diagonal/sub-diagonal block rows are variate-major ([variate 0 nodes,
+ * variate 1 nodes, ...]), the arrowhead holds the fixed effects of all
+ * variates in order.  params: DEVICE array, C_ij[3][4] for every variate pair
+ * (i, j) row-major (nv*nv*12 doubles: q_ij = sum_k sum_m C_ij[k][m] M_k kron G_m),
+ * then exp(prec_o_i) for i < nv.  Replaces CoregionalModel.construct_Q_prior /
+ * construct_Q_conditional with its permutations (models/coregional_model.py:
+ * 279-551, 595-639, 849-915). */
+int dh_assemble_coreg(dh_handle* h, const dh_model* m, int nv, const double* params, int sparsity,
+                      int with_likelihood);
+/* y[p] = s_dev[idx[p]] * x[p] (device arrays): the coregional information
+ * vector A^T D y from the per-variate A_i^T y_i.  Part of
+ * CoregionalModel.construct_information_vector (coregional_model.py:641-680). */
+int dh_scale_indexed(dh_handle* h, const double* x, const int* idx, const double* s_dev, double* y, long long len);
+
 /* Fill the blocks from a symmetric sparse matrix of order n*b (+ a) given as
  * COO with global 0-based indices (device arrays, no duplicates): diagonal
  * blocks take both triangles, then the sub-diagonal blocks, the arrow rows

@@ -1,0 +1,54 @@
+"""GPU: coregional f(theta) (nv = 2, 3) through the HIP path vs the oracle, and batching."""
+
+import numpy as np
+import pytest
+
+from dalia_amd import synth
+from oracle import coreg as ocoreg
+from oracle import inla as oinla
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape", [(4, 3, 2, 2, 1), (5, 4, 1, 3, 2), (6, 5, 2, 2, 3), (4, 6, 3, 3, 4)])
+def test_coreg_f_matches_oracle(shape):
+    from dalia_amd.coreg import CoregInla
+    d = synth.make_coreg_model(*shape)
+    print(f"case coreg nv={d.nv} ns={d.ns} nt={d.nt} nb={d.nb}", flush=True)
+    eng = CoregInla(d)
+    rng = np.random.default_rng(7)
+    for k in range(3):
+        th = d.theta + (0.0 if k == 0 else 0.1 * rng.standard_normal(d.theta.size))
+        f = eng.f(th)
+        ref = ocoreg.f_theta_coreg(d, th)
+        assert abs(f - ref) <= 1e-10 * abs(ref), (k, f, ref)
+
+
+def test_coreg_fd_gradient_matches_oracle():
+    from dalia_amd.coreg import CoregInla
+    d = synth.make_coreg_model(4, 3, 2, 2, 5)
+    eng = CoregInla(d)
+    f, g = eng.fd_gradient(d.theta, h=1e-4)
+    ref = oinla.fd_gradient(d, d.theta, 1e-4, fun=lambda th: ocoreg.f_theta_coreg(d, th))
+    assert abs(f - ref[0]) <= 1e-10 * abs(ref[0])
+    # both sides difference their own f (agreeing to ~1e-12 relative) over 2h = 2e-4
+    assert np.abs(g - ref[1]).max() <= 1e-4 * max(1.0, np.abs(ref[1]).max())
+
+
+def test_coreg_and_univariate_batched_in_one_graph():
+    from dalia_amd.batch import BatchedF
+    from dalia_amd.coreg import CoregInla
+    from dalia_amd.inla import STInla
+    dc = synth.make_coreg_model(5, 4, 2, 2, 6)
+    du = synth.make_st_model(6, 4, 3, 1)
+    engines = [CoregInla(dc), STInla(du)]
+    assignment = [0, 1, 0, 1, 0]
+    bf = BatchedF(engines, assignment)
+    rng = np.random.default_rng(3)
+    for rep in range(2):
+        thetas = [(dc.theta if a == 0 else du.theta) + 0.05 * rng.standard_normal(9 if a == 0 else 4)
+                  for a in assignment]
+        fv = bf(thetas)
+        for e, a in enumerate(assignment):
+            ref = ocoreg.f_theta_coreg(dc, thetas[e]) if a == 0 else oinla.f_theta(du, thetas[e])
+            assert abs(fv[e] - ref) <= 1e-10 * abs(ref), (rep, e)
