@@ -29,7 +29,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = ("∇f(θ) evals/s adjoint vs central-diff, ns=2025 nt=100 nb=6 FP64; pobtaf TFLOP/s")
+def metric_name(ns: int, nt: int, nb: int) -> str:
+    return f"∇f(θ) evals/s adjoint vs central-diff, ns={ns} nt={nt} nb={nb} FP64; pobtaf TFLOP/s"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
@@ -218,7 +219,7 @@ def main():
         log(f"CPU baseline {cpu['value']:.3g} grad evals/s")
 
     out = {
-        "metric": METRIC,
+        "metric": metric_name(ns, nt, nb),
         "value": 1.0 / t_adj,
         "unit": "grad_evals/s",
         "n_gpus": 1,

@@ -1004,9 +1004,20 @@ struct SelArgs {
   int bp, nbt, wld;
 };
 
-// WT = inv(L_ii)^T L_{i+1,i}^T (tiles I, J; k >= 64 I) and Wa = L_{a,i} inv(L_ii) (k >= 64 J).
-__global__ __launch_bounds__(256) void k_sel_w(SelArgs a) {
+#define DH_SELG 8  // blocks whose factor-only products are batched in one launch
+
+struct SelBatch {
+  SelArgs a[DH_SELG];
+  int cnt;
+};
+
+// WT = inv(L_ii)^T L_{i+1,i}^T (tiles I, J; k >= 64 I) and Wa = L_{a,i} inv(L_ii) (k >= 64 J)
+// for block slot blockIdx.y: products of the factor alone, off the recursion's critical chain.
+__global__ __launch_bounds__(256) void k_sel_w(SelBatch sb) {
+  if ((int)blockIdx.y >= sb.cnt) return;
+  const SelArgs& a = sb.a[blockIdx.y];
   const int nw = a.Lb ? a.nbt * a.nbt : 0;
+  if ((int)blockIdx.x >= nw + (a.R ? a.nbt : 0)) return;
   const Opnd linv{a.D, a.bp, M_UPK_ROW, a.dinv};  // opM[r][k] = inv(L)[k][r]
   dh_d4 acc[2][2];
   dh_zero(acc);
@@ -1053,28 +1064,37 @@ __global__ __launch_bounds__(256) void k_sel_x(SelArgs a) {
   dh_store(a.R, a.bp, 0, e * DH_TILE, acc, -1.0, 0.0, false);
 }
 
-// Lower tiles of X_ii = inv(L)^T inv(L) - W^T X_{i+1,i} - Wa^T X_{a,i} into S.
-__global__ __launch_bounds__(256) void k_sel_s(SelArgs a) {
+// Lower tiles of inv(L_ii)^T inv(L_ii) into S, for block slot blockIdx.y (factor only).
+__global__ __launch_bounds__(256) void k_sel_ll(SelBatch sb) {
+  if ((int)blockIdx.y >= sb.cnt) return;
+  const SelArgs& a = sb.a[blockIdx.y];
   const int e = blockIdx.x;
   int I = 0;
   while ((I + 1) * (I + 2) / 2 <= e) ++I;
   const int J = e - I * (I + 1) / 2;
   const Opnd linv{a.D, a.bp, M_UPK_ROW, a.dinv};
-  dh_d4 acc[2][2], acc2[2][2];
+  dh_d4 acc[2][2];
   dh_zero(acc);
-  dh_zero(acc2);
   dh_mma<M_UPK_ROW, M_UPK_ROW>(acc, linv, I * DH_TILE, linv, J * DH_TILE, I * DH_TILE, a.bp);
+  dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
+}
+
+// Lower tiles of X_ii = inv(L)^T inv(L) - W^T X_{i+1,i} - Wa^T X_{a,i}: S (holding the first
+// term) -= the other two.
+__global__ __launch_bounds__(256) void k_sel_s(SelArgs a) {
+  const int e = blockIdx.x;
+  int I = 0;
+  while ((I + 1) * (I + 2) / 2 <= e) ++I;
+  const int J = e - I * (I + 1) / 2;
+  dh_d4 acc2[2][2];
+  dh_zero(acc2);
   if (a.Lb)
     dh_mma<M_ROW, M_COL>(acc2, Opnd{a.WT, a.wld, M_ROW, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
                          J * DH_TILE, 0, a.bp);
   if (a.R)
     dh_mma<M_COL, M_COL>(acc2, Opnd{a.Wa, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.R, a.bp, M_COL, nullptr},
                          J * DH_TILE, 0, DH_TILE);
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int y = 0; y < 2; ++y) acc[x][y] -= acc2[x][y];
-  dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
+  dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc2, -1.0, 1.0, false);
 }
 
 // D[r][c] = S[max(r,c)][min(r,c)] (lower tiles of S mirrored).
@@ -1687,9 +1707,9 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   rc |= dalloc(h, &h->pinv, 64 * 64);
   rc |= dalloc(h, &h->tipinv, 64 * 64);
   rc |= dalloc(h, &h->tipdinv, 64);
-  rc |= dalloc(h, &h->W, h->G * bb);
-  rc |= dalloc(h, &h->S, bb);
-  rc |= dalloc(h, &h->Wa, (long long)64 * h->bp);
+  rc |= dalloc(h, &h->W, h->G * (long long)h->bp * (h->bp + 8));  // pobtasi W^T slots (padded pitch)
+  rc |= dalloc(h, &h->S, h->G * bb);
+  rc |= dalloc(h, &h->Wa, h->G * 64LL * h->bp);
   rc |= dalloc(h, &h->z, (long long)n * h->bp + 64);
   rc |= dalloc(h, &h->v, h->bp);
   h->part_len = (long long)(2 * n) * ((h->bp + 255) / 256 + 1) * 4 + 4096 +
@@ -2087,33 +2107,43 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
     gemm64<M_COL, M_COL>(h, gargs(Opnd{h->tipinv, 64, M_COL, nullptr}, Opnd{h->tipinv, 64, M_COL, nullptr}, 64,
                                   KLO_0, KHI_K, 1.0, 0.0, h->tip, 64, 1, 1, 0));
   }
-  for (int i = n - 1; i >= 0; --i) {
-    double* D = h->diag + i * bb;
-    SelArgs sa;
-    sa.D = D;
-    sa.dinv = h->dinv + (long)i * bp;
-    sa.Lb = (i < n - 1) ? h->lower + i * bb : nullptr;
-    sa.R = arrow ? h->arrow + (long)i * 64 * bp : nullptr;
-    sa.Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
-    sa.Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * 64 * bp : nullptr;
-    sa.tip = h->tip;
-    // W^T with a padded row pitch: rows of a power-of-two pitch read row-wise by one tile share
-    // memory channels (the product reading it by rows runs about 30% slower)
-    sa.WT = W;
-    sa.Wa = h->Wa;
-    sa.S = S;
-    sa.bp = bp;
-    sa.nbt = nbt;
-    sa.wld = bp + 8;
-    const int nw = (sa.Lb ? nbt * nbt : 0) + (sa.R ? nbt : 0);
-    // 1. W = L_{i+1,i} inv(L_ii) (as W^T), Wa = L_{a,i} inv(L_ii)
-    if (nw > 0) DH_LAUNCH(h, 2, k_sel_w, dim3(nw), dim3(256), 0, sa);
-    // 2. X_{i+1,i}, X_{a,i}
-    if (nw > 0) DH_LAUNCH(h, 2, k_sel_x, dim3(nw), dim3(256), 0, sa);
-    // 3. X_ii (lower tiles, into S)
-    DH_LAUNCH(h, 2, k_sel_s, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, sa);
-    // 4. D_i = symmetric X_ii
-    hipLaunchKernelGGL(k_symm_copy, dim3((unsigned)((bb + 255) / 256)), dim3(256), 0, h->stream, D, S, bp);
+  const int wld = bp + 8;  // W^T pitch: rows of a power-of-two pitch read row-wise by one tile share
+                           // memory channels (the product reading them by rows runs ~30% slower)
+  for (int hi = n - 1; hi >= 0; hi -= h->G) {
+    const int cnt = std::min(h->G, hi + 1);
+    SelBatch sb;
+    memset(&sb, 0, sizeof(sb));
+    sb.cnt = cnt;
+    for (int g = 0; g < cnt; ++g) {
+      const int i = hi - g;
+      SelArgs& sa = sb.a[g];
+      sa.D = h->diag + i * bb;
+      sa.dinv = h->dinv + (long)i * bp;
+      sa.Lb = (i < n - 1) ? h->lower + i * bb : nullptr;
+      sa.R = arrow ? h->arrow + (long)i * 64 * bp : nullptr;
+      sa.Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
+      sa.Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * 64 * bp : nullptr;
+      sa.tip = h->tip;
+      sa.WT = W + (long)g * bp * wld;
+      sa.Wa = h->Wa + (long)g * 64 * bp;
+      sa.S = S + (long)g * bb;
+      sa.bp = bp;
+      sa.nbt = nbt;
+      sa.wld = wld;
+    }
+    // 1. products of the factor alone for the whole chunk: W (as W^T), Wa, inv(L)^T inv(L)
+    const int nw = nbt * nbt + (arrow ? nbt : 0);
+    DH_LAUNCH(h, 2, k_sel_w, dim3(nw, cnt), dim3(256), 0, sb);
+    DH_LAUNCH(h, 2, k_sel_ll, dim3(nbt * (nbt + 1) / 2, cnt), dim3(256), 0, sb);
+    // 2. the recursion, block by block
+    for (int g = 0; g < cnt; ++g) {
+      const SelArgs& sa = sb.a[g];
+      const int nx = (sa.Lb ? nbt * nbt : 0) + (sa.R ? nbt : 0);
+      if (nx > 0) DH_LAUNCH(h, 2, k_sel_x, dim3(nx), dim3(256), 0, sa);  // X_{i+1,i}, X_{a,i}
+      DH_LAUNCH(h, 2, k_sel_s, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, sa);  // X_ii lower tiles
+      hipLaunchKernelGGL(k_symm_copy, dim3((unsigned)((bb + 255) / 256)), dim3(256), 0, h->stream,
+                         h->diag + (long)(hi - g) * bb, (const double*)sa.S, bp);
+    }
   }
   HIPCHK(hipGetLastError());
   h->factored = -1;

@@ -249,6 +249,8 @@ WORKLOADS = {
     "st_ns400_nt12_nb4": (20, 12, 4, 0),
     # plain drop-in and adjoint case: 45x45 grid (ns=2025), nt=100, nb=6, seed 1
     "st_ns2025_nt100_nb6": (45, 100, 6, 1),
+    # large adjoint case: 64x64 grid (ns=4096), nt=250, nb=6, seed 1 (67 GB of BTA blocks)
+    "st_ns4096_nt250_nb6": (64, 250, 6, 1),
 }
 
 
