@@ -69,6 +69,20 @@ def update_flops_per_launch(n, b, a):
     return tot / max(cnt, 1), cnt
 
 
+def committed_traffic(kname: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of this kernel and workload
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a copy-kernel calibration), or None."""
+    tag = {"st_ns2025_nt100_nb6": "ns2025_nt100"}.get(workload)
+    path = os.path.join(ROOT, "profiles", f"{kname}_{tag}_traffic.md")
+    if tag is None or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(workload_args, n_full: int, threads: int):
     """Oracle f(theta) on a bounded sample (same ns, nb; nt_sample time steps), scaled to n_full.
 
@@ -110,6 +124,8 @@ def main():
     ap.add_argument("--fd", type=int, default=1, help="central-difference gradients to time (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-kernel", default="k_syrk")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 2 GiB device-to-device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -117,6 +133,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        n = 1 << 28  # doubles: 2 GiB read + 2 GiB written by torch's elementwise copy kernel
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {n * 8 / 2 ** 30:.1f} GiB three times")
+        return
 
     from dalia_amd import _lib, synth
     from dalia_amd.inla import STInla
@@ -207,7 +233,8 @@ def main():
         per = float("nan")
     achieved = per / avg_s / 1e12
     roofline = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": committed_traffic(kname, args.workload),
                 "launches": int(cnt.value), "avg_launch_us": avg_s * 1e6, "flops_per_launch": per}
     log(f"roofline {kname}: {cnt.value} launches, avg {avg_s * 1e6:.1f} us, {achieved:.2f} TFLOP/s")
 
