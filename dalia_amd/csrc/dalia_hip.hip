@@ -1,15 +1,20 @@
 // MI355X (gfx950) inner solve of one f(theta) evaluation on the BTA precision.
 //
 // Kernels (all FP64):
-//   assembly      k_assemble_rows / k_assemble_arrow / k_assemble_tip
-//   pobtaf/pobtf  k_leaf, k_panel, k_update, k_syrk  (right-looking tall-panel
-//                 block Cholesky, 64-wide panels, FP64 MFMA tiles)
+//   assembly      k_assemble_rows / _arrow / _tip (univariate), k_assemble_*_coreg
+//                 (2-3 variates, variate-major time blocks), k_fill_bta + k_scatter_coo
+//   pobtaf/pobtf  k_pobtaf_dag: one persistent launch of left-looking 64x64 tile tasks
+//                 with per-tile readiness flags; k_leaf for the tip.  The per-panel
+//                 schedule (k_panel, k_update, k_syrk, k_syrk128 + k_syrk_tail) is the
+//                 autotuned alternative.
 //   inverse       k_trtri_t, k_trtri_inv (batched recursive inverse of L_ii,
 //                 packed into the strict upper triangle of each diagonal block)
 //   pobtas        k_pack/k_unpack, k_gemv_row, k_gemv_col_part, k_fin_*, k_upk_*
-//   pobtasi       k_gemm (generic MFMA tile GEMM, up to 3 terms) + k_symm_copy
+//   pobtasi       k_sel_w, k_sel_ll (factor-only products, 8 blocks per launch),
+//                 k_sel_x, k_sel_s (the recursion), k_symm_copy
 //   logdet        k_logdet_part / k_logdet_fin
 //   adjoint       k_qp_matvec, k_ata_matvec*, k_contract_*, k_prior_eig
+//   variances     k_diag_extract, k_obs_variances
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
