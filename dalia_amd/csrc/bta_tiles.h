@@ -78,54 +78,52 @@ __device__ __forceinline__ void dh_zero(dh_d4 acc[2][2]) {
 }
 
 // acc += opA[i0:i0+64, klo:khi] * opB[klo:khi, j0:j0+64] for this wave's quadrant.
-// k advances in chunks of 32 (klo, khi multiples of 32); the loads of the
-// next chunk are issued before the 32 MFMAs of the current one.
+// k advances in chunks of 16 (klo, khi multiples of 16) held in a ring of
+// DH_MMA_STAGES register buffers: the loads of chunk q + DH_MMA_STAGES - 1
+// are issued before the 16 MFMAs of chunk q, so that about three chunks
+// (3 x 1024 MFMA cycles) of load latency are covered.
+#ifndef DH_MMA_STAGES
+#define DH_MMA_STAGES 4
+#endif
 template <int MA, int MB>
-__device__ __forceinline__ void dh_load_chunk(const Opnd& A, const Opnd& B, int ra, int rb, int k,
-                                              double a[2][2][4], double b[2][2][4]) {
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    dh_load4<MA>(A, ra, k + 16 * h, a[h][0]);
-    dh_load4<MA>(A, ra + 16, k + 16 * h, a[h][1]);
-    dh_load4<MB>(B, rb, k + 16 * h, b[h][0]);
-    dh_load4<MB>(B, rb + 16, k + 16 * h, b[h][1]);
-  }
+__device__ __forceinline__ void dh_load16(const Opnd& A, const Opnd& B, int ra, int rb, int k, double a[2][4],
+                                          double b[2][4]) {
+  dh_load4<MA>(A, ra, k, a[0]);
+  dh_load4<MA>(A, ra + 16, k, a[1]);
+  dh_load4<MB>(B, rb, k, b[0]);
+  dh_load4<MB>(B, rb + 16, k, b[1]);
 }
 
 template <int MA, int MB>
 __device__ __forceinline__ void dh_mma(dh_d4 acc[2][2], const Opnd& A, int i0, const Opnd& B, int j0,
                                        int klo, int khi) {
+  constexpr int S = DH_MMA_STAGES;
   if (klo >= khi) return;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int ra = i0 + (w >> 1) * 32 + (lane & 15);
   const int rb = j0 + (w & 1) * 32 + (lane & 15);
   const int kq = (lane >> 4) * 2;  // group base offset within a 16-chunk
-  double a[2][2][4], b[2][2][4];
-  dh_load_chunk<MA, MB>(A, B, ra, rb, klo + kq, a, b);
-  for (int k = klo; k < khi; k += 32) {
-    double na[2][2][4], nb[2][2][4];
-    const bool more = k + 32 < khi;
-    if (more) dh_load_chunk<MA, MB>(A, B, ra, rb, k + 32 + kq, na, nb);
+  const int nck = (khi - klo) >> 4;
+  double a[S][2][4], b[S][2][4];
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+  for (int s = 0; s < S - 1; ++s)
+    if (s < nck) dh_load16<MA, MB>(A, B, ra, rb, klo + 16 * s + kq, a[s], b[s]);
+  for (int q = 0; q < nck; q += S) {
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        acc[0][0] = dh_mfma(a[h][0][m], b[h][0][m], acc[0][0]);
-        acc[0][1] = dh_mfma(a[h][0][m], b[h][1][m], acc[0][1]);
-        acc[1][0] = dh_mfma(a[h][1][m], b[h][0][m], acc[1][0]);
-        acc[1][1] = dh_mfma(a[h][1][m], b[h][1][m], acc[1][1]);
+    for (int s = 0; s < S; ++s) {
+      const int ck = q + s;
+      if (ck < nck) {
+        if (ck + S - 1 < nck)
+          dh_load16<MA, MB>(A, B, ra, rb, klo + 16 * (ck + S - 1) + kq, a[(s + S - 1) % S], b[(s + S - 1) % S]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          acc[0][0] = dh_mfma(a[s][0][m], b[s][0][m], acc[0][0]);
+          acc[0][1] = dh_mfma(a[s][0][m], b[s][1][m], acc[0][1]);
+          acc[1][0] = dh_mfma(a[s][1][m], b[s][0][m], acc[1][0]);
+          acc[1][1] = dh_mfma(a[s][1][m], b[s][1][m], acc[1][1]);
+        }
       }
-    if (more) {
-#pragma unroll
-      for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            a[h][x][m] = na[h][x][m];
-            b[h][x][m] = nb[h][x][m];
-          }
     }
   }
 }

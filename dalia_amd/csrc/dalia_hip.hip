@@ -615,14 +615,20 @@ __global__ __launch_bounds__(256) void k_syrk_tail(BtaPtrs P, const double* Pbuf
 // previous block's sub-diagonal tile (0, nbt-1).  Each helper is split so
 // that it is never late: OLD_D/OLD_L(c) (the previous block's terms and own
 // terms k < c - look) are queued look columns ahead, PRE_D/PRE_L(c) (own
-// terms c - look <= k < c - 1) one column ahead.
+// terms c - look <= k < c - 1) one column ahead.  The other tiles of column
+// c are split the same way: an early task queued slook columns ahead
+// (OLD_X for diagonal-block tiles, OLD_LW for sub-diagonal tiles, OLD_A for
+// the arrow tile) applies the previous block's terms and the own terms
+// k < c - slook, so that the task of column c only adds the last slook terms
+// and solves; without it the tiles needed two columns later by the chain
+// are still accumulating when their column is factored.
 // Tasks are handed out by an atomic counter in block-major order (a decode
 // table per block shape), and every task only waits for tasks with smaller
 // ids, so the oldest unfinished task can always proceed and the persistent
 // grid drains; the chain overlaps the bulk of the products of later columns
 // and of the next block.
 enum { DK_PRE_D = 0, DK_PRE_L = 1, DK_CHAIN = 2, DK_DIAG = 3, DK_LOWER = 4, DK_ARROW = 5, DK_TIP = 6,
-       DK_OLD_D = 7, DK_OLD_L = 8 };
+       DK_OLD_D = 7, DK_OLD_L = 8, DK_OLD_X = 9, DK_OLD_LW = 10, DK_OLD_A = 11 };
 
 struct DagArgs {
   double *diag, *lower, *arrow, *tip, *dinv;
@@ -636,14 +642,17 @@ struct DagArgs {
   int tfull;       // tasks of blocks 0..n-2 (the last block has no sub-diagonal tiles)
   int ntasks;
   int look;        // helper lookahead in tile columns (>= 1)
+  int slook;       // early-task lookahead of the other tiles (>= 1)
 };
 
 #define DH_DAG_SPIN (1 << 20)
 #define DH_DAG_LOOK 6
+#define DH_DAG_SLOOK 6
 
 // Flags of one block: final D tiles [nbt][nbt], final L tiles [nbt][nbt],
-// final arrow tiles [nbt], tip partial applied, PRE_D, PRE_L, OLD_D, OLD_L done [nbt each].
-__device__ __host__ __forceinline__ int dag_fstride(int nbt) { return 2 * nbt * nbt + 5 * nbt + 1; }
+// final arrow tiles [nbt], tip partial applied, PRE_D, PRE_L, OLD_D, OLD_L done [nbt each],
+// OLD_X [nbt][nbt], OLD_LW [nbt][nbt], OLD_A [nbt] done.
+__device__ __host__ __forceinline__ int dag_fstride(int nbt) { return 4 * nbt * nbt + 6 * nbt + 1; }
 
 // Number (1..64) of consecutive k in [k, kend) whose tiles are final in both
 // flag rows (fb may be null), after waiting until there is at least one.
@@ -763,7 +772,11 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
   int* fPL = fPD + nbt;
   int* fOD = fPL + nbt;
   int* fOL = fOD + nbt;
+  int* fX = fOL + nbt;        // OLD_X(r, c) at r * nbt + c
+  int* fLW = fX + nbt * nbt;  // OLD_LW(r, c)
+  int* fXA = fLW + nbt * nbt; // OLD_A(c)
   const int kold = max(0, r - g.look);  // helpers of tile row r: own terms below kold are OLD's
+  const int ks = max(0, c - g.slook);   // other tiles of column c: own terms below ks are the early task's
   const int* fLp = fL - FS;  // block i-1
   const int* fAp = fA - FS;
   const int* fTp = fT - FS;
@@ -837,21 +850,39 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
     __syncthreads();
     dh_leaf64_lds(C, bp, dinv + c * DH_TILE, pv, g.info, i * g.b + c * DH_TILE);
     fdone = fD + c * nbt + c;
+  } else if (kind == DK_OLD_X) {  // diagonal-block tile (r, c): previous block's terms, own terms k < ks
+    if (i > 0)
+      dag_accumulate(acc, Lp, r * DH_TILE, Lp, c * DH_TILE, bp, fLp + r * nbt, fLp + c * nbt, nbt, g.err, s_n);
+    dag_accumulate(acc, D, r * DH_TILE, D, c * DH_TILE, bp, fD + r * nbt, fD + c * nbt, ks, g.err, s_n);
+    if (i > 0 || ks > 0) dh_store(D + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    fdone = fX + r * nbt + c;
+  } else if (kind == DK_OLD_LW) {  // sub-diagonal tile (r, c): own terms k < ks
+    dag_accumulate(acc, Lo, r * DH_TILE, D, c * DH_TILE, bp, fL + r * nbt, fD + c * nbt, ks, g.err, s_n);
+    dh_store(Lo + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    fdone = fLW + r * nbt + c;
+  } else if (kind == DK_OLD_A) {  // arrow tile c: previous block's terms, own terms k < ks
+    if (i > 0) dag_accumulate(acc, Ap, 0, Lp, c * DH_TILE, bp, fAp, fLp + c * nbt, nbt, g.err, s_n);
+    dag_accumulate(acc, Ar, 0, D, c * DH_TILE, bp, fA, fD + c * nbt, ks, g.err, s_n);
+    if (i > 0 || ks > 0) dh_store(Ar + c * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    fdone = fXA + c;
   } else {
+    // the early task of the tile exists for every diagonal-block and arrow
+    // tile, and for sub-diagonal tiles with c > slook
     double* C;
     if (kind == DK_DIAG) {  // tile (r, c), r >= c + 2
-      if (i > 0)
-        dag_accumulate(acc, Lp, r * DH_TILE, Lp, c * DH_TILE, bp, fLp + r * nbt, fLp + c * nbt, nbt, g.err, s_n);
-      dag_accumulate(acc, D, r * DH_TILE, D, c * DH_TILE, bp, fD + r * nbt, fD + c * nbt, c, g.err, s_n);
+      dag_ready(fX + r * nbt + c, nullptr, 0, 1, g.err, s_n);
+      dag_accumulate_from(acc, D, r * DH_TILE, D, c * DH_TILE, bp, fD + r * nbt, fD + c * nbt, ks, c, g.err, s_n);
       C = D + r * T64 + c * DH_TILE;
       fdone = fD + r * nbt + c;
     } else if (kind == DK_LOWER) {  // sub-diagonal block tile (r, c)
-      dag_accumulate(acc, Lo, r * DH_TILE, D, c * DH_TILE, bp, fL + r * nbt, fD + c * nbt, c, g.err, s_n);
+      const int k0 = c > g.slook ? ks : 0;
+      if (c > g.slook) dag_ready(fLW + r * nbt + c, nullptr, 0, 1, g.err, s_n);
+      dag_accumulate_from(acc, Lo, r * DH_TILE, D, c * DH_TILE, bp, fL + r * nbt, fD + c * nbt, k0, c, g.err, s_n);
       C = Lo + r * T64 + c * DH_TILE;
       fdone = fL + r * nbt + c;
     } else {  // arrow tile c
-      if (i > 0) dag_accumulate(acc, Ap, 0, Lp, c * DH_TILE, bp, fAp, fLp + c * nbt, nbt, g.err, s_n);
-      dag_accumulate(acc, Ar, 0, D, c * DH_TILE, bp, fA, fD + c * nbt, c, g.err, s_n);
+      dag_ready(fXA + c, nullptr, 0, 1, g.err, s_n);
+      dag_accumulate_from(acc, Ar, 0, D, c * DH_TILE, bp, fA, fD + c * nbt, ks, c, g.err, s_n);
       C = Ar + c * DH_TILE;
       fdone = fA + c;
     }
@@ -1585,7 +1616,7 @@ struct dh_handle {
   long long bytes;
   int syrk_variant;              // 0: 64-tiles, 1: 128-tiles with split k (needs nbt even)
   int pobtaf_alg;                // 0: one launch per panel, 1: persistent tile-task launch
-  int* dagflags;                 // n * (2 nbt^2 + nbt + 1) tile readiness flags
+  int* dagflags;                 // n * dag_fstride(nbt) tile readiness flags
   int* dagcounter;               // task counter of the persistent launch
   int* dagtab;                   // decode tables of the persistent launch (bta and bt schedules)
   double* dagpinv;               // dense inverses of the diagonal tiles, [2][nbt][64][64]
@@ -1628,17 +1659,27 @@ static void build_nodes(int o, int s, int d, std::vector<NodeList>& lv) {
 }
 
 // Task order of one block of the persistent pobtaf (nl sub-diagonal tile
-// rows: nbt, or 0 for the last block): OLD_D(0), OLD_D/OLD_L(1..look), then
-// per tile column c: CHAIN(c), OLD_D/OLD_L(c+look), PRE_D/PRE_L(c+1), the
-// diagonal-block tiles below (c+1, c), the sub-diagonal tiles and the arrow
-// tile of column c; then the tip partial.
-static void dag_table(int nbt, int nl, int arrow, int look, std::vector<int>& tab) {
+// rows: nbt, or 0 for the last block): OLD_D(0), OLD_D/OLD_L(1..look), the
+// early tasks of columns 0..slook-1, then per tile column c: CHAIN(c),
+// OLD_D/OLD_L(c+look), PRE_D/PRE_L(c+1), the early tasks of column
+// c+slook, the diagonal-block tiles below (c+1, c), the sub-diagonal tiles
+// and the arrow tile of column c; then the tip partial.
+static void dag_early(int nbt, int nl, int arrow, int slook, int c, std::vector<int>& tab) {
+  auto enc = [](int kind, int r, int c) { return (kind << 24) | (r << 12) | c; };
+  for (int r = c + 2; r < nbt; ++r) tab.push_back(enc(DK_OLD_X, r, c));
+  if (c > slook)
+    for (int r = 0; r < nl; ++r) tab.push_back(enc(DK_OLD_LW, r, c));
+  if (arrow) tab.push_back(enc(DK_OLD_A, 0, c));
+}
+
+static void dag_table(int nbt, int nl, int arrow, int look, int slook, std::vector<int>& tab) {
   auto enc = [](int kind, int r, int c) { return (kind << 24) | (r << 12) | c; };
   tab.push_back(enc(DK_OLD_D, 0, 0));
   for (int r = 1; r <= look && r < nbt; ++r) {
     tab.push_back(enc(DK_OLD_D, r, r));
     tab.push_back(enc(DK_OLD_L, r, r - 1));
   }
+  for (int c = 0; c < slook && c < nbt; ++c) dag_early(nbt, nl, arrow, slook, c, tab);
   for (int c = 0; c < nbt; ++c) {
     tab.push_back(enc(DK_CHAIN, c, c));
     if (c >= 1 && c + look < nbt) {
@@ -1649,6 +1690,7 @@ static void dag_table(int nbt, int nl, int arrow, int look, std::vector<int>& ta
       tab.push_back(enc(DK_PRE_D, c + 1, c + 1));
       tab.push_back(enc(DK_PRE_L, c + 1, c));
     }
+    if (c + slook < nbt) dag_early(nbt, nl, arrow, slook, c + slook, tab);
     for (int r = c + 2; r < nbt; ++r) tab.push_back(enc(DK_DIAG, r, c));
     for (int r = 0; r < nl; ++r) tab.push_back(enc(DK_LOWER, r, c));
     if (arrow) tab.push_back(enc(DK_ARROW, 0, c));
@@ -1734,7 +1776,7 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
     for (int ar = 0; ar < 2; ++ar)
       for (int last = 0; last < 2; ++last) {
         const size_t before = tab.size();
-        dag_table(h->nbt, last ? 0 : h->nbt, ar, DH_DAG_LOOK, tab);
+        dag_table(h->nbt, last ? 0 : h->nbt, ar, DH_DAG_LOOK, DH_DAG_SLOOK, tab);
         h->dagt[ar][last] = (int)(tab.size() - before);
       }
     if (hipMalloc((void**)&h->dagtab, sizeof(int) * tab.size()) != hipSuccess ||
@@ -1898,6 +1940,7 @@ int dh_cholesky(dh_handle* h, int sparsity) {
     g.tfull = h->dagt[ar][0];
     g.ntasks = (n - 1) * h->dagt[ar][0] + h->dagt[ar][1];
     g.look = DH_DAG_LOOK;
+    g.slook = DH_DAG_SLOOK;
     const unsigned wgs = (unsigned)std::min<int>(g.ntasks, h->dag_wgs);
     DH_LAUNCH(h, 7, k_pobtaf_dag, dim3(wgs), dim3(256), 0, g);
     if (arrow)
