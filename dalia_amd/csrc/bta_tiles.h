@@ -285,25 +285,28 @@ __device__ __forceinline__ void dh_store(double* C, long ldc, int i0, int j0, co
 //
 // Blocked right-looking elimination, 4 columns per step.  Thread
 // (ty, tx) = (tid/16, tid%16) keeps the entries (ty+16u, tx+16v) in
-// registers.  Step c (columns K0 = 4c .. K0+3) publishes those 4 columns of
-// the trailing matrix S and rows K0..K0+3 of the working inverse W (L W = I
-// is solved alongside) through LDS; after a barrier every thread factors the
-// 4x4 pivot block P = L_B L_B^T and inverts it (V_B = inv(L_B)) redundantly,
-// and thread (j, b) = (tid/4, tid%4) computes one entry of
-//   G[j][b] = L[j][K0+b] = sum_m S[j][K0+m] V_B[b][m]     (j >= K0+4)
-//           = V[K0+b][j] = sum_m V_B[b][m] W[K0+m][j]     (j < K0; V_B for j in the block)
-// into LDS; after a second barrier every thread applies
+// registers.  Step c (columns K0 = 4c .. K0+3) publishes through LDS the 4x4
+// pivot block P of the trailing matrix S and, for every index j, the
+// 4-vector U[j][m] = S[j][K0+m] (j >= K0+4), delta(m, j-K0) (j in the block)
+// or W[K0+m][j] (j < K0), W being the working inverse (L W = I is solved
+// alongside).  After a barrier every thread factors P = L_B L_B^T and inverts
+// it (V_B = inv(L_B)) redundantly, and one thread per index j forms
+//   G[j][b] = sum_m U[j][m] V_B[b][m]
+// (= L[j][K0+b] below the block, V[K0+b][j] above and in it) in LDS; after a
+// second barrier every thread applies
 //   S[i][j] -= sum_b G[i][b] G[j][b]  (i, j >= K0+4),  W[i][j] -= sum_b G[i][b] G[j][b]  (j < K0+4)
 // to its own entries of the rows below the block, and the owners of rows
 // K0..K0+3 replace them by the final inverse rows V[K0+a][j] = G[j][a].
-// Entries above the diagonal carry unused values.  No global store is issued
-// inside the loop (a barrier waits for outstanding stores): L goes to LDS,
-// inv(L) stays in registers, and both are written once at the end.
+// FP64 VALU instructions cost about 9 cycles each per wave on gfx950, so the
+// step avoids per-lane selections and divergent paths.  Entries above the
+// diagonal carry unused values.  No global store is issued inside the loop
+// (a barrier waits for outstanding stores): L goes to LDS, inv(L) stays in
+// registers, and both are written once at the end.
 template <int NB>  // tile of 16*NB rows
 struct __align__(16) LeafShmT {
-  double colb[4][16 * NB];  // columns K0..K0+3 of S (rows >= K0)
-  double rowb[4][16 * NB];  // rows K0..K0+3 of W (columns < K0)
-  double G[16 * NB][4];     // column factors of the step
+  double U[16 * NB][4];  // column factor operands of the step
+  double G[16 * NB][4];  // column factors of the step
+  double Pb[4][4];       // pivot block
   double Ls[16 * NB][16 * NB + 1];
 };
 typedef LeafShmT<4> LeafShm;
@@ -316,46 +319,61 @@ __device__ __forceinline__ double dh_rsq_nr(double p) {
   return fma(r * e, fma(e, 0.375, 0.5), r);
 }
 
+__device__ __forceinline__ void dh_ld4(const double* q, double v[4]) {
+  const double2 x0 = reinterpret_cast<const double2*>(q)[0];
+  const double2 x1 = reinterpret_cast<const double2*>(q)[1];
+  v[0] = x0.x; v[1] = x0.y; v[2] = x1.x; v[3] = x1.y;
+}
+
+__device__ __forceinline__ void dh_st4(double* q, const double v[4]) {
+  reinterpret_cast<double2*>(q)[0] = double2{v[0], v[1]};
+  reinterpret_cast<double2*>(q)[1] = double2{v[2], v[3]};
+}
+
 template <int NB, int KV>
 __device__ __forceinline__ void dh_leaf_block(double R[NB][NB], LeafShmT<NB>& sh, int ty, int tx, int tid,
                                               int* info, int row_base, int c4) {
   const int kq = 4 * c4;
   const int K0 = 16 * KV + kq;
-  const bool colown = tx >= kq && tx < kq + 4;
-  const bool rowown = ty >= kq && ty < kq + 4;
   DH_LEAF_TS(0);
-  if (colown) {
+  if (tx >= kq && tx < kq + 4) {  // owners of columns K0..K0+3
+    const int m = tx - kq;
 #pragma unroll
     for (int u = KV; u < NB; ++u) {
-      sh.colb[tx - kq][ty + 16 * u] = R[u][KV];
-      if (u > KV || ty >= kq + 4) R[u][KV] = 0.0;  // becomes W[i][K0+jj] below the block
+      const int i = ty + 16 * u;
+      if (u > KV || ty >= kq + 4) {
+        sh.U[i][m] = R[u][KV];
+        R[u][KV] = 0.0;  // becomes W[i][K0+m] below the block
+      } else if (ty >= kq) {
+        sh.Pb[ty - kq][m] = R[u][KV];
+        sh.U[i][m] = (ty - kq == m) ? 1.0 : 0.0;
+      }
     }
   }
-  if (rowown) {
+  if (ty >= kq && ty < kq + 4) {  // owners of rows K0..K0+3: W rows left of the block
 #pragma unroll
     for (int v = 0; v <= KV; ++v)
-      if (v < KV || tx < kq) sh.rowb[ty - kq][tx + 16 * v] = R[KV][v];
+      if (v < KV || tx < kq) sh.U[tx + 16 * v][ty - kq] = R[KV][v];
   }
   __syncthreads();
   DH_LEAF_TS(1);
   // 4x4 pivot block: Cholesky and inverse (every thread)
   double P[4][4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b <= a; ++b) P[a][b] = sh.colb[b][K0 + a];
+  for (int a = 0; a < 4; ++a) dh_ld4(&sh.Pb[a][0], P[a]);
   double Lb[4][4], Vb[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) Lb[a][b] = Vb[a][b] = 0.0;
+  int bad = 4;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     double piv = P[k][k];
 #pragma unroll
     for (int m = 0; m < k; ++m) piv -= Lb[k][m] * Lb[k][m];
     const double r = dh_rsq_nr(piv);
-    if (tid == 0 && !(piv > 0.0)) atomicMin(info, 1 + row_base + K0 + k);
+    bad = (bad == 4 && !(piv > 0.0)) ? k : bad;
     Lb[k][k] = piv * r;
     Vb[k][k] = r;
 #pragma unroll
@@ -366,6 +384,7 @@ __device__ __forceinline__ void dh_leaf_block(double R[NB][NB], LeafShmT<NB>& sh
       Lb[a][k] = x * r;
     }
   }
+  if (tid == 0 && bad < 4) atomicMin(info, 1 + row_base + K0 + bad);
 #pragma unroll
   for (int a = 1; a < 4; ++a)
 #pragma unroll
@@ -376,72 +395,53 @@ __device__ __forceinline__ void dh_leaf_block(double R[NB][NB], LeafShmT<NB>& sh
       Vb[a][b] = -Vb[a][a] * x;
     }
   DH_LEAF_TS(2);
-  if (tid < 64 * NB) {  // one column factor per thread
-    const int j = tid >> 2, b = tid & 3;
-    double vb[4];  // row b of V_B
+  if (tid < 16 * NB) {  // one index j per thread
+    const int j = tid;
+    double c[4], g[4];
+    dh_ld4(&sh.U[j][0], c);
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
+    for (int b = 0; b < 4; ++b) {
       double x = 0.0;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) x = (a == b) ? Vb[a][m] : x;
-      vb[m] = x;
+      for (int m = 0; m <= b; ++m) x = fma(c[m], Vb[b][m], x);
+      g[b] = x;
     }
-    double g = 0.0;
+    dh_st4(&sh.G[j][0], g);
     if (j >= K0 + 4) {
 #pragma unroll
-      for (int m = 0; m < 4; ++m) g = fma(sh.colb[m][j], vb[m], g);
-      sh.Ls[j][K0 + b] = g;
-    } else if (j >= K0) {
-      const int jj = j - K0;
-#pragma unroll
-      for (int m = 0; m < 4; ++m) g = (m == jj) ? vb[m] : g;
-      double lv = 0.0;
+      for (int b = 0; b < 4; ++b) sh.Ls[j][K0 + b] = g[b];
+    }
+    if (tid == 0) {
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) lv = (a == jj && m == b) ? Lb[a][m] : lv;
-      sh.Ls[j][K0 + b] = lv;
-    } else {
-#pragma unroll
-      for (int m = 0; m < 4; ++m) g = fma(vb[m], sh.rowb[m][j], g);
+        for (int b = 0; b <= a; ++b) sh.Ls[K0 + a][K0 + b] = Lb[a][b];
     }
-    sh.G[j][b] = g;
   }
   __syncthreads();
   DH_LEAF_TS(3);
   double gj[NB][4];
 #pragma unroll
-  for (int v = 0; v < NB; ++v) {
-    const double2* q = reinterpret_cast<const double2*>(&sh.G[tx + 16 * v][0]);
-    const double2 g0 = q[0], g1 = q[1];
-    gj[v][0] = g0.x; gj[v][1] = g0.y; gj[v][2] = g1.x; gj[v][3] = g1.y;
-  }
+  for (int v = 0; v < NB; ++v) dh_ld4(&sh.G[tx + 16 * v][0], gj[v]);
   DH_LEAF_TS(4);
   // rank-4 updates of the rows below the block (lower and diagonal 16x16 sub-blocks only)
 #pragma unroll
   for (int u = KV; u < NB; ++u) {
-    const int i = ty + 16 * u;
     const bool below = (u > KV) || (ty >= kq + 4);
     if (below) {
-      const double2* q = reinterpret_cast<const double2*>(&sh.G[i][0]);
-      const double2 g0 = q[0], g1 = q[1];
-      const double li[4] = {g0.x, g0.y, g1.x, g1.y};
+      double li[4];
+      dh_ld4(&sh.G[ty + 16 * u][0], li);
 #pragma unroll
       for (int v = 0; v <= u; ++v)
 #pragma unroll
         for (int b = 0; b < 4; ++b) R[u][v] = fma(-li[b], gj[v][b], R[u][v]);
     }
   }
-  // rows of the block are final: row K0+aa of inv(L)
-  if (rowown) {
-    const int aa = ty - kq;
+  // rows of the block are final: row K0+a of inv(L)
+  if (ty >= kq && ty < kq + 4) {
+    const int a = ty - kq;
 #pragma unroll
-    for (int v = 0; v <= KV; ++v) {
-      double x = 0.0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) x = (b == aa) ? gj[v][b] : x;
-      R[KV][v] = x;
-    }
+    for (int v = 0; v <= KV; ++v) R[KV][v] = sh.G[tx + 16 * v][a];
   }
 }
 
@@ -583,14 +583,18 @@ __device__ void dh_leaf64_lds(double* tile, long ld, double* dinv, double* inv_o
 #pragma unroll
   for (int r = 0; r < 4; ++r) V[(32 + 16 * bi + 4 * r + orow) * DH_LDS_LD + 16 * bj + ocol] = -x[r];
   __syncthreads();
-#pragma unroll 4
-  for (int e0 = 0; e0 < 64 * 64; e0 += 256) {  // uniform trip count (256 threads)
-    const int e = e0 + tid;
+#pragma unroll 2
+  for (int e0 = 0; e0 < 64 * 64; e0 += 512) {  // two entries per thread, uniform trip count
+    const int e = e0 + 2 * tid;
     const int i = e >> 6, j = e & 63;
-    const double vij = (j <= i) ? V[i * DH_LDS_LD + j] : 0.0;
-    if (j <= i) tile[(long)i * ld + j] = S[i * DH_LDS_LD + j];
-    if (j < i) tile[(long)j * ld + i] = vij;
-    if (j == i) dinv[i] = vij;
-    if (inv_out) inv_out[e] = vij;
+    double2 t, v;
+    t.x = (j <= i) ? S[i * DH_LDS_LD + j] : V[j * DH_LDS_LD + i];  // inv(L)^T above the diagonal
+    t.y = (j + 1 <= i) ? S[i * DH_LDS_LD + j + 1] : V[(j + 1) * DH_LDS_LD + i];
+    v.x = (j <= i) ? V[i * DH_LDS_LD + j] : 0.0;
+    v.y = (j + 1 <= i) ? V[i * DH_LDS_LD + j + 1] : 0.0;
+    *reinterpret_cast<double2*>(tile + (long)i * ld + j) = t;
+    if (j == i) dinv[i] = v.x;
+    if (j + 1 == i) dinv[i] = v.y;
+    if (inv_out) *reinterpret_cast<double2*>(inv_out + e) = v;
   }
 }
