@@ -94,10 +94,9 @@ __device__ __forceinline__ void dh_load16(const Opnd& A, const Opnd& B, int ra, 
   dh_load4<MB>(B, rb + 16, k, b[1]);
 }
 
-template <int MA, int MB>
+template <int MA, int MB, int S = DH_MMA_STAGES>
 __device__ __forceinline__ void dh_mma(dh_d4 acc[2][2], const Opnd& A, int i0, const Opnd& B, int j0,
                                        int klo, int khi) {
-  constexpr int S = DH_MMA_STAGES;
   if (klo >= khi) return;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;

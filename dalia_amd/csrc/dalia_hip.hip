@@ -615,7 +615,10 @@ __global__ __launch_bounds__(256) void k_syrk_tail(BtaPtrs P, const double* Pbuf
 // previous block's sub-diagonal tile (0, nbt-1).  Each helper is split so
 // that it is never late: OLD_D/OLD_L(c) (the previous block's terms and own
 // terms k < c - look) are queued look columns ahead, PRE_D/PRE_L(c) (own
-// terms c - look <= k < c - 1) one column ahead.  The other tiles of column
+// terms c - look <= k < c - 2) two columns ahead, and QX(c) applies the
+// term k = c - 2 as soon as L(c, c-2) is final, so that no helper that the
+// chain waits for has more than one rank-64 product left after the factor
+// two columns back.  The other tiles of column
 // c are split the same way: an early task queued slook columns ahead
 // (OLD_X for diagonal-block tiles, OLD_LW for sub-diagonal tiles, OLD_A for
 // the arrow tile) applies the previous block's terms and the own terms
@@ -628,7 +631,7 @@ __global__ __launch_bounds__(256) void k_syrk_tail(BtaPtrs P, const double* Pbuf
 // grid drains; the chain overlaps the bulk of the products of later columns
 // and of the next block.
 enum { DK_PRE_D = 0, DK_PRE_L = 1, DK_CHAIN = 2, DK_DIAG = 3, DK_LOWER = 4, DK_ARROW = 5, DK_TIP = 6,
-       DK_OLD_D = 7, DK_OLD_L = 8, DK_OLD_X = 9, DK_OLD_LW = 10, DK_OLD_A = 11 };
+       DK_OLD_D = 7, DK_OLD_L = 8, DK_OLD_X = 9, DK_OLD_LW = 10, DK_OLD_A = 11, DK_QX = 12 };
 
 struct DagArgs {
   double *diag, *lower, *arrow, *tip, *dinv;
@@ -646,13 +649,14 @@ struct DagArgs {
 };
 
 #define DH_DAG_SPIN (1 << 20)
-#define DH_DAG_LOOK 6
+#define DAG_STAGES DH_MMA_STAGES  // operand ring depth of the task products
+#define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed before QX(r) and after OLD(r))
 #define DH_DAG_SLOOK 6
 
 // Flags of one block: final D tiles [nbt][nbt], final L tiles [nbt][nbt],
 // final arrow tiles [nbt], tip partial applied, PRE_D, PRE_L, OLD_D, OLD_L done [nbt each],
-// OLD_X [nbt][nbt], OLD_LW [nbt][nbt], OLD_A [nbt] done.
-__device__ __host__ __forceinline__ int dag_fstride(int nbt) { return 4 * nbt * nbt + 6 * nbt + 1; }
+// OLD_X [nbt][nbt], OLD_LW [nbt][nbt], OLD_A [nbt], QX [nbt] done.
+__device__ __host__ __forceinline__ int dag_fstride(int nbt) { return 4 * nbt * nbt + 7 * nbt + 1; }
 
 // Number (1..64) of consecutive k in [k, kend) whose tiles are final in both
 // flag rows (fb may be null), after waiting until there is at least one.
@@ -709,7 +713,7 @@ __device__ __forceinline__ void dag_accumulate(dh_d4 acc[2][2], const double* A,
   const Opnd ob{B, bp, M_ROW, nullptr};
   for (int k = 0; k < kend;) {
     const int cnt = dag_ready(fa, fb, k, kend, err, s_n);
-    dh_mma<M_ROW, M_ROW>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
+    dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
     k += cnt;
   }
 }
@@ -722,17 +726,32 @@ __device__ __forceinline__ void dag_accumulate_from(dh_d4 acc[2][2], const doubl
   const Opnd ob{B, bp, M_ROW, nullptr};
   for (int k = k0; k < kend;) {
     const int cnt = dag_ready(fa, fb, k, kend, err, s_n);
-    dh_mma<M_ROW, M_ROW>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
+    dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
     k += cnt;
   }
 }
 
-// C = C inv(L)^T in place, inv(L) dense row-major (ld 64).
-__device__ __forceinline__ void dag_trsm(double* C, const double* pinv, int bp) {
+// acc = -C (accumulator layout): a task that ends with C -= sum starts from
+// the tile's current value, so that its load overlaps the first products and
+// the result is written with plain stores.
+__device__ __forceinline__ void dag_preload_neg(dh_d4 acc[2][2], const double* C, int bp) {
+  dh_load_acc(C, bp, 0, 0, acc);
+#pragma unroll
+  for (int x0 = 0; x0 < 2; ++x0)
+#pragma unroll
+    for (int x1 = 0; x1 < 2; ++x1) acc[x0][x1] = -acc[x0][x1];
+}
+
+// C = (-acc) inv(L)^T once the factor of the diagonal tile is final: -acc is
+// staged in LDS before the wait, so that only inv(L) is loaded after it.
+__device__ __forceinline__ void dag_solve_store(double* C, const dh_d4 acc[2][2], const double* pinv, int bp,
+                                                const int* fdiag, int* err, int* s_n) {
+  dh_store(dh_lS, DH_LDS_LD, 0, 0, acc, -1.0, 0.0, false);
+  dag_ready(fdiag, nullptr, 0, 1, err, s_n);  // also orders the LDS stores
   dh_d4 x[2][2];
   dh_zero(x);
-  dh_mma<M_ROW, M_ROW>(x, Opnd{C, bp, M_ROW, nullptr}, 0, Opnd{pinv, DH_TILE, M_ROW, nullptr}, 0, 0, DH_TILE);
-  __syncthreads();  // every wave has read C
+  dh_mma<M_ROW, M_ROW, DAG_STAGES>(x, Opnd{dh_lS, DH_LDS_LD, M_ROW, nullptr}, 0, Opnd{pinv, DH_TILE, M_ROW, nullptr}, 0, 0,
+                       DH_TILE);
   dh_store(C, bp, 0, 0, x, 1.0, 0.0, false);
 }
 
@@ -775,7 +794,9 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
   int* fX = fOL + nbt;        // OLD_X(r, c) at r * nbt + c
   int* fLW = fX + nbt * nbt;  // OLD_LW(r, c)
   int* fXA = fLW + nbt * nbt; // OLD_A(c)
+  int* fQ = fXA + nbt;        // QX(r): final partial sums of tiles (r, r) and (r, r-1)
   const int kold = max(0, r - g.look);  // helpers of tile row r: own terms below kold are OLD's
+  const int kpre = r >= 2 ? r - 2 : r - 1;  // PRE(r) stops before the term that QX(r) applies
   const int ks = max(0, c - g.slook);   // other tiles of column c: own terms below ks are the early task's
   const int* fLp = fL - FS;  // block i-1
   const int* fAp = fA - FS;
@@ -789,43 +810,63 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
     dh_store(g.tip, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
     fdone = fT;
   } else if (kind == DK_OLD_D) {  // tile (r, r): previous block's terms and own terms k < kold
+    dag_preload_neg(acc, D + r * T64 + r * DH_TILE, bp);
     if (i > 0)
       dag_accumulate(acc, Lp, r * DH_TILE, Lp, r * DH_TILE, bp, fLp + r * nbt, nullptr, r == 0 ? nbt - 1 : nbt,
                      g.err, s_n);
     dag_accumulate(acc, D, r * DH_TILE, D, r * DH_TILE, bp, fD + r * nbt, nullptr, kold, g.err, s_n);
-    dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fOD + r;
   } else if (kind == DK_OLD_L) {  // tile (r, r-1): previous block's terms and own terms k < kold
+    dag_preload_neg(acc, D + r * T64 + (r - 1) * DH_TILE, bp);
     if (i > 0)
       dag_accumulate(acc, Lp, r * DH_TILE, Lp, (r - 1) * DH_TILE, bp, fLp + r * nbt, fLp + (r - 1) * nbt, nbt,
                      g.err, s_n);
     dag_accumulate(acc, D, r * DH_TILE, D, (r - 1) * DH_TILE, bp, fD + r * nbt, fD + (r - 1) * nbt, kold, g.err,
                    s_n);
-    dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fOL + r;
-  } else if (kind == DK_PRE_D) {  // tile (r, r), r >= 1: own terms kold <= k < r-1
-    dag_accumulate_from(acc, D, r * DH_TILE, D, r * DH_TILE, bp, fD + r * nbt, nullptr, kold, r - 1, g.err, s_n);
+  } else if (kind == DK_PRE_D) {  // tile (r, r), r >= 1: own terms kold <= k < kpre
     dag_ready(fOD + r, nullptr, 0, 1, g.err, s_n);
-    dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    dag_preload_neg(acc, D + r * T64 + r * DH_TILE, bp);
+    dag_accumulate_from(acc, D, r * DH_TILE, D, r * DH_TILE, bp, fD + r * nbt, nullptr, kold, kpre, g.err, s_n);
+    dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fPD + r;
-  } else if (kind == DK_PRE_L) {  // tile (r, r-1), r >= 1: own terms kold <= k < r-1
-    dag_accumulate_from(acc, D, r * DH_TILE, D, (r - 1) * DH_TILE, bp, fD + r * nbt, fD + (r - 1) * nbt, kold,
-                        r - 1, g.err, s_n);
+  } else if (kind == DK_PRE_L) {  // tile (r, r-1), r >= 1: own terms kold <= k < kpre
     dag_ready(fOL + r, nullptr, 0, 1, g.err, s_n);
-    dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    dag_preload_neg(acc, D + r * T64 + (r - 1) * DH_TILE, bp);
+    dag_accumulate_from(acc, D, r * DH_TILE, D, (r - 1) * DH_TILE, bp, fD + r * nbt, fD + (r - 1) * nbt, kold,
+                        kpre, g.err, s_n);
+    dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fPL + r;
+  } else if (kind == DK_QX) {  // r >= 2: term k = r-2 of tiles (r, r) and (r, r-1), once L(r, r-2) is final
+    double* Pd = D + r * T64 + r * DH_TILE;
+    double* Pl = D + r * T64 + (r - 1) * DH_TILE;
+    dag_ready(fPD + r, nullptr, 0, 1, g.err, s_n);
+    dag_ready(fPL + r, nullptr, 0, 1, g.err, s_n);
+    dh_d4 a2[2][2];
+    dag_preload_neg(acc, Pd, bp);
+    dag_preload_neg(a2, Pl, bp);
+    const Opnd lr{D + r * T64 + (r - 2) * DH_TILE, bp, M_ROW, nullptr};
+    dag_ready(fD + r * nbt + (r - 2), nullptr, 0, 1, g.err, s_n);
+    dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, lr, 0, lr, 0, 0, DH_TILE);
+    dag_ready(fD + (r - 1) * nbt + (r - 2), nullptr, 0, 1, g.err, s_n);
+    dh_mma<M_ROW, M_ROW, DAG_STAGES>(a2, lr, 0, Opnd{D + (r - 1) * T64 + (r - 2) * DH_TILE, bp, M_ROW, nullptr}, 0, 0, DH_TILE);
+    dh_store(Pd, bp, 0, 0, acc, -1.0, 0.0, false);
+    dh_store(Pl, bp, 0, 0, a2, -1.0, 0.0, false);
+    fdone = fQ + r;
   } else if (kind == DK_CHAIN) {  // tile (c, c)
     double* C = D + c * T64 + c * DH_TILE;
     double* pv = pinv + c * DH_TILE * DH_TILE;
     // the inverse slot was last read by block i-2's column c, whose arrow tile is the only reader
     // that is not already a dependency of this task
     if (i >= 2 && g.arrow_on) dag_ready(fA - 2 * FS + c, nullptr, 0, 1, g.err, s_n);
-    dag_ready(c == 0 ? fOD : fPD + c, nullptr, 0, 1, g.err, s_n);
+    dag_ready(c == 0 ? fOD : (c == 1 ? fPD + c : fQ + c), nullptr, 0, 1, g.err, s_n);
     dh_d4 part[2][2];
     dh_load_acc(C, bp, 0, 0, part);  // partial sum of tile (c, c) without its last term
     if (c >= 1) {
       double* X = D + c * T64 + (c - 1) * DH_TILE;
-      dag_ready(fPL + c, nullptr, 0, 1, g.err, s_n);
+      if (c == 1) dag_ready(fPL + c, nullptr, 0, 1, g.err, s_n);
       double xa[2][2][2][4];
       dh_load_a64<M_ROW>(Opnd{X, bp, M_ROW, nullptr}, 0, xa);
       dag_ready(fD + (c - 1) * nbt + (c - 1), nullptr, 0, 1, g.err, s_n);  // the chain dependency
@@ -836,11 +877,11 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
       dh_store(dh_lV, DH_LDS_LD, 0, 0, x, 1.0, 0.0, false);
       dag_signal(fD + c * nbt + (c - 1));
       const Opnd xs{dh_lV, DH_LDS_LD, M_ROW, nullptr};
-      dh_mma<M_ROW, M_ROW>(acc, xs, 0, xs, 0, 0, DH_TILE);
+      dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, xs, 0, xs, 0, 0, DH_TILE);
     } else if (i > 0) {
       dag_ready(fLp + (nbt - 1), nullptr, 0, 1, g.err, s_n);
       const Opnd xl{Lp + (nbt - 1) * DH_TILE, bp, M_ROW, nullptr};
-      dh_mma<M_ROW, M_ROW>(acc, xl, 0, xl, 0, 0, DH_TILE);
+      dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, xl, 0, xl, 0, 0, DH_TILE);
     }
 #pragma unroll
     for (int x0 = 0; x0 < 2; ++x0)
@@ -851,44 +892,48 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
     dh_leaf64_lds(C, bp, dinv + c * DH_TILE, pv, g.info, i * g.b + c * DH_TILE);
     fdone = fD + c * nbt + c;
   } else if (kind == DK_OLD_X) {  // diagonal-block tile (r, c): previous block's terms, own terms k < ks
+    if (i > 0 || ks > 0) dag_preload_neg(acc, D + r * T64 + c * DH_TILE, bp);
     if (i > 0)
       dag_accumulate(acc, Lp, r * DH_TILE, Lp, c * DH_TILE, bp, fLp + r * nbt, fLp + c * nbt, nbt, g.err, s_n);
     dag_accumulate(acc, D, r * DH_TILE, D, c * DH_TILE, bp, fD + r * nbt, fD + c * nbt, ks, g.err, s_n);
-    if (i > 0 || ks > 0) dh_store(D + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    if (i > 0 || ks > 0) dh_store(D + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fX + r * nbt + c;
   } else if (kind == DK_OLD_LW) {  // sub-diagonal tile (r, c): own terms k < ks
+    dag_preload_neg(acc, Lo + r * T64 + c * DH_TILE, bp);
     dag_accumulate(acc, Lo, r * DH_TILE, D, c * DH_TILE, bp, fL + r * nbt, fD + c * nbt, ks, g.err, s_n);
-    dh_store(Lo + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    dh_store(Lo + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fLW + r * nbt + c;
   } else if (kind == DK_OLD_A) {  // arrow tile c: previous block's terms, own terms k < ks
+    if (i > 0 || ks > 0) dag_preload_neg(acc, Ar + c * DH_TILE, bp);
     if (i > 0) dag_accumulate(acc, Ap, 0, Lp, c * DH_TILE, bp, fAp, fLp + c * nbt, nbt, g.err, s_n);
     dag_accumulate(acc, Ar, 0, D, c * DH_TILE, bp, fA, fD + c * nbt, ks, g.err, s_n);
-    if (i > 0 || ks > 0) dh_store(Ar + c * DH_TILE, bp, 0, 0, acc, -1.0, 1.0, false);
+    if (i > 0 || ks > 0) dh_store(Ar + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fXA + c;
   } else {
     // the early task of the tile exists for every diagonal-block and arrow
     // tile, and for sub-diagonal tiles with c > slook
     double* C;
     if (kind == DK_DIAG) {  // tile (r, c), r >= c + 2
-      dag_ready(fX + r * nbt + c, nullptr, 0, 1, g.err, s_n);
-      dag_accumulate_from(acc, D, r * DH_TILE, D, c * DH_TILE, bp, fD + r * nbt, fD + c * nbt, ks, c, g.err, s_n);
       C = D + r * T64 + c * DH_TILE;
+      dag_ready(fX + r * nbt + c, nullptr, 0, 1, g.err, s_n);
+      dag_preload_neg(acc, C, bp);
+      dag_accumulate_from(acc, D, r * DH_TILE, D, c * DH_TILE, bp, fD + r * nbt, fD + c * nbt, ks, c, g.err, s_n);
       fdone = fD + r * nbt + c;
     } else if (kind == DK_LOWER) {  // sub-diagonal block tile (r, c)
+      C = Lo + r * T64 + c * DH_TILE;
       const int k0 = c > g.slook ? ks : 0;
       if (c > g.slook) dag_ready(fLW + r * nbt + c, nullptr, 0, 1, g.err, s_n);
+      dag_preload_neg(acc, C, bp);
       dag_accumulate_from(acc, Lo, r * DH_TILE, D, c * DH_TILE, bp, fL + r * nbt, fD + c * nbt, k0, c, g.err, s_n);
-      C = Lo + r * T64 + c * DH_TILE;
       fdone = fL + r * nbt + c;
     } else {  // arrow tile c
-      dag_ready(fXA + c, nullptr, 0, 1, g.err, s_n);
-      dag_accumulate_from(acc, Ar, 0, D, c * DH_TILE, bp, fA, fD + c * nbt, ks, c, g.err, s_n);
       C = Ar + c * DH_TILE;
+      dag_ready(fXA + c, nullptr, 0, 1, g.err, s_n);
+      dag_preload_neg(acc, C, bp);
+      dag_accumulate_from(acc, Ar, 0, D, c * DH_TILE, bp, fA, fD + c * nbt, ks, c, g.err, s_n);
       fdone = fA + c;
     }
-    dh_store(C, bp, 0, 0, acc, -1.0, 1.0, false);
-    dag_ready(fD + c * nbt + c, nullptr, 0, 1, g.err, s_n);  // factor of tile (c, c); orders the store above
-    dag_trsm(C, pinv + c * DH_TILE * DH_TILE, bp);
+    dag_solve_store(C, acc, pinv + c * DH_TILE * DH_TILE, bp, fD + c * nbt + c, g.err, s_n);
   }
   dag_signal(fdone);
 }
@@ -1660,10 +1705,11 @@ static void build_nodes(int o, int s, int d, std::vector<NodeList>& lv) {
 
 // Task order of one block of the persistent pobtaf (nl sub-diagonal tile
 // rows: nbt, or 0 for the last block): OLD_D(0), OLD_D/OLD_L(1..look), the
-// early tasks of columns 0..slook-1, then per tile column c: CHAIN(c),
-// OLD_D/OLD_L(c+look), PRE_D/PRE_L(c+1), the early tasks of column
-// c+slook, the diagonal-block tiles below (c+1, c), the sub-diagonal tiles
-// and the arrow tile of column c; then the tip partial.
+// early tasks of columns 0..slook-1, PRE_D/PRE_L(1), then per tile column c:
+// CHAIN(c), OLD_D/OLD_L(c+look), PRE_D/PRE_L(c+2), QX(c+1), the early tasks
+// of column c+slook, the diagonal-block tiles below (c+1, c), the
+// sub-diagonal tiles and the arrow tile of column c; then the tip partial.
+// Every task waits only for tasks listed before it (look >= 2).
 static void dag_early(int nbt, int nl, int arrow, int slook, int c, std::vector<int>& tab) {
   auto enc = [](int kind, int r, int c) { return (kind << 24) | (r << 12) | c; };
   for (int r = c + 2; r < nbt; ++r) tab.push_back(enc(DK_OLD_X, r, c));
@@ -1680,16 +1726,21 @@ static void dag_table(int nbt, int nl, int arrow, int look, int slook, std::vect
     tab.push_back(enc(DK_OLD_L, r, r - 1));
   }
   for (int c = 0; c < slook && c < nbt; ++c) dag_early(nbt, nl, arrow, slook, c, tab);
+  if (nbt > 1) {
+    tab.push_back(enc(DK_PRE_D, 1, 1));
+    tab.push_back(enc(DK_PRE_L, 1, 0));
+  }
   for (int c = 0; c < nbt; ++c) {
     tab.push_back(enc(DK_CHAIN, c, c));
     if (c >= 1 && c + look < nbt) {
       tab.push_back(enc(DK_OLD_D, c + look, c + look));
       tab.push_back(enc(DK_OLD_L, c + look, c + look - 1));
     }
-    if (c + 1 < nbt) {
-      tab.push_back(enc(DK_PRE_D, c + 1, c + 1));
-      tab.push_back(enc(DK_PRE_L, c + 1, c));
+    if (c + 2 < nbt) {
+      tab.push_back(enc(DK_PRE_D, c + 2, c + 2));
+      tab.push_back(enc(DK_PRE_L, c + 2, c + 1));
     }
+    if (c + 1 >= 2 && c + 1 < nbt) tab.push_back(enc(DK_QX, c + 1, c + 1));
     if (c + slook < nbt) dag_early(nbt, nl, arrow, slook, c + slook, tab);
     for (int r = c + 2; r < nbt; ++r) tab.push_back(enc(DK_DIAG, r, c));
     for (int r = 0; r < nl; ++r) tab.push_back(enc(DK_LOWER, r, c));
