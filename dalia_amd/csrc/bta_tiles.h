@@ -78,12 +78,14 @@ __device__ __forceinline__ void dh_zero(dh_d4 acc[2][2]) {
 }
 
 // acc += opA[i0:i0+64, klo:khi] * opB[klo:khi, j0:j0+64] for this wave's quadrant.
-// k advances in chunks of 16 (klo, khi multiples of 16) held in a ring of
-// DH_MMA_STAGES register buffers: the loads of chunk q + DH_MMA_STAGES - 1
-// are issued before the 16 MFMAs of chunk q, so that about three chunks
-// (3 x 1024 MFMA cycles) of load latency are covered.
+// k advances in chunks of 16 (klo, khi multiples of 16) held in a ring of S
+// register buffers: the loads of chunk q + S - 1 are issued before the 16
+// MFMAs of chunk q.  A wave issues one FP64 16x16x4 MFMA about every 140
+// cycles on gfx950, so one chunk ahead (S = 2, ~2200 cycles) covers the L2
+// latency and keeps the register count low enough for three waves per SIMD;
+// the persistent factorization (one workgroup per CU) uses a deeper ring.
 #ifndef DH_MMA_STAGES
-#define DH_MMA_STAGES 4
+#define DH_MMA_STAGES 2
 #endif
 template <int MA, int MB>
 __device__ __forceinline__ void dh_load16(const Opnd& A, const Opnd& B, int ra, int rb, int k, double a[2][4],

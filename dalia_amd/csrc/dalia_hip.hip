@@ -649,7 +649,7 @@ struct DagArgs {
 };
 
 #define DH_DAG_SPIN (1 << 20)
-#define DAG_STAGES DH_MMA_STAGES  // operand ring depth of the task products
+#define DAG_STAGES 4  // operand ring depth of the task products (one workgroup per CU)
 #define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed before QX(r) and after OLD(r))
 #define DH_DAG_SLOOK 6
 
