@@ -1669,6 +1669,8 @@ struct dh_handle {
   int dag_wgs;                   // persistent workgroups
   int tkind;                     // timed kernel kind, -1 none
   std::vector<hipEvent_t> ev;    // event pairs
+  hipStream_t side;              // selected inversion: products of the factor alone, run ahead of the recursion
+  hipEvent_t sev[5];             // fork, chunk ready (2 slots), chunk slots free (2 slots)
   long long nrec;
 };
 
@@ -1796,6 +1798,16 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   }
   const long long bb = (long long)h->bp * h->bp;
   h->G = n < 8 ? n : 8;
+  h->side = nullptr;
+  for (hipEvent_t& e : h->sev) e = nullptr;
+  if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) h->side = nullptr;
+  for (hipEvent_t& e : h->sev)
+    if (h->side && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+  for (hipEvent_t e : h->sev)
+    if (!e && h->side) {
+      (void)hipStreamDestroy(h->side);
+      h->side = nullptr;
+    }
   int rc = DH_OK;
   rc |= dalloc(h, &h->diag, n * bb);
   rc |= dalloc(h, &h->lower, (n - 1) * bb);
@@ -1856,6 +1868,9 @@ int dh_destroy(dh_handle* h) {
   if (h->dagcounter) (void)hipFree(h->dagcounter);
   if (h->dagtab) (void)hipFree(h->dagtab);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->sev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->side) (void)hipStreamDestroy(h->side);
   delete h;
   return DH_OK;
 }
@@ -2208,13 +2223,23 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   }
   const int wld = bp + 8;  // W^T pitch: rows of a power-of-two pitch read row-wise by one tile share
                            // memory channels (the product reading them by rows runs ~30% slower)
-  for (int hi = n - 1; hi >= 0; hi -= h->G) {
-    const int cnt = std::min(h->G, hi + 1);
-    SelBatch sb;
+  // The products of the factor alone (W, Wa, inv(L)^T inv(L)) of a chunk of blocks do not depend on
+  // the recursion, so they run on a second stream one chunk ahead of it, in two alternating halves
+  // of the scratch slots, and fill the GPU while the recursion's launches drain.  Under stream
+  // capture everything stays on the caller's stream.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  const bool two = h->side && h->G >= 2 && n > h->G / 2 &&
+                   hipStreamIsCapturing(h->stream, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
+  const int G = two ? h->G / 2 : h->G;  // blocks per chunk
+  const int nq = (n + G - 1) / G;
+  auto batch = [&](int q, SelBatch& sb) {
+    const int hi = n - 1 - q * G;
+    const int cnt = std::min(G, hi + 1);
+    const int slot0 = two ? (q & 1) * G : 0;
     memset(&sb, 0, sizeof(sb));
     sb.cnt = cnt;
     for (int g = 0; g < cnt; ++g) {
-      const int i = hi - g;
+      const int i = hi - g, sl = slot0 + g;
       SelArgs& sa = sb.a[g];
       sa.D = h->diag + i * bb;
       sa.dinv = h->dinv + (long)i * bp;
@@ -2223,25 +2248,60 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
       sa.Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
       sa.Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * 64 * bp : nullptr;
       sa.tip = h->tip;
-      sa.WT = W + (long)g * bp * wld;
-      sa.Wa = h->Wa + (long)g * 64 * bp;
-      sa.S = S + (long)g * bb;
+      sa.WT = W + (long)sl * bp * wld;
+      sa.Wa = h->Wa + (long)sl * 64 * bp;
+      sa.S = S + (long)sl * bb;
       sa.bp = bp;
       sa.nbt = nbt;
       sa.wld = wld;
     }
-    // 1. products of the factor alone for the whole chunk: W (as W^T), Wa, inv(L)^T inv(L)
-    const int nw = nbt * nbt + (arrow ? nbt : 0);
-    DH_LAUNCH(h, 2, k_sel_w, dim3(nw, cnt), dim3(256), 0, sb);
-    DH_LAUNCH(h, 2, k_sel_ll, dim3(nbt * (nbt + 1) / 2, cnt), dim3(256), 0, sb);
-    // 2. the recursion, block by block
-    for (int g = 0; g < cnt; ++g) {
+  };
+  const int nw = nbt * nbt + (arrow ? nbt : 0);
+  auto factor_products = [&](int q, hipStream_t st) {  // W (as W^T), Wa, inv(L)^T inv(L) of chunk q
+    SelBatch sb;
+    batch(q, sb);
+    if (st == h->stream) {
+      DH_LAUNCH(h, 2, k_sel_w, dim3(nw, sb.cnt), dim3(256), 0, sb);
+      DH_LAUNCH(h, 2, k_sel_ll, dim3(nbt * (nbt + 1) / 2, sb.cnt), dim3(256), 0, sb);
+    } else {
+      hipLaunchKernelGGL(k_sel_w, dim3(nw, sb.cnt), dim3(256), 0, st, sb);
+      hipLaunchKernelGGL(k_sel_ll, dim3(nbt * (nbt + 1) / 2, sb.cnt), dim3(256), 0, st, sb);
+    }
+  };
+  auto recursion = [&](int q) {  // block by block
+    SelBatch sb;
+    batch(q, sb);
+    const int hi = n - 1 - q * G;
+    for (int g = 0; g < sb.cnt; ++g) {
       const SelArgs& sa = sb.a[g];
       const int nx = (sa.Lb ? nbt * nbt : 0) + (sa.R ? nbt : 0);
       if (nx > 0) DH_LAUNCH(h, 2, k_sel_x, dim3(nx), dim3(256), 0, sa);  // X_{i+1,i}, X_{a,i}
       DH_LAUNCH(h, 2, k_sel_s, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, sa);  // X_ii lower tiles
       hipLaunchKernelGGL(k_symm_copy, dim3((unsigned)((bb + 255) / 256)), dim3(256), 0, h->stream,
                          h->diag + (long)(hi - g) * bb, (const double*)sa.S, bp);
+    }
+  };
+  if (!two) {
+    for (int q = 0; q < nq; ++q) {
+      factor_products(q, h->stream);
+      recursion(q);
+    }
+  } else {
+    hipEvent_t fork = h->sev[0], ready[2] = {h->sev[1], h->sev[2]}, freed[2] = {h->sev[3], h->sev[4]};
+    HIPCHK(hipEventRecord(fork, h->stream));  // the factor and X_aa are complete
+    HIPCHK(hipStreamWaitEvent(h->side, fork, 0));
+    auto ahead = [&](int q) {
+      if (q >= 2) HIPCHK(hipStreamWaitEvent(h->side, freed[q & 1], 0));  // chunk q-2 done with the slots
+      factor_products(q, h->side);
+      HIPCHK(hipEventRecord(ready[q & 1], h->side));
+      return DH_OK;
+    };
+    if (ahead(0)) return DH_EHIP;
+    for (int q = 0; q < nq; ++q) {
+      if (q + 1 < nq && ahead(q + 1)) return DH_EHIP;
+      HIPCHK(hipStreamWaitEvent(h->stream, ready[q & 1], 0));
+      recursion(q);
+      HIPCHK(hipEventRecord(freed[q & 1], h->stream));
     }
   }
   HIPCHK(hipGetLastError());
