@@ -1086,6 +1086,9 @@ struct SelArgs {
 };
 
 #define DH_SELG 8  // blocks whose factor-only products are batched in one launch
+// The selected-inversion products are bound by the FP64 MFMA issue rate, which grows with the
+// waves per SIMD (one wave issues a 16x16x4 MFMA every ~140 cycles): their register budgets are
+// capped for four (three for k_sel_ll) waves per SIMD, which the compiler meets without spills.
 
 struct SelBatch {
   SelArgs a[DH_SELG];
@@ -1094,7 +1097,7 @@ struct SelBatch {
 
 // WT = inv(L_ii)^T L_{i+1,i}^T (tiles I, J; k >= 64 I) and Wa = L_{a,i} inv(L_ii) (k >= 64 J)
 // for block slot blockIdx.y: products of the factor alone, off the recursion's critical chain.
-__global__ __launch_bounds__(256) void k_sel_w(SelBatch sb) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_w(SelBatch sb) {
   if ((int)blockIdx.y >= sb.cnt) return;
   const SelArgs& a = sb.a[blockIdx.y];
   const int nw = a.Lb ? a.nbt * a.nbt : 0;
@@ -1116,7 +1119,7 @@ __global__ __launch_bounds__(256) void k_sel_w(SelBatch sb) {
 
 // X_{i+1,i} = -X_{i+1,i+1} W - X_{a,i+1}^T Wa  (X_{i+1,i+1} read by columns: it is symmetric) and
 // X_{a,i} = -X_{a,i+1} W - X_aa Wa.
-__global__ __launch_bounds__(256) void k_sel_x(SelArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_x(SelArgs a) {
   const int nx = a.Lb ? a.nbt * a.nbt : 0;
   dh_d4 acc[2][2], acc2[2][2];
   dh_zero(acc);
@@ -1146,7 +1149,7 @@ __global__ __launch_bounds__(256) void k_sel_x(SelArgs a) {
 }
 
 // Lower tiles of inv(L_ii)^T inv(L_ii) into S, for block slot blockIdx.y (factor only).
-__global__ __launch_bounds__(256) void k_sel_ll(SelBatch sb) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_sel_ll(SelBatch sb) {
   if ((int)blockIdx.y >= sb.cnt) return;
   const SelArgs& a = sb.a[blockIdx.y];
   const int e = blockIdx.x;
@@ -1162,7 +1165,7 @@ __global__ __launch_bounds__(256) void k_sel_ll(SelBatch sb) {
 
 // Lower tiles of X_ii = inv(L)^T inv(L) - W^T X_{i+1,i} - Wa^T X_{a,i}: S (holding the first
 // term) -= the other two.
-__global__ __launch_bounds__(256) void k_sel_s(SelArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_s(SelArgs a) {
   const int e = blockIdx.x;
   int I = 0;
   while ((I + 1) * (I + 2) / 2 <= e) ++I;
