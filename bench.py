@@ -69,6 +69,26 @@ def update_flops_per_launch(n, b, a):
     return tot / max(cnt, 1), cnt
 
 
+def dgemm_ceiling(n: int = 8192, reps: int = 5) -> float:
+    """A measured FP64 GEMM ceiling on this GPU: the vendor library's DGEMM (torch.mm) at n^3, TFLOP/s."""
+    import torch
+
+    a = torch.randn(n, n, dtype=torch.float64, device="cuda")
+    b = torch.randn(n, n, dtype=torch.float64, device="cuda")
+    c = torch.mm(a, b)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        torch.mm(a, b, out=c)
+    ev1.record()
+    torch.cuda.synchronize()
+    t = ev0.elapsed_time(ev1) * 1e-3 / reps
+    del a, b, c
+    torch.cuda.empty_cache()
+    return 2.0 * n ** 3 / t / 1e12
+
+
 def committed_traffic(kname: str, workload: str):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary of this kernel and workload
     (FETCH_SIZE and WRITE_SIZE, each scaled by a copy-kernel calibration), or None."""
@@ -237,6 +257,10 @@ def main():
                 "traffic": committed_traffic(kname, args.workload),
                 "launches": int(cnt.value), "avg_launch_us": avg_s * 1e6, "flops_per_launch": per}
     log(f"roofline {kname}: {cnt.value} launches, avg {avg_s * 1e6:.1f} us, {achieved:.2f} TFLOP/s")
+    dgemm = dgemm_ceiling()
+    roofline["dgemm_ceiling"] = dgemm
+    roofline["frac_of_dgemm"] = achieved / dgemm
+    log(f"library DGEMM ceiling (torch.mm FP64, 8192^3): {dgemm:.2f} TFLOP/s; {kname} at {achieved / dgemm:.2f} of it")
 
     cpu = None
     if not args.no_cpu:
