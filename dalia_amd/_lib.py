@@ -71,6 +71,10 @@ SIGNATURES = {
     "dh_contract_fem": (_I, [_P, ctypes.POINTER(DhModel), _P]),
     "dh_contract_fem_vec": (_I, [_P, ctypes.POINTER(DhModel), _P, _P]),
     "dh_contract_ata": (_I, [_P, ctypes.POINTER(DhModel), _P]),
+    "dh_contract_fem_v": (_I, [_P, ctypes.POINTER(DhModel), _I, _I, _P]),
+    "dh_contract_fem_vec_v": (_I, [_P, ctypes.POINTER(DhModel), _P, _I, _I, _I, _P]),
+    "dh_contract_ata_v": (_I, [_P, ctypes.POINTER(DhModel), _I, _I, _P]),
+    "dh_ata_quad_v": (_I, [_P, ctypes.POINTER(DhModel), _P, _I, _I, _I, _P]),
     "dh_prior_eig": (_I, [_P, ctypes.POINTER(DhModel), _P, c_double_p, _P]),
     "dh_set_tuning": (_I, [_P, _I, _I]),
     "dh_timing_enable": (_I, [_P, _I]),

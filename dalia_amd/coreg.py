@@ -27,15 +27,19 @@ import torch
 
 from dalia_amd import _lib, hyper
 from dalia_amd.inla import hessian_from_values, hessian_stencil
-from dalia_amd.model import _csr, fem_union, tridiag_parts
+from dalia_amd.model import _csr, closed_form_ok, fem_union, tridiag_parts
 from dalia_amd.solver import HipBTASolver
 
 
 def mixing_weights(nv: int, theta) -> np.ndarray:
-    """w[i, j, k]: q_ij = sum_k w[i,j,k] Qu_k (`coregional_model.py:304-497`)."""
-    s2 = np.array([math.exp(2.0 * theta[3 * nv + k]) for k in range(nv)])
-    lam = np.asarray(theta[4 * nv:], dtype=np.float64)
-    w = np.zeros((nv, nv, nv))
+    """w[i, j, k]: q_ij = sum_k w[i,j,k] Qu_k (`coregional_model.py:304-497`).
+
+    Complex theta is accepted (complex-step derivatives of the weights)."""
+    theta = np.asarray(theta)
+    dt = np.result_type(theta.dtype, np.float64)
+    s2 = np.exp(2.0 * theta[3 * nv:4 * nv].astype(dt))
+    lam = theta[4 * nv:].astype(dt)
+    w = np.zeros((nv, nv, nv), dtype=dt)
     if nv == 2:
         l01 = lam[0]
         w[0, 0] = [1 / s2[0], l01 ** 2 / s2[1]]
@@ -85,7 +89,17 @@ class DeviceCoregModel:
         rp, col, val = fem_union(data.c0, data.g1, data.g2, data.g3)
         self.fem_rowptr, self.fem_col, self.fem_val = up(rp, torch.int32), up(col, torch.int32), up(val)
         md, mo = tridiag_parts(data.m0, data.m1, data.m2, nt)
+        self.m_diag_h, self.m_off_h = md, mo
         self.m_diag, self.m_off = up(md), up(mo)
+        # Kronecker-eigen closed form of each variate's prior log-determinant (the adjoint's prior term)
+        self.logdet_c0 = float(np.sum(np.log(sps.csr_matrix(data.c0).diagonal())))
+        self.closed_form = closed_form_ok(data.c0, data.g1, data.g2, data.g3)
+        self.lam = None
+        if self.closed_form:
+            c = sps.csr_matrix(data.c0).diagonal()
+            sc = 1.0 / np.sqrt(c)
+            S = (sps.diags(sc) @ data.g1 @ sps.diags(sc)).toarray()
+            self.lam = up(np.linalg.eigvalsh(0.5 * (S + S.T)))
         self.keep = []
         structs = []
         nst = ns * nt
@@ -112,6 +126,7 @@ class DeviceCoregModel:
             aty[nv * nst + i * nb: nv * nst + (i + 1) * nb] = X.T @ data.y[i]
             vidx[nv * nst + i * nb: nv * nst + (i + 1) * nb] = i
         self.models = (_lib.DhModel * nv)(*structs)
+        self.nst = nst
         self.aty = up(aty)
         self.vidx = up(vidx, torch.int32)
 
@@ -208,3 +223,127 @@ class CoregInla:
         pts = hessian_stencil(theta, h, offdiag_points)
         fv = np.array([self.f(p) for p in pts])
         return hessian_from_values(fv, len(theta), h, offdiag_points)
+
+    # ------------------------------------------------------------------
+    # adjoint gradient
+    def _coef_grads(self, theta):
+        """dC[p][i, j] (3 x 4) = d C_ij / d theta_p for every theta entry, C_ij = sum_k w_ijk c_k."""
+        nv, sst = self.data.nv, self.data.sigma_st
+        theta = np.asarray(theta, dtype=np.float64)
+        d = theta.size
+        coef = np.stack([hyper.kron_coefficients(np.array([theta[3 * k], theta[3 * k + 1], sst])) for k in range(nv)])
+        w = mixing_weights(nv, theta)
+        dC = np.zeros((d, nv, nv, 3, 4))
+        for k in range(nv):
+            dc = hyper.kron_coefficients_grad(np.array([theta[3 * k], theta[3 * k + 1], sst]))
+            for a in range(2):  # r_s_k, r_t_k
+                dC[3 * k + a] = w[:, :, k, None, None] * dc[a][None, None]
+        hcs = 1e-30
+        for p in range(3 * nv, d):  # sigmas and lambdas: complex-step derivative of the weights
+            tc = theta.astype(np.complex128)
+            tc[p] += 1j * hcs
+            dw = mixing_weights(nv, tc).imag / hcs
+            dC[p] = np.einsum("ijk,kab->ijab", dw, coef)
+        return dC
+
+    def forward(self, theta):
+        """Forward half of the adjoint: f(theta) with each variate's closed-form prior term."""
+        theta = np.asarray(theta, dtype=np.float64)
+        if any(p.kind != "gaussian" for p in self.data.priors):
+            raise ValueError("coregional hyper-priors must be Gaussian")
+        if not self.dm.closed_form:
+            raise ValueError("the coregional adjoint needs g2 = g1 c0^-1 g1 and g3 = g1 c0^-1 g2 (lumped c0)")
+        self._set_params(theta)
+        L, h, nv, N = self.lib, self.h, self.data.nv, self.data.n_latent
+        _lib.check(L.dh_assemble_coreg(h, self.dm.models, nv, self.params.data_ptr(), 1, 1), "dh_assemble_coreg")
+        _lib.check(L.dh_cholesky(h, 1), "dh_cholesky")
+        taus = self.params.data_ptr() + 8 * nv * nv * 12
+        _lib.check(L.dh_scale_indexed(h, self.dm.aty.data_ptr(), self.dm.vidx.data_ptr(), ctypes.c_void_p(taus),
+                                      self.b.data_ptr(), N), "dh_scale_indexed")
+        _lib.check(L.dh_scale(h, self.b.data_ptr(), 1.0, self.mu.data_ptr(), N), "dh_scale")
+        _lib.check(L.dh_solve(h, self.mu.data_ptr(), 1), "dh_solve")
+        _lib.check(L.dh_logdet(h, 1, self.scal[0:1].data_ptr()), "dh_logdet")
+        _lib.check(L.dh_dot(h, self.mu.data_ptr(), self.b.data_ptr(), N, self.scal[1:2].data_ptr()), "dh_dot")
+        eig = torch.zeros((nv, 4), dtype=torch.float64, device="cuda")
+        for k in range(nv):
+            gam = np.array(hyper.interpretable2compute(float(theta[3 * k]), float(theta[3 * k + 1]),
+                                                       float(self.data.sigma_st)), dtype=np.float64)
+            _lib.check(L.dh_prior_eig(h, ctypes.byref(self.dm.models[k]), self.dm.lam.data_ptr(),
+                                      gam.ctypes.data_as(_lib.c_double_p), eig[k].data_ptr()), "dh_prior_eig")
+        s = self.scal[:2].cpu().numpy()
+        e = eig.cpu().numpy()
+        ld_c, quad = float(s[0]), float(s[1])
+        if np.isnan(ld_c):
+            raise ValueError("Logdet is NaN. Check the input matrix.")
+        nst = self.dm.nst
+        ld_p = sum(self.dm.nt * self.dm.logdet_c0 + e[k, 0] for k in range(nv)) \
+            - 2.0 * nst * float(np.sum(theta[3 * nv:4 * nv]))
+        dld_p = np.zeros(theta.size)
+        for k in range(nv):
+            dg = hyper.DGAMMA.T @ e[k, 1:4]
+            dld_p[3 * k], dld_p[3 * k + 1] = dg[0], dg[1]
+            dld_p[3 * nv + k] = -2.0 * nst
+        lp, glp = hyper.log_prior_and_grad(theta, self.data.priors)
+        lik = sum(hyper.likelihood_eta0(float(theta[3 * i + 2]), self.dm.n_obs_v[i], self.dm.yty_v[i])
+                  for i in range(nv))
+        f = 0.5 * ld_c - 0.5 * quad - (lp + lik + 0.5 * ld_p)
+        self._state = dict(theta=theta.copy(), f=float(f), dld_p=dld_p, glp=glp)
+        return float(f)
+
+    def backward(self):
+        """Gradient of the last `forward`: selected inversion of Qc and its contractions."""
+        st = getattr(self, "_state", None)
+        if st is None:
+            raise RuntimeError("backward() needs a preceding forward() on this engine")
+        theta = st["theta"]
+        L, h = self.lib, self.h
+        nv, ns, nb, nt, b = self.data.nv, self.dm.ns, self.dm.nb, self.dm.nt, self.data.b
+        nq = 2 * nt - 1
+        m0 = ctypes.byref(self.dm.models[0])
+        cmu = torch.zeros((nv, nv, nq * 4), dtype=torch.float64, device="cuda")
+        cx = torch.zeros_like(cmu)
+        sc = torch.zeros((2, nv), dtype=torch.float64, device="cuda")
+        for i in range(nv):
+            for j in range(nv):
+                _lib.check(L.dh_contract_fem_vec_v(h, m0, self.mu.data_ptr(), b, i * ns, j * ns, cmu[i, j].data_ptr()),
+                           "dh_contract_fem_vec_v")
+        for k in range(nv):
+            _lib.check(L.dh_ata_quad_v(h, ctypes.byref(self.dm.models[k]), self.mu.data_ptr(), b, k * ns, k * nb,
+                                       sc[1, k:k + 1].data_ptr()), "dh_ata_quad_v")
+        _lib.check(L.dh_selected_inversion(h, 1), "dh_selected_inversion")
+        for i in range(nv):
+            for j in range(nv):
+                _lib.check(L.dh_contract_fem_v(h, m0, i * ns, j * ns, cx[i, j].data_ptr()), "dh_contract_fem_v")
+        for k in range(nv):
+            _lib.check(L.dh_contract_ata_v(h, ctypes.byref(self.dm.models[k]), k * ns, k * nb,
+                                           sc[0, k:k + 1].data_ptr()), "dh_contract_ata_v")
+        prod = self.mu * self.dm.aty
+        bmu = torch.stack([prod[self.dm.vidx == k].sum() for k in range(nv)]).cpu().numpy()
+        cxmu = (cx + cmu).cpu().numpy().reshape(nv, nv, nq, 4)
+        tr_ata, quad = sc.cpu().numpy()
+        mult = np.concatenate([np.ones(nt), 2.0 * np.ones(nt - 1)])[:, None]
+        dC = self._coef_grads(theta)
+        g = np.zeros(theta.size)
+        for p in range(theta.size):
+            if p < 3 * nv and p % 3 == 2:  # likelihood precision of variate k
+                k = p // 3
+                et = math.exp(float(theta[p]))
+                g[p] = 0.5 * et * tr_ata[k] + 0.5 * et * quad[k] - et * bmu[k] \
+                    - 0.5 * self.dm.n_obs_v[k] + 0.5 * et * self.dm.yty_v[k]
+                continue
+            acc = 0.0
+            for i in range(nv):
+                for j in range(nv):
+                    if not np.any(dC[p, i, j]):
+                        continue
+                    w = np.concatenate([self.dm.m_diag_h.T @ dC[p, i, j], self.dm.m_off_h.T @ dC[p, i, j]], 0) * mult
+                    acc += float(np.sum(w * cxmu[i, j]))
+            g[p] = 0.5 * acc - 0.5 * st["dld_p"][p]
+        g -= st["glp"]
+        self._state = None
+        return st["f"], g
+
+    def value_and_grad(self, theta):
+        self.forward(theta)
+        return self.backward()
+

@@ -1432,7 +1432,7 @@ __global__ __launch_bounds__(256) void k_ata_matvec_fe(dh_model m, const double*
 // <X_block, G_m> partials: block q < n is D_q, q >= n is L_{q-n}; one
 // workgroup per (q, 256 rows).  part[(q*nrb + rb)*4 + m].
 __global__ __launch_bounds__(256) void k_contract_fem(const double* diag, const double* lower, int n, int bp,
-                                                      dh_model m, double* part) {
+                                                      dh_model m, int roff, int coff, double* part) {
   const int q = blockIdx.y;
   const int rb = blockIdx.x;
   const int s = rb * 256 + threadIdx.x;
@@ -1440,7 +1440,7 @@ __global__ __launch_bounds__(256) void k_contract_fem(const double* diag, const 
   const long nn = m.fem_nnz;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   if (s < m.ns) {
-    const double* xr = X + (long)s * bp;
+    const double* xr = X + (long)(roff + s) * bp + coff;
     for (int e = m.fem_rowptr[s]; e < m.fem_rowptr[s + 1]; ++e) {
       const double xv = xr[m.fem_col[e]];
       a0 += xv * m.fem_val[e];
@@ -1459,9 +1459,11 @@ __global__ __launch_bounds__(256) void k_contract_fem(const double* diag, const 
   }
 }
 
-// x_t^T G_m x_t' partials (latent-ordered x): q < n -> (t,t) = (q,q);
-// q >= n -> rows at t+1, columns at t with t = q - n.
-__global__ __launch_bounds__(256) void k_contract_fem_vec(const double* x, int n, dh_model m, double* part) {
+// x_t^T G_m x_t' partials (x[t * stride + off + s]; the row side at roff, the
+// column side at coff): q < n -> (t,t) = (q,q); q >= n -> rows at t+1,
+// columns at t with t = q - n.
+__global__ __launch_bounds__(256) void k_contract_fem_vec(const double* x, int n, dh_model m, int stride, int roff,
+                                                          int coff, double* part) {
   const int q = blockIdx.y;
   const int rb = blockIdx.x;
   const int s = rb * 256 + threadIdx.x;
@@ -1470,8 +1472,8 @@ __global__ __launch_bounds__(256) void k_contract_fem_vec(const double* x, int n
   const long nn = m.fem_nnz;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   if (s < m.ns) {
-    const double xs = x[(long)tr * m.ns + s];
-    const double* xc = x + (long)tc * m.ns;
+    const double xs = x[(long)tr * stride + roff + s];
+    const double* xc = x + (long)tc * stride + coff;
     for (int e = m.fem_rowptr[s]; e < m.fem_rowptr[s + 1]; ++e) {
       const double xv = xs * xc[m.fem_col[e]];
       a0 += xv * m.fem_val[e];
@@ -1491,10 +1493,12 @@ __global__ __launch_bounds__(256) void k_contract_fem_vec(const double* x, int n
 }
 
 // <X, A^T A> over the spatio-temporal rows (both triangles), the arrow
-// (twice) and the tip; one thread per spatio-temporal row.
+// (twice) and the tip; one thread per spatio-temporal row.  The model's
+// nodes sit at offset voff inside every time block and its fixed effects at
+// arrow rows aoff .. aoff + a - 1.
 __global__ __launch_bounds__(256) void k_contract_ata(const double* diag, const double* lower, const double* arrow,
                                                       const double* tip, int n, int bp, int a, int ap, dh_model m,
-                                                      double* part) {
+                                                      int voff, int aoff, double* part) {
   const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long nst = (long)m.ns * m.nt;
   double acc = 0.0;
@@ -1504,18 +1508,44 @@ __global__ __launch_bounds__(256) void k_contract_ata(const double* diag, const 
       const int c = m.ata_col[e];
       const int t2 = c / m.ns, s2 = c - t2 * m.ns;
       double xv;
-      if (t2 == t) xv = diag[(long)t * bp * bp + (long)s * bp + s2];
-      else if (t2 == t - 1) xv = lower[(long)t2 * bp * bp + (long)s * bp + s2];
-      else xv = lower[(long)t * bp * bp + (long)s2 * bp + s];
+      if (t2 == t) xv = diag[(long)t * bp * bp + (long)(voff + s) * bp + voff + s2];
+      else if (t2 == t - 1) xv = lower[(long)t2 * bp * bp + (long)(voff + s) * bp + voff + s2];
+      else xv = lower[(long)t * bp * bp + (long)(voff + s2) * bp + voff + s];
       acc += xv * m.ata_val[e];
     }
     if (arrow)
       for (int j = 0; j < a; ++j)
-        acc += 2.0 * arrow[((long)t * ap + j) * bp + s] * m.ata_arrow[(long)j * nst + r];
+        acc += 2.0 * arrow[((long)t * ap + aoff + j) * bp + voff + s] * m.ata_arrow[(long)j * nst + r];
   }
   if (r == 0 && tip)
     for (int j = 0; j < a; ++j)
-      for (int k = 0; k < a; ++k) acc += tip[j * ap + k] * m.ata_tip[j * a + k];
+      for (int k = 0; k < a; ++k) acc += tip[(aoff + j) * ap + aoff + k] * m.ata_tip[j * a + k];
+  acc = dh_block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// x^T [A X]^T [A X] x for one model's observations (x[t * stride + voff + s]
+// for the nodes, xa[j] for its a fixed effects); one thread per node row.
+__global__ __launch_bounds__(256) void k_ata_quad(const double* x, const double* xa, int stride, int voff, int a,
+                                                  dh_model m, double* part) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nst = (long)m.ns * m.nt;
+  double acc = 0.0;
+  if (r < nst) {
+    const int t = (int)(r / m.ns), s = (int)(r % m.ns);
+    const double xr = x[(long)t * stride + voff + s];
+    double y = 0.0;
+    for (long e = m.ata_rowptr[r]; e < m.ata_rowptr[r + 1]; ++e) {
+      const int c = m.ata_col[e];
+      const int t2 = c / m.ns, s2 = c - t2 * m.ns;
+      y += m.ata_val[e] * x[(long)t2 * stride + voff + s2];
+    }
+    for (int j = 0; j < a; ++j) y += 2.0 * m.ata_arrow[(long)j * nst + r] * xa[j];
+    acc = xr * y;
+  }
+  if (r == 0)
+    for (int j = 0; j < a; ++j)
+      for (int k = 0; k < a; ++k) acc += xa[j] * m.ata_tip[j * a + k] * xa[k];
   acc = dh_block_sum(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
@@ -2383,11 +2413,22 @@ int dh_dot(dh_handle* h, const double* x, const double* y, long long len, double
   return DH_OK;
 }
 
-int dh_contract_fem(dh_handle* h, const dh_model* m, double* out_dev) {
-  if (!h || !out_dev || !model_ok(h, m)) return DH_EARG;
+static int contract_fem(dh_handle* h, const dh_model* m, int roff, int coff, double* out_dev) {
   const int nq = 2 * h->n - 1;
   const int nrb = (m->ns + 255) / 256;
   hipLaunchKernelGGL(k_contract_fem, dim3(nrb, nq), dim3(256), 0, h->stream, h->diag, h->lower, h->n, h->bp, *m,
+                     roff, coff, h->part);
+  hipLaunchKernelGGL(k_sum_seg, dim3((nq * 4 + 255) / 256), dim3(256), 0, h->stream, h->part, nq, nrb, 4,
+                     out_dev);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+static int contract_fem_vec(dh_handle* h, const dh_model* m, const double* x, int stride, int roff, int coff,
+                            double* out_dev) {
+  const int nq = 2 * h->n - 1;
+  const int nrb = (m->ns + 255) / 256;
+  hipLaunchKernelGGL(k_contract_fem_vec, dim3(nrb, nq), dim3(256), 0, h->stream, x, h->n, *m, stride, roff, coff,
                      h->part);
   hipLaunchKernelGGL(k_sum_seg, dim3((nq * 4 + 255) / 256), dim3(256), 0, h->stream, h->part, nq, nrb, 4,
                      out_dev);
@@ -2395,31 +2436,71 @@ int dh_contract_fem(dh_handle* h, const dh_model* m, double* out_dev) {
   return DH_OK;
 }
 
-int dh_contract_fem_vec(dh_handle* h, const dh_model* m, const double* x, double* out_dev) {
-  if (!h || !x || !out_dev || !model_ok(h, m)) return DH_EARG;
-  const int nq = 2 * h->n - 1;
-  const int nrb = (m->ns + 255) / 256;
-  hipLaunchKernelGGL(k_contract_fem_vec, dim3(nrb, nq), dim3(256), 0, h->stream, x, h->n, *m, h->part);
-  hipLaunchKernelGGL(k_sum_seg, dim3((nq * 4 + 255) / 256), dim3(256), 0, h->stream, h->part, nq, nrb, 4,
-                     out_dev);
+static int contract_ata(dh_handle* h, const dh_model* m, int voff, int aoff, bool arrow, double* out_dev) {
+  const long nst = (long)m->ns * m->nt;
+  const unsigned g = (unsigned)((nst + 255) / 256);
+  hipLaunchKernelGGL(k_contract_ata, dim3(g), dim3(256), 0, h->stream, h->diag, h->lower,
+                     arrow ? h->arrow : nullptr, arrow ? h->tip : nullptr, h->n, h->bp, m->nb, h->ap, *m, voff, aoff,
+                     h->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, h->stream, h->part, (int)g, 1, 1, out_dev);
   HIPCHK(hipGetLastError());
   return DH_OK;
 }
 
+// A model placed at node offset voff (and fixed-effect offset aoff) inside a handle with larger
+// blocks: the variates of a coregional model.
+static bool model_ok_v(const dh_handle* h, const dh_model* m, int voff, int aoff) {
+  return m && m->nt == h->n && m->nt >= 2 && m->ns >= 1 && voff >= 0 && voff + m->ns <= h->b && m->nb >= 0 &&
+         aoff >= 0 && (m->nb == 0 || aoff + m->nb <= h->a);
+}
+
+int dh_contract_fem(dh_handle* h, const dh_model* m, double* out_dev) {
+  if (!h || !out_dev || !model_ok(h, m)) return DH_EARG;
+  return contract_fem(h, m, 0, 0, out_dev);
+}
+
+int dh_contract_fem_v(dh_handle* h, const dh_model* m, int roff, int coff, double* out_dev) {
+  if (!h || !out_dev || !model_ok_v(h, m, roff, 0) || !model_ok_v(h, m, coff, 0)) return DH_EARG;
+  return contract_fem(h, m, roff, coff, out_dev);
+}
+
+int dh_contract_fem_vec(dh_handle* h, const dh_model* m, const double* x, double* out_dev) {
+  if (!h || !x || !out_dev || !model_ok(h, m)) return DH_EARG;
+  return contract_fem_vec(h, m, x, m->ns, 0, 0, out_dev);
+}
+
+int dh_contract_fem_vec_v(dh_handle* h, const dh_model* m, const double* x, int stride, int roff, int coff,
+                          double* out_dev) {
+  if (!h || !x || !out_dev || stride < h->b || !model_ok_v(h, m, roff, 0) || !model_ok_v(h, m, coff, 0))
+    return DH_EARG;
+  return contract_fem_vec(h, m, x, stride, roff, coff, out_dev);
+}
+
 int dh_contract_ata(dh_handle* h, const dh_model* m, double* out_dev) {
   if (!h || !out_dev || !model_ok(h, m)) return DH_EARG;
+  return contract_ata(h, m, 0, 0, h->a > 0 && m->nb == h->a, out_dev);
+}
+
+int dh_contract_ata_v(dh_handle* h, const dh_model* m, int voff, int aoff, double* out_dev) {
+  if (!h || !out_dev || !model_ok_v(h, m, voff, aoff)) return DH_EARG;
+  return contract_ata(h, m, voff, aoff, h->a > 0 && m->nb > 0, out_dev);
+}
+
+int dh_ata_quad_v(dh_handle* h, const dh_model* m, const double* x, int stride, int voff, int aoff,
+                  double* out_dev) {
+  if (!h || !x || !out_dev || stride < h->b || !model_ok_v(h, m, voff, aoff)) return DH_EARG;
   const long nst = (long)m->ns * m->nt;
   const unsigned g = (unsigned)((nst + 255) / 256);
-  const bool arrow = h->a > 0 && m->nb == h->a;
-  hipLaunchKernelGGL(k_contract_ata, dim3(g), dim3(256), 0, h->stream, h->diag, h->lower,
-                     arrow ? h->arrow : nullptr, arrow ? h->tip : nullptr, h->n, h->bp, h->a, h->ap, *m, h->part);
+  const int a = h->a > 0 ? m->nb : 0;
+  hipLaunchKernelGGL(k_ata_quad, dim3(g), dim3(256), 0, h->stream, x, x + (long)h->n * stride + aoff, stride, voff,
+                     a, *m, h->part);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, h->stream, h->part, (int)g, 1, 1, out_dev);
   HIPCHK(hipGetLastError());
   return DH_OK;
 }
 
 int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const double* gam, double* out_dev) {
-  if (!h || !lam || !gam || !out_dev || !model_ok(h, m)) return DH_EARG;
+  if (!h || !lam || !gam || !out_dev || !model_ok_v(h, m, 0, 0)) return DH_EARG;
   const unsigned g = (unsigned)((m->ns + 255) / 256);
   double* lbuf = h->eigbuf;
   double* dbuf = h->eigbuf + (long)h->n * h->bp;

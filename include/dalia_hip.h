@@ -32,7 +32,8 @@
  *   dh_qc_matvec             Q_conditional @ x inside DALIA._evaluate_conditional_latent_parameters
  *                            src/dalia/core/dalia.py:1161
  *   dh_scale, dh_dot         the vector algebra of core/model.py:446 and core/dalia.py:1161
- *   dh_contract_fem, dh_contract_fem_vec, dh_contract_ata, dh_prior_eig
+ *   dh_contract_fem, dh_contract_fem_vec, dh_contract_ata, dh_prior_eig,
+ *   dh_contract_fem_v, dh_contract_fem_vec_v, dh_contract_ata_v, dh_ata_quad_v
  *                            no counterpart: the adjoint gradient that replaces the
  *                            finite-difference gradient of DALIA._objective_function
  *                            src/dalia/core/dalia.py:513-601
@@ -162,6 +163,22 @@ int dh_contract_fem(dh_handle* h, const dh_model* m, double* out_dev);
 int dh_contract_fem_vec(dh_handle* h, const dh_model* m, const double* x, double* out_dev);
 /* After dh_selected_inversion (BTA): *out_dev = <X, A^T A> over all blocks. */
 int dh_contract_ata(dh_handle* h, const dh_model* m, double* out_dev);
+/* The same contractions for a model whose nodes sit at an offset inside larger
+ * blocks (variate k of a coregional model: offset k ns, fixed effects at arrow
+ * row k nb), the inputs of the coregional adjoint gradient; no counterpart in
+ * the reference, which differentiates the coregional f(theta) by central
+ * differences (core/dalia.py:513-601).
+ *   dh_contract_fem_v: <X sub-block (rows roff.., cols coff..), G_m> per block pair;
+ *   dh_contract_fem_vec_v: x_t[roff..]^T G_m x_t'[coff..] with x[t stride + ...];
+ *   dh_contract_ata_v: <X, [A X]^T [A X]> for the model's rows and fixed effects;
+ *   dh_ata_quad_v: x^T [A X]^T [A X] x, fixed effects read at x[n stride + aoff].
+ * Output layouts as for the unshifted calls. */
+int dh_contract_fem_v(dh_handle* h, const dh_model* m, int roff, int coff, double* out_dev);
+int dh_contract_fem_vec_v(dh_handle* h, const dh_model* m, const double* x, int stride, int roff, int coff,
+                          double* out_dev);
+int dh_contract_ata_v(dh_handle* h, const dh_model* m, int voff, int aoff, double* out_dev);
+int dh_ata_quad_v(dh_handle* h, const dh_model* m, const double* x, int stride, int voff, int aoff,
+                  double* out_dev);
 /* Kronecker-eigen closed form of the prior log-determinant:
  * out_dev[4] = (sum_j logdet T_j, d/dgamma_s, d/dgamma_t, d/dgamma_st). */
 int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const double* gam,
