@@ -389,10 +389,11 @@ class FTheta(torch.autograd.Function):
 
     Forward factors Qc, solves for mu and evaluates the log-determinants;
     backward overwrites the factor with the selected inverse and contracts it.
+    The engine is an `STInla` or a `dalia_amd.coreg.CoregInla`.
     """
 
     @staticmethod
-    def forward(ctx, theta: torch.Tensor, engine: STInla):
+    def forward(ctx, theta: torch.Tensor, engine):
         th = theta.detach().cpu().double().numpy()
         f_val = engine.forward(th)
         if f_val is None:  # no closed form: value needs the BT factorisation of Qp

@@ -81,3 +81,18 @@ def test_coreg_adjoint_gradient_matches_oracle(shape):
     assert np.abs(g - g_ref).max() <= 1e-6 * max(1.0, np.abs(g_ref).max()), (g, g_ref)
     # the engine is reusable after the selected inversion destroyed the factor
     assert abs(eng.f(th) - f_ref) <= 1e-10 * abs(f_ref)
+
+
+def test_coreg_autograd_node():
+    import torch
+
+    from dalia_amd.coreg import CoregInla
+    from dalia_amd.inla import FTheta
+    d = synth.make_coreg_model(4, 3, 2, 2, 13)
+    eng = CoregInla(d)
+    th = torch.tensor(d.theta, dtype=torch.float64, requires_grad=True)
+    f = FTheta.apply(th, eng)
+    f.backward()
+    f_ref, g_ref = eng.value_and_grad(d.theta)
+    assert abs(float(f) - f_ref) <= 1e-12 * abs(f_ref)
+    assert np.abs(th.grad.numpy() - g_ref).max() <= 1e-10 * max(1.0, np.abs(g_ref).max())
