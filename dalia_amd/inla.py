@@ -384,6 +384,23 @@ def hessian_from_values(fv: np.ndarray, d: int, h: float, offdiag_points: int = 
     return H
 
 
+def hessian_adjoint(engine, theta, h: float = 1e-3) -> np.ndarray:
+    """Hessian of f by central differences of the adjoint gradient: 2d gradient
+    evaluations instead of the 1 + 2d + 2d(d-1) values of the f-based stencil
+    (`core/dalia.py:767-910`); symmetrised.  `engine` is an `STInla` or a
+    `dalia_amd.coreg.CoregInla`; the selected inversions destroy its factor."""
+    theta = np.asarray(theta, dtype=np.float64)
+    d = theta.size
+    H = np.zeros((d, d))
+    for i in range(d):
+        e = np.zeros(d)
+        e[i] = h
+        _, gp = engine.value_and_grad(theta + e)
+        _, gm = engine.value_and_grad(theta - e)
+        H[i] = (gp - gm) / (2 * h)
+    return 0.5 * (H + H.T)
+
+
 class FTheta(torch.autograd.Function):
     """f(theta) as an autograd node; backward runs the adjoint on the device.
 

@@ -96,3 +96,14 @@ def test_coreg_autograd_node():
     f_ref, g_ref = eng.value_and_grad(d.theta)
     assert abs(float(f) - f_ref) <= 1e-12 * abs(f_ref)
     assert np.abs(th.grad.numpy() - g_ref).max() <= 1e-10 * max(1.0, np.abs(g_ref).max())
+
+
+def test_coreg_hessian_from_adjoint_gradients():
+    from dalia_amd.coreg import CoregInla
+    from dalia_amd.inla import hessian_adjoint
+    d = synth.make_coreg_model(4, 3, 1, 2, 17)
+    print("case hessian_adjoint coreg", flush=True)
+    eng = CoregInla(d)
+    H = hessian_adjoint(eng, d.theta, h=1e-3)
+    H_ref = oinla.fd_hessian(d, d.theta, 1e-3, fun=lambda th: ocoreg.f_theta_coreg(d, th))
+    assert np.abs(H - H_ref).max() <= 1e-4 * np.abs(H_ref).max(), (H, H_ref)

@@ -109,3 +109,13 @@ def test_marginal_variances(shape):
     assert np.abs(mv2 - ref).max() <= 1e-9 * np.abs(ref).max()
     with pytest.raises(ValueError):
         eng.marginal_variances_latent(theta, None)
+
+
+def test_hessian_from_adjoint_gradients_matches_fd_hessian():
+    from dalia_amd.inla import STInla, hessian_adjoint
+    d = synth.make_st_model(5, 4, 2, 21)
+    print("case hessian_adjoint st", flush=True)
+    eng = STInla(d)
+    H = hessian_adjoint(eng, d.theta, h=1e-3)
+    H_ref = oinla.fd_hessian(d, d.theta, 1e-3)
+    assert np.abs(H - H_ref).max() <= 1e-4 * np.abs(H_ref).max(), (H, H_ref)
