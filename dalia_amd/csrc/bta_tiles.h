@@ -46,21 +46,23 @@ __device__ __forceinline__ void dh_load4(const Opnd& o, int row, int k, double v
     const double2 x0 = *reinterpret_cast<const double2*>(q);
     const double2 x1 = *reinterpret_cast<const double2*>(q + 8);
     v[0] = x0.x; v[1] = x0.y; v[2] = x1.x; v[3] = x1.y;
-    if (MODE == M_UPK_ROW) {
+    if (MODE == M_UPK_ROW) {  // one unconditional load, then selects (no lane-divergent branch)
+      const double d = o.dinv[row];
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         const int kk = k + dh_koff(m);
-        v[m] = kk > row ? v[m] : (kk == row ? o.dinv[row] : 0.0);
+        v[m] = kk > row ? v[m] : (kk == row ? d : 0.0);
       }
     }
   } else {
     const double* q = o.p + (long)k * o.ld + row;
     v[0] = q[0]; v[1] = q[o.ld]; v[2] = q[8 * o.ld]; v[3] = q[9 * o.ld];
     if (MODE == M_UPK_COL) {
+      const double d = o.dinv[row];
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         const int kk = k + dh_koff(m);
-        v[m] = kk < row ? v[m] : (kk == row ? o.dinv[row] : 0.0);
+        v[m] = kk < row ? v[m] : (kk == row ? d : 0.0);
       }
     }
   }
@@ -96,7 +98,10 @@ __device__ __forceinline__ void dh_load16(const Opnd& A, const Opnd& B, int ra, 
   dh_load4<MB>(B, rb + 16, k, b[1]);
 }
 
-template <int MA, int MB, int S = DH_MMA_STAGES>
+// DESC: chunks in decreasing k (from khi down).  Tiles of a triangular operand whose k ranges
+// share their upper end then read the same chunks at the same time, as tiles with a common
+// lower end do in increasing order, and share them in L2.
+template <int MA, int MB, int S = DH_MMA_STAGES, bool DESC = false>
 __device__ __forceinline__ void dh_mma(dh_d4 acc[2][2], const Opnd& A, int i0, const Opnd& B, int j0,
                                        int klo, int khi) {
   if (klo >= khi) return;
@@ -106,17 +111,19 @@ __device__ __forceinline__ void dh_mma(dh_d4 acc[2][2], const Opnd& A, int i0, c
   const int rb = j0 + (w & 1) * 32 + (lane & 15);
   const int kq = (lane >> 4) * 2;  // group base offset within a 16-chunk
   const int nck = (khi - klo) >> 4;
+  const int k0 = (DESC ? khi - 16 : klo) + kq;
+  const int dk = DESC ? -16 : 16;
   double a[S][2][4], b[S][2][4];
 #pragma unroll
   for (int s = 0; s < S - 1; ++s)
-    if (s < nck) dh_load16<MA, MB>(A, B, ra, rb, klo + 16 * s + kq, a[s], b[s]);
+    if (s < nck) dh_load16<MA, MB>(A, B, ra, rb, k0 + dk * s, a[s], b[s]);
   for (int q = 0; q < nck; q += S) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int ck = q + s;
       if (ck < nck) {
         if (ck + S - 1 < nck)
-          dh_load16<MA, MB>(A, B, ra, rb, klo + 16 * (ck + S - 1) + kq, a[(s + S - 1) % S], b[(s + S - 1) % S]);
+          dh_load16<MA, MB>(A, B, ra, rb, k0 + dk * (ck + S - 1), a[(s + S - 1) % S], b[(s + S - 1) % S]);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
           acc[0][0] = dh_mfma(a[s][0][m], b[s][0][m], acc[0][0]);

@@ -11,7 +11,7 @@
 //                 packed into the strict upper triangle of each diagonal block)
 //   pobtas        k_pack/k_unpack, k_gemv_row, k_gemv_col_part, k_fin_*, k_upk_*
 //   pobtasi       k_sel_w, k_sel_ll (factor-only products, 8 blocks per launch),
-//                 k_sel_x, k_sel_s (the recursion), k_symm_copy
+//                 k_sel_x, k_sel_s, k_sel_sym (the recursion)
 //   logdet        k_logdet_part / k_logdet_fin
 //   adjoint       k_qp_matvec, k_ata_matvec*, k_contract_*, k_prior_eig
 //   variances     k_diag_extract, k_obs_variances
@@ -967,7 +967,7 @@ struct NodeList {
 };
 
 // T = L[o+s1 .. o+s, o .. o+s1] * inv(L11)  (T stored at the same position in W)
-__global__ __launch_bounds__(256) void k_trtri_t(double* Dbase, double* dinvbase, double* Wbase, int bp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_trtri_t(double* Dbase, double* dinvbase, double* Wbase, int bp,
                                                  int blk0, NodeList nl) {
   const int nd = blockIdx.y;
   if (nd >= nl.cnt) return;
@@ -985,13 +985,16 @@ __global__ __launch_bounds__(256) void k_trtri_t(double* Dbase, double* dinvbase
   Opnd B{D + off * bp + off, bp, M_UPK_ROW, dinv + off};  // opB = inv(L11)
   dh_d4 acc[2][2];
   dh_zero(acc);
-  dh_mma<M_ROW, M_UPK_ROW>(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, s1 * DH_TILE);
+  // inv(L11) needs the packed-triangle masking only in its diagonal tile (k < 64 (J + 1))
+  dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, true>(acc, A, I * DH_TILE, Opnd{B.p, bp, M_ROW, nullptr}, J * DH_TILE,
+                                            (J + 1) * DH_TILE, s1 * DH_TILE);
+  dh_mma<M_ROW, M_UPK_ROW>(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, (J + 1) * DH_TILE);
   double* C = W + (off + (long)s1 * DH_TILE) * bp + off;
   dh_store(C, bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
 }
 
 // inv21 = -inv(L22) T, stored transposed into the strict upper triangle.
-__global__ __launch_bounds__(256) void k_trtri_inv(double* Dbase, double* dinvbase, double* Wbase, int bp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_trtri_inv(double* Dbase, double* dinvbase, double* Wbase, int bp,
                                                    int blk0, NodeList nl) {
   const int nd = blockIdx.y;
   if (nd >= nl.cnt) return;
@@ -1010,7 +1013,9 @@ __global__ __launch_bounds__(256) void k_trtri_inv(double* Dbase, double* dinvba
   Opnd B{W + off2 * bp + off, bp, M_COL, nullptr};            // opB = T
   dh_d4 acc[2][2];
   dh_zero(acc);
-  dh_mma<M_UPK_COL, M_COL>(acc, A, I * DH_TILE, B, J * DH_TILE, 0, (I + 1) * DH_TILE);
+  // inv(L22) needs the packed-triangle masking only in its diagonal tile (k >= 64 I)
+  dh_mma<M_COL, M_COL>(acc, Opnd{A.p, bp, M_COL, nullptr}, I * DH_TILE, B, J * DH_TILE, 0, I * DH_TILE);
+  dh_mma<M_UPK_COL, M_COL>(acc, A, I * DH_TILE, B, J * DH_TILE, I * DH_TILE, (I + 1) * DH_TILE);
   // C[i][j] = inv21[i][j] lives at D[off + j][off2 + i]
   double* C = D + off * bp + off2;
   dh_store(C, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
@@ -1103,49 +1108,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int nw = a.Lb ? a.nbt * a.nbt : 0;
   if ((int)blockIdx.x >= nw + (a.R ? a.nbt : 0)) return;
   const Opnd linv{a.D, a.bp, M_UPK_ROW, a.dinv};  // opM[r][k] = inv(L)[k][r]
+  const Opnd lt{a.D, a.bp, M_ROW, nullptr};       // the same without the masking, for k >= 64 (row tile + 1)
   dh_d4 acc[2][2];
   dh_zero(acc);
   int e = blockIdx.x;
   if (e < nw) {
     const int I = e / a.nbt, J = e % a.nbt;
-    dh_mma<M_UPK_ROW, M_ROW>(acc, linv, I * DH_TILE, Opnd{a.Lb, a.bp, M_ROW, nullptr}, J * DH_TILE, I * DH_TILE, a.bp);
+    dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, true>(acc, lt, I * DH_TILE, Opnd{a.Lb, a.bp, M_ROW, nullptr}, J * DH_TILE,
+                                              (I + 1) * DH_TILE, a.bp);
+    dh_mma<M_UPK_ROW, M_ROW>(acc, linv, I * DH_TILE, Opnd{a.Lb, a.bp, M_ROW, nullptr}, J * DH_TILE, I * DH_TILE,
+                             (I + 1) * DH_TILE);
     dh_store(a.WT, a.wld, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
     return;
   }
   e -= nw;
-  dh_mma<M_ROW, M_UPK_ROW>(acc, Opnd{a.R, a.bp, M_ROW, nullptr}, 0, linv, e * DH_TILE, e * DH_TILE, a.bp);
+  dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, true>(acc, Opnd{a.R, a.bp, M_ROW, nullptr}, 0, lt, e * DH_TILE,
+                                            (e + 1) * DH_TILE, a.bp);
+  dh_mma<M_ROW, M_UPK_ROW>(acc, Opnd{a.R, a.bp, M_ROW, nullptr}, 0, linv, e * DH_TILE, e * DH_TILE,
+                           (e + 1) * DH_TILE);
   dh_store(a.Wa, a.bp, 0, e * DH_TILE, acc, 1.0, 0.0, false);
 }
 
-// X_{i+1,i} = -X_{i+1,i+1} W - X_{a,i+1}^T Wa  (X_{i+1,i+1} read by columns: it is symmetric) and
-// X_{a,i} = -X_{a,i+1} W - X_aa Wa.
+// X_{i+1,i} = -X_{i+1,i+1} W - X_{a,i+1}^T Wa  (X_{i+1,i+1} read by columns: it is symmetric).
+// nbt^2 workgroups: one round at four workgroups per CU on a 256-CU device when bp = 2048.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_x(SelArgs a) {
-  const int nx = a.Lb ? a.nbt * a.nbt : 0;
   dh_d4 acc[2][2], acc2[2][2];
   dh_zero(acc);
   dh_zero(acc2);
-  int e = blockIdx.x;
-  if (e < nx) {
-    const int I = e / a.nbt, J = e % a.nbt;
-    dh_mma<M_COL, M_ROW>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.WT, a.wld, M_ROW, nullptr},
-                         J * DH_TILE, 0, a.bp);
-    if (a.Rn)
-      dh_mma<M_COL, M_COL>(acc2, Opnd{a.Rn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.Wa, a.bp, M_COL, nullptr},
-                           J * DH_TILE, 0, DH_TILE);
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-      for (int y = 0; y < 2; ++y) acc[x][y] += acc2[x][y];
-    dh_store(a.Lb, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
-    return;
-  }
-  e -= nx;
+  const int e = blockIdx.x;
+  const int I = e / a.nbt, J = e % a.nbt;
+  dh_mma<M_COL, M_ROW>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.WT, a.wld, M_ROW, nullptr},
+                       J * DH_TILE, 0, a.bp);
   if (a.Rn)
-    dh_mma<M_ROW, M_ROW>(acc, Opnd{a.Rn, a.bp, M_ROW, nullptr}, 0, Opnd{a.WT, a.wld, M_ROW, nullptr}, e * DH_TILE,
-                         0, a.bp);
-  dh_mma<M_ROW, M_COL>(acc, Opnd{a.tip, DH_TILE, M_ROW, nullptr}, 0, Opnd{a.Wa, a.bp, M_COL, nullptr}, e * DH_TILE,
-                       0, DH_TILE);
-  dh_store(a.R, a.bp, 0, e * DH_TILE, acc, -1.0, 0.0, false);
+    dh_mma<M_COL, M_COL>(acc2, Opnd{a.Rn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.Wa, a.bp, M_COL, nullptr},
+                         J * DH_TILE, 0, DH_TILE);
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] += acc2[x][y];
+  dh_store(a.Lb, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
 }
 
 // Lower tiles of inv(L_ii)^T inv(L_ii) into S, for block slot blockIdx.y (factor only).
@@ -1159,34 +1160,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const Opnd linv{a.D, a.bp, M_UPK_ROW, a.dinv};
   dh_d4 acc[2][2];
   dh_zero(acc);
-  dh_mma<M_UPK_ROW, M_UPK_ROW>(acc, linv, I * DH_TILE, linv, J * DH_TILE, I * DH_TILE, a.bp);
+  const Opnd lt{a.D, a.bp, M_ROW, nullptr};  // the same rows without the masking (k >= 64 (I + 1))
+  dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, true>(acc, lt, I * DH_TILE, lt, J * DH_TILE, (I + 1) * DH_TILE, a.bp);
+  dh_mma<M_UPK_ROW, M_UPK_ROW>(acc, linv, I * DH_TILE, linv, J * DH_TILE, I * DH_TILE, (I + 1) * DH_TILE);
   dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
 }
 
-// Lower tiles of X_ii = inv(L)^T inv(L) - W^T X_{i+1,i} - Wa^T X_{a,i}: S (holding the first
-// term) -= the other two.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_s(SelArgs a) {
+// Lower tiles of S (holding inv(L)^T inv(L)) -= W^T X_{i+1,i} (workgroups e < ntri, ntri = 0 for
+// the last block), and the arrow row X_{a,i} = -X_{a,i+1} W - X_aa Wa (the nbt workgroups after
+// them): the two are independent, and the rank-64 term Wa^T X_{a,i} of X_ii is left to k_sel_sym.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_s(SelArgs a, int ntri) {
+  int e = blockIdx.x;
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  if (e < ntri) {
+    int I = 0;
+    while ((I + 1) * (I + 2) / 2 <= e) ++I;
+    const int J = e - I * (I + 1) / 2;
+    dh_mma<M_ROW, M_COL>(acc, Opnd{a.WT, a.wld, M_ROW, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
+                         J * DH_TILE, 0, a.bp);
+    dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 1.0, false);
+    return;
+  }
+  e -= ntri;
+  if (a.Rn)
+    dh_mma<M_ROW, M_ROW>(acc, Opnd{a.Rn, a.bp, M_ROW, nullptr}, 0, Opnd{a.WT, a.wld, M_ROW, nullptr}, e * DH_TILE,
+                         0, a.bp);
+  dh_mma<M_ROW, M_COL>(acc, Opnd{a.tip, DH_TILE, M_ROW, nullptr}, 0, Opnd{a.Wa, a.bp, M_COL, nullptr}, e * DH_TILE,
+                       0, DH_TILE);
+  dh_store(a.R, a.bp, 0, e * DH_TILE, acc, -1.0, 0.0, false);
+}
+
+// X_ii = S - Wa^T X_{a,i} on lower tile (I, J) (S alone without the arrowhead), written to D at
+// (I, J) and mirrored to (J, I).
+__global__ __launch_bounds__(256) void k_sel_sym(SelArgs a, double* D) {
   const int e = blockIdx.x;
   int I = 0;
   while ((I + 1) * (I + 2) / 2 <= e) ++I;
   const int J = e - I * (I + 1) / 2;
-  dh_d4 acc2[2][2];
-  dh_zero(acc2);
-  if (a.Lb)
-    dh_mma<M_ROW, M_COL>(acc2, Opnd{a.WT, a.wld, M_ROW, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
-                         J * DH_TILE, 0, a.bp);
+  dh_d4 acc[2][2];
+  dag_preload_neg(acc, a.S + (long)I * DH_TILE * a.bp + J * DH_TILE, a.bp);
   if (a.R)
-    dh_mma<M_COL, M_COL>(acc2, Opnd{a.Wa, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.R, a.bp, M_COL, nullptr},
+    dh_mma<M_COL, M_COL>(acc, Opnd{a.Wa, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.R, a.bp, M_COL, nullptr},
                          J * DH_TILE, 0, DH_TILE);
-  dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc2, -1.0, 1.0, false);
-}
-
-// D[r][c] = S[max(r,c)][min(r,c)] (lower tiles of S mirrored).
-__global__ __launch_bounds__(256) void k_symm_copy(double* D, const double* S, int bp) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)bp * bp) return;
-  const int r = (int)(e / bp), c = (int)(e % bp);
-  D[e] = (r >= c) ? S[(long)r * bp + c] : S[(long)c * bp + r];
+  dh_store(D, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
+  if (I != J) dh_store(D, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1703,6 +1721,7 @@ struct dh_handle {
   int tkind;                     // timed kernel kind, -1 none
   std::vector<hipEvent_t> ev;    // event pairs
   hipStream_t side;              // selected inversion: products of the factor alone, run ahead of the recursion
+  int side_on;                   // 1: use the side stream when there is one
   hipEvent_t sev[5];             // fork, chunk ready (2 slots), chunk slots free (2 slots)
   long long nrec;
 };
@@ -1832,6 +1851,7 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   const long long bb = (long long)h->bp * h->bp;
   h->G = n < 8 ? n : 8;
   h->side = nullptr;
+  h->side_on = 1;
   for (hipEvent_t& e : h->sev) e = nullptr;
   if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) h->side = nullptr;
   for (hipEvent_t& e : h->sev)
@@ -2261,7 +2281,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   // of the scratch slots, and fill the GPU while the recursion's launches drain.  Under stream
   // capture everything stays on the caller's stream.
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  const bool two = h->side && h->G >= 2 && n > h->G / 2 &&
+  const bool two = h->side && h->side_on && h->G >= 2 && n > h->G / 2 &&
                    hipStreamIsCapturing(h->stream, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
   const int G = two ? h->G / 2 : h->G;  // blocks per chunk
   const int nq = (n + G - 1) / G;
@@ -2307,11 +2327,12 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
     const int hi = n - 1 - q * G;
     for (int g = 0; g < sb.cnt; ++g) {
       const SelArgs& sa = sb.a[g];
-      const int nx = (sa.Lb ? nbt * nbt : 0) + (sa.R ? nbt : 0);
-      if (nx > 0) DH_LAUNCH(h, 2, k_sel_x, dim3(nx), dim3(256), 0, sa);  // X_{i+1,i}, X_{a,i}
-      DH_LAUNCH(h, 2, k_sel_s, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, sa);  // X_ii lower tiles
-      hipLaunchKernelGGL(k_symm_copy, dim3((unsigned)((bb + 255) / 256)), dim3(256), 0, h->stream,
-                         h->diag + (long)(hi - g) * bb, (const double*)sa.S, bp);
+      const int ntri = sa.Lb ? nbt * (nbt + 1) / 2 : 0;
+      if (sa.Lb) DH_LAUNCH(h, 2, k_sel_x, dim3(nbt * nbt), dim3(256), 0, sa);  // X_{i+1,i}
+      const int ns = ntri + (sa.R ? nbt : 0);  // S -= W^T X_{i+1,i}; X_{a,i}
+      if (ns > 0) DH_LAUNCH(h, 2, k_sel_s, dim3(ns), dim3(256), 0, sa, ntri);
+      DH_LAUNCH(h, 2, k_sel_sym, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, sa,
+                h->diag + (long)(hi - g) * bb);  // X_ii, both triangles
     }
   };
   if (!two) {
@@ -2515,6 +2536,7 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
   if (!h) return DH_EARG;
   if (key == 0 && (value == 0 || value == 1)) h->syrk_variant = value;
   else if (key == 2 && (value == 0 || value == 1)) h->pobtaf_alg = value;
+  else if (key == 3 && (value == 0 || value == 1)) h->side_on = value;
   else return DH_EARG;
   return DH_OK;
 }

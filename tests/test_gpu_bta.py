@@ -4,6 +4,8 @@ Bar: logdet within 1e-10 relative; factor, solve and selected-inverse blocks
 within 1e-9 relative Frobenius.
 """
 
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -57,14 +59,21 @@ def test_pobtaf_logdet_solve(n, b, a):
     assert rel(x.cpu().numpy(), xr) < 1e-9
 
 
+@pytest.mark.parametrize("side", [1, 0])
 @pytest.mark.parametrize("n,b,a", SHAPES)
-def test_pobtasi(n, b, a):
+def test_pobtasi(n, b, a, side):
+    """Selected inversion with the products of the factor on a second stream or not (side)."""
+    from dalia_amd import _lib
     from dalia_amd.solver import HipBTASolver
-    print(f"case pobtasi n={n} b={b} a={a}", flush=True)
+    print(f"case pobtasi n={n} b={b} a={a} side={side}", flush=True)
     M, S = random_bta(n, b, a, seed=n * 7 + b + a)
     s = HipBTASolver(diagonal_blocksize=b, n_diag_blocks=n, arrowhead_blocksize=a)
+    lib = _lib.load()
+    _lib.check(lib.dh_set_tuning(s.h, 3, side), "dh_set_tuning")
     s.cholesky(S, sparsity="bta")
     s.selected_inversion("bta")
+    info = ctypes.c_int(-1)
+    assert lib.dh_get_info(s.h, ctypes.byref(info)) == 0 and info.value == 0  # no flag wait gave up
     Minv = np.linalg.inv(M)
     Xd, Xl, Xr, Xt = obta.dense_to_bta(Minv, n, b, a)
     assert rel(s.A_diagonal_blocks.cpu().numpy(), Xd) < 1e-9
