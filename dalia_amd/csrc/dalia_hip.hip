@@ -1240,6 +1240,8 @@ __global__ __launch_bounds__(256) void k_unpack(const double* z, double* rhs, in
 }
 
 // y[r] = yin[r] + alpha * sum_c M[r][c] x[c]; one wave per row.
+// y[r] = yin[r] + alpha * M[r, :ncols] x (one wave per row, ncols even; double2 loads issued in
+// groups of four so that each wave keeps several KB in flight).
 __global__ __launch_bounds__(256) void k_gemv_row(const double* M, long ld, int nrows, int ncols, const double* x,
                                                   const double* yin, double* y, double alpha) {
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1247,10 +1249,19 @@ __global__ __launch_bounds__(256) void k_gemv_row(const double* M, long ld, int 
   if (r >= nrows) return;
   const double2* m2 = reinterpret_cast<const double2*>(M + (long)r * ld);
   const double2* x2 = reinterpret_cast<const double2*>(x);
+  const int np = ncols / 2;
   double s = 0.0;
-  for (int c = lane; c < ncols / 2; c += 64) {
-    double2 a = m2[c], v = x2[c];
-    s += a.x * v.x + a.y * v.y;
+  for (int q0 = lane; q0 < np; q0 += 256) {
+    double2 a[4], v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = min(q0 + 64 * u, np - 1);
+      a[u] = m2[q];
+      v[u] = x2[q];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (q0 + 64 * u < np) s += a[u].x * v[u].x + a[u].y * v[u].y;
   }
   s = dh_wave_sum(s);
   if (lane == 0) y[r] = (yin ? yin[r] : 0.0) + alpha * s;
@@ -1258,14 +1269,31 @@ __global__ __launch_bounds__(256) void k_gemv_row(const double* M, long ld, int 
 
 // Packed inverse factor, row form: y[c] = dinv[c] w[c] + sum_{r>c} D[c][r] w[r]
 // (y = inv(L)^T w).  One wave per row c.
+// y[c] = dinv[c] w[c] + sum_{r > c} D[c][r] w[r]: x = inv(L)^T w with inv(L)^T packed in the strict
+// upper triangle (one wave per row, double2 loads in groups of four).
 __global__ __launch_bounds__(256) void k_upk_row(const double* D, const double* dinv, int bp, const double* w,
                                                  double* y) {
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (c >= bp) return;
-  const double* row = D + (long)c * bp;
+  const double2* row = reinterpret_cast<const double2*>(D + (long)c * bp);
+  const double2* w2 = reinterpret_cast<const double2*>(w);
+  const int np = bp / 2, p0 = (c + 1) / 2;  // pairs (2q, 2q+1) with 2q+1 > c
   double s = 0.0;
-  for (int r = c + 1 + lane; r < bp; r += 64) s += row[r] * w[r];
+  for (int q0 = p0 + lane; q0 < np; q0 += 256) {
+    double2 a[4], v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = min(q0 + 64 * u, np - 1);
+      a[u] = row[q];
+      v[u] = w2[q];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + 64 * u;
+      if (q < np) s += (2 * q > c ? a[u].x * v[u].x : 0.0) + a[u].y * v[u].y;
+    }
+  }
   s = dh_wave_sum(s);
   if (lane == 0) y[c] = dinv[c] * w[c] + s;
 }
@@ -1274,47 +1302,112 @@ __global__ __launch_bounds__(256) void k_upk_row(const double* D, const double* 
 
 // part[chunk][c] = sum_{r in chunk} M[r][c] x[r] (only r < c when upk, for
 // the packed strict upper triangle: y = inv(L) v, y[c] = sum_{r<c} D[r][c] v[r]).
+// Column-oriented GEMV partials, one workgroup per 128 columns x DH_CCH rows: lane l of wave w
+// reads columns c0 + 2l, c0 + 2l + 1 (double2) of the rows r0 + 16w .. r0 + 16w + 15, all sixteen
+// loads issued together (rows past r1 are clamped and weighted by zero; with upk only rows r < c
+// count); the four waves' sums are added in LDS in a fixed order and part[chunk][c] written.
+template <bool UPK>
+__device__ __forceinline__ void gv_col_tile(const double* M, long ld, int r0, int r1, int ncols, const double* x,
+                                            double* prow) {
+  __shared__ double2 red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c0 = blockIdx.x * 128;
+  const int c = c0 + 2 * lane;
+  const int cc = min(c, ncols - 2);
+  double2 acc = {0.0, 0.0};
+  if (!(UPK && r0 >= c0 + 128) && r0 < r1) {  // workgroup-uniform
+    double2 m[16];
+    double xv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int r = r0 + 16 * w + u;
+      const int rc = min(r, r1 - 1);
+      m[u] = *reinterpret_cast<const double2*>(M + (long)rc * ld + cc);
+      xv[u] = r < r1 ? x[rc] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int r = r0 + 16 * w + u;
+      if (UPK) {
+        acc.x += (r < c ? m[u].x : 0.0) * xv[u];
+        acc.y += (r < c + 1 ? m[u].y : 0.0) * xv[u];
+      } else {
+        acc.x += m[u].x * xv[u];
+        acc.y += m[u].y * xv[u];
+      }
+    }
+  }
+  red[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && c < ncols) {
+    double2 t = red[0][lane];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+      t.x += red[k][lane].x;
+      t.y += red[k][lane].y;
+    }
+    *reinterpret_cast<double2*>(prow + c) = t;
+  }
+}
+
+// part[chunk][c] = sum_{r in chunk} M[r][c] x[r] (upk: only r < c; M is then the packed inverse
+// inv(L)^T, whose columns are the rows of inv(L)).  Grid (ceil(ncols/128), ceil(nrows/DH_CCH)).
 __global__ __launch_bounds__(256) void k_gemv_col_part(const double* M, long ld, int nrows, int ncols,
                                                        const double* x, double* part, int upk) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
   const int chunk = blockIdx.y;
-  if (c >= ncols) return;
-  const int r0 = chunk * DH_CCH;
-  int r1 = min(nrows, r0 + DH_CCH);
-  if (upk) r1 = min(r1, c);
-  double s = 0.0;
-  for (int r = r0; r < r1; ++r) s += M[(long)r * ld + c] * x[r];
-  part[(long)chunk * ncols + c] = s;
+  const int r0 = chunk * DH_CCH, r1 = min(nrows, r0 + DH_CCH);
+  if (upk)
+    gv_col_tile<true>(M, ld, r0, r1, ncols, x, part + (long)chunk * ncols);
+  else
+    gv_col_tile<false>(M, ld, r0, r1, ncols, x, part + (long)chunk * ncols);
 }
 
 // y[c] = base(c) + sign * sum_chunks part[chunk][c]; base = dinv[c] v[c] (fwd) or v[c] (bwd)
+// y[c] = base[c] + sign * sum_k part[k][c] (base = dinv[c] v[c], or v[c] without dinv): 64 columns
+// per workgroup, wave w adding the partials k = w, w + 4, ... (eight loads in flight), the four
+// wave sums added in LDS in a fixed order.
 __global__ __launch_bounds__(256) void k_fin_col(const double* part, int nch, int ncols, const double* v,
                                                  const double* dinv, double sign, double* y) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncols) return;
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  const int cc = min(c, ncols - 1);
   double s = 0.0;
-  for (int k = 0; k < nch; ++k) s += part[(long)k * ncols + c];
-  const double base = dinv ? dinv[c] * v[c] : v[c];
-  y[c] = base + sign * s;
+  for (int k0 = w; k0 < nch; k0 += 32) {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 + 4 * u;
+      t[u] = part[(long)min(k, nch - 1) * ncols + cc];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k0 + 4 * u < nch) s += t[u];
+  }
+  red[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && c < ncols) {
+    const double tot = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    const double base = dinv ? dinv[c] * v[c] : v[c];
+    y[c] = base + sign * tot;
+  }
 }
 
 // Two-source column partials: rows of L (nrows1) then of R (nrows2).
+// The same over two matrices stacked by rows: chunks of M1 (nr1 rows) then of M2 (nr2 rows).
 __global__ __launch_bounds__(256) void k_gemv_col_part2(const double* M1, int nr1, const double* x1,
                                                         const double* M2, int nr2, const double* x2, long ld,
                                                         int ncols, double* part) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
   const int chunk = blockIdx.y;
-  if (c >= ncols) return;
   const int nch1 = (nr1 + DH_CCH - 1) / DH_CCH;
-  double s = 0.0;
+  double* prow = part + (long)chunk * ncols;
   if (chunk < nch1) {
-    const int r0 = chunk * DH_CCH, r1 = min(nr1, r0 + DH_CCH);
-    for (int r = r0; r < r1; ++r) s += M1[(long)r * ld + c] * x1[r];
+    const int r0 = chunk * DH_CCH;
+    gv_col_tile<false>(M1, ld, r0, min(nr1, r0 + DH_CCH), ncols, x1, prow);
   } else {
-    const int r0 = (chunk - nch1) * DH_CCH, r1 = min(nr2, r0 + DH_CCH);
-    for (int r = r0; r < r1; ++r) s += M2[(long)r * ld + c] * x2[r];
+    const int r0 = (chunk - nch1) * DH_CCH;
+    gv_col_tile<false>(M2, ld, r0, min(nr2, r0 + DH_CCH), ncols, x2, prow);
   }
-  part[(long)chunk * ncols + c] = s;
 }
 
 // Arrowhead tip solves on a 64-vector: y = inv(L_aa) x (trans=0) or inv(L_aa)^T x.
@@ -2165,7 +2258,8 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
   const long N = (long)n * bp + 64;
   hipLaunchKernelGGL(k_pack, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, rhs, h->z, n, b, bp, a,
                      64);
-  const int ncb = (bp + 255) / 256;
+  const int ncb = (bp + 127) / 128;  // column blocks of the partial GEMVs
+  const int nfb = (bp + 63) / 64;    // column blocks of k_fin_col
   const int nch = (bp + DH_CCH - 1) / DH_CCH;
   double* za = h->z + (long)n * bp;
   // forward: y_i = inv(L_ii) (b_i - L_{i-1} y_{i-1}); arrow rows accumulate R_i y_i
@@ -2182,7 +2276,7 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
     }
     hipLaunchKernelGGL(k_gemv_col_part, dim3(ncb, nch), dim3(256), 0, h->stream, h->diag + i * bb, (long)bp, bp,
                        bp, src, h->part, 1);
-    hipLaunchKernelGGL(k_fin_col, dim3(ncb), dim3(256), 0, h->stream, h->part, nch, bp, src,
+    hipLaunchKernelGGL(k_fin_col, dim3(nfb), dim3(256), 0, h->stream, h->part, nch, bp, src,
                        h->dinv + (long)i * bp, 1.0, zi);
     if (arrow)
       hipLaunchKernelGGL(k_gemv_row, dim3(16), dim3(256), 0, h->stream, h->arrow + (long)i * ap * bp, (long)bp, 64,
@@ -2203,7 +2297,7 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
       hipLaunchKernelGGL(k_gemv_col_part2, dim3(ncb, nch1 + nch2), dim3(256), 0, h->stream,
                          hasL ? h->lower + i * bb : h->diag, hasL ? bp : 0, zi + bp,
                          arrow ? h->arrow + (long)i * ap * bp : h->diag, arrow ? 64 : 0, za, (long)bp, bp, h->part);
-      hipLaunchKernelGGL(k_fin_col, dim3(ncb), dim3(256), 0, h->stream, h->part, nch1 + nch2, bp, zi,
+      hipLaunchKernelGGL(k_fin_col, dim3(nfb), dim3(256), 0, h->stream, h->part, nch1 + nch2, bp, zi,
                          (const double*)nullptr, -1.0, h->v);
     } else {
       hipLaunchKernelGGL(k_copy, dim3((bp + 255) / 256), dim3(256), 0, h->stream, zi, h->v, bp);
