@@ -7,7 +7,7 @@
 //                 with per-tile readiness flags; k_leaf for the tip.  The per-panel
 //                 schedule (k_panel, k_update, k_syrk, k_syrk128 + k_syrk_tail) is the
 //                 autotuned alternative.
-//   inverse       k_trtri_t, k_trtri_inv (batched recursive inverse of L_ii,
+//   inverse       k_trtri_u, k_trtri_x (batched recursive inverse of L_ii,
 //                 packed into the strict upper triangle of each diagonal block)
 //   pobtas        k_pack/k_unpack, k_gemv_row, k_gemv_col_part, k_fin_*, k_upk_*
 //   pobtasi       k_sel_w, k_sel_ll (factor-only products, 8 blocks per launch),
@@ -966,36 +966,42 @@ struct NodeList {
   int o[DH_MAXNODES], s1[DH_MAXNODES], s[DH_MAXNODES];
 };
 
-// T = L[o+s1 .. o+s, o .. o+s1] * inv(L11)  (T stored at the same position in W)
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_trtri_t(double* Dbase, double* dinvbase, double* Wbase, int bp,
-                                                 int blk0, NodeList nl) {
+// One level of the recursive inverse, node (o, s1, s) in tile units: L = [[L11, 0], [L21, L22]]
+// with inv(L11), inv(L22) already packed (P[r][k] = inv(L)[k][r] above the diagonal), and
+// inv21 = -inv(L22) L21 inv(L11) in two products.  The first reads both operands by columns;
+// the second reads U by columns and only inv(L11) by rows (at the power-of-two block pitch,
+// products reading both operands by rows run at about three quarters of the speed).
+
+// U = inv(L22) L21, stored transposed (U^T at the position of L21^T in the scratch W).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_trtri_u(double* Dbase, double* dinvbase, double* Wbase, int bp,
+                                                                                      int blk0, NodeList nl) {
   const int nd = blockIdx.y;
   if (nd >= nl.cnt) return;
   const int o = nl.o[nd], s1 = nl.s1[nd], s = nl.s[nd];
   const int s2 = s - s1;
   const int x = blockIdx.x;
   if (x >= s2 * s1) return;
-  const int I = x / s1, J = x % s1;
+  const int I = x / s1, J = x % s1;  // tile of U: row tile I of L22's range, column tile J of L11's
   const long g = blk0 + blockIdx.z;
   double* D = Dbase + g * (long)bp * bp;
   double* W = Wbase + (long)blockIdx.z * bp * bp;
   const double* dinv = dinvbase + g * bp;
   const long off = (long)o * DH_TILE;
-  Opnd A{D + (off + (long)s1 * DH_TILE) * bp + off, bp, M_ROW, nullptr};
-  Opnd B{D + off * bp + off, bp, M_UPK_ROW, dinv + off};  // opB = inv(L11)
+  const long off2 = off + (long)s1 * DH_TILE;
+  Opnd A{D + off2 * bp + off2, bp, M_UPK_COL, dinv + off2};  // opA[i][k] = inv(L22)[i][k]
+  Opnd B{D + off2 * bp + off, bp, M_COL, nullptr};            // opB[j][k] = L21[k][j]
   dh_d4 acc[2][2];
   dh_zero(acc);
-  // inv(L11) needs the packed-triangle masking only in its diagonal tile (k < 64 (J + 1))
-  dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, true>(acc, A, I * DH_TILE, Opnd{B.p, bp, M_ROW, nullptr}, J * DH_TILE,
-                                            (J + 1) * DH_TILE, s1 * DH_TILE);
-  dh_mma<M_ROW, M_UPK_ROW>(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, (J + 1) * DH_TILE);
-  double* C = W + (off + (long)s1 * DH_TILE) * bp + off;
-  dh_store(C, bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
+  // inv(L22) needs the packed-triangle masking only in its diagonal tile (k >= 64 I)
+  dh_mma<M_COL, M_COL>(acc, Opnd{A.p, bp, M_COL, nullptr}, I * DH_TILE, B, J * DH_TILE, 0, I * DH_TILE);
+  dh_mma<M_UPK_COL, M_COL>(acc, A, I * DH_TILE, B, J * DH_TILE, I * DH_TILE, (I + 1) * DH_TILE);
+  // U[i][j] -> W[off + j][off2 + i]
+  dh_store(W + off * bp + off2, bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, true);
 }
 
-// inv21 = -inv(L22) T, stored transposed into the strict upper triangle.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_trtri_inv(double* Dbase, double* dinvbase, double* Wbase, int bp,
-                                                   int blk0, NodeList nl) {
+// inv21 = -U inv(L11), stored transposed into the strict upper triangle.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_trtri_x(double* Dbase, double* dinvbase, double* Wbase, int bp,
+                                                                                      int blk0, NodeList nl) {
   const int nd = blockIdx.y;
   if (nd >= nl.cnt) return;
   const int o = nl.o[nd], s1 = nl.s1[nd], s = nl.s[nd];
@@ -1009,16 +1015,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const double* dinv = dinvbase + g * bp;
   const long off = (long)o * DH_TILE;
   const long off2 = off + (long)s1 * DH_TILE;
-  Opnd A{D + off2 * bp + off2, bp, M_UPK_COL, dinv + off2};  // opA = inv(L22)
-  Opnd B{W + off2 * bp + off, bp, M_COL, nullptr};            // opB = T
+  Opnd A{W + off * bp + off2, bp, M_COL, nullptr};      // opA[i][k] = U[i][k] (U^T stored)
+  Opnd B{D + off * bp + off, bp, M_UPK_ROW, dinv + off};  // opB[j][k] = inv(L11)[k][j]
   dh_d4 acc[2][2];
   dh_zero(acc);
-  // inv(L22) needs the packed-triangle masking only in its diagonal tile (k >= 64 I)
-  dh_mma<M_COL, M_COL>(acc, Opnd{A.p, bp, M_COL, nullptr}, I * DH_TILE, B, J * DH_TILE, 0, I * DH_TILE);
-  dh_mma<M_UPK_COL, M_COL>(acc, A, I * DH_TILE, B, J * DH_TILE, I * DH_TILE, (I + 1) * DH_TILE);
+  // inv(L11) needs the packed-triangle masking only in its diagonal tile (k < 64 (J + 1))
+  dh_mma<M_COL, M_ROW, DH_MMA_STAGES, true>(acc, A, I * DH_TILE, Opnd{B.p, bp, M_ROW, nullptr}, J * DH_TILE,
+                                            (J + 1) * DH_TILE, s1 * DH_TILE);
+  dh_mma<M_COL, M_UPK_ROW>(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, (J + 1) * DH_TILE);
   // C[i][j] = inv21[i][j] lives at D[off + j][off2 + i]
-  double* C = D + off * bp + off2;
-  dh_store(C, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
+  dh_store(D + off * bp + off2, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -2237,8 +2243,8 @@ static int ensure_inverse(dh_handle* h) {
         if (t > maxt) maxt = t;
       }
       dim3 grid(maxt, nl.cnt, gc);
-      hipLaunchKernelGGL(k_trtri_t, grid, dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
-      hipLaunchKernelGGL(k_trtri_inv, grid, dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
+      hipLaunchKernelGGL(k_trtri_u, grid, dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
+      hipLaunchKernelGGL(k_trtri_x, grid, dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
     }
   }
   HIPCHK(hipGetLastError());
