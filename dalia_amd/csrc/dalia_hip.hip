@@ -1000,7 +1000,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 }
 
 // inv21 = -U inv(L11), stored transposed into the strict upper triangle.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_trtri_x(double* Dbase, double* dinvbase, double* Wbase, int bp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_trtri_x(double* Dbase, double* dinvbase, double* Wbase, int bp,
                                                                                       int blk0, NodeList nl) {
   const int nd = blockIdx.y;
   if (nd >= nl.cnt) return;
