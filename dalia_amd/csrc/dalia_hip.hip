@@ -1106,33 +1106,58 @@ struct SelBatch {
   int cnt;
 };
 
-// WT = inv(L_ii)^T L_{i+1,i}^T (tiles I, J; k >= 64 I) and Wa = L_{a,i} inv(L_ii) (k >= 64 J)
-// for block slot blockIdx.y: products of the factor alone, off the recursion's critical chain.
+// One tile of W^T = inv(L_ii)^T L_{i+1,i}^T (row tile I, k >= 64 I) or of Wa = L_{a,i} inv(L_ii)
+// (tile e, k >= 64 e; wa = true), k chunks in decreasing (desc) or increasing order.  The packed
+// inverse needs its masking only in the diagonal tile of the k range.
+template <bool DESC>
+__device__ __forceinline__ void sel_w_tile(const SelArgs& a, bool wa, int t, int J) {
+  const Opnd linv{a.D, a.bp, M_UPK_ROW, a.dinv};  // opM[r][k] = inv(L)[k][r]
+  const Opnd lt{a.D, a.bp, M_ROW, nullptr};       // the same without the masking, for k >= 64 (t + 1)
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  if (!wa) {
+    const Opnd lb{a.Lb, a.bp, M_ROW, nullptr};
+    if (!DESC) dh_mma<M_UPK_ROW, M_ROW>(acc, linv, t * DH_TILE, lb, J * DH_TILE, t * DH_TILE, (t + 1) * DH_TILE);
+    dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, DESC>(acc, lt, t * DH_TILE, lb, J * DH_TILE, (t + 1) * DH_TILE, a.bp);
+    if (DESC) dh_mma<M_UPK_ROW, M_ROW>(acc, linv, t * DH_TILE, lb, J * DH_TILE, t * DH_TILE, (t + 1) * DH_TILE);
+    dh_store(a.WT, a.wld, t * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
+  } else {
+    const Opnd r{a.R, a.bp, M_ROW, nullptr};
+    if (!DESC) dh_mma<M_ROW, M_UPK_ROW>(acc, r, 0, linv, t * DH_TILE, t * DH_TILE, (t + 1) * DH_TILE);
+    dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, DESC>(acc, r, 0, lt, t * DH_TILE, (t + 1) * DH_TILE, a.bp);
+    if (DESC) dh_mma<M_ROW, M_UPK_ROW>(acc, r, 0, linv, t * DH_TILE, t * DH_TILE, (t + 1) * DH_TILE);
+    dh_store(a.Wa, a.bp, 0, t * DH_TILE, acc, 1.0, 0.0, false);
+  }
+}
+
+// W^T and Wa for block slot blockIdx.y: products of the factor alone, off the recursion's
+// critical chain.  Tile row t has nbt - t k chunks, so each workgroup takes the pair of rows
+// (nbt - 1 - p, p), nbt + 1 chunks in all: the short one from the top of its k range down, then
+// the long one from its start up.  All workgroups then have the same length and, at any moment,
+// read only two chunk columns between them, which they share in L2.  With nbt odd the middle row
+// is a workgroup of its own.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_w(SelBatch sb) {
   if ((int)blockIdx.y >= sb.cnt) return;
   const SelArgs& a = sb.a[blockIdx.y];
-  const int nw = a.Lb ? a.nbt * a.nbt : 0;
-  if ((int)blockIdx.x >= nw + (a.R ? a.nbt : 0)) return;
-  const Opnd linv{a.D, a.bp, M_UPK_ROW, a.dinv};  // opM[r][k] = inv(L)[k][r]
-  const Opnd lt{a.D, a.bp, M_ROW, nullptr};       // the same without the masking, for k >= 64 (row tile + 1)
-  dh_d4 acc[2][2];
-  dh_zero(acc);
+  const int nbt = a.nbt, half = nbt / 2, odd = nbt & 1;
+  const int nw = a.Lb ? (half + odd) * nbt : 0;  // workgroups of W^T
+  const int na = a.R ? half + odd : 0;           // workgroups of Wa
   int e = blockIdx.x;
-  if (e < nw) {
-    const int I = e / a.nbt, J = e % a.nbt;
-    dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, true>(acc, lt, I * DH_TILE, Opnd{a.Lb, a.bp, M_ROW, nullptr}, J * DH_TILE,
-                                              (I + 1) * DH_TILE, a.bp);
-    dh_mma<M_UPK_ROW, M_ROW>(acc, linv, I * DH_TILE, Opnd{a.Lb, a.bp, M_ROW, nullptr}, J * DH_TILE, I * DH_TILE,
-                             (I + 1) * DH_TILE);
-    dh_store(a.WT, a.wld, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
-    return;
+  if (e >= nw + na) return;
+  const bool wa = e >= nw;
+  int p, J = 0;
+  if (!wa) {
+    p = e / nbt;
+    J = e % nbt;
+  } else {
+    p = e - nw;
   }
-  e -= nw;
-  dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, true>(acc, Opnd{a.R, a.bp, M_ROW, nullptr}, 0, lt, e * DH_TILE,
-                                            (e + 1) * DH_TILE, a.bp);
-  dh_mma<M_ROW, M_UPK_ROW>(acc, Opnd{a.R, a.bp, M_ROW, nullptr}, 0, linv, e * DH_TILE, e * DH_TILE,
-                           (e + 1) * DH_TILE);
-  dh_store(a.Wa, a.bp, 0, e * DH_TILE, acc, 1.0, 0.0, false);
+  if (p < half) {
+    sel_w_tile<true>(a, wa, nbt - 1 - p, J);
+    sel_w_tile<false>(a, wa, p, J);
+  } else {
+    sel_w_tile<true>(a, wa, p, J);  // the middle row (nbt odd)
+  }
 }
 
 // X_{i+1,i} = -X_{i+1,i+1} W - X_{a,i+1}^T Wa  (X_{i+1,i+1} read by columns: it is symmetric).
@@ -2409,7 +2434,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
       sa.wld = wld;
     }
   };
-  const int nw = nbt * nbt + (arrow ? nbt : 0);
+  const int nw = (nbt / 2 + (nbt & 1)) * (nbt + (arrow ? 1 : 0));  // k_sel_w workgroups per block
   auto factor_products = [&](int q, hipStream_t st) {  // W (as W^T), Wa, inv(L)^T inv(L) of chunk q
     SelBatch sb;
     batch(q, sb);
