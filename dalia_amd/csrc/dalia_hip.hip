@@ -999,8 +999,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   dh_store(W + off * bp + off2, bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, true);
 }
 
-// inv21 = -U inv(L11), stored transposed into the strict upper triangle.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_trtri_x(double* Dbase, double* dinvbase, double* Wbase, int bp,
+// inv21 = -U inv(L11), stored transposed into the strict upper triangle of D and as it is into Q
+// (inv(L) row-major, whose blocks inv(L11) of the deeper levels are complete: read by columns).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_trtri_x(double* Dbase, double* Qbase, double* Wbase, int bp,
                                                                                       int blk0, NodeList nl) {
   const int nd = blockIdx.y;
   if (nd >= nl.cnt) return;
@@ -1011,20 +1012,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const int I = x / s1, J = x % s1;
   const long g = blk0 + blockIdx.z;
   double* D = Dbase + g * (long)bp * bp;
+  double* Q = Qbase + g * (long)bp * bp;
   double* W = Wbase + (long)blockIdx.z * bp * bp;
-  const double* dinv = dinvbase + g * bp;
   const long off = (long)o * DH_TILE;
   const long off2 = off + (long)s1 * DH_TILE;
-  Opnd A{W + off * bp + off2, bp, M_COL, nullptr};      // opA[i][k] = U[i][k] (U^T stored)
-  Opnd B{D + off * bp + off, bp, M_UPK_ROW, dinv + off};  // opB[j][k] = inv(L11)[k][j]
+  const Opnd A{W + off * bp + off2, bp, M_COL, nullptr};  // opA[i][k] = U[i][k] (U^T stored)
+  const Opnd B{Q + off * bp + off, bp, M_COL, nullptr};   // opB[j][k] = inv(L11)[k][j]
   dh_d4 acc[2][2];
   dh_zero(acc);
-  // inv(L11) needs the packed-triangle masking only in its diagonal tile (k < 64 (J + 1))
-  dh_mma<M_COL, M_ROW, DH_MMA_STAGES, true>(acc, A, I * DH_TILE, Opnd{B.p, bp, M_ROW, nullptr}, J * DH_TILE,
-                                            (J + 1) * DH_TILE, s1 * DH_TILE);
-  dh_mma<M_COL, M_UPK_ROW>(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, (J + 1) * DH_TILE);
-  // C[i][j] = inv21[i][j] lives at D[off + j][off2 + i]
+  dh_mma<M_COL, M_COL, DH_MMA_STAGES, true>(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, s1 * DH_TILE);
+  // inv21[i][j] lives at D[off + j][off2 + i] and Q[off2 + i][off + j]
   dh_store(D + off * bp + off2, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
+  dh_store(Q + off2 * bp + off, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
+}
+
+// Q tile (t, t) = inv(L_tt) from the packed 64x64 tile inverse of the factorization
+// (inv(L)^T above the diagonal, 1/L_kk in dinv), zeros above the diagonal.  Grid (nbt, blocks).
+__global__ __launch_bounds__(256) void k_qdiag(const double* Dbase, const double* dinvbase, double* Qbase, int bp,
+                                               int blk0) {
+  __shared__ double t[DH_TILE][DH_TILE + 1];
+  const long g = blk0 + blockIdx.y;
+  const long t0 = (long)blockIdx.x * DH_TILE;
+  const double* D = Dbase + g * (long)bp * bp + t0 * bp + t0;
+  double* Q = Qbase + g * (long)bp * bp + t0 * bp + t0;
+  const double* dinv = dinvbase + g * bp + t0;
+  for (int e = threadIdx.x; e < DH_TILE * DH_TILE; e += 256) t[e >> 6][e & 63] = D[(long)(e >> 6) * bp + (e & 63)];
+  __syncthreads();
+  for (int e = threadIdx.x; e < DH_TILE * DH_TILE; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    Q[(long)r * bp + c] = c < r ? t[c][r] : (c == r ? dinv[r] : 0.0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1085,6 +1102,7 @@ __global__ __launch_bounds__(256) void k_gemm64(GArgs g) {
 struct SelArgs {
   const double* D;     // block i: L_ii below the diagonal, inv(L_ii)^T above it (ld bp)
   const double* dinv;  // 1 / diag(L_ii)
+  const double* Q;     // inv(L_ii) row-major, zeros above the diagonal (ld bp)
   double* Lb;          // L_{i+1,i} on entry, X_{i+1,i} on exit; null for the last block
   double* R;           // L_{a,i} on entry, X_{a,i} on exit; null without the arrowhead
   const double* Dn;    // X_{i+1,i+1} (symmetric, full)
@@ -1106,26 +1124,21 @@ struct SelBatch {
   int cnt;
 };
 
-// One tile of W^T = inv(L_ii)^T L_{i+1,i}^T (row tile I, k >= 64 I) or of Wa = L_{a,i} inv(L_ii)
-// (tile e, k >= 64 e; wa = true), k chunks in decreasing (desc) or increasing order.  The packed
-// inverse needs its masking only in the diagonal tile of the k range.
+// One tile of W^T = inv(L_ii)^T L_{i+1,i}^T (row tile t, k >= 64 t) or of Wa = L_{a,i} inv(L_ii)
+// (tile t; wa = true), k chunks in decreasing (DESC) or increasing order.  inv(L_ii) is read from
+// its row-major copy by columns (zeros above the diagonal: no masking).
 template <bool DESC>
 __device__ __forceinline__ void sel_w_tile(const SelArgs& a, bool wa, int t, int J) {
-  const Opnd linv{a.D, a.bp, M_UPK_ROW, a.dinv};  // opM[r][k] = inv(L)[k][r]
-  const Opnd lt{a.D, a.bp, M_ROW, nullptr};       // the same without the masking, for k >= 64 (t + 1)
+  const Opnd q{a.Q, a.bp, M_COL, nullptr};  // opM[r][k] = inv(L)[k][r], read by columns
   dh_d4 acc[2][2];
   dh_zero(acc);
   if (!wa) {
-    const Opnd lb{a.Lb, a.bp, M_ROW, nullptr};
-    if (!DESC) dh_mma<M_UPK_ROW, M_ROW>(acc, linv, t * DH_TILE, lb, J * DH_TILE, t * DH_TILE, (t + 1) * DH_TILE);
-    dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, DESC>(acc, lt, t * DH_TILE, lb, J * DH_TILE, (t + 1) * DH_TILE, a.bp);
-    if (DESC) dh_mma<M_UPK_ROW, M_ROW>(acc, linv, t * DH_TILE, lb, J * DH_TILE, t * DH_TILE, (t + 1) * DH_TILE);
+    dh_mma<M_COL, M_ROW, DH_MMA_STAGES, DESC>(acc, q, t * DH_TILE, Opnd{a.Lb, a.bp, M_ROW, nullptr}, J * DH_TILE,
+                                              t * DH_TILE, a.bp);
     dh_store(a.WT, a.wld, t * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
   } else {
-    const Opnd r{a.R, a.bp, M_ROW, nullptr};
-    if (!DESC) dh_mma<M_ROW, M_UPK_ROW>(acc, r, 0, linv, t * DH_TILE, t * DH_TILE, (t + 1) * DH_TILE);
-    dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, DESC>(acc, r, 0, lt, t * DH_TILE, (t + 1) * DH_TILE, a.bp);
-    if (DESC) dh_mma<M_ROW, M_UPK_ROW>(acc, r, 0, linv, t * DH_TILE, t * DH_TILE, (t + 1) * DH_TILE);
+    dh_mma<M_ROW, M_COL, DH_MMA_STAGES, DESC>(acc, Opnd{a.R, a.bp, M_ROW, nullptr}, 0, q, t * DH_TILE, t * DH_TILE,
+                                              a.bp);
     dh_store(a.Wa, a.bp, 0, t * DH_TILE, acc, 1.0, 0.0, false);
   }
 }
@@ -1181,19 +1194,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // Lower tiles of inv(L_ii)^T inv(L_ii) into S, for block slot blockIdx.y (factor only).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_sel_ll(SelBatch sb) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_ll(SelBatch sb) {
   if ((int)blockIdx.y >= sb.cnt) return;
   const SelArgs& a = sb.a[blockIdx.y];
   const int e = blockIdx.x;
   int I = 0;
   while ((I + 1) * (I + 2) / 2 <= e) ++I;
   const int J = e - I * (I + 1) / 2;
-  const Opnd linv{a.D, a.bp, M_UPK_ROW, a.dinv};
+  const Opnd q{a.Q, a.bp, M_COL, nullptr};  // opM[r][k] = inv(L)[k][r], read by columns
   dh_d4 acc[2][2];
   dh_zero(acc);
-  const Opnd lt{a.D, a.bp, M_ROW, nullptr};  // the same rows without the masking (k >= 64 (I + 1))
-  dh_mma<M_ROW, M_ROW, DH_MMA_STAGES, true>(acc, lt, I * DH_TILE, lt, J * DH_TILE, (I + 1) * DH_TILE, a.bp);
-  dh_mma<M_UPK_ROW, M_UPK_ROW>(acc, linv, I * DH_TILE, linv, J * DH_TILE, I * DH_TILE, (I + 1) * DH_TILE);
+  dh_mma<M_COL, M_COL, DH_MMA_STAGES, true>(acc, q, I * DH_TILE, q, J * DH_TILE, I * DH_TILE, a.bp);
   dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
 }
 
@@ -1840,6 +1851,7 @@ struct dh_handle {
   int* dagcounter;               // task counter of the persistent launch
   int* dagtab;                   // decode tables of the persistent launch (bta and bt schedules)
   double* dagpinv;               // dense inverses of the diagonal tiles, [2][nbt][64][64]
+  double* qinv;                  // [n][bp][bp] inv(L_ii) row-major (zeros above the diagonal tiles' diagonals)
   int dagt[2][2];                // [arrow][full block, last block] task counts
   int dag_wgs;                   // persistent workgroups
   int tkind;                     // timed kernel kind, -1 none
@@ -1975,7 +1987,7 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   const long long bb = (long long)h->bp * h->bp;
   h->G = n < 8 ? n : 8;
   h->side = nullptr;
-  h->side_on = 1;
+  h->side_on = 0;  // measured: one stream is faster once the factor-only products read inv(L) by columns
   for (hipEvent_t& e : h->sev) e = nullptr;
   if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) h->side = nullptr;
   for (hipEvent_t& e : h->sev)
@@ -2005,6 +2017,7 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   rc |= dalloc(h, &h->arrpart, (long long)n * 64);
   rc |= dalloc(h, &h->eigbuf, 2LL * n * h->bp);
   rc |= dalloc(h, &h->dagpinv, 2LL * h->nbt * 64 * 64);
+  rc |= dalloc(h, &h->qinv, n * bb);
   if (hipMalloc((void**)&h->info, sizeof(int)) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->flags, sizeof(int) * ((size_t)n * h->nbt + 1)) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->err, sizeof(int)) != hipSuccess) rc = DH_EHIP;
@@ -2035,7 +2048,7 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
 int dh_destroy(dh_handle* h) {
   if (!h) return DH_EARG;
   double* ps[] = {h->diag, h->lower, h->arrow, h->tip, h->dinv, h->pinv, h->tipinv, h->tipdinv,
-                  h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->eigbuf, h->dagpinv};
+                  h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->eigbuf, h->dagpinv, h->qinv};
   for (double* p : ps)
     if (p) (void)hipFree(p);
   if (h->info) (void)hipFree(h->info);
@@ -2260,6 +2273,8 @@ static int ensure_inverse(dh_handle* h) {
   const int bp = h->bp;
   for (int g0 = 0; g0 < h->n; g0 += h->G) {
     const int gc = (h->n - g0 < h->G) ? h->n - g0 : h->G;
+    hipLaunchKernelGGL(k_qdiag, dim3(h->nbt, gc), dim3(256), 0, h->stream, (const double*)h->diag,
+                       (const double*)h->dinv, h->qinv, bp, g0);
     for (int d = (int)h->levels.size() - 1; d >= 0; --d) {
       const NodeList& nl = h->levels[d];
       int maxt = 0;
@@ -2269,7 +2284,7 @@ static int ensure_inverse(dh_handle* h) {
       }
       dim3 grid(maxt, nl.cnt, gc);
       hipLaunchKernelGGL(k_trtri_u, grid, dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
-      hipLaunchKernelGGL(k_trtri_x, grid, dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
+      hipLaunchKernelGGL(k_trtri_x, grid, dim3(256), 0, h->stream, h->diag, h->qinv, h->W, bp, g0, nl);
     }
   }
   HIPCHK(hipGetLastError());
@@ -2420,6 +2435,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
       const int i = hi - g, sl = slot0 + g;
       SelArgs& sa = sb.a[g];
       sa.D = h->diag + i * bb;
+      sa.Q = h->qinv + i * bb;
       sa.dinv = h->dinv + (long)i * bp;
       sa.Lb = (i < n - 1) ? h->lower + i * bb : nullptr;
       sa.R = arrow ? h->arrow + (long)i * 64 * bp : nullptr;

@@ -206,7 +206,7 @@ int dh_obs_variances(dh_handle* h, long long n_obs, const long long* rowptr, con
  *        applies), 1 = one persistent launch of left-looking tile tasks
  *        synchronised by per-tile readiness flags (default);
  * key 3: selected inversion, 1 = products of the factor alone on a second
- *        stream ahead of the recursion (default), 0 = one stream.
+ *        stream ahead of the recursion, 0 = one stream (default).
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 
