@@ -1108,7 +1108,7 @@ struct SelArgs {
   const double* Dn;    // X_{i+1,i+1} (symmetric, full)
   const double* Rn;    // X_{a,i+1}, null for the last block or without the arrowhead
   const double* tip;   // X_aa (ld 64)
-  double* WT;          // W^T, W = L_{i+1,i} inv(L_ii) (ld wld)
+  double* W;           // W = L_{i+1,i} inv(L_ii), row-major (ld wld): read by columns as W[k][j]
   double* Wa;          // Wa = L_{a,i} inv(L_ii) (64 x bp)
   double* S;           // lower tiles of X_ii (ld bp)
   int bp, nbt, wld;
@@ -1135,7 +1135,7 @@ __device__ __forceinline__ void sel_w_tile(const SelArgs& a, bool wa, int t, int
   if (!wa) {
     dh_mma<M_COL, M_ROW, DH_MMA_STAGES, DESC>(acc, q, t * DH_TILE, Opnd{a.Lb, a.bp, M_ROW, nullptr}, J * DH_TILE,
                                               t * DH_TILE, a.bp);
-    dh_store(a.WT, a.wld, t * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
+    dh_store(a.W, a.wld, t * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, true);  // tile (t, J) of W^T
   } else {
     dh_mma<M_ROW, M_COL, DH_MMA_STAGES, DESC>(acc, Opnd{a.R, a.bp, M_ROW, nullptr}, 0, q, t * DH_TILE, t * DH_TILE,
                                               a.bp);
@@ -1143,8 +1143,9 @@ __device__ __forceinline__ void sel_w_tile(const SelArgs& a, bool wa, int t, int
   }
 }
 
-// W^T and Wa for block slot blockIdx.y: products of the factor alone, off the recursion's
-// critical chain.  Tile row t has nbt - t k chunks, so each workgroup takes the pair of rows
+// W (computed as W^T tiles, stored transposed) and Wa for block slot blockIdx.y: products of the
+// factor alone, off the recursion's critical chain.  Tile row t of W^T has nbt - t k chunks, so
+// each workgroup takes the pair of rows
 // (nbt - 1 - p, p), nbt + 1 chunks in all: the short one from the top of its k range down, then
 // the long one from its start up.  All workgroups then have the same length and, at any moment,
 // read only two chunk columns between them, which they share in L2.  With nbt odd the middle row
@@ -1181,7 +1182,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   dh_zero(acc2);
   const int e = blockIdx.x;
   const int I = e / a.nbt, J = e % a.nbt;
-  dh_mma<M_COL, M_ROW>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.WT, a.wld, M_ROW, nullptr},
+  dh_mma<M_COL, M_COL>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.W, a.wld, M_COL, nullptr},
                        J * DH_TILE, 0, a.bp);
   if (a.Rn)
     dh_mma<M_COL, M_COL>(acc2, Opnd{a.Rn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.Wa, a.bp, M_COL, nullptr},
@@ -1219,14 +1220,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     int I = 0;
     while ((I + 1) * (I + 2) / 2 <= e) ++I;
     const int J = e - I * (I + 1) / 2;
-    dh_mma<M_ROW, M_COL>(acc, Opnd{a.WT, a.wld, M_ROW, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
+    dh_mma<M_COL, M_COL>(acc, Opnd{a.W, a.wld, M_COL, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
                          J * DH_TILE, 0, a.bp);
     dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 1.0, false);
     return;
   }
   e -= ntri;
   if (a.Rn)
-    dh_mma<M_ROW, M_ROW>(acc, Opnd{a.Rn, a.bp, M_ROW, nullptr}, 0, Opnd{a.WT, a.wld, M_ROW, nullptr}, e * DH_TILE,
+    dh_mma<M_ROW, M_COL>(acc, Opnd{a.Rn, a.bp, M_ROW, nullptr}, 0, Opnd{a.W, a.wld, M_COL, nullptr}, e * DH_TILE,
                          0, a.bp);
   dh_mma<M_ROW, M_COL>(acc, Opnd{a.tip, DH_TILE, M_ROW, nullptr}, 0, Opnd{a.Wa, a.bp, M_COL, nullptr}, e * DH_TILE,
                        0, DH_TILE);
@@ -2006,7 +2007,7 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   rc |= dalloc(h, &h->pinv, 64 * 64);
   rc |= dalloc(h, &h->tipinv, 64 * 64);
   rc |= dalloc(h, &h->tipdinv, 64);
-  rc |= dalloc(h, &h->W, h->G * (long long)h->bp * (h->bp + 8));  // pobtasi W^T slots (padded pitch)
+  rc |= dalloc(h, &h->W, h->G * (long long)h->bp * (h->bp + 8));  // pobtasi W slots (padded pitch)
   rc |= dalloc(h, &h->S, h->G * bb);
   rc |= dalloc(h, &h->Wa, h->G * 64LL * h->bp);
   rc |= dalloc(h, &h->z, (long long)n * h->bp + 64);
@@ -2414,8 +2415,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
     gemm64<M_COL, M_COL>(h, gargs(Opnd{h->tipinv, 64, M_COL, nullptr}, Opnd{h->tipinv, 64, M_COL, nullptr}, 64,
                                   KLO_0, KHI_K, 1.0, 0.0, h->tip, 64, 1, 1, 0));
   }
-  const int wld = bp + 8;  // W^T pitch: rows of a power-of-two pitch read row-wise by one tile share
-                           // memory channels (the product reading them by rows runs ~30% slower)
+  const int wld = bp + 8;  // W pitch (the recursion reads W by columns, which is fast at any pitch)
   // The products of the factor alone (W, Wa, inv(L)^T inv(L)) of a chunk of blocks do not depend on
   // the recursion, so they run on a second stream one chunk ahead of it, in two alternating halves
   // of the scratch slots, and fill the GPU while the recursion's launches drain.  Under stream
@@ -2442,7 +2442,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
       sa.Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
       sa.Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * 64 * bp : nullptr;
       sa.tip = h->tip;
-      sa.WT = W + (long)sl * bp * wld;
+      sa.W = W + (long)sl * bp * wld;
       sa.Wa = h->Wa + (long)sl * 64 * bp;
       sa.S = S + (long)sl * bb;
       sa.bp = bp;
@@ -2451,7 +2451,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
     }
   };
   const int nw = (nbt / 2 + (nbt & 1)) * (nbt + (arrow ? 1 : 0));  // k_sel_w workgroups per block
-  auto factor_products = [&](int q, hipStream_t st) {  // W (as W^T), Wa, inv(L)^T inv(L) of chunk q
+  auto factor_products = [&](int q, hipStream_t st) {  // W, Wa, inv(L)^T inv(L) of chunk q
     SelBatch sb;
     batch(q, sb);
     if (st == h->stream) {
