@@ -1124,6 +1124,21 @@ struct SelBatch {
   int cnt;
 };
 
+// S slot = L_{i+1,i}^T for the blocks of a batch that have one (64x64 tiles through LDS), so that
+// k_sel_w reads both of its operands by columns; k_sel_ll overwrites the slot afterwards.
+__global__ __launch_bounds__(256) void k_lb_transpose(SelBatch sb) {
+  __shared__ double t[DH_TILE][DH_TILE + 1];
+  if ((int)blockIdx.y >= sb.cnt) return;
+  const SelArgs& a = sb.a[blockIdx.y];
+  if (!a.Lb) return;
+  const int I = blockIdx.x / a.nbt, J = blockIdx.x % a.nbt;
+  const double* src = a.Lb + (long)I * DH_TILE * a.bp + J * DH_TILE;
+  double* dst = a.S + (long)J * DH_TILE * a.bp + I * DH_TILE;
+  for (int e = threadIdx.x; e < DH_TILE * DH_TILE; e += 256) t[e >> 6][e & 63] = src[(long)(e >> 6) * a.bp + (e & 63)];
+  __syncthreads();
+  for (int e = threadIdx.x; e < DH_TILE * DH_TILE; e += 256) dst[(long)(e >> 6) * a.bp + (e & 63)] = t[e & 63][e >> 6];
+}
+
 // One tile of W^T = inv(L_ii)^T L_{i+1,i}^T (row tile t, k >= 64 t) or of Wa = L_{a,i} inv(L_ii)
 // (tile t; wa = true), k chunks in decreasing (DESC) or increasing order.  inv(L_ii) is read from
 // its row-major copy by columns (zeros above the diagonal: no masking).
@@ -1132,8 +1147,8 @@ __device__ __forceinline__ void sel_w_tile(const SelArgs& a, bool wa, int t, int
   const Opnd q{a.Q, a.bp, M_COL, nullptr};  // opM[r][k] = inv(L)[k][r], read by columns
   dh_d4 acc[2][2];
   dh_zero(acc);
-  if (!wa) {
-    dh_mma<M_COL, M_ROW, DH_MMA_STAGES, DESC>(acc, q, t * DH_TILE, Opnd{a.Lb, a.bp, M_ROW, nullptr}, J * DH_TILE,
+  if (!wa) {  // L_{i+1,i}^T is in the S slot (k_lb_transpose), read by columns
+    dh_mma<M_COL, M_COL, DH_MMA_STAGES, DESC>(acc, q, t * DH_TILE, Opnd{a.S, a.bp, M_COL, nullptr}, J * DH_TILE,
                                               t * DH_TILE, a.bp);
     dh_store(a.W, a.wld, t * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, true);  // tile (t, J) of W^T
   } else {
@@ -2455,9 +2470,11 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
     SelBatch sb;
     batch(q, sb);
     if (st == h->stream) {
+      DH_LAUNCH(h, 2, k_lb_transpose, dim3(nbt * nbt, sb.cnt), dim3(256), 0, sb);
       DH_LAUNCH(h, 2, k_sel_w, dim3(nw, sb.cnt), dim3(256), 0, sb);
       DH_LAUNCH(h, 2, k_sel_ll, dim3(nbt * (nbt + 1) / 2, sb.cnt), dim3(256), 0, sb);
     } else {
+      hipLaunchKernelGGL(k_lb_transpose, dim3(nbt * nbt, sb.cnt), dim3(256), 0, st, sb);
       hipLaunchKernelGGL(k_sel_w, dim3(nw, sb.cnt), dim3(256), 0, st, sb);
       hipLaunchKernelGGL(k_sel_ll, dim3(nbt * (nbt + 1) / 2, sb.cnt), dim3(256), 0, st, sb);
     }
