@@ -972,58 +972,88 @@ struct NodeList {
 // the second reads U by columns and only inv(L11) by rows (at the power-of-two block pitch,
 // products reading both operands by rows run at about three quarters of the speed).
 
-// U = inv(L22) L21, stored transposed (U^T at the position of L21^T in the scratch W).
+// Tiles (I, J) of U have I + 1 chunks and tiles of inv21 s1 - J: each workgroup takes a short and
+// a long one, the short one in the order that ends where the long one starts, so that all
+// workgroups of a launch have the same length and read only two chunk columns at a time (the odd
+// middle one is a workgroup of its own).
+
+// Tile (I, J) of U = inv(L22) L21, stored transposed (U^T at the position of L21^T in W).
+template <bool DESC>
+__device__ __forceinline__ void trtri_u_tile(double* D, const double* dinv, double* W, int bp, long off, long off2,
+                                             int I, int J) {
+  const Opnd A{D + off2 * bp + off2, bp, M_UPK_COL, dinv + off2};  // opA[i][k] = inv(L22)[i][k]
+  const Opnd Ap{A.p, bp, M_COL, nullptr};                           // the same without the masking, k < 64 I
+  const Opnd B{D + off2 * bp + off, bp, M_COL, nullptr};            // opB[j][k] = L21[k][j]
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  // inv(L22) needs the packed-triangle masking only in its diagonal tile (k >= 64 I)
+  if (DESC) dh_mma<M_UPK_COL, M_COL>(acc, A, I * DH_TILE, B, J * DH_TILE, I * DH_TILE, (I + 1) * DH_TILE);
+  dh_mma<M_COL, M_COL, DH_MMA_STAGES, DESC>(acc, Ap, I * DH_TILE, B, J * DH_TILE, 0, I * DH_TILE);
+  if (!DESC) dh_mma<M_UPK_COL, M_COL>(acc, A, I * DH_TILE, B, J * DH_TILE, I * DH_TILE, (I + 1) * DH_TILE);
+  dh_store(W + off * bp + off2, bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, true);  // U[i][j] -> W[off+j][off2+i]
+}
+
+// Tile (I, J) of inv21 = -U inv(L11), stored transposed into the strict upper triangle of D and as
+// it is into Q (inv(L) row-major, whose blocks inv(L11) of the deeper levels are complete).
+template <bool DESC>
+__device__ __forceinline__ void trtri_x_tile(double* D, double* Q, double* W, int bp, long off, long off2, int s1,
+                                             int I, int J) {
+  const Opnd A{W + off * bp + off2, bp, M_COL, nullptr};  // opA[i][k] = U[i][k] (U^T stored)
+  const Opnd B{Q + off * bp + off, bp, M_COL, nullptr};   // opB[j][k] = inv(L11)[k][j], zeros above the diagonal
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  dh_mma<M_COL, M_COL, DH_MMA_STAGES, DESC>(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, s1 * DH_TILE);
+  // inv21[i][j] lives at D[off + j][off2 + i] and Q[off2 + i][off + j]
+  dh_store(D + off * bp + off2, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
+  dh_store(Q + off2 * bp + off, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
+}
+
+// Grid (ceil(s2 / 2) * s1 over the nodes, nodes, blocks).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_trtri_u(double* Dbase, double* dinvbase, double* Wbase, int bp,
                                                                                       int blk0, NodeList nl) {
   const int nd = blockIdx.y;
   if (nd >= nl.cnt) return;
   const int o = nl.o[nd], s1 = nl.s1[nd], s = nl.s[nd];
-  const int s2 = s - s1;
+  const int s2 = s - s1, h2 = s2 / 2;
   const int x = blockIdx.x;
-  if (x >= s2 * s1) return;
-  const int I = x / s1, J = x % s1;  // tile of U: row tile I of L22's range, column tile J of L11's
+  if (x >= (h2 + (s2 & 1)) * s1) return;
+  const int p = x / s1, J = x % s1;
   const long g = blk0 + blockIdx.z;
   double* D = Dbase + g * (long)bp * bp;
   double* W = Wbase + (long)blockIdx.z * bp * bp;
   const double* dinv = dinvbase + g * bp;
   const long off = (long)o * DH_TILE;
   const long off2 = off + (long)s1 * DH_TILE;
-  Opnd A{D + off2 * bp + off2, bp, M_UPK_COL, dinv + off2};  // opA[i][k] = inv(L22)[i][k]
-  Opnd B{D + off2 * bp + off, bp, M_COL, nullptr};            // opB[j][k] = L21[k][j]
-  dh_d4 acc[2][2];
-  dh_zero(acc);
-  // inv(L22) needs the packed-triangle masking only in its diagonal tile (k >= 64 I)
-  dh_mma<M_COL, M_COL>(acc, Opnd{A.p, bp, M_COL, nullptr}, I * DH_TILE, B, J * DH_TILE, 0, I * DH_TILE);
-  dh_mma<M_UPK_COL, M_COL>(acc, A, I * DH_TILE, B, J * DH_TILE, I * DH_TILE, (I + 1) * DH_TILE);
-  // U[i][j] -> W[off + j][off2 + i]
-  dh_store(W + off * bp + off2, bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, true);
+  if (p < h2) {
+    trtri_u_tile<false>(D, dinv, W, bp, off, off2, p, J);           // p + 1 chunks, from 0 up
+    trtri_u_tile<true>(D, dinv, W, bp, off, off2, s2 - 1 - p, J);   // s2 - p chunks, down to 0
+  } else {
+    trtri_u_tile<false>(D, dinv, W, bp, off, off2, p, J);
+  }
 }
 
-// inv21 = -U inv(L11), stored transposed into the strict upper triangle of D and as it is into Q
-// (inv(L) row-major, whose blocks inv(L11) of the deeper levels are complete: read by columns).
+// Grid (s2 * ceil(s1 / 2) over the nodes, nodes, blocks).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_trtri_x(double* Dbase, double* Qbase, double* Wbase, int bp,
                                                                                       int blk0, NodeList nl) {
   const int nd = blockIdx.y;
   if (nd >= nl.cnt) return;
   const int o = nl.o[nd], s1 = nl.s1[nd], s = nl.s[nd];
-  const int s2 = s - s1;
+  const int s2 = s - s1, h1 = s1 / 2;
   const int x = blockIdx.x;
-  if (x >= s2 * s1) return;
-  const int I = x / s1, J = x % s1;
+  if (x >= s2 * (h1 + (s1 & 1))) return;
+  const int I = x % s2, p = x / s2;
   const long g = blk0 + blockIdx.z;
   double* D = Dbase + g * (long)bp * bp;
   double* Q = Qbase + g * (long)bp * bp;
   double* W = Wbase + (long)blockIdx.z * bp * bp;
   const long off = (long)o * DH_TILE;
   const long off2 = off + (long)s1 * DH_TILE;
-  const Opnd A{W + off * bp + off2, bp, M_COL, nullptr};  // opA[i][k] = U[i][k] (U^T stored)
-  const Opnd B{Q + off * bp + off, bp, M_COL, nullptr};   // opB[j][k] = inv(L11)[k][j]
-  dh_d4 acc[2][2];
-  dh_zero(acc);
-  dh_mma<M_COL, M_COL, DH_MMA_STAGES, true>(acc, A, I * DH_TILE, B, J * DH_TILE, J * DH_TILE, s1 * DH_TILE);
-  // inv21[i][j] lives at D[off + j][off2 + i] and Q[off2 + i][off + j]
-  dh_store(D + off * bp + off2, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
-  dh_store(Q + off2 * bp + off, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
+  if (p < h1) {
+    trtri_x_tile<true>(D, Q, W, bp, off, off2, s1, I, s1 - 1 - p);  // p + 1 chunks, from the top down
+    trtri_x_tile<false>(D, Q, W, bp, off, off2, s1, I, p);          // s1 - p chunks, from 64 p up
+  } else {
+    trtri_x_tile<true>(D, Q, W, bp, off, off2, s1, I, p);
+  }
 }
 
 // Q tile (t, t) = inv(L_tt) from the packed 64x64 tile inverse of the factorization
@@ -2293,14 +2323,14 @@ static int ensure_inverse(dh_handle* h) {
                        (const double*)h->dinv, h->qinv, bp, g0);
     for (int d = (int)h->levels.size() - 1; d >= 0; --d) {
       const NodeList& nl = h->levels[d];
-      int maxt = 0;
+      int mu = 0, mx = 0;  // workgroups per node of the two launches (short and long tiles paired)
       for (int k = 0; k < nl.cnt; ++k) {
-        int t = (nl.s[k] - nl.s1[k]) * nl.s1[k];
-        if (t > maxt) maxt = t;
+        const int s1 = nl.s1[k], s2 = nl.s[k] - nl.s1[k];
+        mu = std::max(mu, (s2 / 2 + (s2 & 1)) * s1);
+        mx = std::max(mx, s2 * (s1 / 2 + (s1 & 1)));
       }
-      dim3 grid(maxt, nl.cnt, gc);
-      hipLaunchKernelGGL(k_trtri_u, grid, dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
-      hipLaunchKernelGGL(k_trtri_x, grid, dim3(256), 0, h->stream, h->diag, h->qinv, h->W, bp, g0, nl);
+      hipLaunchKernelGGL(k_trtri_u, dim3(mu, nl.cnt, gc), dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
+      hipLaunchKernelGGL(k_trtri_x, dim3(mx, nl.cnt, gc), dim3(256), 0, h->stream, h->diag, h->qinv, h->W, bp, g0, nl);
     }
   }
   HIPCHK(hipGetLastError());
