@@ -554,11 +554,10 @@ __device__ __forceinline__ void dh_leaf32_lds(double* S, double* V, LeafShmT<2>&
     }
 }
 
-// Factor the 64x64 tile held in dh_lS (lower part; clobbered) into tile (ld):
-// L below and on the diagonal, inv(L)^T above it, dinv = 1/diag(L), and the
-// dense inv(L) (ld 64) into inv_out when given.  Same error convention as
-// dh_leaf_potrf.
-__device__ void dh_leaf64_lds(double* tile, long ld, double* dinv, double* inv_out, int* info, int row_base) {
+// Factor the 64x64 tile held in dh_lS (lower part): on exit dh_lS holds L in
+// its lower triangle and dh_lV the dense inv(L) (lower triangle); the stores
+// below write them out.  Same error convention as dh_leaf_potrf.
+__device__ void dh_leaf64_core(int* info, int row_base) {
   __shared__ LeafShmT<2> sh;
   const int tid = threadIdx.x;
   const int w = tid >> 6, l = tid & 63;
@@ -591,18 +590,36 @@ __device__ void dh_leaf64_lds(double* tile, long ld, double* dinv, double* inv_o
 #pragma unroll
   for (int r = 0; r < 4; ++r) V[(32 + 16 * bi + 4 * r + orow) * DH_LDS_LD + 16 * bj + ocol] = -x[r];
   __syncthreads();
+}
+
+// Dense inv(L) of the tile factored by dh_leaf64_core (zeros above the diagonal, ld 64).
+__device__ __forceinline__ void dh_leaf64_store_inv(double* inv_out) {
+  const double* V = dh_lV;
 #pragma unroll 2
   for (int e0 = 0; e0 < 64 * 64; e0 += 512) {  // two entries per thread, uniform trip count
-    const int e = e0 + 2 * tid;
+    const int e = e0 + 2 * threadIdx.x;
     const int i = e >> 6, j = e & 63;
-    double2 t, v;
-    t.x = (j <= i) ? S[i * DH_LDS_LD + j] : V[j * DH_LDS_LD + i];  // inv(L)^T above the diagonal
-    t.y = (j + 1 <= i) ? S[i * DH_LDS_LD + j + 1] : V[(j + 1) * DH_LDS_LD + i];
+    double2 v;
     v.x = (j <= i) ? V[i * DH_LDS_LD + j] : 0.0;
     v.y = (j + 1 <= i) ? V[i * DH_LDS_LD + j + 1] : 0.0;
+    *reinterpret_cast<double2*>(inv_out + e) = v;
+  }
+}
+
+// L (lower triangle), inv(L)^T (strict upper triangle) and dinv of the tile factored by
+// dh_leaf64_core.
+__device__ __forceinline__ void dh_leaf64_store_tile(double* tile, long ld, double* dinv) {
+  const double* S = dh_lS;
+  const double* V = dh_lV;
+#pragma unroll 2
+  for (int e0 = 0; e0 < 64 * 64; e0 += 512) {
+    const int e = e0 + 2 * threadIdx.x;
+    const int i = e >> 6, j = e & 63;
+    double2 t;
+    t.x = (j <= i) ? S[i * DH_LDS_LD + j] : V[j * DH_LDS_LD + i];  // inv(L)^T above the diagonal
+    t.y = (j + 1 <= i) ? S[i * DH_LDS_LD + j + 1] : V[(j + 1) * DH_LDS_LD + i];
     *reinterpret_cast<double2*>(tile + (long)i * ld + j) = t;
-    if (j == i) dinv[i] = v.x;
-    if (j + 1 == i) dinv[i] = v.y;
-    if (inv_out) *reinterpret_cast<double2*>(inv_out + e) = v;
+    if (j == i) dinv[i] = V[i * DH_LDS_LD + i];
+    if (j + 1 == i) dinv[i] = V[i * DH_LDS_LD + i];
   }
 }

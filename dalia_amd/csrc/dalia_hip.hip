@@ -889,8 +889,13 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
       for (int x1 = 0; x1 < 2; ++x1) acc[x0][x1] = part[x0][x1] - acc[x0][x1];
     dh_store(dh_lS, DH_LDS_LD, 0, 0, acc, 1.0, 0.0, false);
     __syncthreads();
-    dh_leaf64_lds(C, bp, dinv + c * DH_TILE, pv, g.info, i * g.b + c * DH_TILE);
-    fdone = fD + c * nbt + c;
+    dh_leaf64_core(g.info, i * g.b + c * DH_TILE);
+    // Within the launch the tile (c, c) is only ever used through its dense inverse, so the flag
+    // is raised once that is out; L, inv(L)^T and dinv follow (read after the launch).
+    dh_leaf64_store_inv(pv);
+    dag_signal(fD + c * nbt + c);
+    dh_leaf64_store_tile(C, bp, dinv + c * DH_TILE);
+    fdone = nullptr;
   } else if (kind == DK_OLD_X) {  // diagonal-block tile (r, c): previous block's terms, own terms k < ks
     if (i > 0 || ks > 0) dag_preload_neg(acc, D + r * T64 + c * DH_TILE, bp);
     if (i > 0)
@@ -935,7 +940,7 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
     }
     dag_solve_store(C, acc, pinv + c * DH_TILE * DH_TILE, bp, fD + c * nbt + c, g.err, s_n);
   }
-  dag_signal(fdone);
+  if (fdone) dag_signal(fdone);  // wave-uniform
 }
 
 // Tasks are claimed by wave 0 under a wave-uniform branch: lane 0 adds 1 and
