@@ -7,11 +7,14 @@
 //                 with per-tile readiness flags; k_leaf for the tip.  The per-panel
 //                 schedule (k_panel, k_update, k_syrk, k_syrk128 + k_syrk_tail) is the
 //                 autotuned alternative.
-//   inverse       k_trtri_u, k_trtri_x (batched recursive inverse of L_ii,
-//                 packed into the strict upper triangle of each diagonal block)
+//   inverse       k_qdiag, k_trtri_u, k_trtri_x (batched recursive inverse of L_ii,
+//                 packed into the strict upper triangle of each diagonal block and
+//                 written row-major into qinv)
 //   pobtas        k_pack/k_unpack, k_gemv_row, k_gemv_col_part, k_fin_*, k_upk_*
-//   pobtasi       k_sel_w, k_sel_ll (factor-only products, 8 blocks per launch),
-//                 k_sel_x, k_sel_s, k_sel_sym (the recursion)
+//   pobtasi       k_lb_transpose, k_sel_w, k_sel_ll (factor-only products, 8 blocks per
+//                 launch), k_sel_x, k_sel_s, k_sel_sym (the recursion); every operand of
+//                 these products is read along its columns (16 consecutive entries of 4
+//                 k rows per load), which runs ~40% faster than reading along rows
 //   logdet        k_logdet_part / k_logdet_fin
 //   adjoint       k_qp_matvec, k_ata_matvec*, k_contract_*, k_prior_eig
 //   variances     k_diag_extract, k_obs_variances
