@@ -15,7 +15,8 @@ from tests.bta_cases import random_bta
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(1, 64, 0), (3, 64, 0), (4, 100, 3), (3, 130, 5), (2, 200, 6), (5, 64, 64), (1, 70, 2), (6, 256, 4), (17, 96, 3)]
+SHAPES = [(1, 64, 0), (3, 64, 0), (4, 100, 3), (3, 130, 5), (2, 200, 6), (5, 64, 64), (1, 70, 2), (6, 256, 4), (17, 96, 3),
+          (5, 300, 4)]
 
 
 def rel(a, b):
