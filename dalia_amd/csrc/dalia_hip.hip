@@ -654,7 +654,7 @@ struct DagArgs {
 #define DH_DAG_SPIN (1 << 20)
 #define DAG_STAGES 4  // operand ring depth of the task products (one workgroup per CU)
 #define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed before QX(r) and after OLD(r))
-#define DH_DAG_SLOOK 6
+#define DH_DAG_SLOOK 8  // measured at ns = 2025, nt = 100: 6 -> 83.7 ms, 7 -> 81.1, 8 -> 80.3, 10 -> 81.7, 12 -> 82.9
 
 // Flags of one block: final D tiles [nbt][nbt], final L tiles [nbt][nbt],
 // final arrow tiles [nbt], tip partial applied, PRE_D, PRE_L, OLD_D, OLD_L done [nbt each],
