@@ -85,7 +85,7 @@ __device__ __forceinline__ void dh_zero(dh_d4 acc[2][2]) {
 // MFMAs of chunk q.  A wave issues one FP64 16x16x4 MFMA about every 140
 // cycles on gfx950, so one chunk ahead (S = 2, ~2200 cycles) covers the L2
 // latency and keeps the register count low enough for three waves per SIMD;
-// the persistent factorization (one workgroup per CU) uses a deeper ring.
+// the persistent factorization uses the same depth (deeper rings measured slower).
 #ifndef DH_MMA_STAGES
 #define DH_MMA_STAGES 2
 #endif

@@ -652,7 +652,7 @@ struct DagArgs {
 };
 
 #define DH_DAG_SPIN (1 << 20)
-#define DAG_STAGES 4  // operand ring depth of the task products (one workgroup per CU)
+#define DAG_STAGES 2  // operand ring depth of the task products (measured: 1 -> 77.9 ms, 2 -> 77.2, 3 -> 78.8, 4 -> 80.3, 5 -> 80.9)
 #define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed before QX(r) and after OLD(r))
 #define DH_DAG_SLOOK 8  // measured at ns = 2025, nt = 100: 6 -> 83.7 ms, 7 -> 81.1, 8 -> 80.3, 10 -> 81.7, 12 -> 82.9
 
