@@ -82,10 +82,10 @@ __device__ __forceinline__ void dh_zero(dh_d4 acc[2][2]) {
 // acc += opA[i0:i0+64, klo:khi] * opB[klo:khi, j0:j0+64] for this wave's quadrant.
 // k advances in chunks of 16 (klo, khi multiples of 16) held in a ring of S
 // register buffers: the loads of chunk q + S - 1 are issued before the 16
-// MFMAs of chunk q.  A wave issues one FP64 16x16x4 MFMA about every 140
-// cycles on gfx950, so one chunk ahead (S = 2, ~2200 cycles) covers the L2
-// latency and keeps the register count low enough for three waves per SIMD;
-// the persistent factorization uses the same depth (deeper rings measured slower).
+// MFMAs of chunk q.  One chunk ahead (S = 2) keeps the register count low
+// enough for three waves per SIMD, whose interleaving hides the rest of the
+// L2 latency; the persistent factorization uses the same depth (deeper rings
+// measured slower).
 #ifndef DH_MMA_STAGES
 #define DH_MMA_STAGES 2
 #endif

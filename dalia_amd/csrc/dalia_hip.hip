@@ -1153,9 +1153,9 @@ struct SelArgs {
 };
 
 #define DH_SELG 8  // blocks whose factor-only products are batched in one launch
-// The selected-inversion products are bound by the FP64 MFMA issue rate, which grows with the
-// waves per SIMD (one wave issues a 16x16x4 MFMA every ~140 cycles): their register budgets are
-// capped for four (three for k_sel_ll) waves per SIMD, which the compiler meets without spills.
+// The selected-inversion products are bound by the latency of their operand streams, which more
+// waves per SIMD hide: their register budgets are capped for four (three for k_sel_ll) waves per
+// SIMD, which the compiler meets without spills.
 
 struct SelBatch {
   SelArgs a[DH_SELG];
