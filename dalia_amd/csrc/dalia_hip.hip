@@ -10,6 +10,9 @@
 //   inverse       k_qdiag, k_trtri_u, k_trtri_x (batched recursive inverse of L_ii,
 //                 packed into the strict upper triangle of each diagonal block and
 //                 written row-major into qinv)
+//   overlap       k_dag_gate (+ k_trtri_u_side): optional second stream that runs the
+//                 inverse and the factor-only pobtasi products of finished blocks beside
+//                 the persistent factorization
 //   pobtas        k_pack/k_unpack, k_gemv_row, k_gemv_col_part, k_fin_*, k_upk_*
 //   pobtasi       k_lb_transpose, k_sel_w, k_sel_ll (factor-only products, 8 blocks per
 //                 launch), k_sel_x, k_sel_s, k_sel_sym (the recursion); every operand of
@@ -649,6 +652,7 @@ struct DagArgs {
   int ntasks;
   int look;        // helper lookahead in tile columns (>= 1)
   int slook;       // early-task lookahead of the other tiles (>= 1)
+  int* bdone;      // finished tasks per block (null: not counted)
 };
 
 #define DH_DAG_SPIN (1 << 20)
@@ -766,6 +770,17 @@ __device__ __forceinline__ void dag_signal(int* flag) {
   if (__builtin_amdgcn_readfirstlane(threadIdx.x) < 64) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// End of a task: raise its flag (may be null) and count it as finished for its block (count
+// may be null), both released at agent scope after every thread's stores.
+__device__ __forceinline__ void dag_finish(int* flag, int* count) {
+  __syncthreads();
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x) < 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (flag) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (count && threadIdx.x == 0) __hip_atomic_fetch_add(count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -943,7 +958,30 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
     }
     dag_solve_store(C, acc, pinv + c * DH_TILE * DH_TILE, bp, fD + c * nbt + c, g.err, s_n);
   }
-  if (fdone) dag_signal(fdone);  // wave-uniform
+  if (fdone || g.bdone) dag_finish(fdone, g.bdone ? g.bdone + i : nullptr);  // wave-uniform
+}
+
+// Waits until the persistent factorization has finished every task of blocks lo..lo+cnt-1, so
+// that later launches on this stream may read their factor while the factorization goes on with
+// the next blocks.  One lane per block (cnt <= 64); bounded like the factorization's own waits,
+// and it returns at once when the factorization has recorded a failure.
+#define DH_GATE_SPIN (1 << 24)
+__global__ __launch_bounds__(64) void k_dag_gate(const int* bdone, int lo, int cnt, int n, int tfull, int tlast,
+                                                 int* err) {
+  const int j = threadIdx.x;
+  if (j < cnt) {
+    const int want = (lo + j < n - 1) ? tfull : tlast;
+    for (int it = 0;; ++it) {
+      if (__hip_atomic_load(bdone + lo + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want) break;
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (it > DH_GATE_SPIN) {
+        atomicOr(err, 2);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // Tasks are claimed by wave 0 under a wave-uniform branch: lane 0 adds 1 and
@@ -1016,9 +1054,19 @@ __device__ __forceinline__ void trtri_x_tile(double* D, double* Q, double* W, in
   dh_store(Q + off2 * bp + off, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
 }
 
+// The factor-only kernels (inverse of the diagonal factors, W, inv(L)^T inv(L)) may run on a second
+// stream while the persistent factorization is still writing later blocks (acq = 1): each wave then
+// starts with an agent-scope acquire, so that no cache holds a line of the factor from before its
+// release.  Those launches fit beside the factorization's workgroups, which leave 144 VGPRs per
+// lane and 82 KB of LDS per CU (k_trtri_u_side is k_trtri_u capped at 128 VGPRs; measured: at 134
+// its workgroups were not placed until the factorization ended).  On their own (acq = 0) they
+// skip the acquire, which invalidates L2 under the other workgroups of the launch.
+#define DH_SIDE_ACQUIRE(acq) \
+  if (acq) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent")
+
 // Grid (ceil(s2 / 2) * s1 over the nodes, nodes, blocks).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_trtri_u(double* Dbase, double* dinvbase, double* Wbase, int bp,
-                                                                                      int blk0, NodeList nl) {
+__device__ __forceinline__ void trtri_u_block(double* Dbase, double* dinvbase, double* Wbase, int bp, int blk0,
+                                              const NodeList& nl) {
   const int nd = blockIdx.y;
   if (nd >= nl.cnt) return;
   const int o = nl.o[nd], s1 = nl.s1[nd], s = nl.s[nd];
@@ -1040,9 +1088,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
 }
 
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_trtri_u(double* Dbase, double* dinvbase,
+                                                                                      double* Wbase, int bp, int blk0,
+                                                                                      NodeList nl) {
+  trtri_u_block(Dbase, dinvbase, Wbase, bp, blk0, nl);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_trtri_u_side(double* Dbase, double* dinvbase,
+                                                                                           double* Wbase, int bp, int blk0,
+                                                                                           NodeList nl) {
+  DH_SIDE_ACQUIRE(1);
+  trtri_u_block(Dbase, dinvbase, Wbase, bp, blk0, nl);
+}
+
 // Grid (s2 * ceil(s1 / 2) over the nodes, nodes, blocks).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_trtri_x(double* Dbase, double* Qbase, double* Wbase, int bp,
-                                                                                      int blk0, NodeList nl) {
+                                                                                      int blk0, NodeList nl, int acq) {
+  DH_SIDE_ACQUIRE(acq);
   const int nd = blockIdx.y;
   if (nd >= nl.cnt) return;
   const int o = nl.o[nd], s1 = nl.s1[nd], s = nl.s[nd];
@@ -1067,7 +1129,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // Q tile (t, t) = inv(L_tt) from the packed 64x64 tile inverse of the factorization
 // (inv(L)^T above the diagonal, 1/L_kk in dinv), zeros above the diagonal.  Grid (nbt, blocks).
 __global__ __launch_bounds__(256) void k_qdiag(const double* Dbase, const double* dinvbase, double* Qbase, int bp,
-                                               int blk0) {
+                                               int blk0, int acq) {
+  DH_SIDE_ACQUIRE(acq);
   __shared__ double t[DH_TILE][DH_TILE + 1];
   const long g = blk0 + blockIdx.y;
   const long t0 = (long)blockIdx.x * DH_TILE;
@@ -1160,11 +1223,13 @@ struct SelArgs {
 struct SelBatch {
   SelArgs a[DH_SELG];
   int cnt;
+  int acq;  // beside the factorization: acquire first
 };
 
 // S slot = L_{i+1,i}^T for the blocks of a batch that have one (64x64 tiles through LDS), so that
 // k_sel_w reads both of its operands by columns; k_sel_ll overwrites the slot afterwards.
 __global__ __launch_bounds__(256) void k_lb_transpose(SelBatch sb) {
+  DH_SIDE_ACQUIRE(sb.acq);
   __shared__ double t[DH_TILE][DH_TILE + 1];
   if ((int)blockIdx.y >= sb.cnt) return;
   const SelArgs& a = sb.a[blockIdx.y];
@@ -1204,6 +1269,7 @@ __device__ __forceinline__ void sel_w_tile(const SelArgs& a, bool wa, int t, int
 // read only two chunk columns between them, which they share in L2.  With nbt odd the middle row
 // is a workgroup of its own.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_w(SelBatch sb) {
+  DH_SIDE_ACQUIRE(sb.acq);
   if ((int)blockIdx.y >= sb.cnt) return;
   const SelArgs& a = sb.a[blockIdx.y];
   const int nbt = a.nbt, half = nbt / 2, odd = nbt & 1;
@@ -1249,6 +1315,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // Lower tiles of inv(L_ii)^T inv(L_ii) into S, for block slot blockIdx.y (factor only).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_ll(SelBatch sb) {
+  DH_SIDE_ACQUIRE(sb.acq);
   if ((int)blockIdx.y >= sb.cnt) return;
   const SelArgs& a = sb.a[blockIdx.y];
   const int e = blockIdx.x;
@@ -1914,6 +1981,10 @@ struct dh_handle {
   int side_on;                   // 1: use the side stream when there is one
   hipEvent_t sev[5];             // fork, chunk ready (2 slots), chunk slots free (2 slots)
   long long nrec;
+  int overlap;                   // factor-only work run beside the persistent factorization: 0 none,
+                                 // 1 inverse of the diagonal factors, 2 also the pobtasi W, Wa, inv(L)^T inv(L)
+  double *Wall, *Sall, *Waall;   // per-block W, inv(L)^T inv(L), Wa slots (overlap 2)
+  int fp_ready;                  // sparsity whose factor products are in the per-block slots, -1 none
 };
 
 #define DH_MAXREC 16384
@@ -2028,6 +2099,8 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   h->nrec = 0;
   h->syrk_variant = 0;
   h->pobtaf_alg = 1;
+  h->overlap = 0;
+  h->fp_ready = -1;
   {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -2075,7 +2148,7 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   if (hipMalloc((void**)&h->info, sizeof(int)) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->flags, sizeof(int) * ((size_t)n * h->nbt + 1)) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->err, sizeof(int)) != hipSuccess) rc = DH_EHIP;
-  const size_t nflags = (size_t)n * dag_fstride(h->nbt);
+  const size_t nflags = (size_t)n * dag_fstride(h->nbt) + n;  // + finished tasks per block
   if (hipMalloc((void**)&h->dagflags, sizeof(int) * nflags) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->dagcounter, sizeof(int)) != hipSuccess) rc = DH_EHIP;
   if (rc == DH_OK) {
@@ -2102,7 +2175,8 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
 int dh_destroy(dh_handle* h) {
   if (!h) return DH_EARG;
   double* ps[] = {h->diag, h->lower, h->arrow, h->tip, h->dinv, h->pinv, h->tipinv, h->tipdinv,
-                  h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->eigbuf, h->dagpinv, h->qinv};
+                  h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->eigbuf, h->dagpinv, h->qinv,
+                  h->Wall, h->Sall, h->Waall};
   for (double* p : ps)
     if (p) (void)hipFree(p);
   if (h->info) (void)hipFree(h->info);
@@ -2218,16 +2292,86 @@ int dh_scatter_coo(dh_handle* h, long long nnz, const long long* rows, const lon
   return DH_OK;
 }
 
+// Packed inverse of the diagonal factors of blocks g0..g0+gc-1 on stream st (scratch: the W slots);
+// side: beside the persistent factorization.
+static void inverse_chunk(dh_handle* h, int g0, int gc, hipStream_t st, bool side) {
+  const int bp = h->bp;
+  hipLaunchKernelGGL(k_qdiag, dim3(h->nbt, gc), dim3(256), 0, st, (const double*)h->diag, (const double*)h->dinv,
+                     h->qinv, bp, g0, side ? 1 : 0);
+  for (int d = (int)h->levels.size() - 1; d >= 0; --d) {
+    const NodeList& nl = h->levels[d];
+    int mu = 0, mx = 0;  // workgroups per node of the two launches (short and long tiles paired)
+    for (int k = 0; k < nl.cnt; ++k) {
+      const int s1 = nl.s1[k], s2 = nl.s[k] - nl.s1[k];
+      mu = std::max(mu, (s2 / 2 + (s2 & 1)) * s1);
+      mx = std::max(mx, s2 * (s1 / 2 + (s1 & 1)));
+    }
+    if (side)
+      hipLaunchKernelGGL(k_trtri_u_side, dim3(mu, nl.cnt, gc), dim3(256), 0, st, h->diag, h->dinv, h->W, bp, g0, nl);
+    else
+      hipLaunchKernelGGL(k_trtri_u, dim3(mu, nl.cnt, gc), dim3(256), 0, st, h->diag, h->dinv, h->W, bp, g0, nl);
+    hipLaunchKernelGGL(k_trtri_x, dim3(mx, nl.cnt, gc), dim3(256), 0, st, h->diag, h->qinv, h->W, bp, g0, nl,
+                       side ? 1 : 0);
+  }
+}
+
+// Operands of block i for the selected-inversion kernels, with its W, Wa and S slots.
+static void sel_args(const dh_handle* h, int i, bool arrow, double* W, double* Wa, double* S, SelArgs& sa) {
+  const int n = h->n, bp = h->bp;
+  const long bb = (long)bp * bp;
+  sa.D = h->diag + i * bb;
+  sa.Q = h->qinv + i * bb;
+  sa.dinv = h->dinv + (long)i * bp;
+  sa.Lb = (i < n - 1) ? h->lower + i * bb : nullptr;
+  sa.R = arrow ? h->arrow + (long)i * 64 * bp : nullptr;
+  sa.Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
+  sa.Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * 64 * bp : nullptr;
+  sa.tip = h->tip;
+  sa.W = W;
+  sa.Wa = Wa;
+  sa.S = S;
+  sa.bp = bp;
+  sa.nbt = h->nbt;
+  sa.wld = bp + 8;
+}
+
+// Per-block slots of the factor products (overlap 2): W (ld bp + 8), Wa, S.
+static double* wall_slot(const dh_handle* h, int i) { return h->Wall + (long)i * h->bp * (h->bp + 8); }
+static double* waall_slot(const dh_handle* h, int i) { return h->Waall + (long)i * 64 * h->bp; }
+static double* sall_slot(const dh_handle* h, int i) { return h->Sall + (long)i * h->bp * h->bp; }
+
+// W, Wa and inv(L)^T inv(L) of blocks g0..g0+gc-1 (gc <= DH_SELG) into their per-block slots.
+static void factor_products_chunk(dh_handle* h, int g0, int gc, bool arrow, hipStream_t st) {
+  const int nbt = h->nbt;
+  SelBatch sb;
+  memset(&sb, 0, sizeof(sb));
+  sb.cnt = gc;
+  sb.acq = 1;
+  for (int g = 0; g < gc; ++g) {
+    const int i = g0 + g;
+    sel_args(h, i, arrow, wall_slot(h, i), waall_slot(h, i), sall_slot(h, i), sb.a[g]);
+  }
+  const int nw = (nbt / 2 + (nbt & 1)) * (nbt + (arrow ? 1 : 0));  // k_sel_w workgroups per block
+  hipLaunchKernelGGL(k_lb_transpose, dim3(nbt * nbt, gc), dim3(256), 0, st, sb);
+  hipLaunchKernelGGL(k_sel_w, dim3(nw, gc), dim3(256), 0, st, sb);
+  hipLaunchKernelGGL(k_sel_ll, dim3(nbt * (nbt + 1) / 2, gc), dim3(256), 0, st, sb);
+}
+
 int dh_cholesky(dh_handle* h, int sparsity) {
   if (!h) return DH_EARG;
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
   const int n = h->n, bp = h->bp, nbt = h->nbt;
   const long bb = (long)bp * bp;
+  h->fp_ready = -1;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->info, INT_MAX);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->err, 0);
   if (h->pobtaf_alg == 1) {
-    hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags, 0, n * dag_fstride(nbt));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    const bool ov = h->overlap > 0 && h->side && hipStreamIsCapturing(h->stream, &cap) == hipSuccess &&
+                    cap == hipStreamCaptureStatusNone;
+    const bool fp = ov && h->overlap == 2 && h->Wall;
+    hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags, 0, n * dag_fstride(nbt) + n);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
     const int ar = arrow ? 1 : 0;
     DagArgs g;
@@ -2251,14 +2395,34 @@ int dh_cholesky(dh_handle* h, int sparsity) {
     g.ntasks = (n - 1) * h->dagt[ar][0] + h->dagt[ar][1];
     g.look = DH_DAG_LOOK;
     g.slook = DH_DAG_SLOOK;
+    g.bdone = ov ? h->dagflags + (long)n * dag_fstride(nbt) : nullptr;
+    if (ov) {  // the side stream starts after the flags are cleared
+      HIPCHK(hipEventRecord(h->sev[0], h->stream));
+      HIPCHK(hipStreamWaitEvent(h->side, h->sev[0], 0));
+    }
     const unsigned wgs = (unsigned)std::min<int>(g.ntasks, h->dag_wgs);
     DH_LAUNCH(h, 7, k_pobtaf_dag, dim3(wgs), dim3(256), 0, g);
     if (arrow)
       hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, h->stream, h->tip, (long)h->ap, h->tipdinv, h->tipinv,
                          h->info, n * h->b);
+    if (ov) {
+      // Chunks of G blocks, in block order: each waits for its blocks' tasks, then inverts their
+      // diagonal factors and (overlap 2) forms their W, Wa and inv(L)^T inv(L), beside the
+      // factorization of the later blocks.  The caller's stream joins at the end.
+      for (int g0 = 0; g0 < n; g0 += h->G) {
+        const int gc = std::min(h->G, n - g0);
+        hipLaunchKernelGGL(k_dag_gate, dim3(1), dim3(64), 0, h->side, (const int*)g.bdone, g0, gc, n, g.tfull,
+                           h->dagt[ar][1], h->err);
+        inverse_chunk(h, g0, gc, h->side, true);
+        if (fp) factor_products_chunk(h, g0, gc, arrow, h->side);
+      }
+      HIPCHK(hipEventRecord(h->sev[1], h->side));
+      HIPCHK(hipStreamWaitEvent(h->stream, h->sev[1], 0));
+    }
     HIPCHK(hipGetLastError());
     h->factored = sparsity;
-    h->inverted = 0;
+    h->inverted = ov ? 1 : 0;
+    h->fp_ready = fp ? sparsity : -1;
     return DH_OK;
   }
   hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, h->stream, h->flags, 0, n * nbt);
@@ -2324,23 +2488,7 @@ int dh_cholesky(dh_handle* h, int sparsity) {
 // Packed inverse of every diagonal factor, batched G blocks per launch.
 static int ensure_inverse(dh_handle* h) {
   if (h->inverted) return DH_OK;
-  const int bp = h->bp;
-  for (int g0 = 0; g0 < h->n; g0 += h->G) {
-    const int gc = (h->n - g0 < h->G) ? h->n - g0 : h->G;
-    hipLaunchKernelGGL(k_qdiag, dim3(h->nbt, gc), dim3(256), 0, h->stream, (const double*)h->diag,
-                       (const double*)h->dinv, h->qinv, bp, g0);
-    for (int d = (int)h->levels.size() - 1; d >= 0; --d) {
-      const NodeList& nl = h->levels[d];
-      int mu = 0, mx = 0;  // workgroups per node of the two launches (short and long tiles paired)
-      for (int k = 0; k < nl.cnt; ++k) {
-        const int s1 = nl.s1[k], s2 = nl.s[k] - nl.s1[k];
-        mu = std::max(mu, (s2 / 2 + (s2 & 1)) * s1);
-        mx = std::max(mx, s2 * (s1 / 2 + (s1 & 1)));
-      }
-      hipLaunchKernelGGL(k_trtri_u, dim3(mu, nl.cnt, gc), dim3(256), 0, h->stream, h->diag, h->dinv, h->W, bp, g0, nl);
-      hipLaunchKernelGGL(k_trtri_x, dim3(mx, nl.cnt, gc), dim3(256), 0, h->stream, h->diag, h->qinv, h->W, bp, g0, nl);
-    }
-  }
+  for (int g0 = 0; g0 < h->n; g0 += h->G) inverse_chunk(h, g0, std::min(h->G, h->n - g0), h->stream, false);
   HIPCHK(hipGetLastError());
   h->inverted = 1;
   return DH_OK;
@@ -2473,8 +2621,10 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   // the recursion, so they run on a second stream one chunk ahead of it, in two alternating halves
   // of the scratch slots, and fill the GPU while the recursion's launches drain.  Under stream
   // capture everything stays on the caller's stream.
+  // pre: the factorization already formed them beside its own work, in per-block slots
+  const bool pre = h->fp_ready == sparsity;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  const bool two = h->side && h->side_on && h->G >= 2 && n > h->G / 2 &&
+  const bool two = !pre && h->side && h->side_on && h->G >= 2 && n > h->G / 2 &&
                    hipStreamIsCapturing(h->stream, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
   const int G = two ? h->G / 2 : h->G;  // blocks per chunk
   const int nq = (n + G - 1) / G;
@@ -2486,21 +2636,10 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
     sb.cnt = cnt;
     for (int g = 0; g < cnt; ++g) {
       const int i = hi - g, sl = slot0 + g;
-      SelArgs& sa = sb.a[g];
-      sa.D = h->diag + i * bb;
-      sa.Q = h->qinv + i * bb;
-      sa.dinv = h->dinv + (long)i * bp;
-      sa.Lb = (i < n - 1) ? h->lower + i * bb : nullptr;
-      sa.R = arrow ? h->arrow + (long)i * 64 * bp : nullptr;
-      sa.Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
-      sa.Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * 64 * bp : nullptr;
-      sa.tip = h->tip;
-      sa.W = W + (long)sl * bp * wld;
-      sa.Wa = h->Wa + (long)sl * 64 * bp;
-      sa.S = S + (long)sl * bb;
-      sa.bp = bp;
-      sa.nbt = nbt;
-      sa.wld = wld;
+      if (pre)
+        sel_args(h, i, arrow, wall_slot(h, i), waall_slot(h, i), sall_slot(h, i), sb.a[g]);
+      else
+        sel_args(h, i, arrow, W + (long)sl * bp * wld, h->Wa + (long)sl * 64 * bp, S + (long)sl * bb, sb.a[g]);
     }
   };
   const int nw = (nbt / 2 + (nbt & 1)) * (nbt + (arrow ? 1 : 0));  // k_sel_w workgroups per block
@@ -2533,7 +2672,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   };
   if (!two) {
     for (int q = 0; q < nq; ++q) {
-      factor_products(q, h->stream);
+      if (!pre) factor_products(q, h->stream);
       recursion(q);
     }
   } else {
@@ -2557,6 +2696,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   HIPCHK(hipGetLastError());
   h->factored = -1;
   h->inverted = 0;
+  h->fp_ready = -1;
   return DH_OK;
 }
 
@@ -2733,7 +2873,25 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
   if (key == 0 && (value == 0 || value == 1)) h->syrk_variant = value;
   else if (key == 2 && (value == 0 || value == 1)) h->pobtaf_alg = value;
   else if (key == 3 && (value == 0 || value == 1)) h->side_on = value;
-  else return DH_EARG;
+  else if (key == 4 && value >= 0 && value <= 2) {
+    if (value == 2 && !h->Wall) {  // per-block slots, kept until the handle is destroyed
+      const long long bb = (long long)h->bp * h->bp, before = h->bytes;
+      int rc = dalloc(h, &h->Wall, h->n * (long long)h->bp * (h->bp + 8));
+      rc |= dalloc(h, &h->Sall, h->n * bb);
+      rc |= dalloc(h, &h->Waall, h->n * 64LL * h->bp);
+      if (rc != DH_OK) {
+        double** ps[] = {&h->Wall, &h->Sall, &h->Waall};
+        for (double** p : ps)
+          if (*p) {
+            (void)hipFree(*p);
+            *p = nullptr;
+          }
+        h->bytes = before;  // the handle stays usable without them
+        return DH_EHIP;
+      }
+    }
+    h->overlap = value;
+  } else return DH_EARG;
   return DH_OK;
 }
 

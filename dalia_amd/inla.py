@@ -43,7 +43,11 @@ from dalia_amd.solver import HipBTASolver
 
 
 class STInla:
-    def __init__(self, data, device="cuda", closed_form_prior: bool = True):
+    def __init__(self, data, device="cuda", closed_form_prior: bool = True, overlap: int = 0):
+        """overlap: work run beside each factorization on a second stream (`HipBTASolver.set_overlap`),
+        at most this mode: 0 none (default), 1 the inverse of the diagonal factors, 2 also the
+        selected inversion's products of the factor.  Measured at ns = 2025, nt = 100: the adjoint
+        gradient gains 1.3 % with 2, as the factorization slows down beside that work."""
         self.data = data
         self.dm = DeviceSTModel(data, device=device, eigen=closed_form_prior)
         self.solver = HipBTASolver(diagonal_blocksize=data.ns, n_diag_blocks=data.nt,
@@ -67,6 +71,9 @@ class STInla:
         self._graph = None
         self.n_f_evals = 0
         self._theta_last = None  # last theta evaluated (the reference keeps it in model.theta)
+        self._ov_max = int(overlap)
+        self._ov = 0          # current overlap mode of the handle
+        self._ov2_ok = None   # whether mode 2's per-block slots were reserved
 
     # ------------------------------------------------------------------
     def _set_params(self, theta):
@@ -82,16 +89,34 @@ class STInla:
         _lib.check(self.lib.dh_assemble(self.h, self.dm.ref(), self.params.data_ptr(),
                                         _lib.SPARSITY[sparsity], int(with_like)), "dh_assemble")
 
-    def _conditional(self, theta, out=None):
+    def _overlap(self, mode: int) -> None:
+        """Work the next factorizations run beside themselves: 0 none, 1 the inverse of the
+        diagonal factors (the solve), 2 also the selected inversion's products of the factor
+        (falls back to 1 when their per-block slots do not fit); capped by the engine's option."""
+        mode = min(mode, self._ov_max)
+        if mode == 2:
+            if self._ov2_ok is None:
+                self._ov2_ok = self.solver.set_overlap(2)
+                self._ov = 2 if self._ov2_ok else self._ov
+            if not self._ov2_ok:
+                mode = 1
+        if mode != self._ov:
+            self.solver.set_overlap(mode)
+            self._ov = mode
+
+    def _conditional(self, theta, out=None, grad: bool = False):
         """Assemble Qc, factor, solve for mu; queue logdet and the quadratic-form pieces.
 
         theta None: the parameters already on the device are used (graph replay).
         out: 5-element device tensor for (logdet Qc, mu^T Qp mu, mu^T tau A^T A mu,
         mu^T A^T y, logdet Qp) (default: the engine's own slots).
+        grad: a selected inversion of Qc follows (its products of the factor run beside the
+        factorization).
         """
         out = self.scal if out is None else out
         L = self.lib
         self._assemble(theta, "bta", True)
+        self._overlap(2 if grad else 1)
         _lib.check(L.dh_cholesky(self.h, 1), "dh_cholesky")
         N = self.dm.n_latent
         # mu = e^tau A^T y: scale by params[12] on the device (a 1x1 "scale" kernel via dot-free copy)
@@ -116,6 +141,7 @@ class STInla:
         out = self.scal if out is None else out
         self._conditional(theta, out)
         self._assemble(None, "bt", False)
+        self._overlap(0)  # logdet only
         _lib.check(self.lib.dh_cholesky(self.h, 0), "dh_cholesky")
         _lib.check(self.lib.dh_logdet(self.h, 0, out[4:5].data_ptr()), "dh_logdet")
 
@@ -220,7 +246,7 @@ class STInla:
     def forward(self, theta):
         """Forward half of the adjoint: f(theta) with the closed-form prior term."""
         theta = np.asarray(theta, dtype=np.float64)
-        self._conditional(theta)
+        self._conditional(theta, grad=True)
         if self.dm.closed_form:
             self._prior_logdet_terms(theta)
             s = self.scal[:4].cpu().numpy()
@@ -279,6 +305,7 @@ class STInla:
         """logdet Qp|BT and its theta-gradient by pobtf + pobtsi (general FEM inputs)."""
         L = self.lib
         self._assemble(None, "bt", False)
+        self._overlap(2)
         _lib.check(L.dh_cholesky(self.h, 0), "dh_cholesky")
         _lib.check(L.dh_logdet(self.h, 0, self.scal[6:7].data_ptr()), "dh_logdet")
         _lib.check(L.dh_selected_inversion(self.h, 0), "dh_selected_inversion")
@@ -311,6 +338,7 @@ class STInla:
                 raise ValueError("no stored theta: call f / value_and_grad first or pass theta and x_star")
             theta = self._theta_last
         self._assemble(np.asarray(theta, dtype=np.float64), "bta", True)
+        self._overlap(2)
         _lib.check(self.lib.dh_cholesky(self.h, 1), "dh_cholesky")
         _lib.check(self.lib.dh_selected_inversion(self.h, 1), "dh_selected_inversion")
 

@@ -143,6 +143,17 @@ class HipBTASolver:
             raise ValueError("Logdet is NaN. Check the input matrix.")
         return v
 
+    def set_overlap(self, mode: int) -> bool:
+        """Work that later calls need, run beside the factorization on a second stream (`dh_set_tuning`
+        key 4): 0 none, 1 the inverse of the diagonal factors (`solve`), 2 also the products of the
+        factor alone (`selected_inversion`).  Mode 2 reserves 2 n bp^2 doubles; returns False, and
+        keeps the previous mode, when they do not fit."""
+        rc = self.lib.dh_set_tuning(self.h, 4, int(mode))
+        if rc == -2 and int(mode) == 2:
+            return False
+        _lib.check(rc, "dh_set_tuning")
+        return True
+
     def selected_inversion(self, sparsity: str = "bta") -> None:
         _lib.check(self.lib.dh_selected_inversion(self.h, _check_sparsity(sparsity)), "dh_selected_inversion")
 

@@ -206,7 +206,15 @@ int dh_obs_variances(dh_handle* h, long long n_obs, const long long* rowptr, con
  *        applies), 1 = one persistent launch of left-looking tile tasks
  *        synchronised by per-tile readiness flags (default);
  * key 3: selected inversion, 1 = products of the factor alone on a second
- *        stream ahead of the recursion, 0 = one stream (default).
+ *        stream ahead of the recursion, 0 = one stream (default);
+ * key 4: work run beside the persistent factorization (schedule 1, outside
+ *        stream capture) on a second stream, chunk by chunk as its blocks
+ *        finish: 0 = none (default), 1 = the inverse of the diagonal factors
+ *        that dh_solve needs, 2 = also the products of the factor alone that
+ *        dh_selected_inversion needs (setting 2 reserves per-block slots,
+ *        2 n bp^2 doubles, until dh_destroy; DH_EHIP if they do not fit).
+ *        dh_cholesky then returns with that work queued on the handle's
+ *        stream as well.
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 

@@ -85,15 +85,51 @@ def test_pobtasi(n, b, a, side):
         assert rel(s.A_arrow_tip_block.cpu().numpy(), Xt) < 1e-9
 
 
+@pytest.mark.parametrize("overlap", [1, 2])
+@pytest.mark.parametrize("n,b,a", SHAPES)
+def test_overlap_factor_solve_selinv(n, b, a, overlap):
+    """The inverse of the diagonal factors (1) and the products of the factor (2) formed on a second
+    stream beside the persistent factorization; two factorizations in a row on one handle."""
+    from dalia_amd import _lib
+    from dalia_amd.solver import HipBTASolver
+    lib = _lib.load()
+    s = HipBTASolver(diagonal_blocksize=b, n_diag_blocks=n, arrowhead_blocksize=a)
+    assert s.set_overlap(overlap)
+    for rep in range(2):
+        print(f"case overlap={overlap} n={n} b={b} a={a} rep={rep}", flush=True)
+        M, S = random_bta(n, b, a, seed=n * 31 + b + a + rep)
+        s.cholesky(S, sparsity="bta")
+        info = ctypes.c_int(-1)
+        assert lib.dh_get_info(s.h, ctypes.byref(info)) == 0 and info.value == 0
+        ld = s.logdet("bta")
+        ld_ref = np.linalg.slogdet(M)[1]
+        assert abs(ld - ld_ref) <= 1e-10 * abs(ld_ref) + 1e-12
+        rhs = np.random.default_rng(rep).standard_normal(n * b + a)
+        x = torch.as_tensor(rhs.copy(), device="cuda")
+        s.solve(x, sparsity="bta")
+        assert rel(x.cpu().numpy(), np.linalg.solve(M, rhs)) < 1e-9
+        s.selected_inversion("bta")
+        assert lib.dh_get_info(s.h, ctypes.byref(info)) == 0 and info.value == 0
+        Xd, Xl, Xr, Xt = obta.dense_to_bta(np.linalg.inv(M), n, b, a)
+        assert rel(s.A_diagonal_blocks.cpu().numpy(), Xd) < 1e-9
+        if n > 1:
+            assert rel(s.A_lower_diagonal_blocks.cpu().numpy(), Xl) < 1e-9
+        if a:
+            assert rel(s.A_arrow_bottom_blocks.cpu().numpy(), Xr) < 1e-9
+            assert rel(s.A_arrow_tip_block.cpu().numpy(), Xt) < 1e-9
+
+
+@pytest.mark.parametrize("overlap", [0, 2])
 @pytest.mark.parametrize("n,b", [(4, 100), (3, 192)])
-def test_bt_path_ignores_arrow(n, b):
+def test_bt_path_ignores_arrow(n, b, overlap):
     """sparsity 'bt' factors the block-tridiagonal part only (pobtf / pobts / pobtsi)."""
     from dalia_amd.solver import HipBTASolver
     a = 4
-    print(f"case bt n={n} b={b}", flush=True)
+    print(f"case bt n={n} b={b} overlap={overlap}", flush=True)
     M, S = random_bta(n, b, a, seed=11 + b)
     Mbt = M[: n * b, : n * b]
     s = HipBTASolver(diagonal_blocksize=b, n_diag_blocks=n, arrowhead_blocksize=a)
+    assert s.set_overlap(overlap)
     s.cholesky(S, sparsity="bt")
     ld = s.logdet("bt")
     ld_ref = np.linalg.slogdet(Mbt)[1]
