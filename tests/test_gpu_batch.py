@@ -19,11 +19,13 @@ def rel(a, b):
 THETAS = [np.array([-1.0, 0.7, 0.0, 1.5]), np.array([-0.8, 0.5, 0.2, 1.2]), np.array([-1.2, 0.9, -0.1, 1.8])]
 
 
-def test_graph_replay_matches_eager_every_replay():
+@pytest.mark.parametrize("overlap", [0, 2])
+def test_graph_replay_matches_eager_every_replay(overlap):
+    """overlap 2: the eager evaluations use the second stream, the captured one stays on one stream."""
     from dalia_amd.inla import STInla
     d = synth.make_st_model(7, 5, 3, 2)
-    print(f"case graph ns={d.ns} nt={d.nt}", flush=True)
-    eng = STInla(d)
+    print(f"case graph ns={d.ns} nt={d.nt} overlap={overlap}", flush=True)
+    eng = STInla(d, overlap=overlap)
     ref = [eng.f(t) for t in THETAS]
     eng.capture_f()
     for rep in range(3):

@@ -53,11 +53,13 @@ def test_f_theta_matches_oracle(small):
         assert rel(eng.mu.cpu().numpy(), parts["mu"]) < 1e-9
 
 
-def test_adjoint_gradient(small):
+@pytest.mark.parametrize("overlap", [0, 2])
+def test_adjoint_gradient(small, overlap):
+    """overlap 2: the inverse and the products of the factor formed beside the factorization."""
     from dalia_amd.inla import STInla
     d = small
-    print(f"case adjoint ns={d.ns} nt={d.nt}", flush=True)
-    eng = STInla(d)
+    print(f"case adjoint ns={d.ns} nt={d.nt} overlap={overlap}", flush=True)
+    eng = STInla(d, overlap=overlap)
     th = d.theta
     f_ref, g_ref = oinla.adjoint_gradient(d, th)
     f, g = eng.value_and_grad(th)
@@ -67,11 +69,13 @@ def test_adjoint_gradient(small):
     assert rel(g, gfd) < 1e-5
 
 
-def test_adjoint_without_closed_form(small):
+@pytest.mark.parametrize("overlap", [0, 2])
+def test_adjoint_without_closed_form(small, overlap):
     """General FEM inputs: prior term by pobtf + pobtsi instead of the eigen form."""
     from dalia_amd.inla import STInla
     d = small
-    eng = STInla(d, closed_form_prior=False)
+    print(f"case adjoint-bt ns={d.ns} nt={d.nt} overlap={overlap}", flush=True)
+    eng = STInla(d, closed_form_prior=False, overlap=overlap)
     f_ref, g_ref = oinla.adjoint_gradient(d, d.theta)
     f, g = eng.value_and_grad(d.theta)
     assert abs(f - f_ref) <= 1e-10 * abs(f_ref)
