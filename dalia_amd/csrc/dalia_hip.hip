@@ -784,6 +784,64 @@ __device__ __forceinline__ void dag_finish(int* flag, int* count) {
   }
 }
 
+// x += opA[0:64, 0:64] inv^T for the chain's solve, opA from dh_load_a64 (rows of this wave's
+// half) and inv the dense lower-triangular inverse of the previous diagonal tile (ld 64).  Column
+// block J (16 wide) of the product only needs k < 16 (J + 1), so each wave takes the column blocks
+// {0, 3} or {1, 2} (ca, cb: their first columns) and issues 40 MFMAs instead of 64.
+__device__ __forceinline__ void chain_solve(dh_d4 acc[2][2], const double a[2][2][2][4], const double* inv, int& ca,
+                                            int& cb) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int ja = (w & 1) ? 1 : 0, jb = (w & 1) ? 2 : 3;
+  ca = DH_TILE / 4 * ja;
+  cb = DH_TILE / 4 * jb;
+  const int kq = (lane >> 4) * 2;  // group base offset within a 16-chunk
+  const Opnd B{inv, DH_TILE, M_ROW, nullptr};
+  double b[2][2][2][4];
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = 2 * kc + h;  // k in [16 q, 16 q + 16)
+      if (q <= ja) dh_load4<M_ROW>(B, ca + (lane & 15), 32 * kc + 16 * h + kq, b[kc][h][0]);
+      if (q <= jb) dh_load4<M_ROW>(B, cb + (lane & 15), 32 * kc + 16 * h + kq, b[kc][h][1]);
+    }
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = 2 * kc + h;
+      if (q <= ja) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          acc[0][0] = dh_mfma(a[kc][h][0][m], b[kc][h][0][m], acc[0][0]);
+          acc[1][0] = dh_mfma(a[kc][h][1][m], b[kc][h][0][m], acc[1][0]);
+        }
+      }
+      if (q <= jb) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          acc[0][1] = dh_mfma(a[kc][h][0][m], b[kc][h][1][m], acc[0][1]);
+          acc[1][1] = dh_mfma(a[kc][h][1][m], b[kc][h][1][m], acc[1][1]);
+        }
+      }
+    }
+}
+
+// C[rows of this wave's half, column blocks at ca and cb] = acc (chain_solve's layout).
+__device__ __forceinline__ void chain_store(double* C, long ldc, const dh_d4 acc[2][2], int ca, int cb) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int rbase = (w >> 1) * 32 + (lane >> 4);
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        C[(long)(rbase + bi * 16 + 4 * r) * ldc + (bj ? cb : ca) + (lane & 15)] = acc[bi][bj][r];
+}
+
 // One task (t wave-uniform); every path ends at the single signal.
 __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
   const int nbt = g.nbt, bp = g.bp;
@@ -890,9 +948,10 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
       dag_ready(fD + (c - 1) * nbt + (c - 1), nullptr, 0, 1, g.err, s_n);  // the chain dependency
       dh_d4 x[2][2];
       dh_zero(x);
-      dh_mma_a64<M_ROW>(x, xa, Opnd{pv - DH_TILE * DH_TILE, DH_TILE, M_ROW, nullptr}, 0);
-      dh_store(X, bp, 0, 0, x, 1.0, 0.0, false);  // final L(c, c-1)
-      dh_store(dh_lV, DH_LDS_LD, 0, 0, x, 1.0, 0.0, false);
+      int ca, cb;
+      chain_solve(x, xa, pv - DH_TILE * DH_TILE, ca, cb);
+      chain_store(X, bp, x, ca, cb);  // final L(c, c-1)
+      chain_store(dh_lV, DH_LDS_LD, x, ca, cb);
       dag_signal(fD + c * nbt + (c - 1));
       const Opnd xs{dh_lV, DH_LDS_LD, M_ROW, nullptr};
       dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, xs, 0, xs, 0, 0, DH_TILE);
