@@ -77,6 +77,7 @@ SIGNATURES = {
     "dh_ata_quad_v": (_I, [_P, ctypes.POINTER(DhModel), _P, _I, _I, _I, _P]),
     "dh_prior_eig": (_I, [_P, ctypes.POINTER(DhModel), _P, c_double_p, _P]),
     "dh_set_tuning": (_I, [_P, _I, _I]),
+    "dh_dag_table": (_I, [_I, _I, _I, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "dh_timing_enable": (_I, [_P, _I]),
     "dh_timing_read": (_I, [_P, c_double_p, ctypes.POINTER(_LL)]),
 }

@@ -2927,6 +2927,16 @@ int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const doubl
   return DH_OK;
 }
 
+int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int* slook) {
+  if (nbt < 1 || nbt > 2 * DH_MAXNODES || cap < 0 || (cap > 0 && !out)) return DH_EARG;
+  std::vector<int> tab;
+  dag_table(nbt, last ? 0 : nbt, arrow ? 1 : 0, DH_DAG_LOOK, DH_DAG_SLOOK, tab);
+  for (int e = 0; e < (int)tab.size() && e < cap; ++e) out[e] = tab[e];
+  if (look) *look = DH_DAG_LOOK;
+  if (slook) *slook = DH_DAG_SLOOK;
+  return (int)tab.size();
+}
+
 int dh_set_tuning(dh_handle* h, int key, int value) {
   if (!h) return DH_EARG;
   if (key == 0 && (value == 0 || value == 1)) h->syrk_variant = value;

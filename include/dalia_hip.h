@@ -198,6 +198,14 @@ int dh_marginal_variances(dh_handle* h, double* out);
 int dh_obs_variances(dh_handle* h, long long n_obs, const long long* rowptr, const int* col, const double* val,
                      const double* xcov, double* out);
 
+/* Task table of one block of the persistent pobtaf/pobtf launch for nbt 64-column tiles per block
+ * (last != 0: the last block, without sub-diagonal tiles; arrow != 0: with the arrowhead), as the
+ * launch decodes it: entry = kind << 24 | r << 12 | c, kinds as in the source's DK_* list.  Writes
+ * min(cap, count) entries and the helper and early-task lookaheads; returns the count, or DH_EARG.
+ * Host only (no GPU call): the schedule's ordering invariant is tested from it.  No counterpart in
+ * the reference. */
+int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int* slook);
+
 /* Tuning knobs (chosen per shape by the autotuner in dalia_amd/autotune.py):
  * key 0: Schur-update kernel of the per-panel pobtaf schedule, 0 = 64x64
  *        tiles (one launch), 1 = 128x128 tiles with the k range split in two
