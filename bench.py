@@ -144,6 +144,8 @@ def main():
     ap.add_argument("--workload", default="st_ns2025_nt100_nb6")
     ap.add_argument("--fd", type=int, default=1, help="central-difference gradients to time (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--overlap", type=int, default=0, choices=(0, 1, 2),
+                    help="work run beside each factorization on a second stream (STInla overlap option)")
     ap.add_argument("--roofline-kernel", default="k_syrk")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device-to-device copy (calibrates the HBM byte counters)")
@@ -174,7 +176,7 @@ def main():
     log(f"building synthetic model {args.workload} (ns={ns}, nt={nt}, nb={nb}, seed={seed})")
     t0 = time.perf_counter()
     data = synth.make_st_model(m, nt, nb, seed)
-    eng = STInla(data)
+    eng = STInla(data, overlap=args.overlap)
     torch.cuda.synchronize()
     log(f"set-up {time.perf_counter() - t0:.1f} s (closed-form prior: {eng.dm.closed_form})")
     theta = data.theta.copy()
@@ -284,7 +286,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "ns": ns, "nt": nt, "nb": nb, "seed": seed, "theta_dim": 4,
                    "n_obs": data.n_obs, "kernel_variants": {"pobtaf_schedule": "tile-task" if choice["alg"] == 1 else "per-panel",
-                                                        "syrk": choice["syrk"]}},
+                                                        "syrk": choice["syrk"]},
+                   "overlap": args.overlap},
         "pobtaf_ms": t_pobtaf * 1e3,
         "pobtaf_tflops": pobtaf_tflops,
         "pobtaf_frac_of_peak": pobtaf_tflops / FP64_PEAK_TFLOPS,
