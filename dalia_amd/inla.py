@@ -47,7 +47,7 @@ class STInla:
         """overlap: work run beside each factorization on a second stream (`HipBTASolver.set_overlap`),
         at most this mode: 0 none (default), 1 the inverse of the diagonal factors, 2 also the
         selected inversion's products of the factor.  Measured at ns = 2025, nt = 100: the adjoint
-        gradient gains 1.3 % with 2, as the factorization slows down beside that work."""
+        gradient gains about 2 % with 2, as the factorization slows down beside that work."""
         self.data = data
         self.dm = DeviceSTModel(data, device=device, eigen=closed_form_prior)
         self.solver = HipBTASolver(diagonal_blocksize=data.ns, n_diag_blocks=data.nt,
