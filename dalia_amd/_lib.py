@@ -10,6 +10,8 @@ from __future__ import annotations
 import ctypes
 import os
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdalia_hip.so")
 
@@ -109,6 +111,27 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
 
 class DhError(RuntimeError):
     pass
+
+
+#: bits of the NaN that dh_logdet writes when a readiness wait of the factorization gave up
+NAN_GIVEUP = 0x7FF80000DEAD0004
+
+
+def is_giveup_nan(value) -> bool:
+    return bool(np.isnan(value)) and int(np.float64(value).view(np.uint64)) == NAN_GIVEUP
+
+
+def raise_nan_logdet(lib, h, *values) -> None:
+    """A NaN log-determinant: raise DhError when a readiness wait of the factorization gave up
+    (the factor is invalid; the NaN carries NAN_GIVEUP, or handle h still reports it), else the
+    reference's ValueError for an input that is not SPD (`structured_solver.py:163-164`).
+    Synchronises the handle's stream."""
+    v = ctypes.c_int()
+    rc = lib.dh_get_info(h, ctypes.byref(v))
+    if rc == -4 or any(is_giveup_nan(x) for x in values):
+        raise DhError("dh_cholesky failed: a fused launch gave up waiting for a factored tile "
+                      "(the factor, log-determinant and solve are invalid)")
+    raise ValueError("Logdet is NaN. Check the input matrix.")
 
 
 def check(rc: int, what: str) -> None:

@@ -24,7 +24,8 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from dalia_amd.inla import STInla, hessian_from_values, hessian_stencil
+from dalia_amd.inla import (STInla, check_hessian, hessian_diag_from_values, hessian_diag_stencil,
+                            hessian_from_values, hessian_stencil)
 
 
 class BatchedF:
@@ -90,6 +91,23 @@ def batched_hessians(engines: list[STInla], thetas, h: float = 5e-3, offdiag_poi
     fv = bf([p for st in stencils for p in st])
     out, k0 = [], 0
     for st, t in zip(stencils, thetas):
-        out.append(hessian_from_values(fv[k0:k0 + len(st)], len(t), h, offdiag_points))
+        H = hessian_from_values(fv[k0:k0 + len(st)], len(t), h, offdiag_points)
+        check_hessian(H)
+        out.append(H)
+        k0 += len(st)
+    return out, bf
+
+
+def batched_hessian_diags(engines, thetas, h: float = 5e-3, points: int = 3):
+    """Diagonals of the FD Hessians of every instance at its own theta (1 + 2d stencil points
+    each for points=3, the reference's `hessian_diag_finite_difference_3pt`), all in one graph
+    launch.  Returns (list of diagonals, the BatchedF)."""
+    stencils = [hessian_diag_stencil(t, h, points) for t in thetas]
+    assignment = [k for k, st in enumerate(stencils) for _ in range(len(st))]
+    bf = BatchedF(engines, assignment)
+    fv = bf([p for st in stencils for p in st])
+    out, k0 = [], 0
+    for st, t in zip(stencils, thetas):
+        out.append(hessian_diag_from_values(fv[k0:k0 + len(st)], len(t), h, points))
         k0 += len(st)
     return out, bf

@@ -26,7 +26,7 @@ import scipy.sparse as sps
 import torch
 
 from dalia_amd import _lib, hyper
-from dalia_amd.inla import hessian_from_values, hessian_stencil
+from dalia_amd.inla import check_hessian, hessian_from_values, hessian_stencil
 from dalia_amd.model import _csr, closed_form_ok, fem_union, tridiag_parts
 from dalia_amd.solver import HipBTASolver
 
@@ -191,7 +191,7 @@ class CoregInla:
     def _f_from_scalars(self, theta, s) -> float:
         ld_c, quad, ld_p = float(s[0]), float(s[1]), float(s[4])
         if np.isnan(ld_c) or np.isnan(ld_p):
-            raise ValueError("Logdet is NaN. Check the input matrix.")
+            _lib.raise_nan_logdet(self.lib, self.h, ld_c, ld_p)
         lp, lik = self._scalars(theta)
         f = 0.5 * ld_c - 0.5 * quad - (lp + lik + 0.5 * ld_p)
         if np.isnan(f):
@@ -222,7 +222,9 @@ class CoregInla:
     def hessian(self, theta, h: float = 5e-3, offdiag_points: int = 4):
         pts = hessian_stencil(theta, h, offdiag_points)
         fv = np.array([self.f(p) for p in pts])
-        return hessian_from_values(fv, len(theta), h, offdiag_points)
+        H = hessian_from_values(fv, len(theta), h, offdiag_points)
+        check_hessian(H)
+        return H
 
     # ------------------------------------------------------------------
     # adjoint gradient
@@ -274,7 +276,7 @@ class CoregInla:
         e = eig.cpu().numpy()
         ld_c, quad = float(s[0]), float(s[1])
         if np.isnan(ld_c):
-            raise ValueError("Logdet is NaN. Check the input matrix.")
+            _lib.raise_nan_logdet(self.lib, self.h, ld_c)
         nst = self.dm.nst
         ld_p = sum(self.dm.nt * self.dm.logdet_c0 + e[k, 0] for k in range(nv)) \
             - 2.0 * nst * float(np.sum(theta[3 * nv:4 * nv]))

@@ -361,6 +361,7 @@ __global__ __launch_bounds__(256) void k_scatter_coo(double* diag, double* lower
   if (e >= nnz) return;
   const long nb = (long)n * b;
   const long r = rows[e], c = cols[e];
+  if (r < 0 || c < 0 || r >= nb + a || c >= nb + a) return;  // outside the matrix: ignored
   const long br = r < nb ? r / b : n, bc = c < nb ? c / b : n;
   const int lr = (int)(r < nb ? r % b : r - nb), lc = (int)(c < nb ? c % b : c - nb);
   const long bb = (long)bp * bp;
@@ -1661,9 +1662,16 @@ __global__ __launch_bounds__(256) void k_logdet_part(const double* D, int bp, in
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// A readiness wait of the factorization that gave up (*err != 0) leaves an invalid factor: the
+// log-determinant is then a NaN with the payload DH_NAN_GIVEUP, which every caller already treats
+// as a failed factorization and can tell apart from the NaN of a non-positive pivot.
 __global__ __launch_bounds__(64) void k_logdet_fin(const double* part, int n, const double* T, int a, int ap,
-                                                   double* out) {
+                                                   const int* err, double* out) {
   if (threadIdx.x != 0) return;
+  if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+    *out = __longlong_as_double((long long)DH_NAN_GIVEUP);
+    return;
+  }
   double s = 0.0, c = 0.0;
   for (int i = 0; i < n; ++i) {
     double y = part[i] - c;
@@ -2044,6 +2052,7 @@ struct dh_handle {
                                  // 1 inverse of the diagonal factors, 2 also the pobtasi W, Wa, inv(L)^T inv(L)
   double *Wall, *Sall, *Waall;   // per-block W, inv(L)^T inv(L), Wa slots (overlap 2)
   int fp_ready;                  // sparsity whose factor products are in the per-block slots, -1 none
+  int force_giveup;              // 1: dh_cholesky starts with the give-up flag raised (tuning key 5)
 };
 
 #define DH_MAXREC 16384
@@ -2160,6 +2169,7 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   h->pobtaf_alg = 1;
   h->overlap = 0;
   h->fp_ready = -1;
+  h->force_giveup = 0;
   {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -2424,7 +2434,9 @@ int dh_cholesky(dh_handle* h, int sparsity) {
   const long bb = (long)bp * bp;
   h->fp_ready = -1;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->info, INT_MAX);
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->err, 0);
+  // force_giveup: the flag that a timed-out readiness wait raises is set before the launch, so
+  // that every wait of the persistent launch returns at once (tuning key 5, a test of the error path)
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->err, h->force_giveup ? 4 : 0);
   if (h->pobtaf_alg == 1) {
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     const bool ov = h->overlap > 0 && h->side && hipStreamIsCapturing(h->stream, &cap) == hipSuccess &&
@@ -2624,7 +2636,7 @@ int dh_logdet(dh_handle* h, int sparsity, double* out_dev) {
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
   hipLaunchKernelGGL(k_logdet_part, dim3(h->n), dim3(256), 0, h->stream, h->diag, h->bp, h->b, h->part);
   hipLaunchKernelGGL(k_logdet_fin, dim3(1), dim3(64), 0, h->stream, h->part, h->n, arrow ? h->tip : nullptr,
-                     h->a, h->ap, out_dev);
+                     h->a, h->ap, (const int*)h->err, out_dev);
   HIPCHK(hipGetLastError());
   return DH_OK;
 }
@@ -2942,6 +2954,7 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
   if (key == 0 && (value == 0 || value == 1)) h->syrk_variant = value;
   else if (key == 2 && (value == 0 || value == 1)) h->pobtaf_alg = value;
   else if (key == 3 && (value == 0 || value == 1)) h->side_on = value;
+  else if (key == 5 && (value == 0 || value == 1)) h->force_giveup = value;
   else if (key == 4 && value >= 0 && value <= 2) {
     if (value == 2 && !h->Wall) {  // per-block slots, kept until the handle is destroyed
       const long long bb = (long long)h->bp * h->bp, before = h->bytes;

@@ -49,6 +49,9 @@ class STInla:
         selected inversion's products of the factor.  Measured at ns = 2025, nt = 100: the adjoint
         gradient gains about 2 % with 2, as the factorization slows down beside that work."""
         self.data = data
+        self.manifold = getattr(data, "manifold", "plane")
+        if self.manifold not in hyper.MANIFOLDS:
+            raise ValueError(f"Manifold not supported: {self.manifold}")
         self.dm = DeviceSTModel(data, device=device, eigen=closed_form_prior)
         self.solver = HipBTASolver(diagonal_blocksize=data.ns, n_diag_blocks=data.nt,
                                    arrowhead_blocksize=data.nb)
@@ -79,7 +82,7 @@ class STInla:
     def _set_params(self, theta):
         """Stage the theta-dependent scalars in pinned memory and copy them to the device."""
         self._theta_last = np.array(theta, dtype=np.float64)
-        p = np.concatenate([DeviceSTModel.coef_array(theta), [self.dm.tau_scale(theta)]])
+        p = np.concatenate([DeviceSTModel.coef_array(theta, self.manifold), [self.dm.tau_scale(theta)]])
         self.params_h.copy_(torch.from_numpy(p))
         self.params.copy_(self.params_h, non_blocking=True)
 
@@ -150,7 +153,7 @@ class STInla:
 
     def param_row(self, theta) -> np.ndarray:
         """The 13 device parameters of theta (Kronecker coefficients, exp(theta_obs))."""
-        return np.concatenate([DeviceSTModel.coef_array(theta), [self.dm.tau_scale(theta)]])
+        return np.concatenate([DeviceSTModel.coef_array(theta, self.manifold), [self.dm.tau_scale(theta)]])
 
     def hessian(self, theta, h: float = 5e-3, offdiag_points: int = 4):
         """FD Hessian of f at theta (`core/dalia.py:767-910`), plain f per stencil point.
@@ -160,12 +163,14 @@ class STInla:
         """
         pts = hessian_stencil(theta, h, offdiag_points)
         fv = np.array([self.f(p) for p in pts])
-        return hessian_from_values(fv, len(theta), h, offdiag_points)
+        H = hessian_from_values(fv, len(theta), h, offdiag_points)
+        check_hessian(H)
+        return H
 
     def _f_from_scalars(self, theta, s) -> float:
         ld_c, quad_p, quad_l, _, ld_p = s
         if np.isnan(ld_c) or np.isnan(ld_p):
-            raise ValueError("Logdet is NaN. Check the input matrix.")
+            _lib.raise_nan_logdet(self.lib, self.h, ld_c, ld_p)
         lp, _, lik = self._scalars(theta)
         f = 0.5 * ld_c - 0.5 * (quad_p + quad_l) - (lp + lik + 0.5 * ld_p)
         if np.isnan(f):
@@ -197,7 +202,7 @@ class STInla:
         if self._graph is None:
             self.capture_f()
         theta = np.asarray(theta, dtype=np.float64)
-        p = np.concatenate([DeviceSTModel.coef_array(theta), [self.dm.tau_scale(theta)]])
+        p = np.concatenate([DeviceSTModel.coef_array(theta, self.manifold), [self.dm.tau_scale(theta)]])
         self.params_h.copy_(torch.from_numpy(p))
         self._graph.replay()
         s = self.scal[:5].cpu().numpy()
@@ -213,7 +218,7 @@ class STInla:
         s = self.scal[:5].cpu().numpy()
         ld_c, quad_p, quad_l, _, ld_p = s
         if np.isnan(ld_c) or np.isnan(ld_p):
-            raise ValueError("Logdet is NaN. Check the input matrix.")
+            _lib.raise_nan_logdet(self.lib, self.h, ld_c, ld_p)
         lp, _, lik = self._scalars(theta)
         f = 0.5 * ld_c - 0.5 * (quad_p + quad_l) - (lp + lik + 0.5 * ld_p)
         self.n_f_evals += 1
@@ -238,7 +243,7 @@ class STInla:
     # ------------------------------------------------------------------
     def _prior_logdet_terms(self, theta):
         """(logdet Qp|BT, d/dtheta_{0..2}) via the Kronecker-eigen closed form."""
-        gs, gt, gst = hyper.interpretable2compute(float(theta[0]), float(theta[1]), float(theta[2]))
+        gs, gt, gst = hyper.interpretable2compute(float(theta[0]), float(theta[1]), float(theta[2]), self.manifold)
         gam = np.array([gs, gt, gst], dtype=np.float64)
         _lib.check(self.lib.dh_prior_eig(self.h, self.dm.ref(), self.dm.lam.data_ptr(),
                                          gam.ctypes.data_as(_lib.c_double_p), self.eig.data_ptr()), "dh_prior_eig")
@@ -252,13 +257,13 @@ class STInla:
             s = self.scal[:4].cpu().numpy()
             e = self.eig.cpu().numpy()
             ld_p = self.dm.nt * self.dm.logdet_c0 + e[0]
-            dld_p = hyper.DGAMMA.T @ e[1:4]
+            dld_p = hyper.dgamma(theta, self.manifold).T @ e[1:4]
         else:
             s = self.scal[:4].cpu().numpy()
             ld_p, dld_p = None, None
         ld_c, quad_p, quad_l, bmu = s
         if np.isnan(ld_c):
-            raise ValueError("Logdet is NaN. Check the input matrix.")
+            _lib.raise_nan_logdet(self.lib, self.h, ld_c)
         lp, glp, lik = self._scalars(theta)
         self._state = dict(theta=theta.copy(), ld_c=ld_c, quad_p=quad_p, quad_l=quad_l, bmu=bmu,
                            ld_p=ld_p, dld_p=dld_p, lp=lp, glp=glp, lik=lik)
@@ -287,7 +292,7 @@ class STInla:
         ld_p, dld_p = st["ld_p"], st["dld_p"]
         if ld_p is None:
             ld_p, dld_p = self._prior_by_selected_inversion(theta)
-        dc = hyper.kron_coefficients_grad(theta)
+        dc = hyper.kron_coefficients_grad(theta, self.manifold)
         nt = self.dm.nt
         mult = np.concatenate([np.ones(nt), 2.0 * np.ones(nt - 1)])[:, None]
         g = np.zeros(theta.size)
@@ -313,10 +318,13 @@ class STInla:
         _lib.check(L.dh_contract_fem(self.h, self.dm.ref(), cp.data_ptr()), "dh_contract_fem")
         nq = 2 * self.dm.nt - 1
         cp = cp.cpu().numpy().reshape(nq, 4)
-        dc = hyper.kron_coefficients_grad(theta)
+        dc = hyper.kron_coefficients_grad(theta, self.manifold)
         mult = np.concatenate([np.ones(self.dm.nt), 2.0 * np.ones(self.dm.nt - 1)])[:, None]
         d = np.array([float(np.sum(self.dm.block_weights(dc[j]) * mult * cp)) for j in range(3)])
-        return float(self.scal[6].item()), d
+        ld_p = float(self.scal[6].item())
+        if np.isnan(ld_p):
+            _lib.raise_nan_logdet(self.lib, self.h, ld_p)
+        return ld_p, d
 
     def value_and_grad(self, theta):
         self.forward(theta)
@@ -410,6 +418,55 @@ def hessian_from_values(fv: np.ndarray, d: int, h: float, offdiag_points: int = 
                 H[i, j] = (fpp + fmm - fp[i] - fm[i] - fp[j] - fm[j] + 2 * f0) / (2 * h * h)
             H[j, i] = H[i, j]
     return H
+
+
+def hessian_diag_stencil(theta, h: float, points: int = 3) -> np.ndarray:
+    """Stencil points (rows) of the diagonal of the FD Hessian: theta first, then +-h e_i (points=3,
+    1 + 2d rows) and +-2h e_i (points=5, 1 + 4d rows).  The reference's
+    `hessian_diag_finite_difference_3pt/_5pt` (`utils/finite_difference_stencils.py:81-171`)
+    evaluate f(theta) once per component; it is evaluated once here."""
+    theta = np.asarray(theta, dtype=np.float64)
+    d = theta.size
+    E = h * np.eye(d)
+    pts = [theta.copy()]
+    for i in range(d):
+        pts += [theta + E[i], theta - E[i]]
+    if points == 5:
+        for i in range(d):
+            pts += [theta + 2 * E[i], theta - 2 * E[i]]
+    elif points != 3:
+        raise ValueError("points must be 3 or 5")
+    return np.stack(pts)
+
+
+def hessian_diag_from_values(fv: np.ndarray, d: int, h: float, points: int = 3) -> np.ndarray:
+    """Diagonal of the Hessian from the values at `hessian_diag_stencil` (same formulas as
+    `utils/finite_difference_stencils.py:117-118` and `:163-170`)."""
+    f0 = fv[0]
+    fp, fm = fv[1:1 + 2 * d:2], fv[2:2 + 2 * d:2]
+    if points == 3:
+        return (fp - 2 * f0 + fm) / h ** 2
+    fpp, fmm = fv[1 + 2 * d::2], fv[2 + 2 * d::2]
+    return (-fpp + 16 * fp - 30 * f0 + 16 * fm - fmm) / (12 * h ** 2)
+
+
+def check_hessian(H: np.ndarray) -> np.ndarray:
+    """The reference's eigenvalue check of the FD Hessian (`core/dalia.py:904-908`): a message
+    when an eigenvalue is negative.  Returns the eigenvalues."""
+    ev = np.linalg.eigvalsh(H)
+    if np.any(ev < 0):
+        print(f"Negative eigenvalues detected: {ev}", flush=True)
+    return ev
+
+
+def compute_covariance_hp(engine, theta, h: float = 5e-3, offdiag_points: int = 4) -> np.ndarray:
+    """Covariance of the hyperparameters, the inverse of the FD Hessian of f at theta
+    (`DALIA.compute_covariance_hp`, `core/dalia.py:747-765`); `engine` is an `STInla` or a
+    `dalia_amd.coreg.CoregInla`.  The Hessian's stencil points run in one HIP graph
+    (`dalia_amd.batch.batched_hessians`), the reference's 4-point form by default."""
+    from dalia_amd.batch import batched_hessians
+    (H,), _ = batched_hessians([engine], [np.asarray(theta, dtype=np.float64)], h=h, offdiag_points=offdiag_points)
+    return np.linalg.inv(H)
 
 
 def hessian_adjoint(engine, theta, h: float = 1e-3) -> np.ndarray:

@@ -140,8 +140,8 @@ class DeviceSTModel:
         return ctypes.byref(self.struct)
 
     @staticmethod
-    def coef_array(theta) -> np.ndarray:
-        return np.ascontiguousarray(kron_coefficients(theta).ravel(), dtype=np.float64)
+    def coef_array(theta, manifold: str = "plane") -> np.ndarray:
+        return np.ascontiguousarray(kron_coefficients(theta, manifold).ravel(), dtype=np.float64)
 
     def block_weights(self, dc: np.ndarray):
         """Per-block FEM weights w[q][m] for coefficient array dc[k][m].

@@ -90,15 +90,44 @@ class HipBTASolver:
             pass
 
     # -- Solver interface ------------------------------------------------------
+    def _order(self, sparsity: str) -> int:
+        """Order of the matrix (and length of the right-hand side) for a sparsity."""
+        n, b, a = self.n_diag_blocks, self.diagonal_blocksize, self.arrowhead_blocksize
+        return n * b + (a if sparsity == "bta" else 0)
+
     def _spmatrix_to_structured(self, A, sparsity: str) -> None:
-        """Scatter a host sparse matrix into the device blocks."""
+        """Scatter a sparse matrix into the device blocks.
+
+        A: a scipy sparse matrix (DALIA's NumPy backend), or a device sparse matrix with
+        `tocoo()` whose row / col / data arrays expose `__cuda_array_interface__` (cupyx, DALIA's
+        CuPy backend: `src/dalia/__init__.py:20-54`), read in place on the device.  The order must
+        be n*b + a (for "bt" also n*b: the arrowhead rows are ignored)."""
         sp = _check_sparsity(sparsity)
-        coo = sps.coo_matrix(A)
-        coo.sum_duplicates()
-        rows = torch.as_tensor(coo.row.astype(np.int64), device="cuda")
-        cols = torch.as_tensor(coo.col.astype(np.int64), device="cuda")
-        vals = torch.as_tensor(coo.data.astype(np.float64), device="cuda")
-        _lib.check(self.lib.dh_scatter_coo(self.h, int(coo.nnz), rows.data_ptr(), cols.data_ptr(),
+        N = self._order("bta")
+        shape = tuple(int(x) for x in A.shape)
+        ok = {(N, N)} if sparsity == "bta" else {(N, N), (self._order("bt"),) * 2}
+        if shape not in ok:
+            raise ValueError(f"matrix of shape {shape} does not match the solver's blocks "
+                             f"(n*b + a = {N}, sparsity {sparsity!r})")
+        if sps.issparse(A):
+            coo = sps.coo_matrix(A)
+            coo.sum_duplicates()
+            rows = torch.as_tensor(coo.row.astype(np.int64), device="cuda")
+            cols = torch.as_tensor(coo.col.astype(np.int64), device="cuda")
+            vals = torch.as_tensor(coo.data.astype(np.float64), device="cuda")
+        elif hasattr(A, "tocoo"):
+            coo = A.tocoo()
+            if hasattr(coo, "sum_duplicates"):
+                coo.sum_duplicates()
+            rows = torch.as_tensor(coo.row, device="cuda").to(torch.int64).contiguous()
+            cols = torch.as_tensor(coo.col, device="cuda").to(torch.int64).contiguous()
+            vals = torch.as_tensor(coo.data, device="cuda").to(torch.float64).contiguous()
+        else:
+            raise TypeError("A must be a scipy sparse matrix or a device sparse matrix with tocoo()")
+        nnz = int(vals.numel())
+        if rows.numel() != nnz or cols.numel() != nnz:
+            raise ValueError("COO row, column and value arrays differ in length")
+        _lib.check(self.lib.dh_scatter_coo(self.h, nnz, rows.data_ptr(), cols.data_ptr(),
                                            vals.data_ptr(), sp), "dh_scatter_coo")
         self._keep = (rows, cols, vals)
 
@@ -115,14 +144,33 @@ class HipBTASolver:
     def cholesky_async(self, sparsity: str = "bta") -> None:
         _lib.check(self.lib.dh_cholesky(self.h, _check_sparsity(sparsity)), "dh_cholesky")
 
-    def solve(self, rhs, sparsity: str = "bta"):
-        """Solve in place (device float64 tensor of length n*b (+a)); returns rhs."""
-        sp = _check_sparsity(sparsity)
-        tic = time.perf_counter()
+    def _rhs_view(self, rhs, sparsity: str):
+        """(device view, whether rhs is a host array) of a right-hand side, checked."""
         host = isinstance(rhs, np.ndarray)
-        x = torch.as_tensor(rhs, device="cuda").contiguous() if host else rhs
+        if host or isinstance(rhs, torch.Tensor):
+            length = int(rhs.size if host else rhs.numel())
+        elif hasattr(rhs, "__cuda_array_interface__"):  # CuPy (DALIA's GPU backend)
+            length = int(np.prod(rhs.__cuda_array_interface__["shape"]))
+        else:
+            raise ValueError("rhs must be a numpy array, a CUDA tensor or a CUDA array")
+        if length != self._order(sparsity):  # checked before anything touches the device
+            raise ValueError(f"rhs of length {length} does not match the solver's blocks "
+                             f"({self._order(sparsity)} for sparsity {sparsity!r})")
+        if host:
+            if rhs.dtype != np.float64:
+                raise ValueError("rhs must be float64")
+            x = torch.as_tensor(rhs, device="cuda").contiguous()
+        else:
+            x = rhs if isinstance(rhs, torch.Tensor) else torch.as_tensor(rhs, device="cuda")  # zero-copy view
         if x.dtype != torch.float64 or not x.is_cuda or not x.is_contiguous():
             raise ValueError("rhs must be a contiguous float64 CUDA tensor or a numpy array")
+        return x, host
+
+    def solve(self, rhs, sparsity: str = "bta"):
+        """Solve in place (numpy array, CUDA tensor or CUDA array of length n*b (+a)); returns rhs."""
+        sp = _check_sparsity(sparsity)
+        tic = time.perf_counter()
+        x, host = self._rhs_view(rhs, sparsity)
         _lib.check(self.lib.dh_solve(self.h, x.data_ptr(), sp), "dh_solve")
         self.stream.synchronize()
         self.t_solve += time.perf_counter() - tic
@@ -131,7 +179,11 @@ class HipBTASolver:
         return rhs
 
     def solve_async(self, x: torch.Tensor, sparsity: str = "bta") -> None:
-        _lib.check(self.lib.dh_solve(self.h, x.data_ptr(), _check_sparsity(sparsity)), "dh_solve")
+        sp = _check_sparsity(sparsity)
+        if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64 and x.is_contiguous()
+                and x.numel() == self._order(sparsity)):
+            raise ValueError(f"solve_async needs a contiguous float64 CUDA tensor of length {self._order(sparsity)}")
+        _lib.check(self.lib.dh_solve(self.h, x.data_ptr(), sp), "dh_solve")
 
     def logdet_async(self, sparsity: str, out: torch.Tensor) -> None:
         _lib.check(self.lib.dh_logdet(self.h, _check_sparsity(sparsity), out.data_ptr()), "dh_logdet")
@@ -140,7 +192,7 @@ class HipBTASolver:
         self.logdet_async(sparsity, self._scal[:1])
         v = float(self._scal[0].item())
         if np.isnan(v):
-            raise ValueError("Logdet is NaN. Check the input matrix.")
+            _lib.raise_nan_logdet(self.lib, self.h, v)
         return v
 
     def set_overlap(self, mode: int) -> bool:

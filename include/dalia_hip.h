@@ -118,8 +118,8 @@ int dh_scale_indexed(dh_handle* h, const double* x, const int* idx, const double
 /* Fill the blocks from a symmetric sparse matrix of order n*b (+ a) given as
  * COO with global 0-based indices (device arrays, no duplicates): diagonal
  * blocks take both triangles, then the sub-diagonal blocks, the arrow rows
- * and the tip; entries outside the BTA pattern are ignored, as are the
- * arrowhead rows for BT. */
+ * and the tip; entries outside the BTA pattern or outside the matrix are
+ * ignored, as are the arrowhead rows for BT. */
 int dh_scatter_coo(dh_handle* h, long long nnz, const long long* rows, const long long* cols,
                    const double* vals, int sparsity);
 
@@ -131,7 +131,12 @@ int dh_cholesky(dh_handle* h, int sparsity);
  * length n*b (+ a for BTA), logical (unpadded) layout. */
 int dh_solve(dh_handle* h, double* rhs, int sparsity);
 
-/* logdet = 2 sum log diag(L).  out_dev: device double (async). */
+/* logdet = 2 sum log diag(L).  out_dev: device double (async).  NaN when the factorization
+ * failed: a non-positive pivot, or a readiness wait of the persistent launch that gave up, in
+ * which case the NaN's bits are DH_NAN_GIVEUP (and dh_get_info returns -4 until the next
+ * factorization). */
+#define DH_NAN_GIVEUP 0x7ff80000dead0004ULL
+
 int dh_logdet(dh_handle* h, int sparsity, double* out_dev);
 
 /* Overwrite the factor with the blocks of A^-1 on the BTA pattern; the
@@ -222,7 +227,11 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
  *        dh_selected_inversion needs (setting 2 reserves per-block slots,
  *        2 n bp^2 doubles, until dh_destroy; DH_EHIP if they do not fit).
  *        dh_cholesky then returns with that work queued on the handle's
- *        stream as well.
+ *        stream as well;
+ * key 5: 1 = the next factorizations start with the flag that a timed-out readiness wait
+ *        raises already set, so every wait of the persistent launch returns at once, the
+ *        launch drains with an invalid factor, dh_logdet writes NaN and dh_get_info returns -4
+ *        (a test of that error path in which every kernel exits normally); 0 = off (default).
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 

@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE ONLY (see `oracle/bta.py`).  A numpy/scipy restatement of
 the reference path, function by function:
 
-* `interpretable2compute`  <- `submodels/spatio_temporal.py:151-197` (plane branch)
+* `interpretable2compute`  <- `submodels/spatio_temporal.py:151-197` (plane and sphere branches)
 * `qprior_sparse`          <- `submodels/spatio_temporal.py:114-149`,
                               `submodels/regression.py:28-35`, `core/model.py:306-382`
 * `qcond_sparse`           <- `core/model.py:384-423`, `likelihoods/gaussian.py:98-131`
@@ -42,10 +42,12 @@ from oracle import bta
 # hyperparameter maps
 # ---------------------------------------------------------------------------
 
-def interpretable2compute(r_s: float, r_t: float, sigma_st: float):
-    """(r_s, r_t, sigma_st) -> (gamma_s, gamma_t, gamma_st), plane manifold.
+def interpretable2compute(r_s, r_t, sigma_st, manifold: str = "plane"):
+    """(r_s, r_t, sigma_st) -> (gamma_s, gamma_t, gamma_st) on the plane or the sphere.
 
-    alpha_s=2, alpha_t=1, alpha_e=1 fixed (`spatio_temporal.py:158-160`).
+    alpha_s=2, alpha_t=1, alpha_e=1 fixed (`spatio_temporal.py:158-160`); the sphere's spatial
+    constant is the 50-term spherical-harmonic series of `:171-180`.  Written with numpy
+    functions so that complex arguments work (complex-step derivatives of the sphere branch).
     """
     alpha_s, alpha_t, alpha_e = 2, 1, 1
     alpha = alpha_e + alpha_s * (alpha_t - 0.5)
@@ -53,9 +55,29 @@ def interpretable2compute(r_s: float, r_t: float, sigma_st: float):
     nu_t = alpha_t - 0.5
     gamma_s = 0.5 * math.log(8 * nu_s) - r_s
     gamma_t = r_t - 0.5 * math.log(8 * nu_t) + alpha_s * gamma_s
-    c1 = gamma_fn(nu_t) * gamma_fn(nu_s) / (gamma_fn(alpha_t) * gamma_fn(alpha) * pow(4 * math.pi, 1.5))
-    gamma_st = 0.5 * math.log(c1) - 0.5 * gamma_t - nu_s * gamma_s - sigma_st
+    if manifold == "sphere":
+        cr_t = gamma_fn(nu_t) / (gamma_fn(alpha_t) * pow(4 * math.pi, 0.5))
+        c_s = 0.0
+        for k in range(50):
+            c_s = c_s + (2.0 * k + 1.0) / (4.0 * math.pi * (np.exp(gamma_s) ** 2 + k * (k + 1)) ** alpha)
+        gamma_st = 0.5 * math.log(cr_t) + 0.5 * np.log(c_s) - 0.5 * gamma_t - sigma_st
+    elif manifold == "plane":
+        c1 = gamma_fn(nu_t) * gamma_fn(nu_s) / (gamma_fn(alpha_t) * gamma_fn(alpha) * pow(4 * math.pi, 1.5))
+        gamma_st = 0.5 * math.log(c1) - 0.5 * gamma_t - nu_s * gamma_s - sigma_st
+    else:
+        raise ValueError(f"Manifold not supported: {manifold}")
     return gamma_s, gamma_t, gamma_st
+
+
+def dgamma_dtheta(theta, manifold: str = "plane") -> np.ndarray:
+    """Rows d gamma_s, d gamma_t, d gamma_st over columns (r_s, r_t, sigma_st): exact on the plane
+    (the map is affine), by a complex step in r_s for gamma_st on the sphere."""
+    J = np.array([[-1.0, 0.0, 0.0], [-2.0, 1.0, 0.0], [2.0, -0.5, -1.0]])
+    if manifold == "sphere":
+        h = 1e-30
+        _, _, g = interpretable2compute(complex(theta[0], h), theta[1], theta[2], "sphere")
+        J[2, 0] = g.imag / h
+    return J
 
 
 def compute2interpretable(gamma_s: float, gamma_t: float, gamma_st: float):
@@ -73,9 +95,9 @@ def compute2interpretable(gamma_s: float, gamma_t: float, gamma_st: float):
     return r_s, r_t, sigma_st
 
 
-def kron_coefficients(theta) -> np.ndarray:
+def kron_coefficients(theta, manifold: str = "plane") -> np.ndarray:
     """c[k, m] with Qp_ST = sum_k sum_m c[k,m] (M_k kron G_m), G = (c0, g1, g2, g3)."""
-    gs, gt, gst = interpretable2compute(theta[0], theta[1], theta[2])
+    gs, gt, gst = interpretable2compute(theta[0], theta[1], theta[2], manifold)
     k2 = math.exp(2 * gs)
     e2st = math.exp(2 * gst)
     et = math.exp(gt)
@@ -92,7 +114,7 @@ def kron_coefficients(theta) -> np.ndarray:
 
 def qprior_st_sparse(data, theta) -> sps.csc_matrix:
     """Spatio-temporal prior precision, time-major `kron(m, q)` (`spatio_temporal.py:114-149`)."""
-    gs, gt, gst = interpretable2compute(theta[0], theta[1], theta[2])
+    gs, gt, gst = interpretable2compute(theta[0], theta[1], theta[2], getattr(data, "manifold", "plane"))
     es, et, est = math.exp(gs), math.exp(gt), math.exp(gst)
     q1 = es ** 2 * data.c0 + data.g1
     q2 = es ** 4 * data.c0 + 2 * es ** 2 * data.g1 + data.g2
@@ -311,14 +333,14 @@ def fd_hessian(data, theta, h: float, offdiag_points: int = 4, fun=None) -> np.n
 # derivatives of the Kronecker coefficients
 # ---------------------------------------------------------------------------
 
-def kron_coefficients_grad(theta) -> np.ndarray:
+def kron_coefficients_grad(theta, manifold: str = "plane") -> np.ndarray:
     """dc[j, k, m] = d c[k,m] / d theta_j for j in (r_s, r_t, sigma_st).
 
     c[k,m] = exp(2 gst + k gt + 2 p_m gs) * const with p = (3,2,1,0) for k=0,
     (2,1,0,-) for k=1, (1,0,-,-) for k=2; chain rule through
     gs = C - r_s, gt = r_t - C' + 2 gs, gst = C'' - gt/2 - gs - sigma_st.
     """
-    c = kron_coefficients(theta)
+    c = kron_coefficients(theta, manifold)
     # exponent of exp(gs) in k2 powers: c[k,m] ~ k2^(p) with p listed below
     p = np.array([[3, 2, 1, 0], [2, 1, 0, 0], [1, 0, 0, 0]], dtype=np.float64)
     kk = np.array([0.0, 1.0, 2.0])[:, None]
@@ -326,9 +348,7 @@ def kron_coefficients_grad(theta) -> np.ndarray:
     dg_t = kk * np.ones((1, 4))  # d log c / d gamma_t
     dg_st = 2.0 * np.ones((3, 4))  # d log c / d gamma_st
     # d gamma / d theta: rows (r_s, r_t, sigma)
-    dgs = np.array([-1.0, 0.0, 0.0])
-    dgt = np.array([-2.0, 1.0, 0.0])
-    dgst = np.array([2.0, -0.5, -1.0])
+    dgs, dgt, dgst = dgamma_dtheta(theta, manifold)
     out = np.zeros((3, 3, 4))
     for j in range(3):
         out[j] = c * (dg_s * dgs[j] + dg_t * dgt[j] + dg_st * dgst[j])
@@ -336,7 +356,7 @@ def kron_coefficients_grad(theta) -> np.ndarray:
 
 
 def dqprior_st_sparse(data, theta, j: int) -> sps.csc_matrix:
-    dc = kron_coefficients_grad(theta)[j]
+    dc = kron_coefficients_grad(theta, getattr(data, "manifold", "plane"))[j]
     G = [data.c0, data.g1, data.g2, data.g3]
     Mk = [data.m0, data.m1, data.m2]
     Q = None
@@ -462,7 +482,8 @@ def prior_logdet_closed_form(data, theta, lam=None, with_grad: bool = False):
     """
     if lam is None:
         lam = spatial_eigenvalues(data)
-    gs, gt, gst = interpretable2compute(theta[0], theta[1], theta[2])
+    manifold = getattr(data, "manifold", "plane")
+    gs, gt, gst = interpretable2compute(theta[0], theta[1], theta[2], manifold)
     k2 = math.exp(2 * gs)
     lt = k2 + lam
     pw = np.array([3.0, 2.0, 1.0])
@@ -484,8 +505,6 @@ def prior_logdet_closed_form(data, theta, lam=None, with_grad: bool = False):
     d_gs = float(np.sum(trM * dP_gs))
     d_gt = float(np.sum(trM * dP_gt))
     d_gst = float(np.sum(trM * dP_gst))
-    dgs = np.array([-1.0, 0.0, 0.0])
-    dgt = np.array([-2.0, 1.0, 0.0])
-    dgst = np.array([2.0, -0.5, -1.0])
+    dgs, dgt, dgst = dgamma_dtheta(theta, manifold)
     grad = d_gs * dgs + d_gt * dgt + d_gst * dgst
     return val, grad

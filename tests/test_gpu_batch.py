@@ -107,3 +107,27 @@ def test_autotune_choice_applied(tmp_path):
     f, g = eng.value_and_grad(d.theta)
     _, g_ref = oinla.adjoint_gradient(d, d.theta)
     assert rel(g, g_ref) < 1e-8
+
+
+def test_batched_hessian_diagonals_and_covariance():
+    """The diagonal 3-point stencil (`utils/finite_difference_stencils.py:81-120`) in one graph over
+    a univariate and a coregional instance, and the covariance of theta (`core/dalia.py:747-765`)."""
+    from dalia_amd.batch import batched_hessian_diags
+    from dalia_amd.coreg import CoregInla
+    from dalia_amd.inla import STInla, compute_covariance_hp
+    from oracle import coreg as ocoreg
+    du = synth.make_st_model(5, 3, 2, 0)
+    dc = synth.make_coreg_model(4, 3, 2, 2, 9)
+    engines = [STInla(du), CoregInla(dc)]
+    thetas = [du.theta, dc.theta]
+    print("case batched diagonal Hessians", flush=True)
+    diags, bf = batched_hessian_diags(engines, thetas, h=5e-3, points=3)
+    assert len(bf.assignment) == (1 + 2 * 4) + (1 + 2 * 9)
+    funs = [lambda t: oinla.f_theta(du, t), lambda t: ocoreg.f_theta_coreg(dc, t)]
+    for dg, t, fun in zip(diags, thetas, funs):
+        f0 = fun(t)
+        ref = np.array([(fun(t + 5e-3 * e) - 2 * f0 + fun(t - 5e-3 * e)) / 5e-3 ** 2 for e in np.eye(t.size)])
+        assert np.abs(dg - ref).max() <= 1e-5 * np.abs(ref).max()
+    C = compute_covariance_hp(engines[0], du.theta)
+    Cr = np.linalg.inv(oinla.fd_hessian(du, du.theta, 5e-3, offdiag_points=4))
+    assert np.abs(C - Cr).max() <= 1e-5 * np.abs(Cr).max()

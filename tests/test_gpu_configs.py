@@ -1,0 +1,202 @@
+"""GPU parity at the benchmark configurations themselves.
+
+* ns=400, nt=12, nb=4 (the CPU-runnable case): f, mu, both log-determinants, the adjoint gradient
+  and the marginal variances against the committed oracle fixture.
+* ns=2025, nt=100, nb=6 (the plain drop-in and adjoint case), full size: the same quantities
+  against a committed oracle fixture (scalars, gradient, a strided sample of mu, the diagonals of
+  six selected-inverse blocks), and the BT log-determinant of the prior against its
+  Kronecker-eigen closed form.
+* ns=4096, nt=250, nb=6 (the large adjoint case, 67 GB of blocks): size-independent checks, the
+  prior's BT log-determinant against the closed form and the adjoint gradient against GPU central
+  differences on one theta component.
+* the bivariate coregional instance ns=576, nt=24 (block size 1152) against the oracle, and the
+  trivariate ns=1600, nt=128 (block size 4800) adjoint gradient against GPU central differences.
+
+Bars: f and log-determinants 1e-10 relative, mu and selected-inverse entries 1e-9 relative,
+adjoint gradient 1e-8 relative against the oracle's adjoint and 1e-6 against central differences
+(h = 1e-4).
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from dalia_amd import synth
+from oracle import coreg as ocoreg
+from oracle import inla as oinla
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+
+
+def _model_from_fixture(g):
+    d = synth.make_st_model(int(g["m"]), int(g["nt"]), int(g["nb"]), int(g["seed"]))
+    assert np.array_equal(d.y[:16], g["y_head"])  # the generator reproduces the fixture's inputs
+    assert d.y.sum() == pytest.approx(float(g["y_sum"]), rel=1e-14)
+    return d
+
+
+def test_ns400_nt12_against_fixture():
+    from dalia_amd.inla import STInla
+    g = load("st_ns400_nt12_nb4")
+    d = _model_from_fixture(g)
+    print(f"case fixture ns={d.ns} nt={d.nt} nb={d.nb}", flush=True)
+    eng = STInla(d)
+    th = g["thetas"][0]
+    f = eng.f(th)
+    assert abs(f - g["f"][0]) <= 1e-10 * abs(g["f"][0])
+    assert rel(eng.mu.cpu().numpy(), g["mu"][0]) < 1e-9
+    s = eng.scal[:5].cpu().numpy()
+    assert abs(s[0] - g["logdet_qc"][0]) <= 1e-10 * abs(g["logdet_qc"][0])
+    assert abs(s[4] - g["logdet_qp"][0]) <= 1e-10 * abs(g["logdet_qp"][0])
+    f2, grad = eng.value_and_grad(th)
+    assert abs(f2 - g["f"][0]) <= 1e-10 * abs(g["f"][0])
+    assert rel(grad, g["grad"][0]) < 1e-8
+    assert rel(grad, g["grad_fd"][0]) < 1e-6
+    mv = eng.marginal_variances_latent(th, np.zeros(d.n_latent))
+    assert rel(mv, g["marginal_var"][0]) < 1e-9
+
+
+@pytest.fixture(scope="module")
+def full():
+    from dalia_amd.inla import STInla
+    g = load("st_ns2025_nt100_nb6")
+    d = _model_from_fixture(g)
+    print(f"case full-size ns={d.ns} nt={d.nt} nb={d.nb}", flush=True)
+    return g, d, STInla(d)
+
+
+def test_ns2025_nt100_f_mu_logdets(full):
+    g, d, eng = full
+    th = g["theta"]
+    f = eng.f(th)
+    print(f"f gpu {f:.15g} oracle {float(g['f']):.15g}", flush=True)
+    assert abs(f - g["f"]) <= 1e-10 * abs(g["f"])
+    s = eng.scal[:5].cpu().numpy()
+    assert abs(s[0] - g["logdet_qc"]) <= 1e-10 * abs(g["logdet_qc"])
+    assert abs(s[4] - g["logdet_qp"]) <= 1e-10 * abs(g["logdet_qp"])
+    mu = eng.mu.cpu().numpy()
+    assert rel(mu[::int(g["mu_stride"])], g["mu_sub"]) < 1e-9
+    assert abs(mu.sum() - g["mu_sum"]) <= 1e-9 * np.abs(mu).sum()
+    assert abs(np.linalg.norm(mu) - g["mu_norm"]) <= 1e-9 * g["mu_norm"]
+
+
+def test_ns2025_nt100_prior_logdet_closed_form(full):
+    """pobtf's log-determinant of Qp|BT at full size against the Kronecker-eigen closed form,
+    evaluated on the host from an eigendecomposition of the FEM matrices."""
+    g, d, eng = full
+    th = g["theta"]
+    eng.f(th)
+    ld_bt = float(eng.scal[4].item())
+    ld_cf = oinla.prior_logdet_closed_form(d, th)
+    ld_cf = ld_cf[0] if isinstance(ld_cf, tuple) else ld_cf
+    assert abs(ld_bt - ld_cf) <= 1e-10 * abs(ld_cf), (ld_bt, ld_cf)
+
+
+def test_ns2025_nt100_adjoint_gradient(full):
+    g, d, eng = full
+    th = g["theta"]
+    f, grad = eng.value_and_grad(th)
+    print(f"grad gpu {grad} oracle {g['grad']}", flush=True)
+    assert abs(f - g["f"]) <= 1e-10 * abs(g["f"])
+    assert rel(grad, g["grad"]) < 1e-8
+
+
+def test_ns2025_nt100_selected_inverse_blocks(full):
+    g, d, eng = full
+    th = g["theta"]
+    mv = eng.marginal_variances_latent(th, np.zeros(d.n_latent))
+    b, n, a = d.ns, d.nt, d.nb
+    for k, i in enumerate(g["x_blocks"]):
+        assert rel(mv[i * b:(i + 1) * b], g["x_diag"][k]) < 1e-9, i
+    assert rel(mv[n * b:], np.diag(g["x_tip"])) < 1e-9
+    # the sub-diagonal and arrow blocks of the same selected inverse (the factor was overwritten)
+    Xl = eng.solver.A_lower_diagonal_blocks
+    lows = [i for i in g["x_blocks"] if i < n - 1]
+    for k, i in enumerate(lows):
+        assert rel(np.diagonal(Xl[i].cpu().numpy()), g["x_lower_diag"][k]) < 1e-9, i
+    ar = eng.solver.A_arrow_bottom_blocks.cpu().numpy().sum(axis=(1, 2))
+    assert rel(ar, g["x_arrow_sum"]) < 1e-8
+    assert rel(eng.solver.A_arrow_tip_block.cpu().numpy(), g["x_tip"]) < 1e-9
+
+
+def _central_difference(fun, theta, j, h=1e-4):
+    e = np.zeros(theta.size)
+    e[j] = h
+    return (fun(theta + e) - fun(theta - e)) / (2 * h)
+
+
+@pytest.mark.timeout(900)
+def test_ns4096_nt250_size_checks():
+    """The large adjoint case (67 GB of blocks, 104 GB handle) on one GPU."""
+    import torch
+
+    from dalia_amd.inla import STInla
+    d = synth.make_workload("st_ns4096_nt250_nb6")
+    print(f"case large ns={d.ns} nt={d.nt} nb={d.nb}", flush=True)
+    eng = STInla(d)
+    th = d.theta
+    f = eng.f(th)
+    ld_bt = float(eng.scal[4].item())
+    ld_cf = oinla.prior_logdet_closed_form(d, th)
+    ld_cf = ld_cf[0] if isinstance(ld_cf, tuple) else ld_cf
+    print(f"f {f:.15g} logdet Qp bt {ld_bt:.15g} closed form {ld_cf:.15g}", flush=True)
+    assert abs(ld_bt - ld_cf) <= 1e-10 * abs(ld_cf)
+    f2, grad = eng.value_and_grad(th)
+    assert abs(f2 - f) <= 1e-10 * abs(f)
+    for j in (0, 3):
+        cd = _central_difference(eng.f, th, j)
+        print(f"d/dtheta_{j}: adjoint {grad[j]:.12g} central difference {cd:.12g}", flush=True)
+        assert abs(grad[j] - cd) <= 1e-6 * max(abs(cd), 1e-6 * np.abs(grad).max())
+    del eng
+    torch.cuda.empty_cache()
+
+
+def test_coreg2_ns576_nt24_against_oracle():
+    """One instance of the bivariate batch: f against the oracle, the adjoint gradient against GPU
+    central differences in all 9 components."""
+    from dalia_amd.coreg import CoregInla
+    d = synth.make_coreg_model(*synth.COREG_WORKLOADS["coreg2_ns576_nt24_nb4"])
+    print(f"case coreg nv={d.nv} ns={d.ns} nt={d.nt} nb={d.nb}", flush=True)
+    eng = CoregInla(d)
+    rng = np.random.default_rng(11)
+    for k in range(2):
+        th = d.theta + (0.0 if k == 0 else 0.05 * rng.standard_normal(d.theta.size))
+        f = eng.f(th)
+        ref = ocoreg.f_theta_coreg(d, th)
+        assert abs(f - ref) <= 1e-10 * abs(ref), (k, f, ref)
+    th = d.theta
+    f, grad = eng.value_and_grad(th)
+    cd = np.array([_central_difference(eng.f, th, j) for j in range(th.size)])
+    assert np.abs(grad - cd).max() <= 1e-6 * np.abs(cd).max(), (grad, cd)
+
+
+@pytest.mark.timeout(900)
+def test_coreg3_ns1600_nt128_adjoint_vs_central_differences():
+    """The trivariate case (block size 4800, 47 GB of blocks): the adjoint gradient against GPU
+    central differences on a range component and a mixing weight."""
+    import torch
+
+    from dalia_amd.coreg import CoregInla
+    d = synth.make_coreg_model(*synth.COREG_WORKLOADS["coreg3_ns1600_nt128_nb3"])
+    print(f"case coreg nv={d.nv} ns={d.ns} nt={d.nt} nb={d.nb} b={d.b}", flush=True)
+    eng = CoregInla(d)
+    th = d.theta
+    f, grad = eng.value_and_grad(th)
+    assert abs(eng.f(th) - f) <= 1e-10 * abs(f)
+    for j in (0, th.size - 1):
+        cd = _central_difference(eng.f, th, j)
+        print(f"d/dtheta_{j}: adjoint {grad[j]:.12g} central difference {cd:.12g}", flush=True)
+        assert abs(grad[j] - cd) <= 1e-6 * max(abs(cd), 1e-6 * np.abs(grad).max())
+    del eng
+    torch.cuda.empty_cache()

@@ -123,3 +123,20 @@ def test_hessian_from_adjoint_gradients_matches_fd_hessian():
     H = hessian_adjoint(eng, d.theta, h=1e-3)
     H_ref = oinla.fd_hessian(d, d.theta, 1e-3)
     assert np.abs(H - H_ref).max() <= 1e-4 * np.abs(H_ref).max(), (H, H_ref)
+
+
+def test_sphere_manifold_f_and_adjoint():
+    """The sphere branch of the hyperparameter map (`spatio_temporal.py:171-180`) through the HIP
+    path: f and the adjoint gradient against the oracle."""
+    import dataclasses
+
+    from dalia_amd.inla import STInla
+    d = dataclasses.replace(synth.make_st_model(7, 5, 2, 4), manifold="sphere")
+    print("case sphere manifold ns=49 nt=5", flush=True)
+    eng = STInla(d)
+    th = d.theta + np.array([-0.2, 0.1, 0.05, 0.1])
+    f_ref, g_ref = oinla.adjoint_gradient(d, th)
+    assert abs(eng.f(th) - f_ref) <= 1e-10 * abs(f_ref)
+    f, g = eng.value_and_grad(th)
+    assert abs(f - f_ref) <= 1e-10 * abs(f_ref)
+    assert rel(g, g_ref) < 1e-8

@@ -150,3 +150,26 @@ def test_marginal_variances_match_dense_inverse():
     ov = inla.marginal_variances_observations(d, theta)
     ref = np.einsum("ij,jk,ik->i", A, C, A)
     assert np.abs(ov - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def test_sphere_adjoint_gradient_and_closed_form(tiny):
+    """The sphere branch of the hyperparameter map through f, the adjoint gradient and the
+    closed-form prior term (planar FEM inputs: only the map differs)."""
+    import dataclasses
+    d = dataclasses.replace(tiny, manifold="sphere")
+    th = d.theta + np.array([-0.2, 0.1, 0.05, 0.1])
+    f, g = inla.adjoint_gradient(d, th)
+    f_plane = inla.f_theta(tiny, th)
+    assert abs(f - f_plane) > 1e-6 * abs(f_plane)  # the map matters
+    _, gfd = inla.fd_gradient(d, th, 1e-4)
+    assert np.linalg.norm(g - gfd) / np.linalg.norm(g) < 1e-6
+    _, parts = inla.f_theta(d, th, return_parts=True)
+    v, gv = inla.prior_logdet_closed_form(d, th, with_grad=True)
+    assert abs(v - parts["logdet_qp"]) < 1e-11 * abs(v)
+    e = 1e-5
+    for j in range(3):
+        tp, tm = th.copy(), th.copy()
+        tp[j] += e
+        tm[j] -= e
+        gn = (inla.prior_logdet_closed_form(d, tp) - inla.prior_logdet_closed_form(d, tm)) / (2 * e)
+        assert abs(gv[j] - gn) < 1e-6 * max(1.0, abs(gn))
