@@ -13,8 +13,10 @@
   trivariate ns=1600, nt=128 (block size 4800) adjoint gradient against GPU central differences.
 
 Bars: f and log-determinants 1e-10 relative, mu and selected-inverse entries 1e-9 relative,
-adjoint gradient 1e-8 relative against the oracle's adjoint and 1e-6 against central differences
-(h = 1e-4).
+adjoint gradient 1e-8 relative against the oracle's adjoint and 1e-6 relative (to the gradient's
+largest component) against central differences with h = 1e-4: at these sizes f itself is
+reproducible to ~1e-9 relative (a difference of log-determinants ~10 |f|), so the differences
+carry a rounding noise of ~|f| 1e-9 / 2h in every component, whatever its size.
 """
 
 import os
@@ -153,11 +155,13 @@ def test_ns4096_nt250_size_checks():
     print(f"f {f:.15g} logdet Qp bt {ld_bt:.15g} closed form {ld_cf:.15g}", flush=True)
     assert abs(ld_bt - ld_cf) <= 1e-10 * abs(ld_cf)
     f2, grad = eng.value_and_grad(th)
-    assert abs(f2 - f) <= 1e-10 * abs(f)
+    # f2 takes logdet Qp|BT from the closed form, f from pobtf: they differ by half the difference
+    # of the two log-determinants (checked above to 1e-10 of logdet Qp, which is ~10 |f| here)
+    assert abs(f2 - f) <= 0.5e-10 * abs(ld_cf) + 1e-12 * abs(f)
     for j in (0, 3):
         cd = _central_difference(eng.f, th, j)
         print(f"d/dtheta_{j}: adjoint {grad[j]:.12g} central difference {cd:.12g}", flush=True)
-        assert abs(grad[j] - cd) <= 1e-6 * max(abs(cd), 1e-6 * np.abs(grad).max())
+        assert abs(grad[j] - cd) <= 1e-6 * np.abs(grad).max()  # the vector bar, see above
     del eng
     torch.cuda.empty_cache()
 
@@ -193,10 +197,13 @@ def test_coreg3_ns1600_nt128_adjoint_vs_central_differences():
     eng = CoregInla(d)
     th = d.theta
     f, grad = eng.value_and_grad(th)
-    assert abs(eng.f(th) - f) <= 1e-10 * abs(f)
+    f_plain = eng.f(th)
+    ld_p = float(eng.scal[4].item())
+    # value_and_grad's prior log-determinant is each variate's closed form, f's is pobtf's
+    assert abs(f_plain - f) <= 0.5e-10 * abs(ld_p) + 1e-12 * abs(f), (f_plain, f, ld_p)
     for j in (0, th.size - 1):
         cd = _central_difference(eng.f, th, j)
         print(f"d/dtheta_{j}: adjoint {grad[j]:.12g} central difference {cd:.12g}", flush=True)
-        assert abs(grad[j] - cd) <= 1e-6 * max(abs(cd), 1e-6 * np.abs(grad).max())
+        assert abs(grad[j] - cd) <= 1e-6 * np.abs(grad).max()  # the vector bar, see above
     del eng
     torch.cuda.empty_cache()
