@@ -64,6 +64,7 @@ SIGNATURES = {
     "dh_cholesky": (_I, [_P, _I]),
     "dh_solve": (_I, [_P, _P, _I]),
     "dh_logdet": (_I, [_P, _I, _P]),
+    "dh_logdet_noalloc": (_I, [_P, _I, _I, _I, _LL, _P, _I, _I, _P, _P]),
     "dh_selected_inversion": (_I, [_P, _I]),
     "dh_get_info": (_I, [_P, ctypes.POINTER(_I)]),
     "dh_qc_matvec": (_I, [_P, ctypes.POINTER(DhModel), _P, _P, _P, _P]),

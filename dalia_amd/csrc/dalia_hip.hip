@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/dalia_hip.h"
@@ -1654,10 +1655,11 @@ __global__ __launch_bounds__(64) void k_arrow_acc(const double* part, int n, dou
 // logdet
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_logdet_part(const double* D, int bp, int b, double* part) {
-  const double* Db = D + (long)blockIdx.x * bp * bp;
+// part[i] = sum_k log D_i[k][k] for the blocks D_i = D + i * stride (leading dimension ld).
+__global__ __launch_bounds__(256) void k_logdet_part(const double* D, long stride, int ld, int b, double* part) {
+  const double* Db = D + (long)blockIdx.x * stride;
   double s = 0.0;
-  for (int k = threadIdx.x; k < b; k += blockDim.x) s += log(Db[(long)k * bp + k]);
+  for (int k = threadIdx.x; k < b; k += blockDim.x) s += log(Db[(long)k * ld + k]);
   s = dh_block_sum(s);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -1668,7 +1670,7 @@ __global__ __launch_bounds__(256) void k_logdet_part(const double* D, int bp, in
 __global__ __launch_bounds__(64) void k_logdet_fin(const double* part, int n, const double* T, int a, int ap,
                                                    const int* err, double* out) {
   if (threadIdx.x != 0) return;
-  if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+  if (err && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
     *out = __longlong_as_double((long long)DH_NAN_GIVEUP);
     return;
   }
@@ -2634,9 +2636,60 @@ int dh_logdet(dh_handle* h, int sparsity, double* out_dev) {
   if (!h || !out_dev) return DH_EARG;
   if (h->factored < 0) return DH_ESTATE;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
-  hipLaunchKernelGGL(k_logdet_part, dim3(h->n), dim3(256), 0, h->stream, h->diag, h->bp, h->b, h->part);
+  hipLaunchKernelGGL(k_logdet_part, dim3(h->n), dim3(256), 0, h->stream, h->diag, (long)h->bp * h->bp, h->bp,
+                     h->b, h->part);
   hipLaunchKernelGGL(k_logdet_fin, dim3(1), dim3(64), 0, h->stream, h->part, h->n, arrow ? h->tip : nullptr,
                      h->a, h->ap, (const int*)h->err, out_dev);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the entry point that allocates nothing: its partial sums live in `scratch`, a fixed device
+// array (one copy per GPU, held until the process ends), whose device address is looked up once
+// per device on the first call with that device current
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) double scratch[DH_NOALLOC_MAX_BLOCKS];
+
+#define DH_MAX_DEVICES 64
+static double* g_scratch_dev[DH_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device (the host-side name is not that address).
+static int scratch_ptr(double** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= DH_MAX_DEVICES) return DH_EARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    double* p = nullptr;
+    HIPCHK(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+    if (!p) return DH_EHIP;
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return DH_OK;
+}
+
+extern "C" {
+
+int dh_logdet_noalloc(const double* diag, int n, int b, int ld, long long block_stride, const double* tip, int a,
+                      int ld_tip, double* out_dev, void* hip_stream) {
+  if (!diag || !out_dev || n < 1 || b < 1 || ld < b || block_stride < (long long)ld * b || a < 0 ||
+      (a > 0 && (!tip || ld_tip < a)))
+    return DH_EARG;
+  if (n > DH_NOALLOC_MAX_BLOCKS) return DH_EARG;
+  double* part = nullptr;
+  const int rc = scratch_ptr(&part);
+  if (rc != DH_OK) return rc;
+  hipStream_t st = (hipStream_t)hip_stream;
+  // every one of the n partial sums is written before it is read: nothing has to start at zero
+  hipLaunchKernelGGL(k_logdet_part, dim3(n), dim3(256), 0, st, diag, (long)block_stride, ld, b, part);
+  hipLaunchKernelGGL(k_logdet_fin, dim3(1), dim3(64), 0, st, (const double*)part, n, a > 0 ? tip : nullptr, a,
+                     ld_tip, (const int*)nullptr, out_dev);
   HIPCHK(hipGetLastError());
   return DH_OK;
 }

@@ -137,6 +137,20 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity);
  * factorization). */
 #define DH_NAN_GIVEUP 0x7ff80000dead0004ULL
 
+/* dh_logdet without a handle, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc); no counterpart in the
+ * reference.  logdet = 2 (sum_i sum_k log D_i[k][k] + sum_k log tip[k][k]) of a factor held in
+ * the caller's device buffers: block i of n at diag + i * block_stride, b x b with leading
+ * dimension ld; the tip (a x a, leading dimension ld_tip) is skipped when a == 0 (BT).  The
+ * handle's own layout (dh_get_layout) is diag, n, b, bp, bp * bp, tip, a, ap.  *out_dev is a
+ * device double written on hip_stream (asynchronous).  Calls no allocating routine: the n
+ * partial sums live in one fixed device array of DH_NOALLOC_MAX_BLOCKS doubles per GPU (512 KiB,
+ * held until the process ends) that every call on that GPU shares, so calls that overlap on one
+ * GPU race.  n > DH_NOALLOC_MAX_BLOCKS returns DH_EARG. */
+#define DH_NOALLOC_MAX_BLOCKS 65536
+int dh_logdet_noalloc(const double* diag, int n, int b, int ld, long long block_stride, const double* tip, int a,
+                      int ld_tip, double* out_dev, void* hip_stream);
+
 int dh_logdet(dh_handle* h, int sparsity, double* out_dev);
 
 /* Overwrite the factor with the blocks of A^-1 on the BTA pattern; the
