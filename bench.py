@@ -104,35 +104,86 @@ def committed_traffic(kname: str, workload: str):
     return None
 
 
-def cpu_baseline(workload_args, n_full: int, threads: int):
-    """Oracle f(theta) on a bounded sample (same ns, nb; nt_sample time steps), scaled to n_full.
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    The CPU gradient is the reference's: 2d+1 = 9 plain f evaluations.
+
+def _median_time(fn, reps: int) -> float:
+    fn()  # warm-up
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline(workload_args, n_full: int, threads: int, reps: int = 5):
+    """The oracle (numpy/scipy restatement of the reference's CPU path) on the host cores.
+
+    Measured, each a median of `reps` runs after one warm-up:
+    * one f(theta) (BTA + BT factorisations, solve, logdets) on a bounded sample of the workload:
+      the same ns and nb, nt_s = 4 time steps; scaled linearly in nt to the full workload (every
+      cost of f is linear in nt at fixed ns) and by 9 evaluations per central-difference
+      gradient (the reference's method) -> `value`;
+    * pobtaf alone on the same sample (GFLOP/s);
+    * f(theta) plus the selected inversion at the CPU-runnable ns=400, nt=12, nb=4 case, in full;
+    * f on the sample with one BLAS thread (a single run after the warm-up).
     """
-    from threadpoolctl import threadpool_limits
+    from threadpoolctl import threadpool_info, threadpool_limits
 
     from dalia_amd import synth
+    from oracle import bta as obta
     from oracle import inla as oinla
 
     m, nt, nb, seed = workload_args
     nt_s = 4
     d = synth.make_st_model(m, nt_s, nb, seed)
+    d0 = synth.make_st_model(*synth.WORKLOADS["st_ns400_nt12_nb4"])
+    Qc = oinla.qcond_sparse(d, d.theta)
+
+    def factor():
+        D, L, R, T = oinla.sparse_to_bta(Qc, d.nt, d.ns, d.nb, "bta")
+        t0 = time.perf_counter()
+        obta.pobtaf(D, L, R, T)
+        return time.perf_counter() - t0
+
     with threadpool_limits(limits=threads):
-        oinla.f_theta(d, d.theta)  # warm-up
-        reps = []
-        for _ in range(2):
-            t0 = time.perf_counter()
-            oinla.f_theta(d, d.theta)
-            reps.append(time.perf_counter() - t0)
-    t_f = float(np.median(reps)) * n_full / nt_s
+        blas = sorted({f"{i.get('internal_api')} {i.get('version')}" for i in threadpool_info()})
+        t_f = _median_time(lambda: oinla.f_theta(d, d.theta), reps)
+        factor()
+        t_pobtaf = float(np.median([factor() for _ in range(reps)]))
+        t_c0 = _median_time(lambda: (oinla.f_theta(d0, d0.theta), oinla.selected_inverse_blocks(d0, d0.theta)),
+                            reps)
+    with threadpool_limits(limits=1):
+        t_f1 = _median_time(lambda: oinla.f_theta(d, d.theta), 1)
+    scale = n_full / nt_s
+    t_full = t_f * scale
+    f_pob = f_pobtaf(nt_s, m * m, nb)
     return {
-        "value": 1.0 / (9.0 * t_f),
+        "value": 1.0 / (9.0 * t_full),
         "unit": "grad_evals/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle f(theta) (scipy/LAPACK, BTA + BT factorisations) on ns={m * m}, nt={nt_s}, nb={nb}, "
-                   f"median of {len(reps)} after 1 warm-up = {np.median(reps):.3f} s, scaled linearly in nt to "
-                   f"nt={n_full} and by 9 evaluations per central-difference gradient"),
+        "sample": (f"oracle f(theta) (scipy/LAPACK, BTA + BT factorisations) on ns={m * m}, nt={nt_s}, nb={nb}: "
+                   f"median of {reps} after 1 warm-up = {t_f:.3f} s on {threads} threads, scaled linearly in nt "
+                   f"to nt={n_full} and by 9 evaluations per central-difference gradient"),
+        "cpu_model": _cpu_model(),
+        "blas": blas,
+        "f_sample_s": t_f,
+        "f_full_s_scaled": t_full,
+        "f_sample_s_1thread": t_f1,
+        "pobtaf_sample_s": t_pobtaf,
+        "pobtaf_gflops": f_pob / t_pobtaf / 1e9,
+        "pobtaf_full_s_scaled": t_pobtaf * scale,
+        "ns400_nt12_f_plus_selinv_s": t_c0,
     }
 
 
@@ -223,7 +274,9 @@ def main():
         log(f"central differences: {t_fd * 1e3:.1f} ms/gradient, adjoint speed-up {t_fd / t_adj:.2f}x, "
             f"rel. difference {relerr:.2e}")
 
-    # pobtaf alone, HIP events on the library's stream (torch's current stream)
+    # pobtaf alone, HIP events on the library's stream (torch's current stream); nothing runs
+    # beside it (the overlap option is switched off for this measurement)
+    eng._overlap(0)
     eng._assemble(theta, "bta", True)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     times = []
@@ -264,6 +317,37 @@ def main():
     roofline["frac_of_dgemm"] = achieved / dgemm
     log(f"library DGEMM ceiling (torch.mm FP64, 8192^3): {dgemm:.2f} TFLOP/s; {kname} at {achieved / dgemm:.2f} of it")
 
+    # HBM-bound kernels: the assembly of Qc (k_assemble_rows: the dense diagonal and sub-diagonal
+    # block rows) and the solve (pobtas N then C, every launch of dh_solve), events on the stream
+    hbm = {}
+    _lib.check(lib.dh_timing_enable(eng.h, _lib.TIMED_KERNELS["k_assemble_rows"]), "dh_timing_enable")
+    for _ in range(3):
+        eng._assemble(theta, "bta", True)
+    _lib.check(lib.dh_timing_read(eng.h, ctypes.byref(tot), ctypes.byref(cnt)), "dh_timing_read")
+    _lib.check(lib.dh_timing_enable(eng.h, -1), "dh_timing_enable")
+    asm_bytes = 8.0 * (2 * nt - 1) * ns * ns
+    asm_s = tot.value * 1e-3 / max(cnt.value, 1)
+    hbm["k_assemble_rows"] = {"bound": "hbm", "achieved": asm_bytes / asm_s / 1e9, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": asm_bytes / asm_s / 1e9 / HBM_PEAK_GBS,
+                              "avg_launch_us": asm_s * 1e6, "bytes_per_launch": asm_bytes}
+    _lib.check(lib.dh_cholesky(eng.h, 1), "dh_cholesky")
+    x = eng.mu.clone()
+    _lib.check(lib.dh_solve(eng.h, x.data_ptr(), 1), "dh_solve")  # also forms the packed inverse once
+    ts = []
+    for _ in range(3):
+        ev0.record()
+        _lib.check(lib.dh_solve(eng.h, x.data_ptr(), 1), "dh_solve")
+        ev1.record()
+        torch.cuda.synchronize()
+        ts.append(ev0.elapsed_time(ev1) * 1e-3)
+    sol_s = float(np.median(ts))
+    sol_bytes = 16.0 * (nt * ns * (ns + 1) / 2 + (nt - 1) * ns * ns + nt * nb * ns + nb * (nb + 1) / 2)
+    hbm["pobtas"] = {"bound": "hbm", "achieved": sol_bytes / sol_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": sol_bytes / sol_s / 1e9 / HBM_PEAK_GBS, "ms": sol_s * 1e3, "bytes": sol_bytes,
+                     "launches_per_solve": 5 + 7 * nt}
+    log(f"assembly {asm_s * 1e3:.2f} ms ({hbm['k_assemble_rows']['achieved']:.0f} GB/s); "
+        f"pobtas {sol_s * 1e3:.2f} ms ({hbm['pobtas']['achieved']:.0f} GB/s)")
+
     cpu = None
     if not args.no_cpu:
         threads = min(len(os.sched_getaffinity(0)), 16)
@@ -293,6 +377,7 @@ def main():
         "pobtaf_frac_of_peak": pobtaf_tflops / FP64_PEAK_TFLOPS,
         **fd_info,
         "roofline": roofline,
+        "roofline_hbm": hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

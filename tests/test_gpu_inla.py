@@ -140,3 +140,30 @@ def test_sphere_manifold_f_and_adjoint():
     f, g = eng.value_and_grad(th)
     assert abs(f - f_ref) <= 1e-10 * abs(f_ref)
     assert rel(g, g_ref) < 1e-8
+
+
+def test_loaded_input_tree_through_the_hip_path(tmp_path):
+    """A model written in DALIA's on-disk layout (`core/model.py:239-252`,
+    `submodels/spatio_temporal.py:27-35`) and read back by `io.load_st_model` runs through the HIP
+    path: f, the adjoint gradient and the marginal variances against the oracle on the loaded data;
+    the coregional tree likewise for f."""
+    from dalia_amd import io
+    from dalia_amd.coreg import CoregInla
+    from dalia_amd.inla import STInla
+    from oracle import coreg as ocoreg
+    d = synth.make_st_model(9, 6, 3, 5)
+    io.save_st_model(d, str(tmp_path / "st"))
+    e = io.load_st_model(str(tmp_path / "st"), d.theta, d.priors)
+    print(f"case loaded tree ns={e.ns} nt={e.nt} nb={e.nb}", flush=True)
+    eng = STInla(e)
+    f_ref, g_ref = oinla.adjoint_gradient(e, e.theta)
+    assert abs(eng.f(e.theta) - f_ref) <= 1e-10 * abs(f_ref)
+    f, g = eng.value_and_grad(e.theta)
+    assert rel(g, g_ref) < 1e-8
+    mv = eng.marginal_variances_latent(e.theta, np.zeros(e.n_latent))
+    assert rel(mv, oinla.marginal_variances_latent(e, e.theta)) < 1e-9
+    dc = synth.make_coreg_model(5, 4, 2, 2, 7)
+    io.save_coreg_model(dc, str(tmp_path / "coreg"))
+    ec = io.load_coreg_model(str(tmp_path / "coreg"), 2, dc.theta, dc.priors)
+    fc_ref = ocoreg.f_theta_coreg(ec, ec.theta)
+    assert abs(CoregInla(ec).f(ec.theta) - fc_ref) <= 1e-10 * abs(fc_ref)
