@@ -187,6 +187,188 @@ def cpu_baseline(workload_args, n_full: int, threads: int, reps: int = 5):
     }
 
 
+def _f_pobtf(n, b):
+    return n * b ** 3 / 3 + 2 * (n - 1) * b ** 3
+
+
+def _eager_pobtaf(eng, reps: int = 2):
+    """(ms, TFLOP/s) of the persistent factorization of the engine's Qc, HIP events on its stream."""
+    import torch
+
+    from dalia_amd import _lib
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ts = []
+    for _ in range(reps + 1):
+        eng._assemble(eng.data.theta, "bta", True)
+        ev0.record()
+        _lib.check(eng.lib.dh_cholesky(eng.h, 1), "dh_cholesky")
+        ev1.record()
+        torch.cuda.synchronize()
+        ts.append(ev0.elapsed_time(ev1) * 1e-3)
+    t = float(np.median(ts[1:]))
+    lay = eng.solver.layout
+    return t * 1e3, f_pobtaf(lay.n, lay.b, lay.a) / t / 1e12
+
+
+def _coreg_cpu_baseline(d, flops_total: float, reps: int = 3):
+    """Oracle coregional f(theta) on one instance (median of reps after a warm-up), scaled by the
+    factorization flops of the whole workload."""
+    from oracle import coreg as ocoreg
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=threads):
+        t = _median_time(lambda: ocoreg.f_theta_coreg(d, d.theta), reps)
+    f_one = f_pobtaf(d.nt, d.b, d.a) + _f_pobtf(d.nt, d.b)
+    return t, threads, t * flops_total / f_one
+
+
+def bench_coreg_batch(args):
+    """The batch of bivariate instances of uneven size: every 3-point diagonal FD Hessian stencil
+    point (theta of 9 entries, 19 points per instance) of every instance in one graph replay."""
+    import torch
+
+    from dalia_amd import synth
+    from dalia_amd.batch import BatchedF, batched_hessian_diags
+    from dalia_amd.coreg import CoregInla
+    from dalia_amd.inla import hessian_diag_stencil
+
+    specs = synth.COREG_BATCHES[args.workload]
+    t0 = time.perf_counter()
+    datas = [synth.make_coreg_model(*sp) for sp in specs]
+    engines = [CoregInla(d) for d in datas]
+    torch.cuda.synchronize()
+    log(f"set-up {time.perf_counter() - t0:.1f} s: instances {[(d.ns, d.nt, d.b) for d in datas]}")
+    h = 5e-3
+    thetas = [d.theta for d in datas]
+    stencils = [hessian_diag_stencil(t, h, 3) for t in thetas]
+    pts = [p for st in stencils for p in st]
+    E = len(pts)
+    diags, bf = batched_hessian_diags(engines, thetas, h=h, points=3, tune=True)  # capture + first replay
+    log(f"autotuned per instance: {[c['alg'] for c in bf.choices]}; {E} evaluations in one graph")
+    for _ in range(args.warmup):
+        fv = bf(pts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fv = bf(pts)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    log(f"batch: {t_batch * 1e3:.1f} ms per replay of {E} evaluations ({E / t_batch:.1f} f evals/s)")
+    # graph values against eager evaluations of the same engines (first, middle and last point)
+    k0, worst = 0, 0.0
+    for eng, st in zip(engines, stencils):
+        for j in (0, len(st) // 2, len(st) - 1):
+            fe = eng.f(st[j])
+            worst = max(worst, abs(fe - fv[k0 + j]) / abs(fe))
+        k0 += len(st)
+    # each instance alone (its own graph of its 19 evaluations): the batch hides the smaller ones
+    alone = []
+    k0 = 0
+    for eng, st in zip(engines, stencils):
+        bk = BatchedF([eng], [0] * len(st))
+        bk(list(st))
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        bk(list(st))
+        torch.cuda.synchronize()
+        alone.append((time.perf_counter() - ta) * 1e3)
+    pob = [_eager_pobtaf(eng) for eng in engines]
+    flops = sum(len(st) * (f_pobtaf(d.nt, d.b, d.a) + _f_pobtf(d.nt, d.b)) for st, d in zip(stencils, datas))
+    log(f"alone: {[round(x, 1) for x in alone]} ms (sum {sum(alone):.1f}); pobtaf per instance "
+        f"{[(round(m, 2), round(tf, 1)) for m, tf in pob]} (ms, TFLOP/s); graph vs eager {worst:.1e}")
+    cpu = None
+    if not args.no_cpu:
+        t1, thr, t_cpu = _coreg_cpu_baseline(datas[0], flops)
+        cpu = {"value": E / t_cpu, "unit": "f_evals/s", "cores": thr, "kind": "port",
+               "sample": (f"oracle coregional f(theta) at ns={datas[0].ns}, nt={datas[0].nt} (b={datas[0].b}): "
+                          f"median of 3 after 1 warm-up = {t1:.3f} s, scaled by the factorization flops of the "
+                          f"{E} evaluations ({flops / 1e12:.1f} TFLOP)")}
+    out = {
+        "metric": "f(θ) evals/s, 57 batched diagonal FD Hessian evaluations of 3 uneven bivariate instances, FP64",
+        "value": E / t_batch, "unit": "f_evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "instances": [{"ns": d.ns, "nt": d.nt, "b": d.b, "a": d.a,
+                                                             "evaluations": len(st)}
+                                                            for d, st in zip(datas, stencils)],
+                   "evaluations": E, "stencil": "3-point diagonal, h=5e-3", "theta_dim": 9,
+                   "pobtaf_schedule_per_instance": [c["alg"] for c in bf.choices]},
+        "tflop_per_step": flops / 1e12, "tflops": flops / t_batch / 1e12,
+        "alone_ms": alone, "sum_alone_ms": sum(alone),
+        "pobtaf_per_instance": [{"ms": m, "tflops": tf} for m, tf in pob],
+        "graph_vs_eager_max_rel": worst,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_coreg(args):
+    """The trivariate model: adjoint gradient evaluations, then the 31-point diagonal FD Hessian
+    stencil (theta of 15 entries) in one graph replay."""
+    import torch
+
+    from dalia_amd import autotune, synth
+    from dalia_amd.batch import BatchedF
+    from dalia_amd.coreg import CoregInla
+    from dalia_amd.inla import hessian_diag_from_values, hessian_diag_stencil
+
+    d = synth.make_coreg_model(*synth.COREG_WORKLOADS[args.workload])
+    t0 = time.perf_counter()
+    eng = CoregInla(d)
+    choice = autotune.tune(eng)
+    torch.cuda.synchronize()
+    log(f"set-up {time.perf_counter() - t0:.1f} s (nv={d.nv}, ns={d.ns}, nt={d.nt}, b={d.b}, a={d.a}); "
+        f"handle {eng.lib.dh_memory_bytes(eng.h) / 1e9:.1f} GB")
+    th = d.theta
+    for _ in range(args.warmup):
+        eng.value_and_grad(th)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f, g = eng.value_and_grad(th)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / args.steps
+    log(f"adjoint gradient {t_adj * 1e3:.1f} ms, f={f:.12g}")
+    h = 5e-3
+    pts = hessian_diag_stencil(th, h, 3)
+    bf = BatchedF([eng], [0] * len(pts))
+    bf.capture()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fv = bf(list(pts))
+    torch.cuda.synchronize()
+    t_hess = time.perf_counter() - t0
+    hd = hessian_diag_from_values(fv, th.size, h, 3)
+    f_eager = eng.f(th)
+    log(f"31-point diagonal Hessian stencil: {t_hess:.2f} s in one replay; f(theta) graph {fv[0]:.12g} "
+        f"eager {f_eager:.12g}")
+    ms, tf = _eager_pobtaf(eng)
+    flops = f_pobtaf(d.nt, d.b, d.a)
+    cpu = None
+    if not args.no_cpu:
+        small = synth.make_coreg_model(20, 16, d.nb, d.nv, 4)
+        t1, thr, t_cpu = _coreg_cpu_baseline(small, 31 * (flops + _f_pobtf(d.nt, d.b)))
+        cpu = {"value": len(pts) / t_cpu, "unit": "f_evals/s", "cores": thr, "kind": "port",
+               "sample": (f"oracle trivariate f(theta) at ns=400, nt=16 (b=1200): median of 3 = {t1:.3f} s, "
+                          f"scaled by the factorization flops of the 31 evaluations at b={d.b}, nt={d.nt}")}
+    out = {
+        "metric": f"∇f(θ) evals/s adjoint, trivariate ns={d.ns} nt={d.nt} nb={d.nb} (b={d.b}) FP64; "
+                  "31-point diagonal FD Hessian batch",
+        "value": 1.0 / t_adj, "unit": "grad_evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_adj * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "nv": d.nv, "ns": d.ns, "nt": d.nt, "nb": d.nb, "b": d.b, "a": d.a,
+                   "theta_dim": int(th.size), "pobtaf_schedule": choice["alg"]},
+        "hessian_diag_batch_s": t_hess, "hessian_diag_evaluations": len(pts),
+        "hessian_diag_f_evals_per_s": len(pts) / t_hess, "graph_vs_eager_rel": abs(fv[0] - f_eager) / abs(f_eager),
+        "adjoint_vs_central_diff_cost": 31 * (t_hess / len(pts)) / t_adj,
+        "pobtaf_ms": ms, "pobtaf_tflops": tf, "pobtaf_frac_of_peak": tf / FP64_PEAK_TFLOPS,
+        "handle_gb": eng.lib.dh_memory_bytes(eng.h) / 1e9,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +399,12 @@ def main():
         torch.cuda.synchronize()
         log(f"copied {n * 8 / 2 ** 30:.1f} GiB three times")
         return
+
+    from dalia_amd import synth as _synth
+    if args.workload in _synth.COREG_BATCHES:
+        return bench_coreg_batch(args)
+    if args.workload in _synth.COREG_WORKLOADS:
+        return bench_coreg(args)
 
     from dalia_amd import _lib, synth
     from dalia_amd.inla import STInla

@@ -29,8 +29,13 @@ from dalia_amd.inla import (STInla, check_hessian, hessian_diag_from_values, hes
 
 
 class BatchedF:
-    def __init__(self, engines: list[STInla], assignment: list[int]):
-        """engines: one STInla per model instance; assignment[e] = instance of evaluation e."""
+    def __init__(self, engines: list[STInla], assignment: list[int], tune: bool = False):
+        """engines: one STInla (or CoregInla) per model instance; assignment[e] = instance of
+        evaluation e.  tune: before the capture, let the autotuner pick each instance's pobtaf
+        schedule on that instance's own matrix (cached per shape)."""
+        if tune:
+            from dalia_amd import autotune
+            self.choices = [autotune.tune(eng) for eng in engines]
         self.engines = engines
         self.assignment = list(assignment)
         E = len(self.assignment)
@@ -41,7 +46,10 @@ class BatchedF:
         self.graph = None
         self.thetas = [None] * E
 
-    def _enqueue_all(self, capture_stream):
+    def _enqueue_all(self, capture_stream, first_only: bool = False):
+        """Every evaluation on its instance's stream (first_only: one per instance, the warm-up
+        that loads the kernels before the capture; the evaluations of an instance all run the same
+        launches on the same buffers)."""
         for k, eng in enumerate(self.engines):
             s = self.streams[k]
             s.wait_stream(capture_stream)
@@ -52,6 +60,8 @@ class BatchedF:
                         continue
                     eng.params.copy_(self.table[e, :eng.params.numel()], non_blocking=True)
                     eng._enqueue_f(None, out=self.slots[e])
+                    if first_only:
+                        break
             capture_stream.wait_stream(s)
 
     def capture(self):
@@ -59,7 +69,7 @@ class BatchedF:
             row = self.engines[inst].param_row(self.engines[inst].data.theta)
             self.table[e, :row.size].copy_(torch.from_numpy(row))
         cur = torch.cuda.current_stream()
-        self._enqueue_all(cur)  # eager warm-up (allocator, kernels loaded)
+        self._enqueue_all(cur, first_only=True)  # eager warm-up (allocator, kernels loaded)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
@@ -83,11 +93,11 @@ class BatchedF:
                          for e, inst in enumerate(self.assignment)])
 
 
-def batched_hessians(engines: list[STInla], thetas, h: float = 5e-3, offdiag_points: int = 2):
+def batched_hessians(engines: list[STInla], thetas, h: float = 5e-3, offdiag_points: int = 2, tune: bool = False):
     """FD Hessians of every instance at its own theta, all stencil points in one graph launch."""
     stencils = [hessian_stencil(t, h, offdiag_points) for t in thetas]
     assignment = [k for k, st in enumerate(stencils) for _ in range(len(st))]
-    bf = BatchedF(engines, assignment)
+    bf = BatchedF(engines, assignment, tune=tune)
     fv = bf([p for st in stencils for p in st])
     out, k0 = [], 0
     for st, t in zip(stencils, thetas):
@@ -98,13 +108,13 @@ def batched_hessians(engines: list[STInla], thetas, h: float = 5e-3, offdiag_poi
     return out, bf
 
 
-def batched_hessian_diags(engines, thetas, h: float = 5e-3, points: int = 3):
+def batched_hessian_diags(engines, thetas, h: float = 5e-3, points: int = 3, tune: bool = False):
     """Diagonals of the FD Hessians of every instance at its own theta (1 + 2d stencil points
     each for points=3, the reference's `hessian_diag_finite_difference_3pt`), all in one graph
     launch.  Returns (list of diagonals, the BatchedF)."""
     stencils = [hessian_diag_stencil(t, h, points) for t in thetas]
     assignment = [k for k, st in enumerate(stencils) for _ in range(len(st))]
-    bf = BatchedF(engines, assignment)
+    bf = BatchedF(engines, assignment, tune=tune)
     fv = bf([p for st in stencils for p in st])
     out, k0 = [], 0
     for st, t in zip(stencils, thetas):

@@ -161,6 +161,13 @@ class CoregInla:
     def set_stream(self, stream) -> None:
         _lib.check(self.lib.dh_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)), "dh_set_stream")
 
+    def _assemble(self, theta, sparsity: str, with_like: bool):
+        """Write Qc ("bta", with_like) or the prior ("bt") into the blocks (the autotuner's hook)."""
+        if theta is not None:
+            self._set_params(theta)
+        _lib.check(self.lib.dh_assemble_coreg(self.h, self.dm.models, self.data.nv, self.params.data_ptr(),
+                                              _lib.SPARSITY[sparsity], int(with_like)), "dh_assemble_coreg")
+
     def _enqueue_f(self, theta, out=None):
         """Device work of one f evaluation; out[0] = logdet Qc, out[1] = mu^T b, out[4] = logdet Qp|BT."""
         if theta is not None:

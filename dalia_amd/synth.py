@@ -405,3 +405,10 @@ COREG_WORKLOADS = {
     "coreg2_ns576_nt24_nb4": (24, 24, 4, 2, 3),
     "coreg3_ns1600_nt128_nb3": (40, 128, 3, 3, 4),
 }
+
+#: Batches of bivariate instances of uneven size, (m, nt, nb, nv, seed) each.  coreg2_hess57: the
+#: 3-point diagonal FD Hessian stencils (theta of 9 entries: 1 + 2 * 9 = 19 points) of three
+#: instances, ns = 576, 1024, 1600 with nt = 24, 48, 96: 57 evaluations.
+COREG_BATCHES = {
+    "coreg2_hess57": [(24, 24, 4, 2, 3), (32, 48, 4, 2, 5), (40, 96, 4, 2, 7)],
+}

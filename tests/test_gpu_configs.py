@@ -205,5 +205,68 @@ def test_coreg3_ns1600_nt128_adjoint_vs_central_differences():
         cd = _central_difference(eng.f, th, j)
         print(f"d/dtheta_{j}: adjoint {grad[j]:.12g} central difference {cd:.12g}", flush=True)
         assert abs(grad[j] - cd) <= 1e-6 * np.abs(grad).max()  # the vector bar, see above
-    del eng
+    # the 31-point diagonal Hessian stencil in one graph at this size: the centre value against the
+    # eager f, two diagonal entries against central differences of the adjoint gradient
+    from dalia_amd.batch import batched_hessian_diags
+    (hd,), bf = batched_hessian_diags([eng], [th], h=5e-3, points=3)
+    print(f"31-point batch done: {len(bf.assignment)} evaluations", flush=True)
+    assert len(bf.assignment) == 31
+    f_centre = eng._f_from_scalars(th, bf.slots[0].cpu().numpy())  # the replay's first evaluation
+    assert abs(f_centre - f_plain) <= 1e-12 * abs(f_plain)
+    for j in (0, th.size - 1):
+        e = np.zeros(th.size)
+        e[j] = 1e-3
+        hj = (eng.value_and_grad(th + e)[1][j] - eng.value_and_grad(th - e)[1][j]) / 2e-3
+        print(f"H_{j}{j}: stencil {hd[j]:.10g} adjoint differences {hj:.10g}", flush=True)
+        assert abs(hd[j] - hj) <= 2e-3 * max(abs(hj), 1e-3 * np.abs(hd).max())
+    del eng, bf
     torch.cuda.empty_cache()
+
+
+def test_coreg2_hess57_batch_in_one_graph():
+    """The 57-evaluation batch: the 19-point diagonal Hessian stencils of three bivariate instances of
+    uneven size (ns 576, 1024, 1600; nt 24, 48, 96) in one graph replay, each instance's schedule
+    chosen by the autotuner.  Every value against an eager evaluation of the same engine (1e-12),
+    the smallest instance's values against the oracle (1e-10), and the diagonals against central
+    differences of the adjoint gradient for the two smaller instances."""
+    from dalia_amd.batch import batched_hessian_diags
+    from dalia_amd.coreg import CoregInla
+    from dalia_amd.inla import hessian_adjoint, hessian_diag_stencil
+    datas = [synth.make_coreg_model(*sp) for sp in synth.COREG_BATCHES["coreg2_hess57"]]
+    print("case batch57 instances", [(d.ns, d.nt, d.b) for d in datas], flush=True)
+    engines = [CoregInla(d) for d in datas]
+    thetas = [d.theta for d in datas]
+    diags, bf = batched_hessian_diags(engines, thetas, h=5e-3, points=3, tune=True)
+    assert len(bf.assignment) == 57
+    pts = [p for t in thetas for p in hessian_diag_stencil(t, 5e-3, 3)]
+    fv = bf(pts)
+    for e, inst in enumerate(bf.assignment):
+        fe = engines[inst].f(pts[e])
+        assert abs(fv[e] - fe) <= 1e-12 * abs(fe), (e, inst)
+    for e in range(19):
+        ref = ocoreg.f_theta_coreg(datas[0], pts[e])
+        assert abs(fv[e] - ref) <= 1e-10 * abs(ref), e
+    print("batch57 values checked; Hessian diagonals against adjoint central differences", flush=True)
+    for k in (0, 1):
+        Ha = hessian_adjoint(engines[k], thetas[k], h=1e-3)
+        assert np.abs(diags[k] - np.diag(Ha)).max() <= 2e-3 * np.abs(np.diag(Ha)).max(), (diags[k], np.diag(Ha))
+
+
+def test_coreg3_hessian_diag_31_points_against_oracle():
+    """The 31-point diagonal stencil (theta of 15 entries) of a trivariate model in one graph, every
+    value against the oracle, at a size the oracle finishes in seconds (ns = 100, nt = 6)."""
+    from dalia_amd.batch import batched_hessian_diags
+    from dalia_amd.coreg import CoregInla
+    from dalia_amd.inla import hessian_diag_stencil
+    d = synth.make_coreg_model(10, 6, 3, 3, 21)
+    print(f"case 31-point trivariate ns={d.ns} nt={d.nt}", flush=True)
+    eng = CoregInla(d)
+    diags, bf = batched_hessian_diags([eng], [d.theta], h=5e-3, points=3)
+    pts = hessian_diag_stencil(d.theta, 5e-3, 3)
+    assert len(pts) == 31
+    fv = bf(list(pts))
+    ref = np.array([ocoreg.f_theta_coreg(d, p) for p in pts])
+    assert np.abs(fv - ref).max() <= 1e-10 * np.abs(ref).max()
+    h = 5e-3
+    ref_diag = (ref[1::2] - 2 * ref[0] + ref[2::2]) / h ** 2
+    assert np.abs(diags[0] - ref_diag).max() <= 1e-5 * np.abs(ref_diag).max()
