@@ -81,6 +81,15 @@ SIGNATURES = {
     "dh_prior_eig": (_I, [_P, ctypes.POINTER(DhModel), _P, c_double_p, _P]),
     "dh_set_tuning": (_I, [_P, _I, _I]),
     "dh_dag_table": (_I, [_I, _I, _I, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "dh_ckpt_create": (_I, [_I, _I, _I, _I, _P, ctypes.POINTER(_P)]),
+    "dh_ckpt_destroy": (_I, [_P]),
+    "dh_ckpt_memory_bytes": (_LL, [_P]),
+    "dh_ckpt_vectors": (_P, [_P]),
+    "dh_ckpt_set_stream": (_I, [_P, _P]),
+    "dh_ckpt_get_info": (_I, [_P, ctypes.POINTER(_I)]),
+    "dh_ckpt_set_tuning": (_I, [_P, _I, _I]),
+    "dh_ckpt_forward": (_I, [_P, ctypes.POINTER(DhModel), _P, _I, _P, _P]),
+    "dh_ckpt_backward": (_I, [_P, ctypes.POINTER(DhModel), _P, _P, _I, _P, _P, _P]),
     "dh_timing_enable": (_I, [_P, _I]),
     "dh_timing_read": (_I, [_P, c_double_p, ctypes.POINTER(_LL)]),
 }

@@ -113,8 +113,10 @@ __global__ __launch_bounds__(256) void k_copy(const double* x, double* y, int le
 
 // One workgroup per row of a diagonal or sub-diagonal block; the row is built
 // in LDS (zero, FEM Kronecker entries, tau A^T A entries) and streamed out.
-__global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* lower, int n, int b, int bp,
-                                                       dh_model m, const double* params, int with_like) {
+// Time blocks t0 .. t0 + n - 1 go to diag[0 .. n - 1], the sub-diagonal blocks (t + 1, t) for
+// t = t0 .. t0 + nl - 1 to lower[0 .. nl - 1] (a segment of a longer time axis, or all of it).
+__global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* lower, int n, int nl, int t0, int b,
+                                                       int bp, dh_model m, const double* params, int with_like) {
   extern __shared__ double row[];
   const long r = blockIdx.x;
   const long ndiag = (long)n * bp;
@@ -123,18 +125,20 @@ __global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* low
   double* out;
   if (r < ndiag) {
     isdiag = true;
-    t = (int)(r / bp);
+    const int tl = (int)(r / bp);
     s = (int)(r % bp);
+    t = t0 + tl;
     trow = tcol = t;
-    out = diag + (long)t * bp * bp + (long)s * bp;
+    out = diag + (long)tl * bp * bp + (long)s * bp;
   } else {
     isdiag = false;
     long rr = r - ndiag;
-    t = (int)(rr / bp);
+    const int tl = (int)(rr / bp);
     s = (int)(rr % bp);
+    t = t0 + tl;
     trow = t + 1;
     tcol = t;
-    out = lower + (long)t * bp * bp + (long)s * bp;
+    out = lower + (long)tl * bp * bp + (long)s * bp;
   }
   for (int c = threadIdx.x; c < bp; c += blockDim.x) row[c] = 0.0;
   __syncthreads();
@@ -176,11 +180,12 @@ __global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* low
   for (int c = threadIdx.x; c < bp / 2; c += blockDim.x) o2[c] = make_double2(row[2 * c], row[2 * c + 1]);
 }
 
-__global__ __launch_bounds__(256) void k_assemble_arrow(double* arrow, int n, int b, int a, int bp, int ap,
+// Arrow rows of time blocks t0 .. t0 + n - 1 into arrow[0 .. n - 1].
+__global__ __launch_bounds__(256) void k_assemble_arrow(double* arrow, int n, int t0, int b, int a, int bp, int ap,
                                                         dh_model m, const double* params, int with_like) {
   const double tau = params[12];
-  const long r = blockIdx.x;  // t*ap + j
-  const int t = (int)(r / ap), j = (int)(r % ap);
+  const long r = blockIdx.x;  // local t * ap + j
+  const int t = t0 + (int)(r / ap), j = (int)(r % ap);
   double* out = arrow + r * bp;
   const long nst = (long)m.ns * m.nt;
   for (int s = threadIdx.x; s < bp; s += blockDim.x) {
@@ -655,6 +660,9 @@ struct DagArgs {
   int look;        // helper lookahead in tile columns (>= 1)
   int slook;       // early-task lookahead of the other tiles (>= 1)
   int* bdone;      // finished tasks per block (null: not counted)
+  int has_prev;    // block 0 has a previous block whose factor is final (a segment of a longer matrix)
+  int last_full;   // the last block has sub-diagonal tiles (its successor is factored by a later launch)
+  int row0;        // global row of block 0 (pivot failures are reported as global rows)
 };
 
 #define DH_DAG_SPIN (1 << 20)
@@ -851,13 +859,14 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
   const int FS = dag_fstride(nbt);
   const int i = min(t / g.tfull, g.n - 1);
   const int e = t - i * g.tfull;
-  const int code = g.tab[(i < g.n - 1) ? e : g.tfull + e];
+  const int code = g.tab[(i < g.n - 1 || g.last_full) ? e : g.tfull + e];
+  const bool prev = i > 0 || g.has_prev;  // the previous block's terms apply
   const int kind = code >> 24, r = (code >> 12) & 0xfff, c = code & 0xfff;
   const long T64 = (long)DH_TILE * bp;  // one tile row
   double* D = g.diag + i * bb;
   double* Lo = g.lower + i * bb;         // i < n-1
   double* Ar = g.arrow + (long)i * T64;
-  const double* Lp = g.lower + (i - 1) * bb;       // i > 0
+  const double* Lp = g.lower + (i - 1) * bb;       // prev
   const double* Ap = g.arrow + (long)(i - 1) * T64;
   double* dinv = g.dinv + (long)i * bp;
   double* pinv = g.pinv + (long)(i & 1) * nbt * DH_TILE * DH_TILE;  // this block's dense tile inverses
@@ -889,7 +898,7 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
     fdone = fT;
   } else if (kind == DK_OLD_D) {  // tile (r, r): previous block's terms and own terms k < kold
     dag_preload_neg(acc, D + r * T64 + r * DH_TILE, bp);
-    if (i > 0)
+    if (prev)
       dag_accumulate(acc, Lp, r * DH_TILE, Lp, r * DH_TILE, bp, fLp + r * nbt, nullptr, r == 0 ? nbt - 1 : nbt,
                      g.err, s_n);
     dag_accumulate(acc, D, r * DH_TILE, D, r * DH_TILE, bp, fD + r * nbt, nullptr, kold, g.err, s_n);
@@ -897,7 +906,7 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
     fdone = fOD + r;
   } else if (kind == DK_OLD_L) {  // tile (r, r-1): previous block's terms and own terms k < kold
     dag_preload_neg(acc, D + r * T64 + (r - 1) * DH_TILE, bp);
-    if (i > 0)
+    if (prev)
       dag_accumulate(acc, Lp, r * DH_TILE, Lp, (r - 1) * DH_TILE, bp, fLp + r * nbt, fLp + (r - 1) * nbt, nbt,
                      g.err, s_n);
     dag_accumulate(acc, D, r * DH_TILE, D, (r - 1) * DH_TILE, bp, fD + r * nbt, fD + (r - 1) * nbt, kold, g.err,
@@ -957,7 +966,7 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
       dag_signal(fD + c * nbt + (c - 1));
       const Opnd xs{dh_lV, DH_LDS_LD, M_ROW, nullptr};
       dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, xs, 0, xs, 0, 0, DH_TILE);
-    } else if (i > 0) {
+    } else if (prev) {
       dag_ready(fLp + (nbt - 1), nullptr, 0, 1, g.err, s_n);
       const Opnd xl{Lp + (nbt - 1) * DH_TILE, bp, M_ROW, nullptr};
       dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, xl, 0, xl, 0, 0, DH_TILE);
@@ -968,7 +977,7 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
       for (int x1 = 0; x1 < 2; ++x1) acc[x0][x1] = part[x0][x1] - acc[x0][x1];
     dh_store(dh_lS, DH_LDS_LD, 0, 0, acc, 1.0, 0.0, false);
     __syncthreads();
-    dh_leaf64_core(g.info, i * g.b + c * DH_TILE);
+    dh_leaf64_core(g.info, g.row0 + i * g.b + c * DH_TILE);
     // Within the launch the tile (c, c) is only ever used through its dense inverse, so the flag
     // is raised once that is out; L, inv(L)^T and dinv follow (read after the launch).
     dh_leaf64_store_inv(pv);
@@ -976,11 +985,11 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
     dh_leaf64_store_tile(C, bp, dinv + c * DH_TILE);
     fdone = nullptr;
   } else if (kind == DK_OLD_X) {  // diagonal-block tile (r, c): previous block's terms, own terms k < ks
-    if (i > 0 || ks > 0) dag_preload_neg(acc, D + r * T64 + c * DH_TILE, bp);
-    if (i > 0)
+    if (prev || ks > 0) dag_preload_neg(acc, D + r * T64 + c * DH_TILE, bp);
+    if (prev)
       dag_accumulate(acc, Lp, r * DH_TILE, Lp, c * DH_TILE, bp, fLp + r * nbt, fLp + c * nbt, nbt, g.err, s_n);
     dag_accumulate(acc, D, r * DH_TILE, D, c * DH_TILE, bp, fD + r * nbt, fD + c * nbt, ks, g.err, s_n);
-    if (i > 0 || ks > 0) dh_store(D + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
+    if (prev || ks > 0) dh_store(D + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fX + r * nbt + c;
   } else if (kind == DK_OLD_LW) {  // sub-diagonal tile (r, c): own terms k < ks
     dag_preload_neg(acc, Lo + r * T64 + c * DH_TILE, bp);
@@ -988,10 +997,10 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
     dh_store(Lo + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fLW + r * nbt + c;
   } else if (kind == DK_OLD_A) {  // arrow tile c: previous block's terms, own terms k < ks
-    if (i > 0 || ks > 0) dag_preload_neg(acc, Ar + c * DH_TILE, bp);
-    if (i > 0) dag_accumulate(acc, Ap, 0, Lp, c * DH_TILE, bp, fAp, fLp + c * nbt, nbt, g.err, s_n);
+    if (prev || ks > 0) dag_preload_neg(acc, Ar + c * DH_TILE, bp);
+    if (prev) dag_accumulate(acc, Ap, 0, Lp, c * DH_TILE, bp, fAp, fLp + c * nbt, nbt, g.err, s_n);
     dag_accumulate(acc, Ar, 0, D, c * DH_TILE, bp, fA, fD + c * nbt, ks, g.err, s_n);
-    if (i > 0 || ks > 0) dh_store(Ar + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
+    if (prev || ks > 0) dh_store(Ar + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fXA + c;
   } else {
     // the early task of the tile exists for every diagonal-block and arrow
@@ -1867,6 +1876,48 @@ __global__ __launch_bounds__(256) void k_contract_ata(const double* diag, const 
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
+// <X, A^T A> restricted to a segment of time blocks t0 .. t0 + cnt - 1 whose selected-inverse
+// blocks sit at slots 0 .. cnt - 1 of diag / arrow, and the couplings (t + 1, t) at lower[t - t0]
+// for t < t0 + nl (nl = cnt, or cnt - 1 at the end of the time axis).  Row blocks t0 .. t0 + nl
+// are visited: diagonal-block entries of rows t < t0 + cnt, coupling entries (t, t + 1) of rows
+// t < t0 + nl and (t, t - 1) of rows t0 < t <= t0 + nl, so that summing over all segments counts
+// every entry of X .* A^T A once; the arrow rows of the segment twice; the tip when tip != null.
+__global__ __launch_bounds__(256) void k_contract_ata_seg(const double* diag, const double* lower,
+                                                          const double* arrow, const double* tip, int t0, int cnt,
+                                                          int nl, int bp, int a, int ap, dh_model m, double* part) {
+  const long rl = (long)blockIdx.x * blockDim.x + threadIdx.x;  // row within row blocks t0 .. t0 + nl
+  const long nrow = (long)(nl + 1) * m.ns;
+  const long nst = (long)m.ns * m.nt;
+  double acc = 0.0;
+  if (rl < nrow) {
+    const int tl = (int)(rl / m.ns), s = (int)(rl % m.ns);
+    const int t = t0 + tl;
+    const long r = (long)t * m.ns + s;
+    for (long e = m.ata_rowptr[r]; e < m.ata_rowptr[r + 1]; ++e) {
+      const int c = m.ata_col[e];
+      const int t2 = c / m.ns, s2 = c - t2 * m.ns;
+      double xv = 0.0;
+      if (t2 == t) {
+        if (tl < cnt) xv = diag[(long)tl * bp * bp + (long)s * bp + s2];
+      } else if (t2 == t - 1) {
+        if (tl >= 1) xv = lower[(long)(tl - 1) * bp * bp + (long)s * bp + s2];
+      } else if (tl < nl) {
+        xv = lower[(long)tl * bp * bp + (long)s2 * bp + s];
+      }
+      acc += xv * m.ata_val[e];
+    }
+    if (arrow && tl < cnt)
+      for (int j = 0; j < a; ++j) acc += 2.0 * arrow[((long)tl * ap + j) * bp + s] * m.ata_arrow[(long)j * nst + r];
+  }
+  if (rl == 0 && tip)
+    for (int j = 0; j < a; ++j)
+      for (int k = 0; k < a; ++k) acc += tip[j * ap + k] * m.ata_tip[j * a + k];
+  acc = dh_block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ void k_add_to(const double* x, double* y) { *y += *x; }
+
 // x^T [A X]^T [A X] x for one model's observations (x[t * stride + voff + s]
 // for the nodes, xa[j] for its a fixed effects); one thread per node row.
 __global__ __launch_bounds__(256) void k_ata_quad(const double* x, const double* xa, int stride, int voff, int a,
@@ -2148,7 +2199,11 @@ extern "C" {
 
 int dh_version(void) { return 1; }
 
-int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
+}  // extern "C"
+
+// storage = false: a handle for the vector operations of an n-block model only (no block storage,
+// no factorization scratch), the companion of a checkpointed evaluation.
+static int create_handle(int n, int b, int a, void* stream, bool storage, dh_handle** out) {
   if (!out || n < 1 || b < 1 || a < 0 || a > 64) return DH_EARG;
   dh_handle* h = new dh_handle();
   h->n = n;
@@ -2196,17 +2251,18 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
       h->side = nullptr;
     }
   int rc = DH_OK;
-  rc |= dalloc(h, &h->diag, n * bb);
-  rc |= dalloc(h, &h->lower, (n - 1) * bb);
-  rc |= dalloc(h, &h->arrow, (long long)n * h->ap * h->bp);
-  rc |= dalloc(h, &h->tip, (long long)h->ap * h->ap);
-  rc |= dalloc(h, &h->dinv, (long long)n * h->bp);
-  rc |= dalloc(h, &h->pinv, 64 * 64);
-  rc |= dalloc(h, &h->tipinv, 64 * 64);
-  rc |= dalloc(h, &h->tipdinv, 64);
-  rc |= dalloc(h, &h->W, h->G * (long long)h->bp * (h->bp + 8));  // pobtasi W slots (padded pitch)
-  rc |= dalloc(h, &h->S, h->G * bb);
-  rc |= dalloc(h, &h->Wa, h->G * 64LL * h->bp);
+  const long long sb = storage ? 1 : 0;  // block storage and factorization scratch, or none
+  rc |= dalloc(h, &h->diag, sb * n * bb);
+  rc |= dalloc(h, &h->lower, sb * (n - 1) * bb);
+  rc |= dalloc(h, &h->arrow, sb * n * h->ap * h->bp);
+  rc |= dalloc(h, &h->tip, sb * h->ap * h->ap);
+  rc |= dalloc(h, &h->dinv, sb * n * h->bp);
+  rc |= dalloc(h, &h->pinv, sb * 64 * 64);
+  rc |= dalloc(h, &h->tipinv, sb * 64 * 64);
+  rc |= dalloc(h, &h->tipdinv, sb * 64);
+  rc |= dalloc(h, &h->W, sb * h->G * h->bp * (h->bp + 8));  // pobtasi W slots (padded pitch)
+  rc |= dalloc(h, &h->S, sb * h->G * bb);
+  rc |= dalloc(h, &h->Wa, sb * h->G * 64LL * h->bp);
   rc |= dalloc(h, &h->z, (long long)n * h->bp + 64);
   rc |= dalloc(h, &h->v, h->bp);
   h->part_len = (long long)(2 * n) * ((h->bp + 255) / 256 + 1) * 4 + 4096 +
@@ -2214,15 +2270,15 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   rc |= dalloc(h, &h->part, h->part_len);
   rc |= dalloc(h, &h->arrpart, (long long)n * 64);
   rc |= dalloc(h, &h->eigbuf, 2LL * n * h->bp);
-  rc |= dalloc(h, &h->dagpinv, 2LL * h->nbt * 64 * 64);
-  rc |= dalloc(h, &h->qinv, n * bb);
+  rc |= dalloc(h, &h->dagpinv, sb * 2 * h->nbt * 64 * 64);
+  rc |= dalloc(h, &h->qinv, sb * n * bb);
   if (hipMalloc((void**)&h->info, sizeof(int)) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->flags, sizeof(int) * ((size_t)n * h->nbt + 1)) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->err, sizeof(int)) != hipSuccess) rc = DH_EHIP;
-  const size_t nflags = (size_t)n * dag_fstride(h->nbt) + n;  // + finished tasks per block
+  const size_t nflags = storage ? (size_t)n * dag_fstride(h->nbt) + n : 1;  // + finished tasks per block
   if (hipMalloc((void**)&h->dagflags, sizeof(int) * nflags) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->dagcounter, sizeof(int)) != hipSuccess) rc = DH_EHIP;
-  if (rc == DH_OK) {
+  if (rc == DH_OK && storage) {
     std::vector<int> tab;
     for (int ar = 0; ar < 2; ++ar)
       for (int last = 0; last < 2; ++last) {
@@ -2241,6 +2297,12 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
   build_nodes(0, h->nbt, 0, h->levels);
   *out = h;
   return DH_OK;
+}
+
+extern "C" {
+
+int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
+  return create_handle(n, b, a, stream, true, out);
 }
 
 int dh_destroy(dh_handle* h) {
@@ -2297,11 +2359,11 @@ int dh_assemble(dh_handle* h, const dh_model* m, const double* params, int spars
   if (sparsity == DH_SPARSITY_BTA && m->nb != h->a) return DH_EARG;
   const long rows = (long)(2 * h->n - 1) * h->bp;
   DH_LAUNCH(h, 5, k_assemble_rows, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double), h->diag,
-            h->lower, h->n, h->b, h->bp, *m, params, with_like);
+            h->lower, h->n, h->n - 1, 0, h->b, h->bp, *m, params, with_like);
   HIPCHK(hipGetLastError());
   if (h->a > 0) {
     const int wl = (sparsity == DH_SPARSITY_BTA) ? with_like : 0;
-    hipLaunchKernelGGL(k_assemble_arrow, dim3(h->n * h->ap), dim3(256), 0, h->stream, h->arrow, h->n, h->b, h->a,
+    hipLaunchKernelGGL(k_assemble_arrow, dim3(h->n * h->ap), dim3(256), 0, h->stream, h->arrow, h->n, 0, h->b, h->a,
                        h->bp, h->ap, *m, params, wl);
     hipLaunchKernelGGL(k_assemble_tip, dim3(1), dim3(256), 0, h->stream, h->tip, h->a, h->ap, *m, params, wl);
     HIPCHK(hipGetLastError());
@@ -2428,6 +2490,49 @@ static void factor_products_chunk(dh_handle* h, int g0, int gc, bool arrow, hipS
   hipLaunchKernelGGL(k_sel_ll, dim3(nbt * (nbt + 1) / 2, gc), dim3(256), 0, st, sb);
 }
 
+// Arguments of one persistent factorization launch over blocks slot0 .. slot0 + cnt - 1 of the
+// handle's arrays.  has_prev: the factor of block slot0 - 1 (its sub-diagonal and arrow tiles) is
+// final and its terms apply to block slot0; last_full: block slot0 + cnt - 1 also gets its
+// sub-diagonal tiles (its successor is factored by a later launch); tip: where the tip partial
+// accumulates; row0: global row of block slot0 for the pivot report.
+static DagArgs dag_args(const dh_handle* h, int ar, int slot0, int cnt, bool has_prev, bool last_full, double* tip,
+                        int row0) {
+  const long bb = (long)h->bp * h->bp;
+  DagArgs g;
+  g.diag = h->diag + slot0 * bb;
+  g.lower = h->lower ? h->lower + slot0 * bb : nullptr;
+  g.arrow = h->arrow ? h->arrow + (long)slot0 * h->ap * h->bp : nullptr;
+  g.tip = tip;
+  g.dinv = h->dinv + (long)slot0 * h->bp;
+  g.pinv = h->dagpinv;
+  g.flags = h->dagflags + (long)slot0 * dag_fstride(h->nbt);
+  g.counter = h->dagcounter;
+  g.tab = h->dagtab + (ar ? h->dagt[0][0] + h->dagt[0][1] : 0);
+  g.info = h->info;
+  g.err = h->err;
+  g.n = cnt;
+  g.b = h->b;
+  g.bp = h->bp;
+  g.nbt = h->nbt;
+  g.arrow_on = ar;
+  g.tfull = h->dagt[ar][0];
+  g.ntasks = last_full ? cnt * h->dagt[ar][0] : (cnt - 1) * h->dagt[ar][0] + h->dagt[ar][1];
+  g.look = DH_DAG_LOOK;
+  g.slook = DH_DAG_SLOOK;
+  g.bdone = nullptr;
+  g.has_prev = has_prev ? 1 : 0;
+  g.last_full = last_full ? 1 : 0;
+  g.row0 = row0;
+  return g;
+}
+
+// The launch itself (the flags of its blocks must be clear, those of a previous block raised).
+static void dag_launch(dh_handle* h, const DagArgs& g) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
+  const unsigned wgs = (unsigned)std::min<int>(g.ntasks, h->dag_wgs);
+  DH_LAUNCH(h, 7, k_pobtaf_dag, dim3(wgs), dim3(256), 0, g);
+}
+
 int dh_cholesky(dh_handle* h, int sparsity) {
   if (!h) return DH_EARG;
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
@@ -2445,36 +2550,14 @@ int dh_cholesky(dh_handle* h, int sparsity) {
                     cap == hipStreamCaptureStatusNone;
     const bool fp = ov && h->overlap == 2 && h->Wall;
     hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags, 0, n * dag_fstride(nbt) + n);
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
     const int ar = arrow ? 1 : 0;
-    DagArgs g;
-    g.diag = h->diag;
-    g.lower = h->lower;
-    g.arrow = h->arrow;
-    g.tip = h->tip;
-    g.dinv = h->dinv;
-    g.pinv = h->dagpinv;
-    g.flags = h->dagflags;
-    g.counter = h->dagcounter;
-    g.tab = h->dagtab + (ar ? h->dagt[0][0] + h->dagt[0][1] : 0);
-    g.info = h->info;
-    g.err = h->err;
-    g.n = n;
-    g.b = h->b;
-    g.bp = bp;
-    g.nbt = nbt;
-    g.arrow_on = ar;
-    g.tfull = h->dagt[ar][0];
-    g.ntasks = (n - 1) * h->dagt[ar][0] + h->dagt[ar][1];
-    g.look = DH_DAG_LOOK;
-    g.slook = DH_DAG_SLOOK;
+    DagArgs g = dag_args(h, ar, 0, n, false, false, h->tip, 0);
     g.bdone = ov ? h->dagflags + (long)n * dag_fstride(nbt) : nullptr;
     if (ov) {  // the side stream starts after the flags are cleared
       HIPCHK(hipEventRecord(h->sev[0], h->stream));
       HIPCHK(hipStreamWaitEvent(h->side, h->sev[0], 0));
     }
-    const unsigned wgs = (unsigned)std::min<int>(g.ntasks, h->dag_wgs);
-    DH_LAUNCH(h, 7, k_pobtaf_dag, dim3(wgs), dim3(256), 0, g);
+    dag_launch(h, g);
     if (arrow)
       hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, h->stream, h->tip, (long)h->ap, h->tipdinv, h->tipinv,
                          h->info, n * h->b);
@@ -2567,6 +2650,74 @@ static int ensure_inverse(dh_handle* h) {
   return DH_OK;
 }
 
+}  // extern "C"
+
+// Forward sweep of pobtas over blocks slot0 .. slot0 + cnt - 1 of the handle's factor:
+// y_i = inv(L_ii) (b_i - L_{i,i-1} y_{i-1}) in place in z (padded, block pitch bp; z[0] is block
+// slot0's part, z[-bp..] the previous block's y when has_prev), and za -= sum_i L_{a,i} y_i.
+static void solve_forward(dh_handle* h, double* z, int slot0, int cnt, bool has_prev, bool arrow, double* za) {
+  const int bp = h->bp;
+  const long bb = (long)bp * bp;
+  const int ncb = (bp + 127) / 128;  // column blocks of the partial GEMVs
+  const int nfb = (bp + 63) / 64;    // column blocks of k_fin_col
+  const int nch = (bp + DH_CCH - 1) / DH_CCH;
+  for (int l = 0; l < cnt; ++l) {
+    const int i = slot0 + l;
+    double* zi = z + (long)l * bp;
+    if (l > 0 || has_prev) {
+      hipLaunchKernelGGL(k_gemv_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, h->lower + (i - 1) * bb, (long)bp,
+                         bp, bp, zi - bp, zi, h->v, -1.0);
+    } else {
+      hipLaunchKernelGGL(k_copy, dim3((bp + 255) / 256), dim3(256), 0, h->stream, zi, h->v, bp);
+    }
+    hipLaunchKernelGGL(k_gemv_col_part, dim3(ncb, nch), dim3(256), 0, h->stream, h->diag + i * bb, (long)bp, bp,
+                       bp, (const double*)h->v, h->part, 1);
+    hipLaunchKernelGGL(k_fin_col, dim3(nfb), dim3(256), 0, h->stream, h->part, nch, bp, (const double*)h->v,
+                       h->dinv + (long)i * bp, 1.0, zi);
+    if (arrow)
+      hipLaunchKernelGGL(k_gemv_row, dim3(16), dim3(256), 0, h->stream, h->arrow + (long)i * h->ap * bp, (long)bp, 64,
+                         bp, (const double*)zi, (const double*)nullptr, h->arrpart + (long)l * 64, 1.0);
+  }
+  if (arrow) hipLaunchKernelGGL(k_arrow_acc, dim3(1), dim3(64), 0, h->stream, h->arrpart, cnt, za);
+}
+
+// za = inv(L_aa)^T inv(L_aa) za (the tip of both sweeps).
+static void solve_tip(dh_handle* h, double* za) {
+  hipLaunchKernelGGL(k_tip_apply, dim3(1), dim3(64), 0, h->stream, h->tipinv, za, h->v, 0);
+  hipLaunchKernelGGL(k_tip_apply, dim3(1), dim3(64), 0, h->stream, h->tipinv, h->v, za, 1);
+}
+
+// Backward sweep over blocks slot0 + cnt - 1 down to slot0: x_i = inv(L_ii)^T (y_i - L_{i+1,i}^T x_{i+1}
+// - L_{a,i}^T x_a), z as for solve_forward (z[cnt * bp..] holds x of the next block when has_next).
+static void solve_backward(dh_handle* h, double* z, int slot0, int cnt, bool has_next, bool arrow, const double* za) {
+  const int bp = h->bp;
+  const long bb = (long)bp * bp;
+  const int ncb = (bp + 127) / 128;
+  const int nfb = (bp + 63) / 64;
+  const int nch = (bp + DH_CCH - 1) / DH_CCH;
+  for (int l = cnt - 1; l >= 0; --l) {
+    const int i = slot0 + l;
+    double* zi = z + (long)l * bp;
+    const bool hasL = l < cnt - 1 || has_next;
+    if (hasL || arrow) {
+      const int nch1 = hasL ? nch : 0;
+      const int nch2 = arrow ? 1 : 0;
+      hipLaunchKernelGGL(k_gemv_col_part2, dim3(ncb, nch1 + nch2), dim3(256), 0, h->stream,
+                         (const double*)(hasL ? h->lower + i * bb : h->diag), hasL ? bp : 0, (const double*)(zi + bp),
+                         (const double*)(arrow ? h->arrow + (long)i * h->ap * bp : h->diag), arrow ? 64 : 0, za,
+                         (long)bp, bp, h->part);
+      hipLaunchKernelGGL(k_fin_col, dim3(nfb), dim3(256), 0, h->stream, h->part, nch1 + nch2, bp, (const double*)zi,
+                         (const double*)nullptr, -1.0, h->v);
+    } else {
+      hipLaunchKernelGGL(k_copy, dim3((bp + 255) / 256), dim3(256), 0, h->stream, zi, h->v, bp);
+    }
+    hipLaunchKernelGGL(k_upk_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, h->diag + i * bb,
+                       h->dinv + (long)i * bp, bp, (const double*)h->v, zi);
+  }
+}
+
+extern "C" {
+
 int dh_solve(dh_handle* h, double* rhs, int sparsity) {
   if (!h || !rhs) return DH_EARG;
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
@@ -2574,58 +2725,14 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
   int rc = ensure_inverse(h);
   if (rc) return rc;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
-  const int n = h->n, b = h->b, bp = h->bp, ap = arrow ? h->ap : 0, a = arrow ? h->a : 0;
-  const long bb = (long)bp * bp;
+  const int n = h->n, b = h->b, bp = h->bp, a = arrow ? h->a : 0;
   const long N = (long)n * bp + 64;
   hipLaunchKernelGGL(k_pack, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, rhs, h->z, n, b, bp, a,
                      64);
-  const int ncb = (bp + 127) / 128;  // column blocks of the partial GEMVs
-  const int nfb = (bp + 63) / 64;    // column blocks of k_fin_col
-  const int nch = (bp + DH_CCH - 1) / DH_CCH;
   double* za = h->z + (long)n * bp;
-  // forward: y_i = inv(L_ii) (b_i - L_{i-1} y_{i-1}); arrow rows accumulate R_i y_i
-  for (int i = 0; i < n; ++i) {
-    double* zi = h->z + (long)i * bp;
-    const double* src = zi;
-    if (i > 0) {
-      hipLaunchKernelGGL(k_gemv_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, h->lower + (i - 1) * bb, (long)bp,
-                         bp, bp, zi - bp, zi, h->v, -1.0);
-      src = h->v;
-    } else {
-      hipLaunchKernelGGL(k_copy, dim3((bp + 255) / 256), dim3(256), 0, h->stream, zi, h->v, bp);
-      src = h->v;
-    }
-    hipLaunchKernelGGL(k_gemv_col_part, dim3(ncb, nch), dim3(256), 0, h->stream, h->diag + i * bb, (long)bp, bp,
-                       bp, src, h->part, 1);
-    hipLaunchKernelGGL(k_fin_col, dim3(nfb), dim3(256), 0, h->stream, h->part, nch, bp, src,
-                       h->dinv + (long)i * bp, 1.0, zi);
-    if (arrow)
-      hipLaunchKernelGGL(k_gemv_row, dim3(16), dim3(256), 0, h->stream, h->arrow + (long)i * ap * bp, (long)bp, 64,
-                         bp, zi, (const double*)nullptr, h->arrpart + i * 64, 1.0);
-  }
-  if (arrow) {
-    hipLaunchKernelGGL(k_arrow_acc, dim3(1), dim3(64), 0, h->stream, h->arrpart, n, za);
-    hipLaunchKernelGGL(k_tip_apply, dim3(1), dim3(64), 0, h->stream, h->tipinv, za, h->v, 0);
-    hipLaunchKernelGGL(k_tip_apply, dim3(1), dim3(64), 0, h->stream, h->tipinv, h->v, za, 1);
-  }
-  // backward: x_i = inv(L_ii)^T (y_i - L_i^T x_{i+1} - R_i^T x_a)
-  for (int i = n - 1; i >= 0; --i) {
-    double* zi = h->z + (long)i * bp;
-    const bool hasL = i < n - 1;
-    if (hasL || arrow) {
-      const int nch1 = hasL ? nch : 0;
-      const int nch2 = arrow ? 1 : 0;
-      hipLaunchKernelGGL(k_gemv_col_part2, dim3(ncb, nch1 + nch2), dim3(256), 0, h->stream,
-                         hasL ? h->lower + i * bb : h->diag, hasL ? bp : 0, zi + bp,
-                         arrow ? h->arrow + (long)i * ap * bp : h->diag, arrow ? 64 : 0, za, (long)bp, bp, h->part);
-      hipLaunchKernelGGL(k_fin_col, dim3(nfb), dim3(256), 0, h->stream, h->part, nch1 + nch2, bp, zi,
-                         (const double*)nullptr, -1.0, h->v);
-    } else {
-      hipLaunchKernelGGL(k_copy, dim3((bp + 255) / 256), dim3(256), 0, h->stream, zi, h->v, bp);
-    }
-    hipLaunchKernelGGL(k_upk_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, h->diag + i * bb,
-                       h->dinv + (long)i * bp, bp, h->v, zi);
-  }
+  solve_forward(h, h->z, 0, n, false, arrow, za);
+  if (arrow) solve_tip(h, za);
+  solve_backward(h, h->z, 0, n, false, arrow, za);
   const long Nl = (long)n * b + a;
   hipLaunchKernelGGL(k_unpack, dim3((unsigned)((Nl + 255) / 256)), dim3(256), 0, h->stream, h->z, rhs, n, b, bp, a);
   HIPCHK(hipGetLastError());
@@ -2725,21 +2832,22 @@ static void gemm64(dh_handle* h, const GArgs& g) {
 
 extern "C" {
 
-int dh_selected_inversion(dh_handle* h, int sparsity) {
-  if (!h) return DH_EARG;
-  if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
-  if (h->factored != sparsity) return DH_ESTATE;
-  int rc = ensure_inverse(h);
-  if (rc) return rc;
-  const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
+}  // extern "C"
+
+// X_aa = inv(L_aa)^T inv(L_aa) into the tip (the start of the selected inversion).
+static void selinv_tip(dh_handle* h) {
+  gemm64<M_COL, M_COL>(h, gargs(Opnd{h->tipinv, 64, M_COL, nullptr}, Opnd{h->tipinv, 64, M_COL, nullptr}, 64, KLO_0,
+                                KHI_K, 1.0, 0.0, h->tip, 64, 1, 1, 0));
+}
+
+// The Takahashi recursion over blocks hi down to lo of the handle's arrays, which hold the
+// factor; block hi + 1 (when hi < n - 1) already holds X_{hi+1,hi+1} and X_{a,hi+1}, and
+// lower[hi] the factor's L_{hi+1,hi}.  Overwrites the blocks with the selected inverse.
+static int selinv_range(dh_handle* h, int sparsity, bool arrow, int lo, int hi) {
   const int n = h->n, bp = h->bp, nbt = h->nbt;
   const long bb = (long)bp * bp;
   double* W = h->W;
   double* S = h->S;
-  if (arrow) {  // X_aa = inv(L_aa)^T inv(L_aa)
-    gemm64<M_COL, M_COL>(h, gargs(Opnd{h->tipinv, 64, M_COL, nullptr}, Opnd{h->tipinv, 64, M_COL, nullptr}, 64,
-                                  KLO_0, KHI_K, 1.0, 0.0, h->tip, 64, 1, 1, 0));
-  }
   const int wld = bp + 8;  // W pitch (the recursion reads W by columns, which is fast at any pitch)
   // The products of the factor alone (W, Wa, inv(L)^T inv(L)) of a chunk of blocks do not depend on
   // the recursion, so they run on a second stream one chunk ahead of it, in two alternating halves
@@ -2748,18 +2856,19 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   // pre: the factorization already formed them beside its own work, in per-block slots
   const bool pre = h->fp_ready == sparsity;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  const bool two = !pre && h->side && h->side_on && h->G >= 2 && n > h->G / 2 &&
+  const int nr = hi - lo + 1;  // blocks of the range
+  const bool two = !pre && h->side && h->side_on && h->G >= 2 && nr > h->G / 2 &&
                    hipStreamIsCapturing(h->stream, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
   const int G = two ? h->G / 2 : h->G;  // blocks per chunk
-  const int nq = (n + G - 1) / G;
+  const int nq = (nr + G - 1) / G;
   auto batch = [&](int q, SelBatch& sb) {
-    const int hi = n - 1 - q * G;
-    const int cnt = std::min(G, hi + 1);
+    const int top = hi - q * G;
+    const int cnt = std::min(G, top - lo + 1);
     const int slot0 = two ? (q & 1) * G : 0;
     memset(&sb, 0, sizeof(sb));
     sb.cnt = cnt;
     for (int g = 0; g < cnt; ++g) {
-      const int i = hi - g, sl = slot0 + g;
+      const int i = top - g, sl = slot0 + g;
       if (pre)
         sel_args(h, i, arrow, wall_slot(h, i), waall_slot(h, i), sall_slot(h, i), sb.a[g]);
       else
@@ -2783,7 +2892,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   auto recursion = [&](int q) {  // block by block
     SelBatch sb;
     batch(q, sb);
-    const int hi = n - 1 - q * G;
+    const int top = hi - q * G;
     for (int g = 0; g < sb.cnt; ++g) {
       const SelArgs& sa = sb.a[g];
       const int ntri = sa.Lb ? nbt * (nbt + 1) / 2 : 0;
@@ -2791,7 +2900,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
       const int ns = ntri + (sa.R ? nbt : 0);  // S -= W^T X_{i+1,i}; X_{a,i}
       if (ns > 0) DH_LAUNCH(h, 2, k_sel_s, dim3(ns), dim3(256), 0, sa, ntri);
       DH_LAUNCH(h, 2, k_sel_sym, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, sa,
-                h->diag + (long)(hi - g) * bb);  // X_ii, both triangles
+                h->diag + (long)(top - g) * bb);  // X_ii, both triangles
     }
   };
   if (!two) {
@@ -2818,10 +2927,25 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
     }
   }
   HIPCHK(hipGetLastError());
+  (void)n;
+  return DH_OK;
+}
+
+extern "C" {
+
+int dh_selected_inversion(dh_handle* h, int sparsity) {
+  if (!h) return DH_EARG;
+  if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
+  if (h->factored != sparsity) return DH_ESTATE;
+  int rc = ensure_inverse(h);
+  if (rc) return rc;
+  const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
+  if (arrow) selinv_tip(h);
+  rc = selinv_range(h, sparsity, arrow, 0, h->n - 1);
   h->factored = -1;
   h->inverted = 0;
   h->fp_ready = -1;
-  return DH_OK;
+  return rc;
 }
 
 int dh_marginal_variances(dh_handle* h, double* out) {
@@ -3053,6 +3177,268 @@ int dh_timing_read(dh_handle* h, double* total_ms, long long* count) {
   }
   *total_ms = t;
   *count = h->nrec;
+  return DH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// checkpointed evaluation: the factor is formed segment by segment in a ring of K + 2 block slots
+// and recomputed, segment by segment in reverse, for the backward solve and the selected inversion
+// ---------------------------------------------------------------------------
+//
+// Segment s holds time blocks [s K, min((s + 1) K, n)).  All segments but the last sit at slots
+// 1 .. K; the last one ends at slot K + 1, so that its last block is the ring's last block.  The
+// slot before a segment receives the boundary factor of the previous segment (L_{j,j-1} in its
+// sub-diagonal slot and L_{a,j-1} in its arrow slot, j the segment's first block), which is all
+// the factorization of block j needs from earlier blocks besides the tip partial; these two blocks
+// are the checkpoint kept per segment.  In the backward pass slot K + 1 carries X_{j+K,j+K} and
+// X_{a,j+K} of the segment above.  The tip partial accumulates in the ring's tip over the forward
+// pass; the recomputation writes its own copy into a scratch tip.
+
+struct dh_ckpt {
+  int n, b, a, K, S;
+  dh_handle* ring;   // K + 2 slots of block storage and all factorization scratch
+  dh_handle* vec;    // n time blocks of vector scratch (vector operations on the whole model)
+  double* ck_lower;  // [S][bp][bp] boundary factor L_{j,j-1} entering segment s (s >= 1)
+  double* ck_arrow;  // [S][64][bp] L_{a,j-1}
+  double* z;         // padded global vector, n bp + 64
+  double* ldpart;    // [n] log-determinant partials per block
+  double* tmp;       // contraction sums of one segment, (2 K + 2) * 4
+  double* tipscr;    // 64 x 64 tip partial of the recomputation (discarded)
+  long long bytes;
+};
+
+static void ckpt_seg(const dh_ckpt* c, int s, int& t0, int& cnt, int& slot0) {
+  t0 = s * c->K;
+  cnt = std::min(c->K, c->n - t0);
+  slot0 = (s < c->S - 1) ? 1 : c->K + 2 - cnt;
+}
+
+static int ckpt_copy(dh_handle* h, double* dst, const double* src, long long count) {
+  if (dst == src || count <= 0) return DH_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, (size_t)count * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  return DH_OK;
+}
+
+// Assemble time blocks t0 .. t0 + cnt - 1 of Qc (BTA, with the likelihood) or of the prior (BT)
+// into slots slot0 .., with the coupling to block t0 + cnt when it exists.
+static int ckpt_assemble(dh_ckpt* c, const dh_model* m, const double* params, bool bta, int t0, int cnt, int slot0) {
+  dh_handle* h = c->ring;
+  const long bb = (long)h->bp * h->bp;
+  const int nl = cnt - 1 + ((t0 + cnt < c->n) ? 1 : 0);
+  const long rows = (long)(cnt + nl) * h->bp;
+  DH_LAUNCH(h, 5, k_assemble_rows, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double),
+            h->diag + slot0 * bb, h->lower + slot0 * bb, cnt, nl, t0, h->b, h->bp, *m, params, bta ? 1 : 0);
+  if (bta && h->a > 0)
+    hipLaunchKernelGGL(k_assemble_arrow, dim3(cnt * h->ap), dim3(256), 0, h->stream,
+                       h->arrow + (long)slot0 * h->ap * h->bp, cnt, t0, h->b, h->a, h->bp, h->ap, *m, params, 1);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+// Factor the segment in its slots (the boundary factor already in slot0 - 1 when t0 > 0).
+static void ckpt_factor(dh_ckpt* c, bool arrow, int t0, int cnt, int slot0, double* tip) {
+  dh_handle* h = c->ring;
+  const int FS = dag_fstride(h->nbt);
+  const bool has_prev = t0 > 0, has_next = t0 + cnt < c->n;
+  hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags + (long)slot0 * FS, 0, cnt * FS);
+  if (has_prev)
+    hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, h->stream, h->dagflags + (long)(slot0 - 1) * FS, 1, FS);
+  DagArgs g = dag_args(h, arrow ? 1 : 0, slot0, cnt, has_prev, has_next, tip, t0 * h->b);
+  dag_launch(h, g);
+  for (int g0 = slot0; g0 < slot0 + cnt; g0 += h->G) inverse_chunk(h, g0, std::min(h->G, slot0 + cnt - g0), h->stream, false);
+}
+
+extern "C" {
+
+int dh_ckpt_create(int n, int b, int a, int K, void* stream, dh_ckpt** out) {
+  if (!out || n < 2 || b < 1 || a < 0 || a > 64 || K < 1) return DH_EARG;
+  K = std::min(K, n);
+  dh_ckpt* c = new dh_ckpt();
+  memset(c, 0, sizeof(*c));
+  c->n = n;
+  c->b = b;
+  c->a = a;
+  c->K = K;
+  c->S = (n + K - 1) / K;
+  int rc = create_handle(K + 2, b, a, stream, true, &c->ring);
+  if (rc == DH_OK) rc = create_handle(n, b, a, stream, false, &c->vec);
+  if (rc != DH_OK) {
+    dh_ckpt_destroy(c);
+    return rc;
+  }
+  dh_handle* h = c->ring;
+  const long long bb = (long long)h->bp * h->bp;
+  rc |= dalloc(h, &c->ck_lower, c->S * bb);
+  rc |= dalloc(h, &c->ck_arrow, a > 0 ? c->S * 64LL * h->bp : 0);
+  rc |= dalloc(h, &c->z, (long long)n * h->bp + 64);
+  rc |= dalloc(h, &c->ldpart, n);
+  rc |= dalloc(h, &c->tmp, (2LL * K + 2) * 4 + 8);
+  rc |= dalloc(h, &c->tipscr, 64 * 64);
+  if (rc != DH_OK) {
+    dh_ckpt_destroy(c);
+    return DH_EHIP;
+  }
+  *out = c;
+  return DH_OK;
+}
+
+int dh_ckpt_destroy(dh_ckpt* c) {
+  if (!c) return DH_EARG;
+  double* ps[] = {c->ck_lower, c->ck_arrow, c->z, c->ldpart, c->tmp, c->tipscr};
+  for (double* p : ps)
+    if (p) (void)hipFree(p);
+  if (c->ring) dh_destroy(c->ring);
+  if (c->vec) dh_destroy(c->vec);
+  delete c;
+  return DH_OK;
+}
+
+long long dh_ckpt_memory_bytes(const dh_ckpt* c) {
+  return c ? c->ring->bytes + c->vec->bytes : -1;
+}
+
+dh_handle* dh_ckpt_vectors(dh_ckpt* c) { return c ? c->vec : nullptr; }
+
+int dh_ckpt_set_stream(dh_ckpt* c, void* stream) {
+  if (!c) return DH_EARG;
+  c->ring->stream = (hipStream_t)stream;
+  c->vec->stream = (hipStream_t)stream;
+  return DH_OK;
+}
+
+int dh_ckpt_get_info(dh_ckpt* c, int* info) { return c ? dh_get_info(c->ring, info) : DH_EARG; }
+
+int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value) {
+  if (!c || key == 4) return DH_EARG;  // nothing runs beside the segment factorizations
+  return dh_set_tuning(c->ring, key, value);
+}
+
+int dh_ckpt_forward(dh_ckpt* c, const dh_model* m, const double* params, int sparsity, const double* rhs,
+                    double* out_dev) {
+  if (!c || !m || !params || !out_dev) return DH_EARG;
+  if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
+  dh_handle* h = c->ring;
+  if (m->nt != c->n || m->ns != c->b || m->nb < 0 || m->nb > 64) return DH_EARG;
+  const bool bta = sparsity == DH_SPARSITY_BTA;
+  if (bta && m->nb != c->a) return DH_EARG;
+  if (rhs && !bta) return DH_EARG;
+  const bool arrow = bta && c->a > 0;
+  const int bp = h->bp;
+  const long bb = (long)bp * bp;
+  const long A64 = 64L * bp;
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->info, INT_MAX);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->err, h->force_giveup ? 4 : 0);
+  if (arrow) hipLaunchKernelGGL(k_assemble_tip, dim3(1), dim3(256), 0, h->stream, h->tip, h->a, h->ap, *m, params, 1);
+  double* za = c->z + (long)c->n * bp;
+  if (rhs) {
+    const long N = (long)c->n * bp + 64;
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, rhs, c->z, c->n, c->b,
+                       bp, c->a, 64);
+  }
+  for (int s = 0; s < c->S; ++s) {
+    int t0, cnt, slot0;
+    ckpt_seg(c, s, t0, cnt, slot0);
+    if (s > 0) {  // the boundary factor left at slot K by the previous segment goes before this one
+      int rc = ckpt_copy(h, h->lower + (slot0 - 1) * bb, h->lower + c->K * bb, bb);
+      if (!rc && arrow) rc = ckpt_copy(h, h->arrow + (slot0 - 1) * A64, h->arrow + c->K * A64, A64);
+      if (!rc && bta) rc = ckpt_copy(h, c->ck_lower + s * bb, h->lower + c->K * bb, bb);
+      if (!rc && arrow) rc = ckpt_copy(h, c->ck_arrow + s * A64, h->arrow + c->K * A64, A64);
+      if (rc) return rc;
+    }
+    int rc = ckpt_assemble(c, m, params, bta, t0, cnt, slot0);
+    if (rc) return rc;
+    ckpt_factor(c, arrow, t0, cnt, slot0, arrow ? h->tip : c->tipscr);
+    hipLaunchKernelGGL(k_logdet_part, dim3(cnt), dim3(256), 0, h->stream, (const double*)(h->diag + slot0 * bb), bb,
+                       bp, h->b, c->ldpart + t0);
+    if (rhs) solve_forward(h, c->z + (long)t0 * bp, slot0, cnt, s > 0, arrow, za);
+  }
+  if (arrow) {
+    hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, h->stream, h->tip, (long)h->ap, h->tipdinv, h->tipinv, h->info,
+                       c->n * h->b);
+    if (rhs) solve_tip(h, za);
+  }
+  hipLaunchKernelGGL(k_logdet_fin, dim3(1), dim3(64), 0, h->stream, (const double*)c->ldpart, c->n,
+                     arrow ? (const double*)h->tip : nullptr, c->a, h->ap, (const int*)h->err, out_dev);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_ckpt_backward(dh_ckpt* c, const dh_model* m, const double* params, double* rhs, int selinv, double* cx_dev,
+                     double* ata_dev, double* var_dev) {
+  if (!c || !m || !params || !rhs) return DH_EARG;
+  if (m->nt != c->n || m->ns != c->b || m->nb != c->a) return DH_EARG;
+  if (selinv && (!cx_dev || !ata_dev)) return DH_EARG;
+  dh_handle* h = c->ring;
+  const bool arrow = c->a > 0;
+  const int bp = h->bp, K = c->K;
+  const long bb = (long)bp * bp;
+  const long A64 = 64L * bp;
+  double* za = c->z + (long)c->n * bp;
+  if (selinv) {
+    selinv_tip(h);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h->stream, (int*)ata_dev, 0, 2);  // *ata_dev = 0.0
+    if (var_dev && arrow)  // the tip's variances
+      hipLaunchKernelGGL(k_diag_extract, dim3(1), dim3(256), 0, h->stream, (const double*)h->diag,
+                         (const double*)h->tip, 0, c->b, bp, c->a, var_dev + (long)c->n * c->b);
+  }
+  int prev_slot0 = -1;
+  for (int s = c->S - 1; s >= 0; --s) {
+    int t0, cnt, slot0;
+    ckpt_seg(c, s, t0, cnt, slot0);
+    const bool has_next = s < c->S - 1;
+    int rc = DH_OK;
+    if (has_next && selinv) {  // X of the first block of the segment above goes to slot K + 1
+      rc = ckpt_copy(h, h->diag + (K + 1) * bb, h->diag + prev_slot0 * bb, bb);
+      if (!rc && arrow) rc = ckpt_copy(h, h->arrow + (K + 1) * A64, h->arrow + prev_slot0 * A64, A64);
+    }
+    // the last segment's factor is still in its slots from the forward pass; the others are
+    // recomputed from their boundary factor
+    if (!rc && has_next && s > 0) {
+      rc = ckpt_copy(h, h->lower + (slot0 - 1) * bb, c->ck_lower + s * bb, bb);
+      if (!rc && arrow) rc = ckpt_copy(h, h->arrow + (slot0 - 1) * A64, c->ck_arrow + s * A64, A64);
+    }
+    if (!rc && has_next) {
+      rc = ckpt_assemble(c, m, params, true, t0, cnt, slot0);
+      if (!rc) ckpt_factor(c, arrow, t0, cnt, slot0, c->tipscr);
+    }
+    if (rc) return rc;
+    solve_backward(h, c->z + (long)t0 * bp, slot0, cnt, has_next, arrow, za);
+    if (selinv) {
+      rc = selinv_range(h, DH_SPARSITY_BTA, arrow, slot0, slot0 + cnt - 1);
+      if (rc) return rc;
+      const int nl = cnt - 1 + (has_next ? 1 : 0);
+      const int nq = cnt + nl;
+      const int nrb = (m->ns + 255) / 256;
+      hipLaunchKernelGGL(k_contract_fem, dim3(nrb, nq), dim3(256), 0, h->stream, (const double*)(h->diag + slot0 * bb),
+                         (const double*)(h->lower + slot0 * bb), cnt, bp, *m, 0, 0, h->part);
+      hipLaunchKernelGGL(k_sum_seg, dim3((nq * 4 + 255) / 256), dim3(256), 0, h->stream, (const double*)h->part, nq,
+                         nrb, 4, c->tmp);
+      rc = ckpt_copy(h, cx_dev + (long)t0 * 4, c->tmp, (long long)cnt * 4);
+      if (!rc && nl > 0) rc = ckpt_copy(h, cx_dev + ((long)c->n + t0) * 4, c->tmp + cnt * 4, (long long)nl * 4);
+      if (rc) return rc;
+      const long nrow = (long)(nl + 1) * m->ns;
+      const unsigned ga = (unsigned)((nrow + 255) / 256);
+      hipLaunchKernelGGL(k_contract_ata_seg, dim3(ga), dim3(256), 0, h->stream, (const double*)(h->diag + slot0 * bb),
+                         (const double*)(h->lower + slot0 * bb),
+                         (const double*)(arrow ? h->arrow + (long)slot0 * A64 : nullptr),
+                         (const double*)((arrow && !has_next) ? h->tip : nullptr), t0, cnt, nl, bp, arrow ? c->a : 0,
+                         h->ap, *m, h->part);
+      hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, h->stream, (const double*)h->part, (int)ga, 1, 1,
+                         c->tmp + 8 * K + 6);
+      hipLaunchKernelGGL(k_add_to, dim3(1), dim3(1), 0, h->stream, (const double*)(c->tmp + 8 * K + 6), ata_dev);
+      if (var_dev)
+        hipLaunchKernelGGL(k_diag_extract, dim3((unsigned)(((long)cnt * c->b + 255) / 256)), dim3(256), 0, h->stream,
+                           (const double*)(h->diag + slot0 * bb), (const double*)h->tip, cnt, c->b, bp, 0,
+                           var_dev + (long)t0 * c->b);
+    }
+    prev_slot0 = slot0;
+  }
+  const long Nl = (long)c->n * c->b + c->a;
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)((Nl + 255) / 256)), dim3(256), 0, h->stream, (const double*)c->z, rhs,
+                     c->n, c->b, bp, c->a);
+  HIPCHK(hipGetLastError());
   return DH_OK;
 }
 

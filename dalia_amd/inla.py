@@ -285,13 +285,21 @@ class STInla:
         _lib.check(L.dh_selected_inversion(self.h, 1), "dh_selected_inversion")
         _lib.check(L.dh_contract_fem(self.h, self.dm.ref(), self.cx.data_ptr()), "dh_contract_fem")
         _lib.check(L.dh_contract_ata(self.h, self.dm.ref(), self.scal[5:6].data_ptr()), "dh_contract_ata")
+        ld_p, dld_p = st["ld_p"], st["dld_p"]
+        if ld_p is None:
+            ld_p, dld_p = self._prior_by_selected_inversion(theta)
+        return self._gradient(st, ld_p, dld_p)
+
+    def _gradient(self, st, ld_p, dld_p):
+        """f and the adjoint gradient from the state of `forward` and the contractions in
+        self.cx (selected inverse against the FEM blocks), self.cmu (mu against them) and
+        self.scal[5] (selected inverse against A^T A)."""
+        theta = st["theta"]
+        et = self.dm.tau_scale(theta)
         nq = 2 * self.dm.nt - 1
         cx = self.cx.cpu().numpy().reshape(nq, 4)
         cmu = self.cmu.cpu().numpy().reshape(nq, 4)
         tr_ata = float(self.scal[5].item())
-        ld_p, dld_p = st["ld_p"], st["dld_p"]
-        if ld_p is None:
-            ld_p, dld_p = self._prior_by_selected_inversion(theta)
         dc = hyper.kron_coefficients_grad(theta, self.manifold)
         nt = self.dm.nt
         mult = np.concatenate([np.ones(nt), 2.0 * np.ones(nt - 1)])[:, None]

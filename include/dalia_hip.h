@@ -217,6 +217,38 @@ int dh_marginal_variances(dh_handle* h, double* out);
 int dh_obs_variances(dh_handle* h, long long n_obs, const long long* rowptr, const int* col, const double* val,
                      const double* xcov, double* out);
 
+/* Checkpointed evaluation (for matrices whose full factor does not fit the memory budget).  No
+ * counterpart in the reference, which keeps the whole factor in the solver's buffers
+ * (src/dalia/solvers/structured_solver.py:38-60) and bounds memory only by splitting the time
+ * axis over processes (distributed_structured_solver.py:62-64).  A context holds a ring of K + 2
+ * block slots, the boundary factor of every segment of K time blocks (L_{j,j-1}, L_{a,j-1}: the
+ * checkpoints) and an n-block handle for vector operations (dh_ckpt_vectors: use it for
+ * dh_qc_matvec, dh_dot, dh_contract_fem_vec, dh_prior_eig on the whole model).  Memory: about
+ * (3 K + 22 + n / K) bp^2 doubles instead of 3 n bp^2.
+ *   dh_ckpt_forward: factors Qc (BTA, assembled from the model with the likelihood) or the prior
+ *     (BT) segment by segment; *out_dev = logdet (NaN on a failed factorization, as dh_logdet);
+ *     BTA with rhs (device, n b + a, logical layout): also the forward sweep of the solve and the
+ *     tip solve (rhs is read only).
+ *   dh_ckpt_backward: directly after dh_ckpt_forward(BTA, rhs) with the same params (no other
+ *     dh_ckpt_forward in between: the last segment's factor is taken from it): recomputes the
+ *     segments in reverse from their checkpoints, finishes the solve (rhs = Qc^-1 rhs) and, when
+ *     selinv != 0, forms the selected inverse segment by segment and contracts it:
+ *     cx_dev[(2n-1)*4] as dh_contract_fem, *ata_dev = <X, A^T A> as dh_contract_ata, and
+ *     var_dev (n b + a, may be null) = diag(Qc^-1) as dh_marginal_variances. */
+typedef struct dh_ckpt dh_ckpt;
+int dh_ckpt_create(int n, int b, int a, int K, void* hip_stream, dh_ckpt** out);
+int dh_ckpt_destroy(dh_ckpt* c);
+long long dh_ckpt_memory_bytes(const dh_ckpt* c);
+dh_handle* dh_ckpt_vectors(dh_ckpt* c);
+int dh_ckpt_set_stream(dh_ckpt* c, void* hip_stream);
+int dh_ckpt_get_info(dh_ckpt* c, int* info);
+/* dh_set_tuning on the context's segment factorizations (keys 0, 2, 3, 5; key 4 is refused). */
+int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value);
+int dh_ckpt_forward(dh_ckpt* c, const dh_model* m, const double* params, int sparsity, const double* rhs,
+                    double* out_dev);
+int dh_ckpt_backward(dh_ckpt* c, const dh_model* m, const double* params, double* rhs, int selinv, double* cx_dev,
+                     double* ata_dev, double* var_dev);
+
 /* Task table of one block of the persistent pobtaf/pobtf launch for nbt 64-column tiles per block
  * (last != 0: the last block, without sub-diagonal tiles; arrow != 0: with the arrowhead), as the
  * launch decodes it: entry = kind << 24 | r << 12 | c, kinds as in the source's DK_* list.  Writes

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_symbols():
     txt = open(os.path.join(ROOT, "include", "dalia_hip.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return set(re.findall(r"^\s*(?:int|long long)\s+(dh_\w+)\s*\(", txt, flags=re.M))
+    return set(re.findall(r"^\s*(?:int|long long|dh_handle\s*\*)\s*(dh_\w+)\s*\(", txt, flags=re.M))
 
 
 def test_header_declares_expected_surface():
