@@ -369,6 +369,79 @@ def bench_coreg(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_ckpt(args):
+    """The spatio-temporal workload with the factor checkpointed every K time blocks
+    (`dalia_amd.ckpt.STInlaCkpt`): adjoint gradient and central differences, device memory."""
+    import torch
+
+    from dalia_amd import _lib, synth
+    from dalia_amd.ckpt import STInlaCkpt, full_handle_bytes
+
+    wl = synth.WORKLOADS[args.workload]
+    m, nt, nb, seed = wl
+    ns = m * m
+    K = args.checkpoint
+    t0 = time.perf_counter()
+    data = synth.make_st_model(m, nt, nb, seed)
+    eng = STInlaCkpt(data, K=K)
+    torch.cuda.synchronize()
+    mem = eng.memory_bytes()
+    full = full_handle_bytes(nt, ns, nb)
+    log(f"set-up {time.perf_counter() - t0:.1f} s; checkpointed every {K} blocks: {mem / 1e9:.1f} GB of device "
+        f"memory for the solver against {full / 1e9:.1f} GB with the whole factor stored")
+    theta = data.theta.copy()
+    lib = eng.lib
+    for _ in range(args.warmup):
+        eng.value_and_grad(theta)
+    torch.cuda.synchronize()
+    _lib.check(lib.dh_ckpt_timing_enable(eng.ck, _lib.TIMED_KERNELS["k_pobtaf_dag"]), "dh_ckpt_timing_enable")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f, g = eng.value_and_grad(theta)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / args.steps
+    tot, cnt = ctypes.c_double(), ctypes.c_longlong()
+    _lib.check(lib.dh_ckpt_timing_read(eng.ck, ctypes.byref(tot), ctypes.byref(cnt)), "dh_ckpt_timing_read")
+    _lib.check(lib.dh_ckpt_timing_enable(eng.ck, -1), "dh_ckpt_timing_enable")
+    log(f"adjoint gradient (checkpointed): {t_adj * 1e3:.1f} ms/eval, f={f:.12g}")
+    fd_info = {}
+    if args.fd > 0:
+        t0 = time.perf_counter()
+        f_fd, g_fd = eng.fd_gradient(theta, h=1e-4)
+        torch.cuda.synchronize()
+        t_fd = time.perf_counter() - t0
+        relerr = float(np.linalg.norm(g - g_fd) / np.linalg.norm(g_fd))
+        fd_info = {"fd_grad_evals_per_s": 1.0 / t_fd, "fd_ms_per_grad": t_fd * 1e3,
+                   "adjoint_vs_fd_speedup": t_fd / t_adj, "adjoint_vs_fd_relerr": relerr}
+        log(f"central differences (checkpointed f): {t_fd * 1e3:.0f} ms, speed-up {t_fd / t_adj:.2f}x, "
+            f"rel. difference {relerr:.2e}")
+    # per evaluation: the forward pass factors every segment, the backward pass all but the last
+    S = (nt + K - 1) // K
+    per_block = ns ** 3 / 3 + 2 * ns ** 3 + 3 * nb * ns ** 2 + nb * nb * ns
+    flops_eval = (f_pobtaf(nt, ns, nb) - nb ** 3 / 3) + (S - 1) * K * per_block
+    t_launch = tot.value * 1e-3 / max(cnt.value, 1)
+    achieved = flops_eval * args.steps / (tot.value * 1e-3) / 1e12
+    roofline = {"kernel": "k_pobtaf_dag", "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "launches": int(cnt.value), "avg_launch_us": t_launch * 1e6,
+                "flops_per_launch": flops_eval * args.steps / max(cnt.value, 1)}
+    cpu = None
+    if not args.no_cpu:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        cpu = cpu_baseline(wl, nt, threads)
+    out = {
+        "metric": metric_name(ns, nt, nb) + f", factor checkpointed every {K} time blocks",
+        "value": 1.0 / t_adj, "unit": "grad_evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_adj * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "ns": ns, "nt": nt, "nb": nb, "seed": seed, "theta_dim": 4,
+                   "checkpoint_every": K, "segments": S},
+        "solver_memory_gb": mem / 1e9, "full_factor_memory_gb": full / 1e9,
+        **fd_info, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -380,6 +453,8 @@ def main():
     ap.add_argument("--overlap", type=int, default=0, choices=(0, 1, 2),
                     help="work run beside each factorization on a second stream (STInla overlap option)")
     ap.add_argument("--roofline-kernel", default="k_syrk")
+    ap.add_argument("--checkpoint", type=int, default=0,
+                    help="factor checkpointed every K time blocks and recomputed in the backward pass (0: stored)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device-to-device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -405,6 +480,8 @@ def main():
         return bench_coreg_batch(args)
     if args.workload in _synth.COREG_WORKLOADS:
         return bench_coreg(args)
+    if args.checkpoint > 0:
+        return bench_ckpt(args)
 
     from dalia_amd import _lib, synth
     from dalia_amd.inla import STInla

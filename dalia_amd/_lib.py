@@ -88,6 +88,8 @@ SIGNATURES = {
     "dh_ckpt_set_stream": (_I, [_P, _P]),
     "dh_ckpt_get_info": (_I, [_P, ctypes.POINTER(_I)]),
     "dh_ckpt_set_tuning": (_I, [_P, _I, _I]),
+    "dh_ckpt_timing_enable": (_I, [_P, _I]),
+    "dh_ckpt_timing_read": (_I, [_P, c_double_p, ctypes.POINTER(_LL)]),
     "dh_ckpt_forward": (_I, [_P, ctypes.POINTER(DhModel), _P, _I, _P, _P]),
     "dh_ckpt_backward": (_I, [_P, ctypes.POINTER(DhModel), _P, _P, _I, _P, _P, _P]),
     "dh_timing_enable": (_I, [_P, _I]),

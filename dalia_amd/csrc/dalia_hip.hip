@@ -3310,6 +3310,12 @@ int dh_ckpt_set_stream(dh_ckpt* c, void* stream) {
 
 int dh_ckpt_get_info(dh_ckpt* c, int* info) { return c ? dh_get_info(c->ring, info) : DH_EARG; }
 
+int dh_ckpt_timing_enable(dh_ckpt* c, int kind) { return c ? dh_timing_enable(c->ring, kind) : DH_EARG; }
+
+int dh_ckpt_timing_read(dh_ckpt* c, double* total_ms, long long* count) {
+  return c ? dh_timing_read(c->ring, total_ms, count) : DH_EARG;
+}
+
 int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value) {
   if (!c || key == 4) return DH_EARG;  // nothing runs beside the segment factorizations
   return dh_set_tuning(c->ring, key, value);

@@ -244,6 +244,9 @@ int dh_ckpt_set_stream(dh_ckpt* c, void* hip_stream);
 int dh_ckpt_get_info(dh_ckpt* c, int* info);
 /* dh_set_tuning on the context's segment factorizations (keys 0, 2, 3, 5; key 4 is refused). */
 int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value);
+/* dh_timing_enable / dh_timing_read on the context's segment launches. */
+int dh_ckpt_timing_enable(dh_ckpt* c, int kind);
+int dh_ckpt_timing_read(dh_ckpt* c, double* total_ms, long long* count);
 int dh_ckpt_forward(dh_ckpt* c, const dh_model* m, const double* params, int sparsity, const double* rhs,
                     double* out_dev);
 int dh_ckpt_backward(dh_ckpt* c, const dh_model* m, const double* params, double* rhs, int selinv, double* cx_dev,
