@@ -125,7 +125,7 @@ def _median_time(fn, reps: int) -> float:
     return float(np.median(ts))
 
 
-def cpu_baseline(workload_args, n_full: int, threads: int, reps: int = 5):
+def cpu_baseline(workload_args, n_full: int, threads: int, reps: int = 5, nt_s: int = 4):
     """The oracle (numpy/scipy restatement of the reference's CPU path) on the host cores.
 
     Measured, each a median of `reps` runs after one warm-up:
@@ -144,7 +144,6 @@ def cpu_baseline(workload_args, n_full: int, threads: int, reps: int = 5):
     from oracle import inla as oinla
 
     m, nt, nb, seed = workload_args
-    nt_s = 4
     d = synth.make_st_model(m, nt_s, nb, seed)
     d0 = synth.make_st_model(*synth.WORKLOADS["st_ns400_nt12_nb4"])
     Qc = oinla.qcond_sparse(d, d.theta)
@@ -158,12 +157,16 @@ def cpu_baseline(workload_args, n_full: int, threads: int, reps: int = 5):
     with threadpool_limits(limits=threads):
         blas = sorted({f"{i.get('internal_api')} {i.get('version')}" for i in threadpool_info()})
         t_f = _median_time(lambda: oinla.f_theta(d, d.theta), reps)
+        log(f"CPU baseline: f on the sample {t_f:.2f} s")
         factor()
         t_pobtaf = float(np.median([factor() for _ in range(reps)]))
+        log(f"CPU baseline: pobtaf on the sample {t_pobtaf:.2f} s")
         t_c0 = _median_time(lambda: (oinla.f_theta(d0, d0.theta), oinla.selected_inverse_blocks(d0, d0.theta)),
                             reps)
+        log(f"CPU baseline: ns=400 nt=12 f + selected inversion {t_c0:.2f} s")
     with threadpool_limits(limits=1):
         t_f1 = _median_time(lambda: oinla.f_theta(d, d.theta), 1)
+        log(f"CPU baseline: f on the sample with one thread {t_f1:.2f} s")
     scale = n_full / nt_s
     t_full = t_f * scale
     f_pob = f_pobtaf(nt_s, m * m, nb)
@@ -428,7 +431,8 @@ def bench_ckpt(args):
     cpu = None
     if not args.no_cpu:
         threads = min(len(os.sched_getaffinity(0)), 16)
-        cpu = cpu_baseline(wl, nt, threads)
+        log(f"CPU baseline on {threads} threads (bounded sample: nt=2)")
+        cpu = cpu_baseline(wl, nt, threads, reps=3, nt_s=2)
     out = {
         "metric": metric_name(ns, nt, nb) + f", factor checkpointed every {K} time blocks",
         "value": 1.0 / t_adj, "unit": "grad_evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
