@@ -79,6 +79,7 @@ SIGNATURES = {
     "dh_contract_ata_v": (_I, [_P, ctypes.POINTER(DhModel), _I, _I, _P]),
     "dh_ata_quad_v": (_I, [_P, ctypes.POINTER(DhModel), _P, _I, _I, _I, _P]),
     "dh_prior_eig": (_I, [_P, ctypes.POINTER(DhModel), _P, c_double_p, _P]),
+    "dh_prior_eig_dev": (_I, [_P, ctypes.POINTER(DhModel), _P, _P, _P]),
     "dh_set_tuning": (_I, [_P, _I, _I]),
     "dh_dag_table": (_I, [_I, _I, _I, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "dh_ckpt_create": (_I, [_I, _I, _I, _I, _P, ctypes.POINTER(_P)]),

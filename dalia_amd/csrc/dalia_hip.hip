@@ -1948,8 +1948,15 @@ __global__ __launch_bounds__(256) void k_ata_quad(const double* x, const double*
 // T_j = sum_k p_k M_k (tridiagonal), p = (e^{2gst} lt^3, e^{2gst+gt} lt^2,
 // e^{2gst+2gt} lt), lt = e^{2gs} + lam_j.  Outputs per workgroup partial sums
 // of (logdet T_j, d/dgs, d/dgt, d/dgst).
+// gdev: when not null, (gamma_s, gamma_t, gamma_st) are read from device memory instead (graph replay).
 __global__ __launch_bounds__(256) void k_prior_eig(dh_model m, const double* lam, double gs, double gt,
-                                                   double gst, double* lbuf, double* dbuf, double* part) {
+                                                   double gst, const double* gdev, double* lbuf, double* dbuf,
+                                                   double* part) {
+  if (gdev) {
+    gs = gdev[0];
+    gt = gdev[1];
+    gst = gdev[2];
+  }
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int nt = m.nt;
   double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
@@ -3109,7 +3116,19 @@ int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const doubl
   const unsigned g = (unsigned)((m->ns + 255) / 256);
   double* lbuf = h->eigbuf;
   double* dbuf = h->eigbuf + (long)h->n * h->bp;
-  hipLaunchKernelGGL(k_prior_eig, dim3(g), dim3(256), 0, h->stream, *m, lam, gam[0], gam[1], gam[2], lbuf, dbuf,
+  hipLaunchKernelGGL(k_prior_eig, dim3(g), dim3(256), 0, h->stream, *m, lam, gam[0], gam[1], gam[2],
+                     (const double*)nullptr, lbuf, dbuf, h->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, h->stream, h->part, (int)g, 4, 4, out_dev);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_prior_eig_dev(dh_handle* h, const dh_model* m, const double* lam, const double* gam_dev, double* out_dev) {
+  if (!h || !lam || !gam_dev || !out_dev || !model_ok_v(h, m, 0, 0)) return DH_EARG;
+  const unsigned g = (unsigned)((m->ns + 255) / 256);
+  double* lbuf = h->eigbuf;
+  double* dbuf = h->eigbuf + (long)h->n * h->bp;
+  hipLaunchKernelGGL(k_prior_eig, dim3(g), dim3(256), 0, h->stream, *m, lam, 0.0, 0.0, 0.0, gam_dev, lbuf, dbuf,
                      h->part);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, h->stream, h->part, (int)g, 4, 4, out_dev);
   HIPCHK(hipGetLastError());

@@ -69,9 +69,13 @@ class STInla:
         # theta-dependent kernel parameters: 12 Kronecker coefficients + exp(theta_obs)
         self.params = torch.zeros(13, dtype=torch.float64, device="cuda")
         self.params_h = torch.zeros(13, dtype=torch.float64).pin_memory()
+        # (gamma_s, gamma_t, gamma_st) for the closed-form prior term of the captured gradient
+        self.gam = torch.zeros(3, dtype=torch.float64, device="cuda")
+        self.gam_h = torch.zeros(3, dtype=torch.float64).pin_memory()
         self.ones = torch.ones(1, dtype=torch.float64, device="cuda")
         self._state = None
         self._graph = None
+        self._ggraph = None
         self.n_f_evals = 0
         self._theta_last = None  # last theta evaluated (the reference keeps it in model.theta)
         self._ov_max = int(overlap)
@@ -337,6 +341,74 @@ class STInla:
     def value_and_grad(self, theta):
         self.forward(theta)
         return self.backward()
+
+    # ------------------------------------------------------------------
+    # the whole adjoint evaluation as one HIP graph
+    def _enqueue_grad(self):
+        """Device work of one adjoint evaluation with the parameters already on the device: the
+        forward half, the closed-form prior term, mu^T G_m mu, the selected inversion and its
+        contractions (no host synchronisation in between)."""
+        L, m = self.lib, self.dm.ref()
+        self._conditional(None, grad=True)
+        _lib.check(L.dh_prior_eig_dev(self.h, m, self.dm.lam.data_ptr(), self.gam.data_ptr(), self.eig.data_ptr()),
+                   "dh_prior_eig_dev")
+        _lib.check(L.dh_contract_fem_vec(self.h, m, self.mu.data_ptr(), self.cmu.data_ptr()), "dh_contract_fem_vec")
+        _lib.check(L.dh_selected_inversion(self.h, 1), "dh_selected_inversion")
+        _lib.check(L.dh_contract_fem(self.h, m, self.cx.data_ptr()), "dh_contract_fem")
+        _lib.check(L.dh_contract_ata(self.h, m, self.scal[5:6].data_ptr()), "dh_contract_ata")
+
+    def _stage_grad_params(self, theta):
+        theta = np.asarray(theta, dtype=np.float64)
+        self._theta_last = theta.copy()
+        self.params_h.copy_(torch.from_numpy(self.param_row(theta)))
+        gam = hyper.interpretable2compute(float(theta[0]), float(theta[1]), float(theta[2]), self.manifold)
+        self.gam_h.copy_(torch.tensor(gam, dtype=torch.float64))
+
+    def capture_grad(self):
+        """Capture one adjoint evaluation (parameter upload included) in a HIP graph.  Needs the
+        closed-form prior term; the overlap option does not apply under capture."""
+        if not self.dm.closed_form:
+            raise ValueError("the captured gradient needs the closed-form prior term")
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):  # warm-up on the side stream before capture
+            _lib.check(self.lib.dh_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)), "dh_set_stream")
+            self._stage_grad_params(self.data.theta)
+            self.params.copy_(self.params_h, non_blocking=True)
+            self.gam.copy_(self.gam_h, non_blocking=True)
+            self._enqueue_grad()
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            cs = torch.cuda.current_stream()
+            _lib.check(self.lib.dh_set_stream(self.h, ctypes.c_void_p(cs.cuda_stream)), "dh_set_stream")
+            self.params.copy_(self.params_h, non_blocking=True)
+            self.gam.copy_(self.gam_h, non_blocking=True)
+            self._enqueue_grad()
+        _lib.check(self.lib.dh_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                   "dh_set_stream")
+        self._ggraph = g
+        return g
+
+    def value_and_grad_graph(self, theta):
+        """`value_and_grad` by replaying the captured adjoint evaluation (captured on first use)."""
+        if self._ggraph is None:
+            self.capture_grad()
+        theta = np.asarray(theta, dtype=np.float64)
+        self._stage_grad_params(theta)
+        self._ggraph.replay()
+        s = self.scal[:4].cpu().numpy()
+        e = self.eig.cpu().numpy()
+        ld_c, quad_p, quad_l, bmu = s
+        if np.isnan(ld_c):
+            _lib.raise_nan_logdet(self.lib, self.h, ld_c)
+        ld_p = self.dm.nt * self.dm.logdet_c0 + e[0]
+        dld_p = hyper.dgamma(theta, self.manifold).T @ e[1:4]
+        lp, glp, lik = self._scalars(theta)
+        st = dict(theta=theta.copy(), ld_c=ld_c, quad_p=quad_p, quad_l=quad_l, bmu=bmu, ld_p=ld_p, dld_p=dld_p,
+                  lp=lp, glp=glp, lik=lik)
+        self._state = None
+        return self._gradient(st, ld_p, dld_p)
 
     # ------------------------------------------------------------------
     # marginal variances (the selected inverse of Qc at theta)

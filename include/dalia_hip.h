@@ -202,6 +202,10 @@ int dh_ata_quad_v(dh_handle* h, const dh_model* m, const double* x, int stride, 
  * out_dev[4] = (sum_j logdet T_j, d/dgamma_s, d/dgamma_t, d/dgamma_st). */
 int dh_prior_eig(dh_handle* h, const dh_model* m, const double* lam, const double* gam,
                  double* out_dev);
+/* The same with (gamma_s, gamma_t, gamma_st) in device memory, read when the kernel runs (so that a
+ * captured graph can be replayed for a new theta).  No counterpart in the reference. */
+int dh_prior_eig_dev(dh_handle* h, const dh_model* m, const double* lam, const double* gam_dev,
+                     double* out_dev);
 
 /* Marginal variances of the latent field, after dh_selected_inversion (BTA):
  * out[i b + s] = X_ii[s][s], out[n b + j] = X_aa[j][j]  (length n b + a, device).

@@ -131,3 +131,24 @@ def test_batched_hessian_diagonals_and_covariance():
     C = compute_covariance_hp(engines[0], du.theta)
     Cr = np.linalg.inv(oinla.fd_hessian(du, du.theta, 5e-3, offdiag_points=4))
     assert np.abs(C - Cr).max() <= 1e-5 * np.abs(Cr).max()
+
+
+def test_captured_adjoint_gradient_every_replay():
+    """The whole adjoint evaluation captured once and replayed for new theta, every replay against
+    the eager evaluation (eager work between the replays)."""
+    from dalia_amd.inla import STInla
+    d = synth.make_st_model(7, 6, 3, 4)
+    print(f"case captured gradient ns={d.ns} nt={d.nt}", flush=True)
+    eng = STInla(d)
+    ref = [eng.value_and_grad(t) for t in THETAS]
+    eng.capture_grad()
+    for rep in range(3):
+        for t, (fr, gr) in zip(THETAS, ref):
+            f, g = eng.value_and_grad_graph(t)
+            assert abs(f - fr) <= 1e-12 * abs(fr)
+            assert rel(g, gr) < 1e-10
+            eng.f(THETAS[(rep + 1) % 3])  # eager work between replays
+            torch.cuda.synchronize()
+    f_or, g_or = oinla.adjoint_gradient(d, THETAS[2])
+    f, g = eng.value_and_grad_graph(THETAS[2])
+    assert abs(f - f_or) <= 1e-10 * abs(f_or) and rel(g, g_or) < 1e-8
