@@ -246,8 +246,9 @@ def bench_coreg_batch(args):
     stencils = [hessian_diag_stencil(t, h, 3) for t in thetas]
     pts = [p for st in stencils for p in st]
     E = len(pts)
-    diags, bf = batched_hessian_diags(engines, thetas, h=h, points=3, tune=True)  # capture + first replay
-    log(f"autotuned per instance: {[c['alg'] for c in bf.choices]}; {E} evaluations in one graph")
+    diags, bf = batched_hessian_diags(engines, thetas, h=h, points=3, tune=True, fused=args.fused)  # + 1st replay
+    log(f"autotuned per instance: {[(c['alg'], c.get('wgs', 0)) for c in bf.choices]} (schedule, workgroups); "
+        f"{E} evaluations in one graph")
     for _ in range(args.warmup):
         fv = bf(pts)
     torch.cuda.synchronize()
@@ -295,7 +296,9 @@ def bench_coreg_batch(args):
                                                              "evaluations": len(st)}
                                                             for d, st in zip(datas, stencils)],
                    "evaluations": E, "stencil": "3-point diagonal, h=5e-3", "theta_dim": 9,
-                   "pobtaf_schedule_per_instance": [c["alg"] for c in bf.choices]},
+                   "fused_rounds": bool(args.fused),
+                   "pobtaf_schedule_per_instance": [c["alg"] for c in bf.choices],
+                   "persistent_workgroups_per_instance": [c.get("wgs", 0) for c in bf.choices]},
         "tflop_per_step": flops / 1e12, "tflops": flops / t_batch / 1e12,
         "alone_ms": alone, "sum_alone_ms": sum(alone),
         "pobtaf_per_instance": [{"ms": m, "tflops": tf} for m, tf in pob],
@@ -457,6 +460,8 @@ def main():
     ap.add_argument("--overlap", type=int, default=0, choices=(0, 1, 2),
                     help="work run beside each factorization on a second stream (STInla overlap option)")
     ap.add_argument("--roofline-kernel", default="k_syrk")
+    ap.add_argument("--fused", action="store_true",
+                    help="batches: the factorizations of each round as one launch over all instances")
     ap.add_argument("--graph", action="store_true",
                     help="time the adjoint evaluation as one captured HIP graph replayed per step")
     ap.add_argument("--checkpoint", type=int, default=0,
@@ -515,10 +520,9 @@ def main():
     kind = _lib.TIMED_KERNELS[kname]
 
     if args.graph:
-        # the whole adjoint evaluation as one HIP graph: the event pair around the roofline kernel is
-        # captured with it and re-recorded by every replay (the reading is the last timed replay's)
+        # the whole adjoint evaluation as one HIP graph (events recorded inside a graph cannot be
+        # timed, so the roofline kernel is timed on eager evaluations of the same step afterwards)
         f_e, g_e = eng.value_and_grad(theta)
-        _lib.check(lib.dh_timing_enable(eng.h, kind), "dh_timing_enable")
         eng.capture_grad()
         step = eng.value_and_grad_graph
     else:
@@ -534,6 +538,10 @@ def main():
         f, g = step(theta)
     torch.cuda.synchronize()
     t_adj = (time.perf_counter() - t0) / args.steps
+    if args.graph:
+        _lib.check(lib.dh_timing_enable(eng.h, kind), "dh_timing_enable")
+        for _ in range(2):
+            eng.value_and_grad(theta)
     tot = ctypes.c_double()
     cnt = ctypes.c_longlong()
     _lib.check(lib.dh_timing_read(eng.h, ctypes.byref(tot), ctypes.byref(cnt)), "dh_timing_read")

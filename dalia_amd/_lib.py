@@ -82,6 +82,9 @@ SIGNATURES = {
     "dh_prior_eig_dev": (_I, [_P, ctypes.POINTER(DhModel), _P, _P, _P]),
     "dh_set_tuning": (_I, [_P, _I, _I]),
     "dh_dag_table": (_I, [_I, _I, _I, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "dh_multi_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P)]),
+    "dh_multi_destroy": (_I, [_P]),
+    "dh_multi_cholesky": (_I, [_P, _I, _P]),
     "dh_ckpt_create": (_I, [_I, _I, _I, _I, _P, ctypes.POINTER(_P)]),
     "dh_ckpt_destroy": (_I, [_P]),
     "dh_ckpt_memory_bytes": (_LL, [_P]),

@@ -3,7 +3,8 @@
 Knobs (`dh_set_tuning`): key 2 = the pobtaf schedule (one persistent launch
 of left-looking tile tasks, or one launch per 64-column panel), key 0 = the
 Schur-update kernel of the per-panel schedule (64x64 tiles, or 128x128 tiles
-with the k range split in two).  `tune(engine)` times every
+with the k range split in two), key 6 = the number of workgroups of the persistent
+launch (for the instances of a batch).  `tune(engine)` times every
 value of every knob on the engine's own matrix with HIP events, applies the
 fastest and records it in a JSON file keyed by (padded block size, number of
 blocks, arrowhead size, device name); later calls read the file.
@@ -25,7 +26,7 @@ DEFAULT_PATH = os.environ.get("DALIA_AMD_TUNING",
 
 def _key(engine) -> str:
     lay = engine.solver.layout
-    return f"v3_bp{lay.bp}_n{lay.n}_a{lay.a}_{torch.cuda.get_device_name(0)}"
+    return f"v4_bp{lay.bp}_n{lay.n}_a{lay.a}_{torch.cuda.get_device_name(0)}"
 
 
 def _load(path):
@@ -40,6 +41,7 @@ def apply(engine, choice: dict) -> None:
     lib, h = engine.lib, engine.h
     _lib.check(lib.dh_set_tuning(h, 2, int(choice["alg"])), "dh_set_tuning")
     _lib.check(lib.dh_set_tuning(h, 0, int(choice["syrk"])), "dh_set_tuning")
+    _lib.check(lib.dh_set_tuning(h, 6, int(choice.get("wgs", 0))), "dh_set_tuning")
 
 
 def _time(fn, reps: int) -> float:
@@ -54,9 +56,11 @@ def _time(fn, reps: int) -> float:
     return float(np.median(ts))
 
 
-def tune(engine, path: str = DEFAULT_PATH, reps: int = 2, force: bool = False) -> dict:
+def tune(engine, path: str = DEFAULT_PATH, reps: int = 2, force: bool = False, workgroups: bool = False) -> dict:
+    """workgroups: also choose the number of persistent workgroups (key 6) — for an instance of a
+    batch, whose factorization runs beside those of the other instances."""
     cache = _load(path)
-    key = _key(engine)
+    key = _key(engine) + ("_w" if workgroups else "")
     if key in cache and not force:
         apply(engine, cache[key])
         return cache[key]
@@ -79,8 +83,24 @@ def tune(engine, path: str = DEFAULT_PATH, reps: int = 2, force: bool = False) -
     best_alg, best_syrk = cands[best]
     _lib.check(lib.dh_set_tuning(h, 2, best_alg), "dh_set_tuning")
     _lib.check(lib.dh_set_tuning(h, 0, best_syrk), "dh_set_tuning")
+    wgs = 0
+    if best_alg == 1 and workgroups:
+        # the fewest persistent workgroups (of the default, a half, a quarter, an eighth) that keep the
+        # factorization within 3 % of its fastest: a launch bound by its chain of diagonal tiles then
+        # leaves CUs to the other instances of a batch
+        full = torch.cuda.get_device_properties(0).multi_processor_count
+        tw = {0: times["dag"]}
+        for cap in (full // 2, full // 4, full // 8):
+            _lib.check(lib.dh_set_tuning(h, 6, cap), "dh_set_tuning")
+            factor()
+            tw[cap] = _time(factor, reps)
+        fastest = min(tw.values())
+        ok = [c for c, t in tw.items() if c > 0 and t <= 1.03 * fastest]
+        wgs = min(ok) if ok else 0
+        times["wgs"] = {str(k): v for k, v in tw.items()}
+        _lib.check(lib.dh_set_tuning(h, 6, wgs), "dh_set_tuning")
 
-    choice = {"alg": best_alg, "syrk": best_syrk, "ms": times}
+    choice = {"alg": best_alg, "syrk": best_syrk, "wgs": wgs, "ms": times}
     apply(engine, choice)
     cache[key] = choice
     try:

@@ -173,10 +173,20 @@ class CoregInla:
         if theta is not None:
             self._set_params(theta)
         out = self.scal if out is None else out
+        self._fused_a(out)
+        _lib.check(self.lib.dh_cholesky(self.h, 1), "dh_cholesky")
+        self._fused_b(out)
+        _lib.check(self.lib.dh_cholesky(self.h, 0), "dh_cholesky")
+        self._fused_c(out)
+
+    # the phases between the factorizations (see STInla._fused_a)
+    def _fused_a(self, out):
+        _lib.check(self.lib.dh_assemble_coreg(self.h, self.dm.models, self.data.nv, self.params.data_ptr(), 1, 1),
+                   "dh_assemble_coreg")
+
+    def _fused_b(self, out):
         L, h, nv = self.lib, self.h, self.data.nv
         N = self.data.n_latent
-        _lib.check(L.dh_assemble_coreg(h, self.dm.models, nv, self.params.data_ptr(), 1, 1), "dh_assemble_coreg")
-        _lib.check(L.dh_cholesky(h, 1), "dh_cholesky")
         taus = self.params.data_ptr() + 8 * nv * nv * 12
         _lib.check(L.dh_scale_indexed(h, self.dm.aty.data_ptr(), self.dm.vidx.data_ptr(), ctypes.c_void_p(taus),
                                       self.b.data_ptr(), N), "dh_scale_indexed")
@@ -185,8 +195,9 @@ class CoregInla:
         _lib.check(L.dh_logdet(h, 1, out[0:1].data_ptr()), "dh_logdet")
         _lib.check(L.dh_dot(h, self.mu.data_ptr(), self.b.data_ptr(), N, out[1:2].data_ptr()), "dh_dot")
         _lib.check(L.dh_assemble_coreg(h, self.dm.models, nv, self.params.data_ptr(), 0, 0), "dh_assemble_coreg")
-        _lib.check(L.dh_cholesky(h, 0), "dh_cholesky")
-        _lib.check(L.dh_logdet(h, 0, out[4:5].data_ptr()), "dh_logdet")
+
+    def _fused_c(self, out):
+        _lib.check(self.lib.dh_logdet(self.h, 0, out[4:5].data_ptr()), "dh_logdet")
 
     def _scalars(self, theta):
         theta = np.asarray(theta, dtype=np.float64)

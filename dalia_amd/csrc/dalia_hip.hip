@@ -1031,6 +1031,28 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
   if (fdone || g.bdone) dag_finish(fdone, g.bdone ? g.bdone + i : nullptr);  // wave-uniform
 }
 
+// One persistent launch over the factorizations of several handles: ticket t runs task order[t].y
+// of instance order[t].x.  The order keeps every instance's own task order, so each task still
+// waits only on tasks with smaller tickets, and the grid drains as for one instance.
+__global__ __launch_bounds__(256) void k_pobtaf_dag_multi(const DagArgs* gs, const int2* order, int ntotal,
+                                                          int* counter) {
+  __shared__ int s_task, s_n;
+  for (;;) {
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x) < 64) {
+      const int old = atomicAdd(counter, threadIdx.x == 0 ? 1 : 0);
+      s_task = __builtin_amdgcn_readfirstlane(old);
+    }
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_task);
+    __syncthreads();
+    if (t >= ntotal) break;
+    const int2 o = order[t];
+    const int inst = __builtin_amdgcn_readfirstlane(o.x);
+    const int task = __builtin_amdgcn_readfirstlane(o.y);
+    dag_task(gs[inst], task, &s_n);
+  }
+}
+
 // Waits until the persistent factorization has finished every task of blocks lo..lo+cnt-1, so
 // that later launches on this stream may read their factor while the factorization goes on with
 // the next blocks.  One lane per block (cnt <= 64); bounded like the factorization's own waits,
@@ -2113,6 +2135,7 @@ struct dh_handle {
   double *Wall, *Sall, *Waall;   // per-block W, inv(L)^T inv(L), Wa slots (overlap 2)
   int fp_ready;                  // sparsity whose factor products are in the per-block slots, -1 none
   int force_giveup;              // 1: dh_cholesky starts with the give-up flag raised (tuning key 5)
+  int dag_cap;                   // at most this many persistent workgroups (tuning key 6), 0: no cap
 };
 
 #define DH_MAXREC 16384
@@ -2234,6 +2257,7 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
   h->overlap = 0;
   h->fp_ready = -1;
   h->force_giveup = 0;
+  h->dag_cap = 0;
   {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -2536,7 +2560,8 @@ static DagArgs dag_args(const dh_handle* h, int ar, int slot0, int cnt, bool has
 // The launch itself (the flags of its blocks must be clear, those of a previous block raised).
 static void dag_launch(dh_handle* h, const DagArgs& g) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
-  const unsigned wgs = (unsigned)std::min<int>(g.ntasks, h->dag_wgs);
+  const int cap = h->dag_cap > 0 ? std::min(h->dag_cap, h->dag_wgs) : h->dag_wgs;
+  const unsigned wgs = (unsigned)std::min<int>(g.ntasks, cap);
   DH_LAUNCH(h, 7, k_pobtaf_dag, dim3(wgs), dim3(256), 0, g);
 }
 
@@ -3151,6 +3176,7 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
   else if (key == 2 && (value == 0 || value == 1)) h->pobtaf_alg = value;
   else if (key == 3 && (value == 0 || value == 1)) h->side_on = value;
   else if (key == 5 && (value == 0 || value == 1)) h->force_giveup = value;
+  else if (key == 6 && value >= 0) h->dag_cap = value;
   else if (key == 4 && value >= 0 && value <= 2) {
     if (value == 2 && !h->Wall) {  // per-block slots, kept until the handle is destroyed
       const long long bb = (long long)h->bp * h->bp, before = h->bytes;
@@ -3463,6 +3489,105 @@ int dh_ckpt_backward(dh_ckpt* c, const dh_model* m, const double* params, double
   const long Nl = (long)c->n * c->b + c->a;
   hipLaunchKernelGGL(k_unpack, dim3((unsigned)((Nl + 255) / 256)), dim3(256), 0, h->stream, (const double*)c->z, rhs,
                      c->n, c->b, bp, c->a);
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// several factorizations in one persistent launch (the instances of a batch)
+// ---------------------------------------------------------------------------
+
+struct dh_multi {
+  std::vector<dh_handle*> hs;
+  DagArgs* args[2];  // device copies per sparsity (BT, BTA)
+  int2* order[2];    // merged task order per sparsity
+  int ntotal[2];
+  int* counter;
+  int wgs;
+};
+
+extern "C" {
+
+int dh_multi_destroy(dh_multi* mc) {
+  if (!mc) return DH_EARG;
+  for (int sp = 0; sp < 2; ++sp) {
+    if (mc->args[sp]) (void)hipFree(mc->args[sp]);
+    if (mc->order[sp]) (void)hipFree(mc->order[sp]);
+  }
+  if (mc->counter) (void)hipFree(mc->counter);
+  delete mc;
+  return DH_OK;
+}
+
+int dh_multi_create(dh_handle* const* hs, int count, dh_multi** out) {
+  if (!hs || !out || count < 1 || count > 1024) return DH_EARG;
+  for (int k = 0; k < count; ++k)
+    if (!hs[k] || !hs[k]->diag || hs[k]->pobtaf_alg != 1) return DH_EARG;
+  dh_multi* mc = new dh_multi();
+  mc->hs.assign(hs, hs + count);
+  mc->args[0] = mc->args[1] = nullptr;
+  mc->order[0] = mc->order[1] = nullptr;
+  mc->counter = nullptr;
+  mc->wgs = 0;
+  for (int k = 0; k < count; ++k) mc->wgs = std::max(mc->wgs, hs[k]->dag_wgs);
+  int rc = DH_OK;
+  for (int sp = 0; sp < 2 && rc == DH_OK; ++sp) {
+    std::vector<DagArgs> ga(count);
+    struct Item {
+      double key;
+      int inst, task;
+    };
+    std::vector<Item> items;
+    for (int k = 0; k < count; ++k) {
+      dh_handle* h = hs[k];
+      const int ar = (sp == DH_SPARSITY_BTA && h->a > 0) ? 1 : 0;
+      ga[k] = dag_args(h, ar, 0, h->n, false, false, h->tip, 0);
+      const int nt = ga[k].ntasks;
+      for (int j = 0; j < nt; ++j) items.push_back(Item{(j + 0.5) / nt, k, j});
+    }
+    // every instance progresses in proportion to its length; each keeps its own order
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.key < y.key; });
+    std::vector<int2> ord(items.size());
+    for (size_t e = 0; e < items.size(); ++e) ord[e] = make_int2(items[e].inst, items[e].task);
+    mc->ntotal[sp] = (int)ord.size();
+    if (hipMalloc((void**)&mc->args[sp], sizeof(DagArgs) * count) != hipSuccess ||
+        hipMemcpy(mc->args[sp], ga.data(), sizeof(DagArgs) * count, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc((void**)&mc->order[sp], sizeof(int2) * ord.size()) != hipSuccess ||
+        hipMemcpy(mc->order[sp], ord.data(), sizeof(int2) * ord.size(), hipMemcpyHostToDevice) != hipSuccess)
+      rc = DH_EHIP;
+  }
+  if (rc == DH_OK && hipMalloc((void**)&mc->counter, sizeof(int)) != hipSuccess) rc = DH_EHIP;
+  if (rc != DH_OK) {
+    dh_multi_destroy(mc);
+    return rc;
+  }
+  *out = mc;
+  return DH_OK;
+}
+
+int dh_multi_cholesky(dh_multi* mc, int sparsity, void* hip_stream) {
+  if (!mc) return DH_EARG;
+  if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
+  hipStream_t st = (hipStream_t)hip_stream;
+  for (dh_handle* h : mc->hs) {
+    h->fp_ready = -1;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->info, INT_MAX);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->err, h->force_giveup ? 4 : 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, st, h->dagflags, 0, h->n * dag_fstride(h->nbt) + h->n);
+  }
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, mc->counter, 0);
+  const unsigned wgs = (unsigned)std::min(mc->ntotal[sparsity], mc->wgs);
+  hipLaunchKernelGGL(k_pobtaf_dag_multi, dim3(wgs), dim3(256), 0, st, (const DagArgs*)mc->args[sparsity],
+                     (const int2*)mc->order[sparsity], mc->ntotal[sparsity], mc->counter);
+  for (dh_handle* h : mc->hs) {
+    if (sparsity == DH_SPARSITY_BTA && h->a > 0)
+      hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, st, h->tip, (long)h->ap, h->tipdinv, h->tipinv, h->info,
+                         h->n * h->b);
+    h->factored = sparsity;
+    h->inverted = 0;
+  }
   HIPCHK(hipGetLastError());
   return DH_OK;
 }

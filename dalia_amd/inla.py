@@ -121,10 +121,14 @@ class STInla:
         factorization).
         """
         out = self.scal if out is None else out
-        L = self.lib
         self._assemble(theta, "bta", True)
         self._overlap(2 if grad else 1)
-        _lib.check(L.dh_cholesky(self.h, 1), "dh_cholesky")
+        _lib.check(self.lib.dh_cholesky(self.h, 1), "dh_cholesky")
+        self._after_qc(out)
+
+    def _after_qc(self, out):
+        """After the factorization of Qc: mu, logdet Qc and the quadratic-form pieces into out."""
+        L = self.lib
         N = self.dm.n_latent
         # mu = e^tau A^T y: scale by params[12] on the device (a 1x1 "scale" kernel via dot-free copy)
         _lib.check(L.dh_scale_dev(self.h, self.dm.aty.data_ptr(), self.params[12:13].data_ptr(),
@@ -154,6 +158,18 @@ class STInla:
 
     def set_stream(self, stream) -> None:
         _lib.check(self.lib.dh_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)), "dh_set_stream")
+
+    # the three phases of one plain f evaluation between the factorizations, for a batch whose
+    # factorizations run as one launch over all instances (`dalia_amd.batch.BatchedF`)
+    def _fused_a(self, out):
+        self._assemble(None, "bta", True)
+
+    def _fused_b(self, out):
+        self._after_qc(out)
+        self._assemble(None, "bt", False)
+
+    def _fused_c(self, out):
+        _lib.check(self.lib.dh_logdet(self.h, 0, out[4:5].data_ptr()), "dh_logdet")
 
     def param_row(self, theta) -> np.ndarray:
         """The 13 device parameters of theta (Kronecker coefficients, exp(theta_obs))."""

@@ -256,6 +256,19 @@ int dh_ckpt_forward(dh_ckpt* c, const dh_model* m, const double* params, int spa
 int dh_ckpt_backward(dh_ckpt* c, const dh_model* m, const double* params, double* rhs, int selinv, double* cx_dev,
                      double* ata_dev, double* var_dev);
 
+/* Several factorizations in one persistent launch: the handles of a batch (each with its own
+ * sizes, assembled beforehand, all with the persistent schedule).  The tasks of all instances are
+ * merged into one order in which every instance keeps its own order and progresses in proportion
+ * to its length, so the instances' chains of diagonal tiles run side by side.
+ * dh_multi_cholesky(mc, sparsity, stream) then does what dh_cholesky(h, sparsity) does for every
+ * handle, on that stream (which every handle's preceding and following work must be ordered
+ * with).  No counterpart in the reference, which runs such evaluations on separate processes
+ * (core/dalia.py:541-573). */
+typedef struct dh_multi dh_multi;
+int dh_multi_create(dh_handle* const* hs, int count, dh_multi** out);
+int dh_multi_destroy(dh_multi* mc);
+int dh_multi_cholesky(dh_multi* mc, int sparsity, void* hip_stream);
+
 /* Task table of one block of the persistent pobtaf/pobtf launch for nbt 64-column tiles per block
  * (last != 0: the last block, without sub-diagonal tiles; arrow != 0: with the arrowhead), as the
  * launch decodes it: entry = kind << 24 | r << 12 | c, kinds as in the source's DK_* list.  Writes
@@ -284,7 +297,10 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
  * key 5: 1 = the next factorizations start with the flag that a timed-out readiness wait
  *        raises already set, so every wait of the persistent launch returns at once, the
  *        launch drains with an invalid factor, dh_logdet writes NaN and dh_get_info returns -4
- *        (a test of that error path in which every kernel exits normally); 0 = off (default).
+ *        (a test of that error path in which every kernel exits normally); 0 = off (default);
+ * key 6: at most this many workgroups in the persistent launch (0 = one per CU that fits, the
+ *        default); a factorization bound by its chain of diagonal tiles needs fewer, and leaves the
+ *        other CUs to concurrent work of other handles (other streams).
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 

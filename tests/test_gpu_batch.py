@@ -152,3 +152,30 @@ def test_captured_adjoint_gradient_every_replay():
     f_or, g_or = oinla.adjoint_gradient(d, THETAS[2])
     f, g = eng.value_and_grad_graph(THETAS[2])
     assert abs(f - f_or) <= 1e-10 * abs(f_or) and rel(g, g_or) < 1e-8
+
+
+@pytest.mark.parametrize("assignment", [[0, 1, 2, 1, 0, 2, 2, 1], [0, 0, 0, 1, 2]])
+def test_fused_rounds_one_launch_per_round(assignment):
+    """Uneven instances (univariate and coregional) whose factorizations of each round run as one
+    persistent launch over all instances (`dh_multi_cholesky`); rounds with fewer instances when
+    their evaluation counts differ.  Every replay against the oracle."""
+    from dalia_amd.batch import BatchedF
+    from dalia_amd.coreg import CoregInla
+    from dalia_amd.inla import STInla
+    from oracle import coreg as ocoreg
+    du = [synth.make_st_model(5, 3, 2, 0), synth.make_st_model(9, 6, 3, 1)]
+    dc = synth.make_coreg_model(5, 4, 2, 2, 6)
+    engines = [STInla(du[0]), CoregInla(dc), STInla(du[1])]
+    datas = [du[0], dc, du[1]]
+    print("case fused rounds", assignment, flush=True)
+    bf = BatchedF(engines, assignment, fused=True)
+    rng = np.random.default_rng(2)
+    for rep in range(2):
+        thetas = [datas[a].theta + 0.05 * rng.standard_normal(datas[a].theta.size) for a in assignment]
+        fv = bf(thetas)
+        for e, a in enumerate(assignment):
+            ref = ocoreg.f_theta_coreg(dc, thetas[e]) if a == 1 else oinla.f_theta(datas[a], thetas[e])
+            assert abs(fv[e] - ref) <= 1e-10 * abs(ref), (rep, e)
+        for eng, d in zip(engines, datas):  # the engines still evaluate eagerly afterwards
+            ref = ocoreg.f_theta_coreg(d, d.theta) if eng is engines[1] else oinla.f_theta(d, d.theta)
+            assert abs(eng.f(d.theta) - ref) <= 1e-10 * abs(ref)
