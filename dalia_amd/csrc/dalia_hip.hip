@@ -1360,14 +1360,10 @@ __device__ __forceinline__ void sel_w_tile(const SelArgs& a, bool wa, int t, int
 // the long one from its start up.  All workgroups then have the same length and, at any moment,
 // read only two chunk columns between them, which they share in L2.  With nbt odd the middle row
 // is a workgroup of its own.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_w(SelBatch sb) {
-  DH_SIDE_ACQUIRE(sb.acq);
-  if ((int)blockIdx.y >= sb.cnt) return;
-  const SelArgs& a = sb.a[blockIdx.y];
+__device__ __forceinline__ void sel_w_work(const SelArgs& a, int e) {
   const int nbt = a.nbt, half = nbt / 2, odd = nbt & 1;
   const int nw = a.Lb ? (half + odd) * nbt : 0;  // workgroups of W^T
   const int na = a.R ? half + odd : 0;           // workgroups of Wa
-  int e = blockIdx.x;
   if (e >= nw + na) return;
   const bool wa = e >= nw;
   int p, J = 0;
@@ -1383,6 +1379,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   } else {
     sel_w_tile<true>(a, wa, p, J);  // the middle row (nbt odd)
   }
+}
+
+// Workgroups of sel_w_work per block: W^T tile-row pairs times tile columns, and the Wa tiles.
+__device__ __host__ __forceinline__ int sel_w_wgs(int nbt, bool lower, bool arrow) {
+  return (nbt / 2 + (nbt & 1)) * ((lower ? nbt : 0) + (arrow ? 1 : 0));
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_w(SelBatch sb) {
+  DH_SIDE_ACQUIRE(sb.acq);
+  if ((int)blockIdx.y >= sb.cnt) return;
+  sel_w_work(sb.a[blockIdx.y], blockIdx.x);
 }
 
 // X_{i+1,i} = -X_{i+1,i+1} W - X_{a,i+1}^T Wa  (X_{i+1,i+1} read by columns: it is symmetric).
@@ -1406,11 +1413,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // Lower tiles of inv(L_ii)^T inv(L_ii) into S, for block slot blockIdx.y (factor only).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_ll(SelBatch sb) {
-  DH_SIDE_ACQUIRE(sb.acq);
-  if ((int)blockIdx.y >= sb.cnt) return;
-  const SelArgs& a = sb.a[blockIdx.y];
-  const int e = blockIdx.x;
+__device__ __forceinline__ void sel_ll_work(const SelArgs& a, int e) {
   int I = 0;
   while ((I + 1) * (I + 2) / 2 <= e) ++I;
   const int J = e - I * (I + 1) / 2;
@@ -1421,11 +1424,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
 }
 
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_ll(SelBatch sb) {
+  DH_SIDE_ACQUIRE(sb.acq);
+  if ((int)blockIdx.y >= sb.cnt) return;
+  sel_ll_work(sb.a[blockIdx.y], blockIdx.x);
+}
+
 // Lower tiles of S (holding inv(L)^T inv(L)) -= W^T X_{i+1,i} (workgroups e < ntri, ntri = 0 for
 // the last block), and the arrow row X_{a,i} = -X_{a,i+1} W - X_aa Wa (the nbt workgroups after
 // them): the two are independent, and the rank-64 term Wa^T X_{a,i} of X_ii is left to k_sel_sym.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_s(SelArgs a, int ntri) {
-  int e = blockIdx.x;
+__device__ __forceinline__ void sel_s_work(const SelArgs& a, int ntri, int e) {
   dh_d4 acc[2][2];
   dh_zero(acc);
   if (e < ntri) {
@@ -1446,10 +1454,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   dh_store(a.R, a.bp, 0, e * DH_TILE, acc, -1.0, 0.0, false);
 }
 
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_s(SelArgs a, int ntri) {
+  sel_s_work(a, ntri, blockIdx.x);
+}
+
+// k_sel_s of block i with, in its spare workgroups, the W and Wa products of a later block (fut,
+// cnt 0 or 1): the recursion's S phase fills only part of the GPU (ntri + nbt tiles), and those
+// products depend on the factor alone.  Workgroups e < ns are k_sel_s's.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_s_w(SelArgs a, int ntri, int ns,
+                                                                                     SelBatch fut) {
+  const int e = blockIdx.x;
+  if (e < ns) sel_s_work(a, ntri, e);
+  else if (fut.cnt > 0) sel_w_work(fut.a[0], e - ns);
+}
+
 // X_ii = S - Wa^T X_{a,i} on lower tile (I, J) (S alone without the arrowhead), written to D at
 // (I, J) and mirrored to (J, I).
-__global__ __launch_bounds__(256) void k_sel_sym(SelArgs a, double* D) {
-  const int e = blockIdx.x;
+__device__ __forceinline__ void sel_sym_work(const SelArgs& a, double* D, int e) {
   int I = 0;
   while ((I + 1) * (I + 2) / 2 <= e) ++I;
   const int J = e - I * (I + 1) / 2;
@@ -1460,6 +1481,16 @@ __global__ __launch_bounds__(256) void k_sel_sym(SelArgs a, double* D) {
                          J * DH_TILE, 0, DH_TILE);
   dh_store(D, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
   if (I != J) dh_store(D, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
+}
+
+__global__ __launch_bounds__(256) void k_sel_sym(SelArgs a, double* D) { sel_sym_work(a, D, blockIdx.x); }
+
+// k_sel_sym of block i with inv(L)^T inv(L) of a later block (fut) in its spare workgroups.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_sym_ll(SelArgs a, double* D,
+                                                                                        int nsym, SelBatch fut) {
+  const int e = blockIdx.x;
+  if (e < nsym) sel_sym_work(a, D, e);
+  else if (fut.cnt > 0) sel_ll_work(fut.a[0], e - nsym);
 }
 
 // ---------------------------------------------------------------------------

@@ -18,9 +18,10 @@ here is generated.  The construction follows the reference's model family
 * observations at uniform points in [0,1]^2 x [0, nt); the spatio-temporal
   design row holds barycentric weights times linear-in-time weights (6
   entries), the regression design block holds N(0,1) covariates;
-* ``y = A_st x_st + X beta + eps`` with a smooth random space-time field
-  ``x_st`` (a sum of random Fourier modes; drawing from the SPDE prior at
-  the full size would need a factorisation of Q_prior just to make data),
+* ``y = A_st x_st + X beta + eps`` with ``x_st`` an exact draw from the
+  spatio-temporal prior N(0, Qp^-1) at theta (`prior_transform`: the
+  Kronecker-eigen factorisation of Qp, one eigendecomposition of the FEM
+  matrices and one tridiagonal factorisation per spatial eigenvalue),
   ``beta ~ N(0,1)`` and ``eps ~ N(0, exp(-theta_obs))``.
 
 Everything here is host-side numpy/scipy set-up data, built once per model.
@@ -200,6 +201,43 @@ DEFAULT_PRIORS = [
 ]
 
 
+def prior_transform(c0, g1, m0, m1, m2, theta, z: np.ndarray, manifold: str = "plane") -> np.ndarray:
+    """x = B z with B B^T = Qp^-1 for the spatio-temporal prior at theta (time-major x).
+
+    With c0 diagonal, g2 = g1 c0^-1 g1 and g3 = g1 c0^-1 g2, Qp = (I (x) V) blkdiag_j(T_j) (I (x) V^T)
+    after ordering by spatial eigenvector, V = c0^1/2 U, U diag(lam) U^T = c0^-1/2 g1 c0^-1/2 and
+    T_j = sum_k p_k(kappa^2 + lam_j) M_k tridiagonal; so x = (I (x) c0^-1/2 U) w with
+    w_j = L_j^-T z_j, T_j = L_j L_j^T.  z: (nt * ns,) standard normal."""
+    from dalia_amd.hyper import kron_coefficients
+    ns, nt = c0.shape[0], m0.shape[0]
+    ci = 1.0 / np.sqrt(sps.csr_matrix(c0).diagonal())
+    S = (sps.diags(ci) @ g1 @ sps.diags(ci)).toarray()
+    lam, U = np.linalg.eigh(0.5 * (S + S.T))
+    c = kron_coefficients(theta, manifold)  # rows M0, M1, M2; columns c0, g1, g2, g3
+    k2 = c[2, 0] / c[2, 1]  # kappa^2
+    lt = k2 + lam
+    p = np.stack([c[0, 3] * lt ** 3, c[1, 2] * lt ** 2, c[2, 1] * lt])  # p_k(lt_j) (c[k, m] = p_k coefficient)
+    Md = np.stack([M.diagonal() for M in (m0, m1, m2)])      # (3, nt)
+    Mo = np.stack([M.diagonal(-1) for M in (m0, m1, m2)])    # (3, nt - 1)
+    d = p.T @ Md   # (ns, nt): diagonals of T_j
+    e = p.T @ Mo   # (ns, nt - 1): sub-diagonals
+    # Cholesky of every T_j at once: l_t = sqrt(d_t - o_{t-1}^2), o_t = e_t / l_t
+    L = np.zeros((ns, nt))
+    O = np.zeros((ns, max(nt - 1, 0)))
+    for t in range(nt):
+        v = d[:, t] - (O[:, t - 1] ** 2 if t > 0 else 0.0)
+        L[:, t] = np.sqrt(v)
+        if t < nt - 1:
+            O[:, t] = e[:, t] / L[:, t]
+    Z = np.asarray(z, dtype=np.float64).reshape(nt, ns).T  # (ns, nt), row j = eigenvector j's series
+    W = np.zeros_like(Z)
+    for t in range(nt - 1, -1, -1):  # L_j^T w = z
+        r = Z[:, t] - (O[:, t] * W[:, t + 1] if t < nt - 1 else 0.0)
+        W[:, t] = r / L[:, t]
+    X = (ci[:, None] * U) @ W  # (ns, nt)
+    return np.ascontiguousarray(X.T).ravel()
+
+
 def make_st_model(m: int, nt: int, nb: int, seed: int, obs_per_node: int = 4,
                   theta=None, priors=None) -> STModelData:
     """Build the seeded spatio-temporal model on an m x m grid."""
@@ -220,20 +258,8 @@ def make_st_model(m: int, nt: int, nb: int, seed: int, obs_per_node: int = 4,
     a_st = design_matrix(m, nt, s_xy, t)
     x_cov = rng.standard_normal((n_obs, nb))
     beta = rng.standard_normal(nb)
-    # smooth space-time field: random Fourier modes
-    n_modes = 8
-    kx = rng.integers(1, 4, n_modes)
-    ky = rng.integers(1, 4, n_modes)
-    kt = rng.uniform(0.0, 2.0 * math.pi / max(nt, 2) * 3.0, n_modes)
-    amp = rng.standard_normal(n_modes) / math.sqrt(n_modes)
-    ph = rng.uniform(0.0, 2.0 * math.pi, n_modes)
-    px = pts[:, 0][None, :]
-    py = pts[:, 1][None, :]
-    tn = np.arange(nt, dtype=np.float64)[:, None]
-    field = np.zeros((nt, ns))
-    for k in range(n_modes):
-        field += amp[k] * np.cos(math.pi * (kx[k] * px + ky[k] * py) + kt[k] * tn + ph[k])
-    x_st = field.ravel()
+    # the spatio-temporal field: a draw from its prior at theta
+    x_st = prior_transform(c0, g1, m0, m1, m2, theta, rng.standard_normal(ns * nt))
     eps = rng.standard_normal(n_obs) * math.exp(-0.5 * theta[3])
     y = a_st @ x_st + x_cov @ beta + eps
     return STModelData(

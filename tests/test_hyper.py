@@ -104,3 +104,16 @@ def test_negative_eigenvalue_message(capsys):
     assert ev.min() < 0 and "Negative eigenvalues detected" in capsys.readouterr().out
     check_hessian(np.eye(3))
     assert capsys.readouterr().out == ""
+
+
+@pytest.mark.parametrize("manifold", ["plane", "sphere"])
+def test_synthetic_field_is_an_exact_prior_draw(manifold):
+    """The data generator draws x_st from N(0, Qp^-1): its linear map B satisfies B B^T = Qp^-1
+    (dense inverse of the oracle's assembled prior) on a small model."""
+    import dataclasses
+    d = dataclasses.replace(synth.make_st_model(4, 3, 1, 0), manifold=manifold)
+    th = np.array([-0.5, 0.3, 0.2, 1.0])
+    N = d.ns * d.nt
+    B = np.stack([synth.prior_transform(d.c0, d.g1, d.m0, d.m1, d.m2, th, e, manifold) for e in np.eye(N)], axis=1)
+    C = np.linalg.inv(inla.qprior_st_sparse(d, th).toarray())
+    assert np.linalg.norm(B @ B.T - C) / np.linalg.norm(C) < 1e-12
