@@ -466,6 +466,8 @@ def main():
                     help="time the adjoint evaluation as one captured HIP graph replayed per step")
     ap.add_argument("--checkpoint", type=int, default=0,
                     help="factor checkpointed every K time blocks and recomputed in the backward pass (0: stored)")
+    ap.add_argument("--twisted", type=int, default=1, choices=(0, 1),
+                    help="two-ended factorizations (blocks eliminated from both ends, meeting in the middle)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device-to-device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -503,7 +505,7 @@ def main():
     log(f"building synthetic model {args.workload} (ns={ns}, nt={nt}, nb={nb}, seed={seed})")
     t0 = time.perf_counter()
     data = synth.make_st_model(m, nt, nb, seed)
-    eng = STInla(data, overlap=args.overlap)
+    eng = STInla(data, overlap=args.overlap, twisted=bool(args.twisted))
     torch.cuda.synchronize()
     log(f"set-up {time.perf_counter() - t0:.1f} s (closed-form prior: {eng.dm.closed_form})")
     theta = data.theta.copy()
@@ -665,7 +667,7 @@ def main():
         "config": {"workload": args.workload, "ns": ns, "nt": nt, "nb": nb, "seed": seed, "theta_dim": 4,
                    "n_obs": data.n_obs, "kernel_variants": {"pobtaf_schedule": "tile-task" if choice["alg"] == 1 else "per-panel",
                                                         "syrk": choice["syrk"]},
-                   "overlap": args.overlap, "graph": bool(args.graph)},
+                   "overlap": args.overlap, "graph": bool(args.graph), "twisted": bool(eng.twisted)},
         "pobtaf_ms": t_pobtaf * 1e3,
         "pobtaf_tflops": pobtaf_tflops,
         "pobtaf_frac_of_peak": pobtaf_tflops / FP64_PEAK_TFLOPS,

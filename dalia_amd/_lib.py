@@ -116,6 +116,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         raise RuntimeError(
             f"{path} is missing: build it with `python -m dalia_amd._build` "
             "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    # torch first: the library then binds to the HIP runtime that torch has loaded (torch ships
+    # its own), so that torch's streams and device pointers are valid in the library's calls
+    import torch  # noqa: F401
     lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

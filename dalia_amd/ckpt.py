@@ -31,10 +31,13 @@ from dalia_amd.model import DeviceSTModel
 
 
 def full_handle_bytes(n: int, b: int, a: int) -> int:
-    """Device bytes of a full `dh_create` handle (factor, inverse, scratch) for n blocks of b."""
+    """Device bytes of a full `dh_create` handle (factor, inverse, scratch) for n blocks of b, an
+    upper bound: two sets of G = min(n, 8) selected-inversion slots (W, S, Wa) and the
+    factorization's own."""
     bp = (b + 63) // 64 * 64
     bb = bp * bp
-    return 8 * ((3 * n + 2 * 8) * bb + 8 * bp * (bp + 8) + 8 * 64 * bp + (n + 2) * 64 * bp)
+    G = min(n, 8)
+    return 8 * ((3 * n + 3 * G) * bb + 2 * G * bp * (bp + 8) + 2 * G * 64 * bp + (n + 2) * 64 * bp)
 
 
 def ckpt_bytes(n: int, b: int, a: int, K: int) -> int:

@@ -97,6 +97,10 @@ __global__ __launch_bounds__(256) void k_fill_int(int* p, int v, int len) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < len; e += gridDim.x * blockDim.x) p[e] = v;
 }
 
+__global__ __launch_bounds__(256) void k_fill_dbl(double* p, double v, int len) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < len; e += gridDim.x * blockDim.x) p[e] = v;
+}
+
 __global__ __launch_bounds__(256) void k_copy(const double* x, double* y, int len) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < len) y[e] = x[e];
@@ -114,9 +118,13 @@ __global__ __launch_bounds__(256) void k_copy(const double* x, double* y, int le
 // One workgroup per row of a diagonal or sub-diagonal block; the row is built
 // in LDS (zero, FEM Kronecker entries, tau A^T A entries) and streamed out.
 // Time blocks t0 .. t0 + n - 1 go to diag[0 .. n - 1], the sub-diagonal blocks (t + 1, t) for
-// t = t0 .. t0 + nl - 1 to lower[0 .. nl - 1] (a segment of a longer time axis, or all of it).
+// t = t0 .. t0 + nl - 1 to lower[0 .. nl - 1] (a segment of a longer time axis, or all of it);
+// those with local index >= tlo are written transposed, as block (t, t + 1) (the layout of the
+// upward side of a two-ended factorization).  The FEM and temporal matrices are symmetric and so is
+// A^T A, so row s of block (t, t + 1) is row (t, s) of Q restricted to time block t + 1.
 __global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* lower, int n, int nl, int t0, int b,
-                                                       int bp, dh_model m, const double* params, int with_like) {
+                                                       int bp, dh_model m, const double* params, int with_like,
+                                                       int tlo) {
   extern __shared__ double row[];
   const long r = blockIdx.x;
   const long ndiag = (long)n * bp;
@@ -136,8 +144,8 @@ __global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* low
     const int tl = (int)(rr / bp);
     s = (int)(rr % bp);
     t = t0 + tl;
-    trow = t + 1;
-    tcol = t;
+    trow = tl >= tlo ? t : t + 1;
+    tcol = tl >= tlo ? t + 1 : t;
     out = lower + (long)tl * bp * bp + (long)s * bp;
   }
   for (int c = threadIdx.x; c < bp; c += blockDim.x) row[c] = 0.0;
@@ -663,10 +671,22 @@ struct DagArgs {
   int has_prev;    // block 0 has a previous block whose factor is final (a segment of a longer matrix)
   int last_full;   // the last block has sub-diagonal tiles (its successor is factored by a later launch)
   int row0;        // global row of block 0 (pivot failures are reported as global rows)
+  int dir;         // +1: local block i is slot i of the arrays; -1: slot -i (an instance that walks the
+                   // slots downward; its sub-diagonal block of local i then sits at lower[-i], i.e. at
+                   // the lower of the two slots it couples, holding the rows of the later block)
+  // A second side in the same launch (the two-ended factorization): ticket t runs task order[t].y
+  // of side order[t].x, side 1 with the arrays, flags, tile inverses and tip below (same block
+  // shape and schedule); order null: one side, ticket t is task t.
+  const int2* order;
+  double *diag2, *lower2, *arrow2, *tip2, *dinv2, *pinv2;
+  int* flags2;
+  int n2, row02, dir2;
 };
 
 #define DH_DAG_SPIN (1 << 20)
+#ifndef DAG_STAGES
 #define DAG_STAGES 2  // operand ring depth of the task products (measured: 1 -> 77.9 ms, 2 -> 77.2, 3 -> 78.8, 4 -> 80.3, 5 -> 80.9)
+#endif
 #define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed before QX(r) and after OLD(r))
 #define DH_DAG_SLOOK 8  // measured at ns = 2025, nt = 100: 6 -> 83.7 ms, 7 -> 81.1, 8 -> 80.3, 10 -> 81.7, 12 -> 82.9
 
@@ -853,7 +873,24 @@ __device__ __forceinline__ void chain_store(double* C, long ldc, const dh_d4 acc
 }
 
 // One task (t wave-uniform); every path ends at the single signal.
-__device__ void dag_task(const DagArgs& g, int t, int* s_n) {
+__device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
+  DagArgs g = g0;
+  if (g0.order) {  // uniform: the side and its task, and the side's arrays
+    const int2 o = g0.order[t];
+    t = __builtin_amdgcn_readfirstlane(o.y);
+    if (__builtin_amdgcn_readfirstlane(o.x)) {
+      g.diag = g0.diag2;
+      g.lower = g0.lower2;
+      g.arrow = g0.arrow2;
+      g.tip = g0.tip2;
+      g.dinv = g0.dinv2;
+      g.pinv = g0.pinv2;
+      g.flags = g0.flags2;
+      g.n = g0.n2;
+      g.row0 = g0.row02;
+      g.dir = g0.dir2;
+    }
+  }
   const int nbt = g.nbt, bp = g.bp;
   const long bb = (long)bp * bp;
   const int FS = dag_fstride(nbt);
@@ -863,12 +900,13 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
   const bool prev = i > 0 || g.has_prev;  // the previous block's terms apply
   const int kind = code >> 24, r = (code >> 12) & 0xfff, c = code & 0xfff;
   const long T64 = (long)DH_TILE * bp;  // one tile row
-  double* D = g.diag + i * bb;
-  double* Lo = g.lower + i * bb;         // i < n-1
-  double* Ar = g.arrow + (long)i * T64;
-  const double* Lp = g.lower + (i - 1) * bb;       // prev
-  const double* Ap = g.arrow + (long)(i - 1) * T64;
-  double* dinv = g.dinv + (long)i * bp;
+  const long si = (long)g.dir * i;       // slot offset of local block i
+  double* D = g.diag + si * bb;
+  double* Lo = g.lower + si * bb;        // i < n-1
+  double* Ar = g.arrow + si * T64;
+  const double* Lp = g.lower + (si - g.dir) * bb;  // prev
+  const double* Ap = g.arrow + (si - g.dir) * T64;
+  double* dinv = g.dinv + si * bp;
   double* pinv = g.pinv + (long)(i & 1) * nbt * DH_TILE * DH_TILE;  // this block's dense tile inverses
   int* fD = g.flags + (long)i * FS;
   int* fL = fD + nbt * nbt;
@@ -977,7 +1015,7 @@ __device__ void dag_task(const DagArgs& g, int t, int* s_n) {
       for (int x1 = 0; x1 < 2; ++x1) acc[x0][x1] = part[x0][x1] - acc[x0][x1];
     dh_store(dh_lS, DH_LDS_LD, 0, 0, acc, 1.0, 0.0, false);
     __syncthreads();
-    dh_leaf64_core(g.info, g.row0 + i * g.b + c * DH_TILE);
+    dh_leaf64_core(g.info, g.row0 + g.dir * i * g.b + c * DH_TILE);
     // Within the launch the tile (c, c) is only ever used through its dense inverse, so the flag
     // is raised once that is out; L, inv(L)^T and dinv follow (read after the launch).
     dh_leaf64_store_inv(pv);
@@ -1074,6 +1112,58 @@ __global__ __launch_bounds__(64) void k_dag_gate(const int* bdone, int lo, int c
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// In-place transpose of blockIdx.y-th bp x bp block at base: the workgroup of lower tile pair
+// (I, J), I >= J, swaps tiles (I, J) and (J, I) transposed (a diagonal tile is transposed alone).
+__global__ __launch_bounds__(256) void k_transpose_blocks(double* base, int bp) {
+  __shared__ double t0[DH_TILE][DH_TILE + 1], t1[DH_TILE][DH_TILE + 1];
+  int e = blockIdx.x, I = 0;
+  while ((I + 1) * (I + 2) / 2 <= e) ++I;
+  const int J = e - I * (I + 1) / 2;
+  double* B = base + (long)blockIdx.y * bp * bp;
+  double* p = B + (long)I * DH_TILE * bp + J * DH_TILE;  // tile (I, J)
+  double* q = B + (long)J * DH_TILE * bp + I * DH_TILE;  // tile (J, I)
+  for (int x = threadIdx.x; x < DH_TILE * DH_TILE; x += 256) {
+    const int r = x / DH_TILE, c = x % DH_TILE;
+    t0[r][c] = p[(long)r * bp + c];
+    t1[r][c] = q[(long)r * bp + c];
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < DH_TILE * DH_TILE; x += 256) {
+    const int r = x / DH_TILE, c = x % DH_TILE;
+    p[(long)r * bp + c] = t1[c][r];
+    if (I != J) q[(long)r * bp + c] = t0[c][r];
+  }
+}
+
+// Junction of the two-ended factorization: the block m where the downward and the upward
+// eliminations meet receives the upward side's terms, D_m -= L L^T on its lower tiles and
+// (arrow) Ar_m -= R L^T, with L the upward side's last coupling (rows of block m) and R that
+// block's arrow rows; the last workgroup adds the upward side's tip partial tp to the tip.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_twist_join(
+    const double* L, const double* R, double* D, double* Ar, const double* tp, double* tip, int nbt, int bp) {
+  int e = blockIdx.x;
+  const int ntri = nbt * (nbt + 1) / 2;
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  const Opnd ol{L, bp, M_ROW, nullptr};
+  if (e < ntri) {
+    int I = 0;
+    while ((I + 1) * (I + 2) / 2 <= e) ++I;
+    const int J = e - I * (I + 1) / 2;
+    dh_mma<M_ROW, M_ROW>(acc, ol, I * DH_TILE, ol, J * DH_TILE, 0, bp);
+    dh_store(D, bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 1.0, false);
+    return;
+  }
+  e -= ntri;
+  if (!R) return;
+  if (e < nbt) {
+    dh_mma<M_ROW, M_ROW>(acc, Opnd{R, bp, M_ROW, nullptr}, 0, ol, e * DH_TILE, 0, bp);
+    dh_store(Ar, bp, 0, e * DH_TILE, acc, -1.0, 1.0, false);
+    return;
+  }
+  for (int x = threadIdx.x; x < DH_TILE * DH_TILE; x += 256) tip[x] += tp[x];
 }
 
 // Tasks are claimed by wave 0 under a wave-uniform branch: lane 0 adds 1 and
@@ -1305,6 +1395,8 @@ struct SelArgs {
   double* Wa;          // Wa = L_{a,i} inv(L_ii) (64 x bp)
   double* S;           // lower tiles of X_ii (ld bp)
   int bp, nbt, wld;
+  int xt;              // 1: X_{i+1,i} is stored transposed into Lb (the upward side of a two-ended factor,
+                       // whose coupling block holds the rows of block i's successor)
 };
 
 #define DH_SELG 8  // blocks whose factor-only products are batched in one launch
@@ -1394,11 +1486,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // X_{i+1,i} = -X_{i+1,i+1} W - X_{a,i+1}^T Wa  (X_{i+1,i+1} read by columns: it is symmetric).
 // nbt^2 workgroups: one round at four workgroups per CU on a 256-CU device when bp = 2048.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_x(SelArgs a) {
+__device__ __forceinline__ void sel_x_work(const SelArgs& a, int e) {
   dh_d4 acc[2][2], acc2[2][2];
   dh_zero(acc);
   dh_zero(acc2);
-  const int e = blockIdx.x;
   const int I = e / a.nbt, J = e % a.nbt;
   dh_mma<M_COL, M_COL>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.W, a.wld, M_COL, nullptr},
                        J * DH_TILE, 0, a.bp);
@@ -1409,7 +1500,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) acc[x][y] += acc2[x][y];
-  dh_store(a.Lb, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
+  dh_store(a.Lb, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, a.xt != 0);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_x(SelArgs a) {
+  sel_x_work(a, blockIdx.x);
+}
+
+// k_sel_x of two independent blocks in one launch (the two sides of a two-ended factor):
+// workgroups e < n0 are a0's.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_x2(SelArgs a0, SelArgs a1,
+                                                                                    int n0) {
+  const int e = blockIdx.x;
+  if (e < n0) sel_x_work(a0, e);
+  else sel_x_work(a1, e - n0);
 }
 
 // Lower tiles of inv(L_ii)^T inv(L_ii) into S, for block slot blockIdx.y (factor only).
@@ -1440,8 +1544,12 @@ __device__ __forceinline__ void sel_s_work(const SelArgs& a, int ntri, int e) {
     int I = 0;
     while ((I + 1) * (I + 2) / 2 <= e) ++I;
     const int J = e - I * (I + 1) / 2;
-    dh_mma<M_COL, M_COL>(acc, Opnd{a.W, a.wld, M_COL, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
-                         J * DH_TILE, 0, a.bp);
+    if (a.xt)  // X_{i+1,i} stored transposed
+      dh_mma<M_COL, M_ROW>(acc, Opnd{a.W, a.wld, M_COL, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_ROW, nullptr},
+                           J * DH_TILE, 0, a.bp);
+    else
+      dh_mma<M_COL, M_COL>(acc, Opnd{a.W, a.wld, M_COL, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
+                           J * DH_TILE, 0, a.bp);
     dh_store(a.S, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 1.0, false);
     return;
   }
@@ -1458,14 +1566,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   sel_s_work(a, ntri, blockIdx.x);
 }
 
-// k_sel_s of block i with, in its spare workgroups, the W and Wa products of a later block (fut,
-// cnt 0 or 1): the recursion's S phase fills only part of the GPU (ntri + nbt tiles), and those
-// products depend on the factor alone.  Workgroups e < ns are k_sel_s's.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_s_w(SelArgs a, int ntri, int ns,
-                                                                                     SelBatch fut) {
+// k_sel_s of one or two independent blocks (workgroups e < ns0 are a0's, e < ns a1's), and in the
+// workgroups after them the W and Wa products of the next chunk's blocks (fut): workgroups
+// w0 + e - ns of its flat range of nwb per block.  The S phase alone fills only part of the
+// GPU's workgroup slots (ntri + nbt tiles per block), or leaves a partial last round.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_s2w(
+    SelArgs a0, int ntri0, int ns0, SelArgs a1, int ntri1, int ns, SelBatch fut, int w0, int nwb) {
   const int e = blockIdx.x;
-  if (e < ns) sel_s_work(a, ntri, e);
-  else if (fut.cnt > 0) sel_w_work(fut.a[0], e - ns);
+  if (e < ns) {
+    if (e < ns0) sel_s_work(a0, ntri0, e);
+    else sel_s_work(a1, ntri1, e - ns0);
+    return;
+  }
+  const int f = w0 + e - ns;
+  const int g = f / nwb;
+  if (g < fut.cnt) sel_w_work(fut.a[g], f - g * nwb);
+}
+
+// The W and Wa products of fut not yet formed: workgroups w0 + blockIdx.x of its flat range.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_wr(SelBatch fut, int w0,
+                                                                                    int nwb) {
+  const int f = w0 + blockIdx.x;
+  const int g = f / nwb;
+  if (g < fut.cnt) sel_w_work(fut.a[g], f - g * nwb);
 }
 
 // X_ii = S - Wa^T X_{a,i} on lower tile (I, J) (S alone without the arrowhead), written to D at
@@ -1485,12 +1608,11 @@ __device__ __forceinline__ void sel_sym_work(const SelArgs& a, double* D, int e)
 
 __global__ __launch_bounds__(256) void k_sel_sym(SelArgs a, double* D) { sel_sym_work(a, D, blockIdx.x); }
 
-// k_sel_sym of block i with inv(L)^T inv(L) of a later block (fut) in its spare workgroups.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sel_sym_ll(SelArgs a, double* D,
-                                                                                        int nsym, SelBatch fut) {
+// k_sel_sym of two independent blocks in one launch; workgroups e < nsym are a0's.
+__global__ __launch_bounds__(256) void k_sel_sym2(SelArgs a0, double* D0, int nsym, SelArgs a1, double* D1) {
   const int e = blockIdx.x;
-  if (e < nsym) sel_sym_work(a, D, e);
-  else if (fut.cnt > 0) sel_ll_work(fut.a[0], e - nsym);
+  if (e < nsym) sel_sym_work(a0, D0, e);
+  else sel_sym_work(a1, D1, e - nsym);
 }
 
 // ---------------------------------------------------------------------------
@@ -2167,6 +2289,18 @@ struct dh_handle {
   int fp_ready;                  // sparsity whose factor products are in the per-block slots, -1 none
   int force_giveup;              // 1: dh_cholesky starts with the give-up flag raised (tuning key 5)
   int dag_cap;                   // at most this many persistent workgroups (tuning key 6), 0: no cap
+  // two-ended factorization (tuning key 7): blocks 0 .. m-1 are eliminated downward and blocks
+  // n-1 .. m+1 upward, both in one persistent launch, and block m last; the sub-diagonal blocks
+  // m .. n-2 then hold the transposed coupling (rows of the upper block)
+  int twist;                     // 1: two-ended when the tile-task schedule runs and n >= 3
+  int tw_m;                      // the junction block m = (n - 1) / 2
+  int fact_tw;                   // junction of the current factor, -1: one-ended
+  int asm_tw;                    // 1: dh_assemble wrote blocks m .. n-2 transposed for the next factorization
+  DagArgs tw_g[2];               // the two-sided launch, per arrow flag
+  int2* tw_order[2];             // its merged task order (device)
+  int tw_ntotal[2];
+  double* tipB;                  // 64 x 64 tip partial of the upward instance
+  int sel_slots;                 // workgroups of k_sel_s2w resident at once on the device
 };
 
 #define DH_MAXREC 16384
@@ -2182,6 +2316,11 @@ struct dh_handle {
       (h)->nrec++;                                                                            \
     }                                                                                         \
   } while (0)
+
+// The two-ended factorization runs for this handle's next dh_cholesky.
+static bool twist_active(const dh_handle* h) {
+  return h->twist && h->tw_order[0] && h->n >= 3 && h->pobtaf_alg == 1 && h->overlap == 0;
+}
 
 static void build_nodes(int o, int s, int d, std::vector<NodeList>& lv) {
   if (s <= 1) return;
@@ -2289,6 +2428,12 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
   h->fp_ready = -1;
   h->force_giveup = 0;
   h->dag_cap = 0;
+  h->twist = 0;
+  h->tw_m = (n - 1) / 2;
+  h->fact_tw = -1;
+  h->asm_tw = 0;
+  h->tw_order[0] = h->tw_order[1] = nullptr;
+  h->tw_ntotal[0] = h->tw_ntotal[1] = 0;
   {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -2298,6 +2443,10 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pobtaf_dag, 256, 0) != hipSuccess || per_cu < 1)
       per_cu = 1;
     h->dag_wgs = per_cu * cus;
+    int per_cu_s = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, k_sel_s2w, 256, 0) != hipSuccess || per_cu_s < 1)
+      per_cu_s = 1;
+    h->sel_slots = per_cu_s * cus;
   }
   const long long bb = (long long)h->bp * h->bp;
   h->G = n < 8 ? n : 8;
@@ -2322,9 +2471,10 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
   rc |= dalloc(h, &h->pinv, sb * 64 * 64);
   rc |= dalloc(h, &h->tipinv, sb * 64 * 64);
   rc |= dalloc(h, &h->tipdinv, sb * 64);
-  rc |= dalloc(h, &h->W, sb * h->G * h->bp * (h->bp + 8));  // pobtasi W slots (padded pitch)
-  rc |= dalloc(h, &h->S, sb * h->G * bb);
-  rc |= dalloc(h, &h->Wa, sb * h->G * 64LL * h->bp);
+  // pobtasi W (padded pitch), S and Wa slots: two sets of G blocks (a chunk and the next one)
+  rc |= dalloc(h, &h->W, sb * 2 * h->G * h->bp * (h->bp + 8));
+  rc |= dalloc(h, &h->S, sb * 2 * h->G * bb);
+  rc |= dalloc(h, &h->Wa, sb * 2 * h->G * 64LL * h->bp);
   rc |= dalloc(h, &h->z, (long long)n * h->bp + 64);
   rc |= dalloc(h, &h->v, h->bp);
   h->part_len = (long long)(2 * n) * ((h->bp + 255) / 256 + 1) * 4 + 4096 +
@@ -2332,7 +2482,8 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
   rc |= dalloc(h, &h->part, h->part_len);
   rc |= dalloc(h, &h->arrpart, (long long)n * 64);
   rc |= dalloc(h, &h->eigbuf, 2LL * n * h->bp);
-  rc |= dalloc(h, &h->dagpinv, sb * 2 * h->nbt * 64 * 64);
+  rc |= dalloc(h, &h->dagpinv, sb * 4 * h->nbt * 64 * 64);  // [instance][parity][nbt][64][64]
+  rc |= dalloc(h, &h->tipB, sb * 64 * 64);
   rc |= dalloc(h, &h->qinv, sb * n * bb);
   if (hipMalloc((void**)&h->info, sizeof(int)) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->flags, sizeof(int) * ((size_t)n * h->nbt + 1)) != hipSuccess) rc = DH_EHIP;
@@ -2371,7 +2522,7 @@ int dh_destroy(dh_handle* h) {
   if (!h) return DH_EARG;
   double* ps[] = {h->diag, h->lower, h->arrow, h->tip, h->dinv, h->pinv, h->tipinv, h->tipdinv,
                   h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->eigbuf, h->dagpinv, h->qinv,
-                  h->Wall, h->Sall, h->Waall};
+                  h->Wall, h->Sall, h->Waall, h->tipB};
   for (double* p : ps)
     if (p) (void)hipFree(p);
   if (h->info) (void)hipFree(h->info);
@@ -2380,6 +2531,8 @@ int dh_destroy(dh_handle* h) {
   if (h->dagflags) (void)hipFree(h->dagflags);
   if (h->dagcounter) (void)hipFree(h->dagcounter);
   if (h->dagtab) (void)hipFree(h->dagtab);
+  for (int ar = 0; ar < 2; ++ar)
+    if (h->tw_order[ar]) (void)hipFree(h->tw_order[ar]);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->sev)
     if (e) (void)hipEventDestroy(e);
@@ -2420,9 +2573,12 @@ int dh_assemble(dh_handle* h, const dh_model* m, const double* params, int spars
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
   if (sparsity == DH_SPARSITY_BTA && m->nb != h->a) return DH_EARG;
   const long rows = (long)(2 * h->n - 1) * h->bp;
+  // for a two-ended factorization the upward side's coupling blocks are written transposed
+  const bool tw = twist_active(h);
   DH_LAUNCH(h, 5, k_assemble_rows, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double), h->diag,
-            h->lower, h->n, h->n - 1, 0, h->b, h->bp, *m, params, with_like);
+            h->lower, h->n, h->n - 1, 0, h->b, h->bp, *m, params, with_like, tw ? h->tw_m : h->n);
   HIPCHK(hipGetLastError());
+  h->asm_tw = tw ? 1 : 0;
   if (h->a > 0) {
     const int wl = (sparsity == DH_SPARSITY_BTA) ? with_like : 0;
     hipLaunchKernelGGL(k_assemble_arrow, dim3(h->n * h->ap), dim3(256), 0, h->stream, h->arrow, h->n, 0, h->b, h->a,
@@ -2450,6 +2606,7 @@ int dh_assemble_coreg(dh_handle* h, const dh_model* m, int nv, const double* par
   DH_LAUNCH(h, 5, k_assemble_rows_coreg, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double), h->diag,
             h->lower, h->n, h->bp, cm, params, with_like);
   HIPCHK(hipGetLastError());
+  h->asm_tw = 0;
   if (h->a > 0) {
     const int wl = (sparsity == DH_SPARSITY_BTA) ? with_like : 0;
     hipLaunchKernelGGL(k_assemble_arrow_coreg, dim3(h->n * h->ap), dim3(256), 0, h->stream, h->arrow, h->n, h->bp,
@@ -2482,6 +2639,7 @@ int dh_scatter_coo(dh_handle* h, long long nnz, const long long* rows, const lon
                        h->lower, h->arrow, h->tip, h->n, h->b, h->bp, h->a, h->ap, (long)nnz, rows, cols, vals,
                        (sparsity == DH_SPARSITY_BTA && h->a > 0) ? 1 : 0);
   HIPCHK(hipGetLastError());
+  h->asm_tw = 0;
   h->factored = -1;
   h->inverted = 0;
   return DH_OK;
@@ -2510,17 +2668,20 @@ static void inverse_chunk(dh_handle* h, int g0, int gc, hipStream_t st, bool sid
   }
 }
 
-// Operands of block i for the selected-inversion kernels, with its W, Wa and S slots.
-static void sel_args(const dh_handle* h, int i, bool arrow, double* W, double* Wa, double* S, SelArgs& sa) {
-  const int n = h->n, bp = h->bp;
+// Operands of block i for the selected-inversion kernels, with its W, Wa and S slots.  succ: the
+// block eliminated after i that it couples to (-1: none), whose coupling block sits at the lower of
+// the two slots; xt: X_{succ,i} is stored transposed (the upward side of a two-ended factor).
+static void sel_args_succ(const dh_handle* h, int i, int succ, bool arrow, double* W, double* Wa, double* S,
+                          int xt, SelArgs& sa) {
+  const int bp = h->bp;
   const long bb = (long)bp * bp;
   sa.D = h->diag + i * bb;
   sa.Q = h->qinv + i * bb;
   sa.dinv = h->dinv + (long)i * bp;
-  sa.Lb = (i < n - 1) ? h->lower + i * bb : nullptr;
+  sa.Lb = succ >= 0 ? h->lower + std::min(i, succ) * bb : nullptr;
   sa.R = arrow ? h->arrow + (long)i * 64 * bp : nullptr;
-  sa.Dn = (i < n - 1) ? h->diag + (i + 1) * bb : nullptr;
-  sa.Rn = (arrow && i < n - 1) ? h->arrow + (long)(i + 1) * 64 * bp : nullptr;
+  sa.Dn = succ >= 0 ? h->diag + succ * bb : nullptr;
+  sa.Rn = (arrow && succ >= 0) ? h->arrow + (long)succ * 64 * bp : nullptr;
   sa.tip = h->tip;
   sa.W = W;
   sa.Wa = Wa;
@@ -2528,6 +2689,11 @@ static void sel_args(const dh_handle* h, int i, bool arrow, double* W, double* W
   sa.bp = bp;
   sa.nbt = h->nbt;
   sa.wld = bp + 8;
+  sa.xt = xt;
+}
+
+static void sel_args(const dh_handle* h, int i, bool arrow, double* W, double* Wa, double* S, SelArgs& sa) {
+  sel_args_succ(h, i, i < h->n - 1 ? i + 1 : -1, arrow, W, Wa, S, 0, sa);
 }
 
 // Per-block slots of the factor products (overlap 2): W (ld bp + 8), Wa, S.
@@ -2585,7 +2751,106 @@ static DagArgs dag_args(const dh_handle* h, int ar, int slot0, int cnt, bool has
   g.has_prev = has_prev ? 1 : 0;
   g.last_full = last_full ? 1 : 0;
   g.row0 = row0;
+  g.dir = 1;
+  g.order = nullptr;
+  g.diag2 = g.lower2 = g.arrow2 = g.tip2 = g.dinv2 = g.pinv2 = nullptr;
+  g.flags2 = nullptr;
+  g.n2 = g.row02 = 0;
+  g.dir2 = 1;
   return g;
+}
+
+// The two-sided launch of the two-ended factorization (per arrow flag) and its merged task order:
+// downward over slots 0 .. m-1 and upward over slots n-1 .. m+1, each side with the sub-diagonal
+// tiles of its last block (the coupling to block m).  Each side keeps its own task order and both
+// progress in proportion, so every task waits only for tasks with smaller tickets.
+static int build_twist(dh_handle* h) {
+  if (h->tw_order[0]) return DH_OK;
+  const int n = h->n, m = h->tw_m, bp = h->bp, nbt = h->nbt;
+  const long bb = (long)bp * bp;
+  if (n < 3 || m < 1 || n - 1 - m < 1) return DH_EARG;
+  for (int ar = 0; ar < 2; ++ar) {
+    DagArgs g = dag_args(h, ar, 0, m, false, true, h->tip, 0);
+    const DagArgs u = dag_args(h, ar, m + 1, n - 1 - m, false, true, h->tipB, (n - 1) * h->b);
+    g.diag2 = h->diag + (n - 1) * bb;
+    g.lower2 = h->lower + (n - 2) * bb;
+    g.arrow2 = h->arrow ? h->arrow + (long)(n - 1) * h->ap * bp : nullptr;
+    g.tip2 = h->tipB;
+    g.dinv2 = h->dinv + (long)(n - 1) * bp;
+    g.pinv2 = h->dagpinv + 2L * nbt * DH_TILE * DH_TILE;
+    g.flags2 = u.flags;
+    g.n2 = u.n;
+    g.row02 = u.row0;
+    g.dir2 = -1;
+    struct Item {
+      double key;
+      int side, task;
+    };
+    std::vector<Item> items;
+    for (int j = 0; j < g.ntasks; ++j) items.push_back(Item{(j + 0.5) / g.ntasks, 0, j});
+    for (int j = 0; j < u.ntasks; ++j) items.push_back(Item{(j + 0.5) / u.ntasks, 1, j});
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.key < y.key; });
+    std::vector<int2> ord(items.size());
+    for (size_t e = 0; e < items.size(); ++e) ord[e] = make_int2(items[e].side, items[e].task);
+    h->tw_ntotal[ar] = (int)ord.size();
+    g.ntasks = (int)ord.size();
+    if (hipMalloc((void**)&h->tw_order[ar], sizeof(int2) * ord.size()) != hipSuccess ||
+        hipMemcpy(h->tw_order[ar], ord.data(), sizeof(int2) * ord.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      for (int q = 0; q < 2; ++q) {
+        if (h->tw_order[q]) (void)hipFree(h->tw_order[q]);
+        h->tw_order[q] = nullptr;
+      }
+      return DH_EHIP;
+    }
+    g.order = h->tw_order[ar];
+    h->tw_g[ar] = g;
+  }
+  return DH_OK;
+}
+
+// Sub-diagonal blocks m .. n-2 transposed in place (into or out of the two-ended layout).
+static void twist_lower(dh_handle* h) {
+  const int n = h->n, m = h->tw_m, nbt = h->nbt;
+  hipLaunchKernelGGL(k_transpose_blocks, dim3(nbt * (nbt + 1) / 2, n - 1 - m), dim3(256), 0, h->stream,
+                     h->lower + (long)m * h->bp * h->bp, h->bp);
+}
+
+// The two-ended factorization: both sides in one persistent launch, the upward side's terms
+// applied to block m, block m factored (the downward side's terms through has_prev), then the tip.
+static int cholesky_twisted(dh_handle* h, int sparsity, int ar) {
+  const int n = h->n, m = h->tw_m, bp = h->bp, nbt = h->nbt, FS = dag_fstride(nbt);
+  const long bb = (long)bp * bp;
+  if (!h->asm_tw) twist_lower(h);
+  h->asm_tw = 0;
+  hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags, 0, n * FS + n);
+  if (ar) hipLaunchKernelGGL(k_fill_dbl, dim3(16), dim3(256), 0, h->stream, h->tipB, 0.0, 64 * 64);
+  const bool timed = h->tkind == 7 && h->nrec < DH_MAXREC;
+  if (timed) (void)hipEventRecord(h->ev[2 * h->nrec], h->stream);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
+  const int cap = h->dag_cap > 0 ? std::min(h->dag_cap, h->dag_wgs) : h->dag_wgs;
+  hipLaunchKernelGGL(k_pobtaf_dag, dim3((unsigned)std::min(h->tw_ntotal[ar], cap)), dim3(256), 0, h->stream,
+                     h->tw_g[ar]);
+  const int ntri = nbt * (nbt + 1) / 2;
+  hipLaunchKernelGGL(k_twist_join, dim3(ntri + (ar ? nbt + 1 : 0)), dim3(256), 0, h->stream,
+                     (const double*)(h->lower + m * bb), (const double*)(ar ? h->arrow + (long)(m + 1) * 64 * bp : nullptr),
+                     h->diag + m * bb, ar ? h->arrow + (long)m * 64 * bp : nullptr, (const double*)h->tipB, h->tip,
+                     nbt, bp);
+  hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, h->stream, h->dagflags + (long)(m - 1) * FS, 1, FS);
+  DagArgs g = dag_args(h, ar, m, 1, true, false, h->tip, m * h->b);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
+  hipLaunchKernelGGL(k_pobtaf_dag, dim3((unsigned)std::min<int>(g.ntasks, cap)), dim3(256), 0, h->stream, g);
+  if (timed) {
+    (void)hipEventRecord(h->ev[2 * h->nrec + 1], h->stream);
+    h->nrec++;
+  }
+  if (ar)
+    hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, h->stream, h->tip, (long)h->ap, h->tipdinv, h->tipinv,
+                       h->info, n * h->b);
+  HIPCHK(hipGetLastError());
+  h->factored = sparsity;
+  h->inverted = 0;
+  h->fact_tw = m;
+  return DH_OK;
 }
 
 // The launch itself (the flags of its blocks must be clear, those of a previous block raised).
@@ -2607,6 +2872,12 @@ int dh_cholesky(dh_handle* h, int sparsity) {
   // force_giveup: the flag that a timed-out readiness wait raises is set before the launch, so
   // that every wait of the persistent launch returns at once (tuning key 5, a test of the error path)
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->err, h->force_giveup ? 4 : 0);
+  h->fact_tw = -1;
+  if (twist_active(h)) return cholesky_twisted(h, sparsity, arrow ? 1 : 0);
+  if (h->asm_tw) {  // assembled for a two-ended factorization that is no longer selected
+    twist_lower(h);
+    h->asm_tw = 0;
+  }
   if (h->pobtaf_alg == 1) {
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     const bool ov = h->overlap > 0 && h->side && hipStreamIsCapturing(h->stream, &cap) == hipSuccess &&
@@ -2715,24 +2986,31 @@ static int ensure_inverse(dh_handle* h) {
 
 }  // extern "C"
 
-// Forward sweep of pobtas over blocks slot0 .. slot0 + cnt - 1 of the handle's factor:
-// y_i = inv(L_ii) (b_i - L_{i,i-1} y_{i-1}) in place in z (padded, block pitch bp; z[0] is block
-// slot0's part, z[-bp..] the previous block's y when has_prev), and za -= sum_i L_{a,i} y_i.
-static void solve_forward(dh_handle* h, double* z, int slot0, int cnt, bool has_prev, bool arrow, double* za) {
+// Forward sweep of pobtas over blocks slot0, slot0 + dir, ..., cnt blocks of the handle's factor
+// (dir = -1: the upward side of a two-ended factor): y_i = inv(L_ii) (b_i - L_{i,p} y_p) with p the
+// previous block, in place in z (padded, block pitch bp; z[0] is block slot0's part, z[-dir bp..]
+// the previous block's y when has_prev), and za -= sum_i L_{a,i} y_i.  The coupling of two
+// neighbouring slots sits at the lower one.  L2, z2: a second previous block for block slot0
+// (the junction of a two-ended factor), y -= L2 z2.
+static void solve_forward(dh_handle* h, double* z, int slot0, int cnt, bool has_prev, bool arrow, double* za,
+                          int dir = 1, const double* L2 = nullptr, const double* z2 = nullptr) {
   const int bp = h->bp;
   const long bb = (long)bp * bp;
   const int ncb = (bp + 127) / 128;  // column blocks of the partial GEMVs
   const int nfb = (bp + 63) / 64;    // column blocks of k_fin_col
   const int nch = (bp + DH_CCH - 1) / DH_CCH;
   for (int l = 0; l < cnt; ++l) {
-    const int i = slot0 + l;
-    double* zi = z + (long)l * bp;
+    const int i = slot0 + dir * l;
+    double* zi = z + (long)dir * l * bp;
     if (l > 0 || has_prev) {
-      hipLaunchKernelGGL(k_gemv_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, h->lower + (i - 1) * bb, (long)bp,
-                         bp, bp, zi - bp, zi, h->v, -1.0);
+      hipLaunchKernelGGL(k_gemv_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream,
+                         h->lower + std::min(i, i - dir) * bb, (long)bp, bp, bp, zi - dir * bp, zi, h->v, -1.0);
     } else {
       hipLaunchKernelGGL(k_copy, dim3((bp + 255) / 256), dim3(256), 0, h->stream, zi, h->v, bp);
     }
+    if (l == 0 && L2)
+      hipLaunchKernelGGL(k_gemv_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, L2, (long)bp, bp, bp, z2,
+                         (const double*)h->v, h->v, -1.0);
     hipLaunchKernelGGL(k_gemv_col_part, dim3(ncb, nch), dim3(256), 0, h->stream, h->diag + i * bb, (long)bp, bp,
                        bp, (const double*)h->v, h->part, 1);
     hipLaunchKernelGGL(k_fin_col, dim3(nfb), dim3(256), 0, h->stream, h->part, nch, bp, (const double*)h->v,
@@ -2750,23 +3028,26 @@ static void solve_tip(dh_handle* h, double* za) {
   hipLaunchKernelGGL(k_tip_apply, dim3(1), dim3(64), 0, h->stream, h->tipinv, h->v, za, 1);
 }
 
-// Backward sweep over blocks slot0 + cnt - 1 down to slot0: x_i = inv(L_ii)^T (y_i - L_{i+1,i}^T x_{i+1}
-// - L_{a,i}^T x_a), z as for solve_forward (z[cnt * bp..] holds x of the next block when has_next).
-static void solve_backward(dh_handle* h, double* z, int slot0, int cnt, bool has_next, bool arrow, const double* za) {
+// Backward sweep over the blocks of solve_forward in reverse: x_i = inv(L_ii)^T (y_i - L_{q,i}^T x_q
+// - L_{a,i}^T x_a) with q the next block (slot i + dir), z as for solve_forward (z[dir cnt bp..]
+// holds x of the next block when has_next).
+static void solve_backward(dh_handle* h, double* z, int slot0, int cnt, bool has_next, bool arrow, const double* za,
+                           int dir = 1) {
   const int bp = h->bp;
   const long bb = (long)bp * bp;
   const int ncb = (bp + 127) / 128;
   const int nfb = (bp + 63) / 64;
   const int nch = (bp + DH_CCH - 1) / DH_CCH;
   for (int l = cnt - 1; l >= 0; --l) {
-    const int i = slot0 + l;
-    double* zi = z + (long)l * bp;
+    const int i = slot0 + dir * l;
+    double* zi = z + (long)dir * l * bp;
     const bool hasL = l < cnt - 1 || has_next;
     if (hasL || arrow) {
       const int nch1 = hasL ? nch : 0;
       const int nch2 = arrow ? 1 : 0;
       hipLaunchKernelGGL(k_gemv_col_part2, dim3(ncb, nch1 + nch2), dim3(256), 0, h->stream,
-                         (const double*)(hasL ? h->lower + i * bb : h->diag), hasL ? bp : 0, (const double*)(zi + bp),
+                         (const double*)(hasL ? h->lower + std::min(i, i + dir) * bb : h->diag), hasL ? bp : 0,
+                         (const double*)(zi + dir * bp),
                          (const double*)(arrow ? h->arrow + (long)i * h->ap * bp : h->diag), arrow ? 64 : 0, za,
                          (long)bp, bp, h->part);
       hipLaunchKernelGGL(k_fin_col, dim3(nfb), dim3(256), 0, h->stream, h->part, nch1 + nch2, bp, (const double*)zi,
@@ -2793,9 +3074,22 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
   hipLaunchKernelGGL(k_pack, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, rhs, h->z, n, b, bp, a,
                      64);
   double* za = h->z + (long)n * bp;
-  solve_forward(h, h->z, 0, n, false, arrow, za);
-  if (arrow) solve_tip(h, za);
-  solve_backward(h, h->z, 0, n, false, arrow, za);
+  const int m = h->fact_tw;
+  if (m >= 0) {  // two-ended factor: both sides inward, the junction block m, and back outward
+    const long bb = (long)bp * bp;
+    double* z = h->z;
+    solve_forward(h, z, 0, m, false, arrow, za);
+    solve_forward(h, z + (long)(n - 1) * bp, n - 1, n - 1 - m, false, arrow, za, -1);
+    solve_forward(h, z + (long)m * bp, m, 1, true, arrow, za, 1, h->lower + m * bb, z + (long)(m + 1) * bp);
+    if (arrow) solve_tip(h, za);
+    solve_backward(h, z + (long)m * bp, m, 1, false, arrow, za);
+    solve_backward(h, z, 0, m, true, arrow, za);
+    solve_backward(h, z + (long)(n - 1) * bp, n - 1, n - 1 - m, true, arrow, za, -1);
+  } else {
+    solve_forward(h, h->z, 0, n, false, arrow, za);
+    if (arrow) solve_tip(h, za);
+    solve_backward(h, h->z, 0, n, false, arrow, za);
+  }
   const long Nl = (long)n * b + a;
   hipLaunchKernelGGL(k_unpack, dim3((unsigned)((Nl + 255) / 256)), dim3(256), 0, h->stream, h->z, rhs, n, b, bp, a);
   HIPCHK(hipGetLastError());
@@ -2903,6 +3197,95 @@ static void selinv_tip(dh_handle* h) {
                                 KHI_K, 1.0, 0.0, h->tip, 64, 1, 1, 0));
 }
 
+// One block of the Takahashi recursion: block i, the block eliminated after it that it couples to
+// (-1: none) and whether X_{succ,i} is stored transposed.
+struct SelBlk {
+  int i, succ, xt;
+};
+
+// The recursion as a sequence of steps (step s: blocks blk[step0[s] .. step0[s+1]), one or two
+// independent ones whose successors are final, their launches paired).  The factor-only products
+// (inv(L)^T inv(L), W = L_{succ,i} inv(L_ii), Wa) of a chunk of steps (at most G blocks) are formed
+// in one of two slot sets before its recursion; the next chunk's W and Wa fill the workgroup slots
+// that the S launches of this chunk leave free (the rest, and its inv(L)^T inv(L), run once the
+// chunk is done).
+static int selinv_steps(dh_handle* h, bool arrow, const std::vector<SelBlk>& blk, const std::vector<int>& step0) {
+  const int bp = h->bp, nbt = h->nbt, G = h->G, wld = bp + 8;
+  const long bb = (long)bp * bp;
+  const int ntri = nbt * (nbt + 1) / 2;
+  const int nwb = (nbt / 2 + (nbt & 1)) * (nbt + (arrow ? 1 : 0));  // W/Wa workgroups per block
+  const int nsteps = (int)step0.size() - 1;
+  std::vector<int> chunk0{0};  // first step of each chunk
+  for (int s = 0, used = 0; s < nsteps; ++s) {
+    const int c = step0[s + 1] - step0[s];
+    if (c < 1 || c > 2 || c > G) return DH_EARG;
+    if (used + c > G) {
+      chunk0.push_back(s);
+      used = 0;
+    }
+    used += c;
+  }
+  chunk0.push_back(nsteps);
+  const int nq = (int)chunk0.size() - 1;
+  auto batch = [&](int q, SelBatch& sb) {
+    memset(&sb, 0, sizeof(sb));
+    const int b0 = step0[chunk0[q]], b1 = step0[chunk0[q + 1]];
+    sb.cnt = b1 - b0;
+    for (int g = 0; g < sb.cnt; ++g) {
+      const SelBlk& k = blk[b0 + g];
+      const long sl = (long)(q & 1) * G + g;
+      sel_args_succ(h, k.i, k.succ, arrow, h->W + sl * bp * wld, h->Wa + sl * 64 * bp, h->S + sl * bb, k.xt,
+                    sb.a[g]);
+    }
+  };
+  SelBatch cur, fut;
+  batch(0, cur);
+  DH_LAUNCH(h, 2, k_lb_transpose, dim3(nbt * nbt, cur.cnt), dim3(256), 0, cur);
+  DH_LAUNCH(h, 2, k_sel_w, dim3(nwb, cur.cnt), dim3(256), 0, cur);
+  DH_LAUNCH(h, 2, k_sel_ll, dim3(ntri, cur.cnt), dim3(256), 0, cur);
+  for (int q = 0; q < nq; ++q) {
+    const bool has_fut = q + 1 < nq;
+    int w0 = 0, wt = 0;
+    if (has_fut) {  // its slot set was last used by chunk q - 1, which is complete
+      batch(q + 1, fut);
+      wt = fut.cnt * nwb;
+      DH_LAUNCH(h, 2, k_lb_transpose, dim3(nbt * nbt, fut.cnt), dim3(256), 0, fut);
+    } else {
+      memset(&fut, 0, sizeof(fut));
+    }
+    const int b0 = step0[chunk0[q]];
+    for (int s = chunk0[q]; s < chunk0[q + 1]; ++s) {
+      const int g0 = step0[s] - b0, c = step0[s + 1] - step0[s];
+      const SelArgs& a0 = cur.a[g0];
+      const SelArgs& a1 = cur.a[g0 + c - 1];
+      double* D0 = h->diag + (long)blk[step0[s]].i * bb;
+      double* D1 = h->diag + (long)blk[step0[s] + c - 1].i * bb;
+      const int x0 = a0.Lb ? nbt * nbt : 0, x1 = (c > 1 && a1.Lb) ? nbt * nbt : 0;
+      if (x0 + x1 > 0)  // X_{succ,i}
+        DH_LAUNCH(h, 2, k_sel_x2, dim3(x0 + x1), dim3(256), 0, x0 ? a0 : a1, a1, x0 ? x0 : x1);
+      const int t0 = a0.Lb ? ntri : 0, t1 = (c > 1 && a1.Lb) ? ntri : 0;
+      const int s0 = t0 + (a0.R ? nbt : 0), s1 = c > 1 ? t1 + (a1.R ? nbt : 0) : 0;
+      const int ns = s0 + s1;  // S -= W^T X_{succ,i}; X_{a,i}
+      int take = 0;
+      if (w0 < wt) {
+        const int spare = ns > 0 ? (ns + h->sel_slots - 1) / h->sel_slots * h->sel_slots - ns : 0;
+        take = std::min(spare, wt - w0);
+      }
+      if (ns + take > 0)
+        DH_LAUNCH(h, 2, k_sel_s2w, dim3(ns + take), dim3(256), 0, a0, t0, s0, a1, t1, ns, fut, w0, nwb);
+      w0 += take;
+      DH_LAUNCH(h, 2, k_sel_sym2, dim3(ntri * c), dim3(256), 0, a0, D0, ntri, a1, D1);  // X_ii
+    }
+    if (has_fut) {
+      if (w0 < wt) DH_LAUNCH(h, 2, k_sel_wr, dim3(wt - w0), dim3(256), 0, fut, w0, nwb);
+      DH_LAUNCH(h, 2, k_sel_ll, dim3(ntri, fut.cnt), dim3(256), 0, fut);
+      cur = fut;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
 // The Takahashi recursion over blocks hi down to lo of the handle's arrays, which hold the
 // factor; block hi + 1 (when hi < n - 1) already holds X_{hi+1,hi+1} and X_{a,hi+1}, and
 // lower[hi] the factor's L_{hi+1,hi}.  Overwrites the blocks with the selected inverse.
@@ -2966,11 +3349,17 @@ static int selinv_range(dh_handle* h, int sparsity, bool arrow, int lo, int hi) 
                 h->diag + (long)(top - g) * bb);  // X_ii, both triangles
     }
   };
-  if (!two) {
-    for (int q = 0; q < nq; ++q) {
-      if (!pre) factor_products(q, h->stream);
-      recursion(q);
+  if (!two && !pre) {
+    std::vector<SelBlk> blk;
+    std::vector<int> step0{0};
+    for (int i = hi; i >= lo; --i) {
+      blk.push_back(SelBlk{i, i < n - 1 ? i + 1 : -1, 0});
+      step0.push_back((int)blk.size());
     }
+    return selinv_steps(h, arrow, blk, step0);
+  }
+  if (!two) {
+    for (int q = 0; q < nq; ++q) recursion(q);
   } else {
     hipEvent_t fork = h->sev[0], ready[2] = {h->sev[1], h->sev[2]}, freed[2] = {h->sev[3], h->sev[4]};
     HIPCHK(hipEventRecord(fork, h->stream));  // the factor and X_aa are complete
@@ -2994,6 +3383,22 @@ static int selinv_range(dh_handle* h, int sparsity, bool arrow, int lo, int hi) 
   return DH_OK;
 }
 
+// The Takahashi recursion of a two-ended factor with junction m: block m first (it has no
+// successor), then the two sides outward in lockstep, step k taking block m - k (successor
+// m - k + 1) and block m + k (successor m + k - 1, X stored transposed so that the result is in
+// the one-ended layout).
+static int selinv_twisted(dh_handle* h, bool arrow) {
+  const int n = h->n, m = h->fact_tw;
+  std::vector<SelBlk> blk{SelBlk{m, -1, 0}};
+  std::vector<int> step0{0, 1};
+  for (int k = 1; k <= std::max(m, n - 1 - m); ++k) {
+    if (k <= m) blk.push_back(SelBlk{m - k, m - k + 1, 0});
+    if (m + k <= n - 1) blk.push_back(SelBlk{m + k, m + k - 1, 1});
+    step0.push_back((int)blk.size());
+  }
+  return selinv_steps(h, arrow, blk, step0);
+}
+
 extern "C" {
 
 int dh_selected_inversion(dh_handle* h, int sparsity) {
@@ -3004,7 +3409,8 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   if (rc) return rc;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
   if (arrow) selinv_tip(h);
-  rc = selinv_range(h, sparsity, arrow, 0, h->n - 1);
+  rc = h->fact_tw >= 0 ? selinv_twisted(h, arrow) : selinv_range(h, sparsity, arrow, 0, h->n - 1);
+  h->fact_tw = -1;
   h->factored = -1;
   h->inverted = 0;
   h->fp_ready = -1;
@@ -3208,6 +3614,14 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
   else if (key == 3 && (value == 0 || value == 1)) h->side_on = value;
   else if (key == 5 && (value == 0 || value == 1)) h->force_giveup = value;
   else if (key == 6 && value >= 0) h->dag_cap = value;
+  else if (key == 7 && (value == 0 || value == 1)) {
+    if (value == 1) {
+      if (!h->diag || h->n < 3) return DH_EARG;
+      const int rc = build_twist(h);
+      if (rc != DH_OK) return rc;
+    }
+    h->twist = value;
+  }
   else if (key == 4 && value >= 0 && value <= 2) {
     if (value == 2 && !h->Wall) {  // per-block slots, kept until the handle is destroyed
       const long long bb = (long long)h->bp * h->bp, before = h->bytes;
@@ -3305,7 +3719,7 @@ static int ckpt_assemble(dh_ckpt* c, const dh_model* m, const double* params, bo
   const int nl = cnt - 1 + ((t0 + cnt < c->n) ? 1 : 0);
   const long rows = (long)(cnt + nl) * h->bp;
   DH_LAUNCH(h, 5, k_assemble_rows, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double),
-            h->diag + slot0 * bb, h->lower + slot0 * bb, cnt, nl, t0, h->b, h->bp, *m, params, bta ? 1 : 0);
+            h->diag + slot0 * bb, h->lower + slot0 * bb, cnt, nl, t0, h->b, h->bp, *m, params, bta ? 1 : 0, nl);
   if (bta && h->a > 0)
     hipLaunchKernelGGL(k_assemble_arrow, dim3(cnt * h->ap), dim3(256), 0, h->stream,
                        h->arrow + (long)slot0 * h->ap * h->bp, cnt, t0, h->b, h->a, h->bp, h->ap, *m, params, 1);
@@ -3393,7 +3807,8 @@ int dh_ckpt_timing_read(dh_ckpt* c, double* total_ms, long long* count) {
 }
 
 int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value) {
-  if (!c || key == 4) return DH_EARG;  // nothing runs beside the segment factorizations
+  // nothing runs beside the segment factorizations, and segments are factored one-ended
+  if (!c || key == 4 || key == 7) return DH_EARG;
   return dh_set_tuning(c->ring, key, value);
 }
 
@@ -3604,6 +4019,12 @@ int dh_multi_cholesky(dh_multi* mc, int sparsity, void* hip_stream) {
   hipStream_t st = (hipStream_t)hip_stream;
   for (dh_handle* h : mc->hs) {
     h->fp_ready = -1;
+    h->fact_tw = -1;
+    if (h->asm_tw) {  // assembled for a two-ended factorization: back to the one-ended layout
+      hipLaunchKernelGGL(k_transpose_blocks, dim3(h->nbt * (h->nbt + 1) / 2, h->n - 1 - h->tw_m), dim3(256), 0,
+                         st, h->lower + (long)h->tw_m * h->bp * h->bp, h->bp);
+      h->asm_tw = 0;
+    }
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->info, INT_MAX);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->err, h->force_giveup ? 4 : 0);
     hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, st, h->dagflags, 0, h->n * dag_fstride(h->nbt) + h->n);

@@ -43,11 +43,15 @@ from dalia_amd.solver import HipBTASolver
 
 
 class STInla:
-    def __init__(self, data, device="cuda", closed_form_prior: bool = True, overlap: int = 0):
+    def __init__(self, data, device="cuda", closed_form_prior: bool = True, overlap: int = 0,
+                 twisted: bool = True):
         """overlap: work run beside each factorization on a second stream (`HipBTASolver.set_overlap`),
         at most this mode: 0 none (default), 1 the inverse of the diagonal factors, 2 also the
         selected inversion's products of the factor.  Measured at ns = 2025, nt = 100: the adjoint
-        gradient gains about 2 % with 2, as the factorization slows down beside that work."""
+        gradient gains about 2 % with 2, as the factorization slows down beside that work.
+        twisted: two-ended factorizations (`HipBTASolver.set_twisted`, default, for nt >= 3;
+        evaluations with overlap > 0 stay one-ended).  Measured at ns = 2025, nt = 100: pobtaf
+        77.9 -> 70.7 ms, the adjoint gradient 169.1 -> 162.3 ms."""
         self.data = data
         self.manifold = getattr(data, "manifold", "plane")
         if self.manifold not in hyper.MANIFOLDS:
@@ -57,6 +61,9 @@ class STInla:
                                    arrowhead_blocksize=data.nb)
         self.lib = self.solver.lib
         self.h = self.solver.h
+        self.twisted = bool(twisted) and data.nt >= 3
+        if self.twisted:
+            self.solver.set_twisted(True)
         N = self.dm.n_latent
         self.mu = torch.zeros(N, dtype=torch.float64, device="cuda")
         self.yp = torch.zeros(N, dtype=torch.float64, device="cuda")

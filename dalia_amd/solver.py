@@ -206,6 +206,14 @@ class HipBTASolver:
         _lib.check(rc, "dh_set_tuning")
         return True
 
+    def set_twisted(self, on: bool = True) -> None:
+        """Two-ended factorization for the next `cholesky` calls (`dh_set_tuning` key 7): blocks
+        0 .. m-1 eliminated downward and n-1 .. m+1 upward in one persistent launch, meeting at block
+        m = (n - 1) // 2, which halves the chain of diagonal tiles.  logdet, solve and the selected
+        inverse are those of the one-ended factor; the factor blocks themselves differ (the buffers
+        then hold a factor of the matrix with the blocks after m in reverse order).  Needs n >= 3."""
+        _lib.check(self.lib.dh_set_tuning(self.h, 7, 1 if on else 0), "dh_set_tuning")
+
     def selected_inversion(self, sparsity: str = "bta") -> None:
         _lib.check(self.lib.dh_selected_inversion(self.h, _check_sparsity(sparsity)), "dh_selected_inversion")
 

@@ -207,7 +207,7 @@ def prior_transform(c0, g1, m0, m1, m2, theta, z: np.ndarray, manifold: str = "p
     With c0 diagonal, g2 = g1 c0^-1 g1 and g3 = g1 c0^-1 g2, Qp = (I (x) V) blkdiag_j(T_j) (I (x) V^T)
     after ordering by spatial eigenvector, V = c0^1/2 U, U diag(lam) U^T = c0^-1/2 g1 c0^-1/2 and
     T_j = sum_k p_k(kappa^2 + lam_j) M_k tridiagonal; so x = (I (x) c0^-1/2 U) w with
-    w_j = L_j^-T z_j, T_j = L_j L_j^T.  z: (nt * ns,) standard normal."""
+    w_j = L_j^-T (U^T z)_j, T_j = L_j L_j^T.  z: (nt * ns,) standard normal."""
     from dalia_amd.hyper import kron_coefficients
     ns, nt = c0.shape[0], m0.shape[0]
     ci = 1.0 / np.sqrt(sps.csr_matrix(c0).diagonal())
@@ -229,7 +229,10 @@ def prior_transform(c0, g1, m0, m1, m2, theta, z: np.ndarray, manifold: str = "p
         L[:, t] = np.sqrt(v)
         if t < nt - 1:
             O[:, t] = e[:, t] / L[:, t]
-    Z = np.asarray(z, dtype=np.float64).reshape(nt, ns).T  # (ns, nt), row j = eigenvector j's series
+    # the noise is mapped into eigen coordinates (Z = U^T z) rather than drawn there: x = c0^-1/2 U
+    # L(lam)^-T U^T z then depends on the eigenvalues alone, not on the basis LAPACK returns for a
+    # degenerate eigenspace (a regular grid has many) or on the eigenvectors' signs
+    Z = U.T @ np.asarray(z, dtype=np.float64).reshape(nt, ns).T  # (ns, nt)
     W = np.zeros_like(Z)
     for t in range(nt - 1, -1, -1):  # L_j^T w = z
         r = Z[:, t] - (O[:, t] * W[:, t + 1] if t < nt - 1 else 0.0)

@@ -246,7 +246,8 @@ long long dh_ckpt_memory_bytes(const dh_ckpt* c);
 dh_handle* dh_ckpt_vectors(dh_ckpt* c);
 int dh_ckpt_set_stream(dh_ckpt* c, void* hip_stream);
 int dh_ckpt_get_info(dh_ckpt* c, int* info);
-/* dh_set_tuning on the context's segment factorizations (keys 0, 2, 3, 5; key 4 is refused). */
+/* dh_set_tuning on the context's segment factorizations (keys 0, 2, 3, 5, 6; keys 4 and 7 are
+ * refused). */
 int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value);
 /* dh_timing_enable / dh_timing_read on the context's segment launches. */
 int dh_ckpt_timing_enable(dh_ckpt* c, int kind);
@@ -301,6 +302,16 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
  * key 6: at most this many workgroups in the persistent launch (0 = one per CU that fits, the
  *        default); a factorization bound by its chain of diagonal tiles needs fewer, and leaves the
  *        other CUs to concurrent work of other handles (other streams).
+ * key 7: 1 = two-ended factorization (n >= 3, schedule 1, key 4 = 0; DH_EARG for n < 3): blocks
+ *        0 .. m-1 are eliminated downward and blocks n-1 .. m+1 upward in one persistent launch,
+ *        m = (n - 1) / 2, then block m and the tip, which halves the chain of diagonal tiles.
+ *        The handle's buffers then hold the Cholesky factor of the matrix with its blocks ordered
+ *        0 .. m-1, n-1 .. m+1, m: diag[i] and arrow[i] the factor of block i, lower[i] (i >= m)
+ *        the coupling of blocks i and i + 1 with the rows of block i.  dh_logdet, dh_solve and
+ *        dh_selected_inversion (whose result is in the usual layout) follow that factor;
+ *        dh_assemble writes blocks m .. n-2 of the sub-diagonal transposed for it, and a
+ *        factorization that is not two-ended (or dh_multi_cholesky) transposes them back
+ *        first.  0 = one-ended (default).
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 

