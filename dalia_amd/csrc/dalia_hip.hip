@@ -681,9 +681,22 @@ struct DagArgs {
   double *diag2, *lower2, *arrow2, *tip2, *dinv2, *pinv2;
   int* flags2;
   int n2, row02, dir2;
+  // transposed copies of the final operand tiles (cols): diagT/lowerT blocks bp x bp, arrowT
+  // blocks bp x 64, with the slot offsets of diag/lower/arrow; the second side's likewise
+  double *diagT, *lowerT, *arrowT, *diagT2, *lowerT2, *arrowT2;
+  int cols;  // 1: the launch runs the COLS variant (products read the transposed copies by columns)
 };
 
 #define DH_DAG_SPIN (1 << 20)
+// COLS (a template parameter of the tile-task code): every final tile that is a product operand is
+// also stored transposed (diagT, lowerT, arrowT), and the tasks' products read both operands from
+// those copies by columns (16 consecutive entries of 4 k-rows per load instruction, whole 128-byte
+// lines) instead of along the rows of the factor (16 rows x 16 bytes per instruction).  Faster
+// for the products in bulk; slower on the chain's critical path, whose helpers' single products
+// are latency bound, so a launch bound by its chain reads by rows.
+#define DAG_MODE (COLS ? M_COL : M_ROW)
+#define DAG_OP(P, PT, LDT) (COLS ? (const double*)(PT) : (const double*)(P)), (COLS ? (long)(LDT) : (long)bp)
+
 #ifndef DAG_STAGES
 #define DAG_STAGES 2  // operand ring depth of the task products (measured: 1 -> 77.9 ms, 2 -> 77.2, 3 -> 78.8, 4 -> 80.3, 5 -> 80.9)
 #endif
@@ -743,27 +756,29 @@ __device__ int dag_ready(const int* fa, const int* fb, int k, int kend, int* err
 
 // acc += opA[ra.., k] opB[rb.., k] over tile columns k in [0, kend), each
 // tile column consumed once its flags are set.
-__device__ __forceinline__ void dag_accumulate(dh_d4 acc[2][2], const double* A, int ra, const double* B, int rb,
-                                               int bp, const int* fa, const int* fb, int kend, int* err,
+template <bool COLS>
+__device__ __forceinline__ void dag_accumulate(dh_d4 acc[2][2], const double* A, long lda, int ra, const double* B,
+                                               long ldb, int rb, const int* fa, const int* fb, int kend, int* err,
                                                int* s_n) {
-  const Opnd oa{A, bp, M_ROW, nullptr};
-  const Opnd ob{B, bp, M_ROW, nullptr};
+  const Opnd oa{A, lda, DAG_MODE, nullptr};
+  const Opnd ob{B, ldb, DAG_MODE, nullptr};
   for (int k = 0; k < kend;) {
     const int cnt = dag_ready(fa, fb, k, kend, err, s_n);
-    dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
+    dh_mma<DAG_MODE, DAG_MODE, DAG_STAGES>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
     k += cnt;
   }
 }
 
 // dag_accumulate over tile columns k in [k0, kend).
-__device__ __forceinline__ void dag_accumulate_from(dh_d4 acc[2][2], const double* A, int ra, const double* B,
-                                                    int rb, int bp, const int* fa, const int* fb, int k0, int kend,
-                                                    int* err, int* s_n) {
-  const Opnd oa{A, bp, M_ROW, nullptr};
-  const Opnd ob{B, bp, M_ROW, nullptr};
+template <bool COLS>
+__device__ __forceinline__ void dag_accumulate_from(dh_d4 acc[2][2], const double* A, long lda, int ra,
+                                                    const double* B, long ldb, int rb, const int* fa, const int* fb,
+                                                    int k0, int kend, int* err, int* s_n) {
+  const Opnd oa{A, lda, DAG_MODE, nullptr};
+  const Opnd ob{B, ldb, DAG_MODE, nullptr};
   for (int k = k0; k < kend;) {
     const int cnt = dag_ready(fa, fb, k, kend, err, s_n);
-    dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
+    dh_mma<DAG_MODE, DAG_MODE, DAG_STAGES>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
     k += cnt;
   }
 }
@@ -781,8 +796,9 @@ __device__ __forceinline__ void dag_preload_neg(dh_d4 acc[2][2], const double* C
 
 // C = (-acc) inv(L)^T once the factor of the diagonal tile is final: -acc is
 // staged in LDS before the wait, so that only inv(L) is loaded after it.
-__device__ __forceinline__ void dag_solve_store(double* C, const dh_d4 acc[2][2], const double* pinv, int bp,
-                                                const int* fdiag, int* err, int* s_n) {
+template <bool COLS>
+__device__ __forceinline__ void dag_solve_store(double* C, double* CT, long ldt, const dh_d4 acc[2][2],
+                                                const double* pinv, int bp, const int* fdiag, int* err, int* s_n) {
   dh_store(dh_lS, DH_LDS_LD, 0, 0, acc, -1.0, 0.0, false);
   dag_ready(fdiag, nullptr, 0, 1, err, s_n);  // also orders the LDS stores
   dh_d4 x[2][2];
@@ -790,6 +806,7 @@ __device__ __forceinline__ void dag_solve_store(double* C, const dh_d4 acc[2][2]
   dh_mma<M_ROW, M_ROW, DAG_STAGES>(x, Opnd{dh_lS, DH_LDS_LD, M_ROW, nullptr}, 0, Opnd{pinv, DH_TILE, M_ROW, nullptr}, 0, 0,
                        DH_TILE);
   dh_store(C, bp, 0, 0, x, 1.0, 0.0, false);
+  if constexpr (COLS) dh_store(CT, ldt, 0, 0, x, 1.0, 0.0, true);  // the transposed copy, tile (c, r)
 }
 
 // Publish a finished tile: wave 0 (a wave-uniform branch, so that no lane
@@ -872,7 +889,22 @@ __device__ __forceinline__ void chain_store(double* C, long ldc, const dh_d4 acc
         C[(long)(rbase + bi * 16 + 4 * r) * ldc + (bj ? cb : ca) + (lane & 15)] = acc[bi][bj][r];
 }
 
+// chain_store into the transposed position: C[col][row] (ld ldc).
+__device__ __forceinline__ void chain_store_t(double* C, long ldc, const dh_d4 acc[2][2], int ca, int cb) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int rbase = (w >> 1) * 32 + (lane >> 4);
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        C[(long)((bj ? cb : ca) + (lane & 15)) * ldc + rbase + bi * 16 + 4 * r] = acc[bi][bj][r];
+}
+
 // One task (t wave-uniform); every path ends at the single signal.
+template <bool COLS>
 __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   DagArgs g = g0;
   if (g0.order) {  // uniform: the side and its task, and the side's arrays
@@ -889,6 +921,9 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
       g.n = g0.n2;
       g.row0 = g0.row02;
       g.dir = g0.dir2;
+      g.diagT = g0.diagT2;
+      g.lowerT = g0.lowerT2;
+      g.arrowT = g0.arrowT2;
     }
   }
   const int nbt = g.nbt, bp = g.bp;
@@ -907,6 +942,11 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   const double* Lp = g.lower + (si - g.dir) * bb;  // prev
   const double* Ap = g.arrow + (si - g.dir) * T64;
   double* dinv = g.dinv + si * bp;
+  double* DT = g.diagT + si * bb;     // transposed copies (COLS)
+  double* LoT = g.lowerT + si * bb;
+  double* ArT = g.arrowT + si * T64;
+  const double* LpT = g.lowerT + (si - g.dir) * bb;
+  const double* ApT = g.arrowT + (si - g.dir) * T64;
   double* pinv = g.pinv + (long)(i & 1) * nbt * DH_TILE * DH_TILE;  // this block's dense tile inverses
   int* fD = g.flags + (long)i * FS;
   int* fL = fD + nbt * nbt;
@@ -930,37 +970,43 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   dh_zero(acc);
   int* fdone;
   if (kind == DK_TIP) {
-    dag_accumulate(acc, Ar, 0, Ar, 0, bp, fA, nullptr, nbt, g.err, s_n);
+    dag_accumulate<COLS>(acc, DAG_OP(Ar, ArT, DH_TILE), 0, DAG_OP(Ar, ArT, DH_TILE), 0, fA, nullptr, nbt, g.err, s_n);
     if (i > 0) dag_ready(fTp, nullptr, 0, 1, g.err, s_n);
     dh_store(g.tip, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
     fdone = fT;
   } else if (kind == DK_OLD_D) {  // tile (r, r): previous block's terms and own terms k < kold
     dag_preload_neg(acc, D + r * T64 + r * DH_TILE, bp);
     if (prev)
-      dag_accumulate(acc, Lp, r * DH_TILE, Lp, r * DH_TILE, bp, fLp + r * nbt, nullptr, r == 0 ? nbt - 1 : nbt,
+      dag_accumulate<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), r * DH_TILE, fLp + r * nbt, nullptr,
+                     r == 0 ? nbt - 1 : nbt,
                      g.err, s_n);
-    dag_accumulate(acc, D, r * DH_TILE, D, r * DH_TILE, bp, fD + r * nbt, nullptr, kold, g.err, s_n);
+    dag_accumulate<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), r * DH_TILE, fD + r * nbt, nullptr, kold, g.err,
+                   s_n);
     dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fOD + r;
   } else if (kind == DK_OLD_L) {  // tile (r, r-1): previous block's terms and own terms k < kold
     dag_preload_neg(acc, D + r * T64 + (r - 1) * DH_TILE, bp);
     if (prev)
-      dag_accumulate(acc, Lp, r * DH_TILE, Lp, (r - 1) * DH_TILE, bp, fLp + r * nbt, fLp + (r - 1) * nbt, nbt,
+      dag_accumulate<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), (r - 1) * DH_TILE, fLp + r * nbt,
+                     fLp + (r - 1) * nbt, nbt,
                      g.err, s_n);
-    dag_accumulate(acc, D, r * DH_TILE, D, (r - 1) * DH_TILE, bp, fD + r * nbt, fD + (r - 1) * nbt, kold, g.err,
+    dag_accumulate<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), (r - 1) * DH_TILE, fD + r * nbt,
+                   fD + (r - 1) * nbt, kold, g.err,
                    s_n);
     dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fOL + r;
   } else if (kind == DK_PRE_D) {  // tile (r, r), r >= 1: own terms kold <= k < kpre
     dag_ready(fOD + r, nullptr, 0, 1, g.err, s_n);
     dag_preload_neg(acc, D + r * T64 + r * DH_TILE, bp);
-    dag_accumulate_from(acc, D, r * DH_TILE, D, r * DH_TILE, bp, fD + r * nbt, nullptr, kold, kpre, g.err, s_n);
+    dag_accumulate_from<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), r * DH_TILE, fD + r * nbt, nullptr,
+                        kold, kpre, g.err, s_n);
     dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fPD + r;
   } else if (kind == DK_PRE_L) {  // tile (r, r-1), r >= 1: own terms kold <= k < kpre
     dag_ready(fOL + r, nullptr, 0, 1, g.err, s_n);
     dag_preload_neg(acc, D + r * T64 + (r - 1) * DH_TILE, bp);
-    dag_accumulate_from(acc, D, r * DH_TILE, D, (r - 1) * DH_TILE, bp, fD + r * nbt, fD + (r - 1) * nbt, kold,
+    dag_accumulate_from<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), (r - 1) * DH_TILE, fD + r * nbt,
+                        fD + (r - 1) * nbt, kold,
                         kpre, g.err, s_n);
     dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fPL + r;
@@ -972,11 +1018,14 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
     dh_d4 a2[2][2];
     dag_preload_neg(acc, Pd, bp);
     dag_preload_neg(a2, Pl, bp);
-    const Opnd lr{D + r * T64 + (r - 2) * DH_TILE, bp, M_ROW, nullptr};
+    // L(r, r-2) and L(r-1, r-2), from their transposed copies when COLS
+    const Opnd lr{COLS ? DT + (r - 2) * T64 + r * DH_TILE : D + r * T64 + (r - 2) * DH_TILE, bp, DAG_MODE, nullptr};
+    const Opnd lq{COLS ? DT + (r - 2) * T64 + (r - 1) * DH_TILE : D + (r - 1) * T64 + (r - 2) * DH_TILE, bp, DAG_MODE,
+                  nullptr};
     dag_ready(fD + r * nbt + (r - 2), nullptr, 0, 1, g.err, s_n);
-    dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, lr, 0, lr, 0, 0, DH_TILE);
+    dh_mma<DAG_MODE, DAG_MODE, DAG_STAGES>(acc, lr, 0, lr, 0, 0, DH_TILE);
     dag_ready(fD + (r - 1) * nbt + (r - 2), nullptr, 0, 1, g.err, s_n);
-    dh_mma<M_ROW, M_ROW, DAG_STAGES>(a2, lr, 0, Opnd{D + (r - 1) * T64 + (r - 2) * DH_TILE, bp, M_ROW, nullptr}, 0, 0, DH_TILE);
+    dh_mma<DAG_MODE, DAG_MODE, DAG_STAGES>(a2, lr, 0, lq, 0, 0, DH_TILE);
     dh_store(Pd, bp, 0, 0, acc, -1.0, 0.0, false);
     dh_store(Pl, bp, 0, 0, a2, -1.0, 0.0, false);
     fdone = fQ + r;
@@ -1000,6 +1049,7 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
       int ca, cb;
       chain_solve(x, xa, pv - DH_TILE * DH_TILE, ca, cb);
       chain_store(X, bp, x, ca, cb);  // final L(c, c-1)
+      if constexpr (COLS) chain_store_t(DT + (c - 1) * T64 + c * DH_TILE, bp, x, ca, cb);  // its transposed copy
       chain_store(dh_lV, DH_LDS_LD, x, ca, cb);
       dag_signal(fD + c * nbt + (c - 1));
       const Opnd xs{dh_lV, DH_LDS_LD, M_ROW, nullptr};
@@ -1025,46 +1075,62 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   } else if (kind == DK_OLD_X) {  // diagonal-block tile (r, c): previous block's terms, own terms k < ks
     if (prev || ks > 0) dag_preload_neg(acc, D + r * T64 + c * DH_TILE, bp);
     if (prev)
-      dag_accumulate(acc, Lp, r * DH_TILE, Lp, c * DH_TILE, bp, fLp + r * nbt, fLp + c * nbt, nbt, g.err, s_n);
-    dag_accumulate(acc, D, r * DH_TILE, D, c * DH_TILE, bp, fD + r * nbt, fD + c * nbt, ks, g.err, s_n);
+      dag_accumulate<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), c * DH_TILE, fLp + r * nbt,
+                     fLp + c * nbt, nbt, g.err, s_n);
+    dag_accumulate<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fD + r * nbt, fD + c * nbt,
+                   ks, g.err, s_n);
     if (prev || ks > 0) dh_store(D + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fX + r * nbt + c;
   } else if (kind == DK_OLD_LW) {  // sub-diagonal tile (r, c): own terms k < ks
     dag_preload_neg(acc, Lo + r * T64 + c * DH_TILE, bp);
-    dag_accumulate(acc, Lo, r * DH_TILE, D, c * DH_TILE, bp, fL + r * nbt, fD + c * nbt, ks, g.err, s_n);
+    dag_accumulate<COLS>(acc, DAG_OP(Lo, LoT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fL + r * nbt, fD + c * nbt,
+                   ks, g.err, s_n);
     dh_store(Lo + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fLW + r * nbt + c;
   } else if (kind == DK_OLD_A) {  // arrow tile c: previous block's terms, own terms k < ks
     if (prev || ks > 0) dag_preload_neg(acc, Ar + c * DH_TILE, bp);
-    if (prev) dag_accumulate(acc, Ap, 0, Lp, c * DH_TILE, bp, fAp, fLp + c * nbt, nbt, g.err, s_n);
-    dag_accumulate(acc, Ar, 0, D, c * DH_TILE, bp, fA, fD + c * nbt, ks, g.err, s_n);
+    if (prev)
+      dag_accumulate<COLS>(acc, DAG_OP(Ap, ApT, DH_TILE), 0, DAG_OP(Lp, LpT, bp), c * DH_TILE, fAp, fLp + c * nbt, nbt,
+                     g.err, s_n);
+    dag_accumulate<COLS>(acc, DAG_OP(Ar, ArT, DH_TILE), 0, DAG_OP(D, DT, bp), c * DH_TILE, fA, fD + c * nbt, ks, g.err, s_n);
     if (prev || ks > 0) dh_store(Ar + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fXA + c;
   } else {
     // the early task of the tile exists for every diagonal-block and arrow
     // tile, and for sub-diagonal tiles with c > slook
     double* C;
+    double* CT;  // its transposed position
+    long ldt;
     if (kind == DK_DIAG) {  // tile (r, c), r >= c + 2
       C = D + r * T64 + c * DH_TILE;
+      CT = DT + c * T64 + r * DH_TILE;
+      ldt = bp;
       dag_ready(fX + r * nbt + c, nullptr, 0, 1, g.err, s_n);
       dag_preload_neg(acc, C, bp);
-      dag_accumulate_from(acc, D, r * DH_TILE, D, c * DH_TILE, bp, fD + r * nbt, fD + c * nbt, ks, c, g.err, s_n);
+      dag_accumulate_from<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fD + r * nbt,
+                          fD + c * nbt, ks, c, g.err, s_n);
       fdone = fD + r * nbt + c;
     } else if (kind == DK_LOWER) {  // sub-diagonal block tile (r, c)
       C = Lo + r * T64 + c * DH_TILE;
+      CT = LoT + c * T64 + r * DH_TILE;
+      ldt = bp;
       const int k0 = c > g.slook ? ks : 0;
       if (c > g.slook) dag_ready(fLW + r * nbt + c, nullptr, 0, 1, g.err, s_n);
       dag_preload_neg(acc, C, bp);
-      dag_accumulate_from(acc, Lo, r * DH_TILE, D, c * DH_TILE, bp, fL + r * nbt, fD + c * nbt, k0, c, g.err, s_n);
+      dag_accumulate_from<COLS>(acc, DAG_OP(Lo, LoT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fL + r * nbt,
+                          fD + c * nbt, k0, c, g.err, s_n);
       fdone = fL + r * nbt + c;
     } else {  // arrow tile c
       C = Ar + c * DH_TILE;
+      CT = ArT + (long)c * DH_TILE * DH_TILE;
+      ldt = DH_TILE;
       dag_ready(fXA + c, nullptr, 0, 1, g.err, s_n);
       dag_preload_neg(acc, C, bp);
-      dag_accumulate_from(acc, Ar, 0, D, c * DH_TILE, bp, fA, fD + c * nbt, ks, c, g.err, s_n);
+      dag_accumulate_from<COLS>(acc, DAG_OP(Ar, ArT, DH_TILE), 0, DAG_OP(D, DT, bp), c * DH_TILE, fA, fD + c * nbt, ks, c,
+                          g.err, s_n);
       fdone = fA + c;
     }
-    dag_solve_store(C, acc, pinv + c * DH_TILE * DH_TILE, bp, fD + c * nbt + c, g.err, s_n);
+    dag_solve_store<COLS>(C, CT, ldt, acc, pinv + c * DH_TILE * DH_TILE, bp, fD + c * nbt + c, g.err, s_n);
   }
   if (fdone || g.bdone) dag_finish(fdone, g.bdone ? g.bdone + i : nullptr);  // wave-uniform
 }
@@ -1072,6 +1138,7 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
 // One persistent launch over the factorizations of several handles: ticket t runs task order[t].y
 // of instance order[t].x.  The order keeps every instance's own task order, so each task still
 // waits only on tasks with smaller tickets, and the grid drains as for one instance.
+template <bool COLS>
 __global__ __launch_bounds__(256) void k_pobtaf_dag_multi(const DagArgs* gs, const int2* order, int ntotal,
                                                           int* counter) {
   __shared__ int s_task, s_n;
@@ -1087,7 +1154,7 @@ __global__ __launch_bounds__(256) void k_pobtaf_dag_multi(const DagArgs* gs, con
     const int2 o = order[t];
     const int inst = __builtin_amdgcn_readfirstlane(o.x);
     const int task = __builtin_amdgcn_readfirstlane(o.y);
-    dag_task(gs[inst], task, &s_n);
+    dag_task<COLS>(gs[inst], task, &s_n);
   }
 }
 
@@ -1137,6 +1204,18 @@ __global__ __launch_bounds__(256) void k_transpose_blocks(double* base, int bp) 
   }
 }
 
+// dst = src^T for a rows x cols row-major src (ld lds; dst ld ldd), 64 x 64 tiles through LDS:
+// workgroup (J, I) moves tile (I, J).
+__global__ __launch_bounds__(256) void k_transpose_oop(const double* src, long lds, double* dst, long ldd) {
+  __shared__ double t[DH_TILE][DH_TILE + 1];
+  const int I = blockIdx.y, J = blockIdx.x;
+  const double* p = src + (long)I * DH_TILE * lds + J * DH_TILE;
+  for (int x = threadIdx.x; x < DH_TILE * DH_TILE; x += 256) t[x / DH_TILE][x % DH_TILE] = p[(long)(x / DH_TILE) * lds + x % DH_TILE];
+  __syncthreads();
+  double* q = dst + (long)J * DH_TILE * ldd + I * DH_TILE;
+  for (int x = threadIdx.x; x < DH_TILE * DH_TILE; x += 256) q[(long)(x / DH_TILE) * ldd + x % DH_TILE] = t[x % DH_TILE][x / DH_TILE];
+}
+
 // Junction of the two-ended factorization: the block m where the downward and the upward
 // eliminations meet receives the upward side's terms, D_m -= L L^T on its lower tiles and
 // (arrow) Ar_m -= R L^T, with L the upward side's last coupling (rows of block m) and R that
@@ -1169,6 +1248,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // Tasks are claimed by wave 0 under a wave-uniform branch: lane 0 adds 1 and
 // the other lanes add 0, and every lane writes lane 0's ticket, so the loop
 // contains no lane-divergent branch around its barriers.
+template <bool COLS>
 __global__ __launch_bounds__(256) void k_pobtaf_dag(DagArgs g) {
   __shared__ int s_task, s_n;
   for (;;) {
@@ -1180,7 +1260,7 @@ __global__ __launch_bounds__(256) void k_pobtaf_dag(DagArgs g) {
     const int t = __builtin_amdgcn_readfirstlane(s_task);
     __syncthreads();
     if (t >= g.ntasks) break;
-    dag_task(g, t, &s_n);
+    dag_task<COLS>(g, t, &s_n);
   }
 }
 
@@ -2301,7 +2381,16 @@ struct dh_handle {
   int tw_ntotal[2];
   double* tipB;                  // 64 x 64 tip partial of the upward instance
   int sel_slots;                 // workgroups of k_sel_s2w resident at once on the device
+  double *diagT, *lowerT, *arrowT;  // transposed copies of the factor's operand tiles (reserved on first use)
+  int dag_cols;                  // tile-task operands by columns: 0 never, 1 always, 2 in two-ended launches (key 8)
 };
+
+// Products of the tile-task launches read by columns from transposed copies of the operand tiles.
+static bool cols_for(const dh_handle* h, bool twisted) {
+  return h->diagT && (h->dag_cols == 1 || (h->dag_cols == 2 && twisted));
+}
+
+
 
 #define DH_MAXREC 16384
 
@@ -2395,6 +2484,26 @@ static int dalloc(dh_handle* h, double** p, long long count) {
   return DH_OK;
 }
 
+// The transposed copies (2 n bp^2 + n 64 bp doubles), kept until dh_destroy.
+static int alloc_transposed(dh_handle* h) {
+  if (h->diagT || !h->diag) return DH_OK;
+  const long long bb = (long long)h->bp * h->bp, before = h->bytes;
+  int rc = dalloc(h, &h->diagT, h->n * bb);
+  rc |= dalloc(h, &h->lowerT, (h->n - 1) * bb);
+  rc |= dalloc(h, &h->arrowT, (long long)h->n * h->ap * h->bp);
+  if (rc != DH_OK) {
+    double** ps[] = {&h->diagT, &h->lowerT, &h->arrowT};
+    for (double** q : ps)
+      if (*q) {
+        (void)hipFree(*q);
+        *q = nullptr;
+      }
+    h->bytes = before;
+    return DH_EHIP;
+  }
+  return DH_OK;
+}
+
 extern "C" {
 
 int dh_version(void) { return 1; }
@@ -2434,13 +2543,15 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
   h->asm_tw = 0;
   h->tw_order[0] = h->tw_order[1] = nullptr;
   h->tw_ntotal[0] = h->tw_ntotal[1] = 0;
+  h->diagT = h->lowerT = h->arrowT = nullptr;
+  h->dag_cols = 2;
   {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
       cus = 256;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pobtaf_dag, 256, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pobtaf_dag<true>, 256, 0) != hipSuccess || per_cu < 1)
       per_cu = 1;
     h->dag_wgs = per_cu * cus;
     int per_cu_s = 0;
@@ -2522,7 +2633,7 @@ int dh_destroy(dh_handle* h) {
   if (!h) return DH_EARG;
   double* ps[] = {h->diag, h->lower, h->arrow, h->tip, h->dinv, h->pinv, h->tipinv, h->tipdinv,
                   h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->eigbuf, h->dagpinv, h->qinv,
-                  h->Wall, h->Sall, h->Waall, h->tipB};
+                  h->Wall, h->Sall, h->Waall, h->tipB, h->diagT, h->lowerT, h->arrowT};
   for (double* p : ps)
     if (p) (void)hipFree(p);
   if (h->info) (void)hipFree(h->info);
@@ -2752,6 +2863,11 @@ static DagArgs dag_args(const dh_handle* h, int ar, int slot0, int cnt, bool has
   g.last_full = last_full ? 1 : 0;
   g.row0 = row0;
   g.dir = 1;
+  g.diagT = h->diagT ? h->diagT + slot0 * bb : nullptr;
+  g.lowerT = h->lowerT ? h->lowerT + slot0 * bb : nullptr;
+  g.arrowT = h->arrowT ? h->arrowT + (long)slot0 * h->ap * h->bp : nullptr;
+  g.diagT2 = g.lowerT2 = g.arrowT2 = nullptr;
+  g.cols = cols_for(h, false) ? 1 : 0;
   g.order = nullptr;
   g.diag2 = g.lower2 = g.arrow2 = g.tip2 = g.dinv2 = g.pinv2 = nullptr;
   g.flags2 = nullptr;
@@ -2808,6 +2924,19 @@ static int build_twist(dh_handle* h) {
   return DH_OK;
 }
 
+// The transposed-copy pointers of both sides of a two-sided launch, from the handle's current
+// buffers (null when they are not reserved).
+static void twist_set_t(const dh_handle* h, DagArgs& g) {
+  const int n = h->n, bp = h->bp;
+  const long bb = (long)bp * bp;
+  g.diagT = h->diagT;
+  g.lowerT = h->lowerT;
+  g.arrowT = h->arrowT;
+  g.diagT2 = h->diagT ? h->diagT + (n - 1) * bb : nullptr;
+  g.lowerT2 = h->lowerT ? h->lowerT + (n - 2) * bb : nullptr;
+  g.arrowT2 = h->arrowT ? h->arrowT + (long)(n - 1) * h->ap * bp : nullptr;
+}
+
 // Sub-diagonal blocks m .. n-2 transposed in place (into or out of the two-ended layout).
 static void twist_lower(dh_handle* h) {
   const int n = h->n, m = h->tw_m, nbt = h->nbt;
@@ -2828,8 +2957,12 @@ static int cholesky_twisted(dh_handle* h, int sparsity, int ar) {
   if (timed) (void)hipEventRecord(h->ev[2 * h->nrec], h->stream);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
   const int cap = h->dag_cap > 0 ? std::min(h->dag_cap, h->dag_wgs) : h->dag_wgs;
-  hipLaunchKernelGGL(k_pobtaf_dag, dim3((unsigned)std::min(h->tw_ntotal[ar], cap)), dim3(256), 0, h->stream,
-                     h->tw_g[ar]);
+  DagArgs gt = h->tw_g[ar];
+  twist_set_t(h, gt);
+  gt.cols = cols_for(h, true) ? 1 : 0;
+  const dim3 gw((unsigned)std::min(h->tw_ntotal[ar], cap));
+  if (gt.cols) hipLaunchKernelGGL(k_pobtaf_dag<true>, gw, dim3(256), 0, h->stream, gt);
+  else hipLaunchKernelGGL(k_pobtaf_dag<false>, gw, dim3(256), 0, h->stream, gt);
   const int ntri = nbt * (nbt + 1) / 2;
   hipLaunchKernelGGL(k_twist_join, dim3(ntri + (ar ? nbt + 1 : 0)), dim3(256), 0, h->stream,
                      (const double*)(h->lower + m * bb), (const double*)(ar ? h->arrow + (long)(m + 1) * 64 * bp : nullptr),
@@ -2837,8 +2970,11 @@ static int cholesky_twisted(dh_handle* h, int sparsity, int ar) {
                      nbt, bp);
   hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, h->stream, h->dagflags + (long)(m - 1) * FS, 1, FS);
   DagArgs g = dag_args(h, ar, m, 1, true, false, h->tip, m * h->b);
+  g.cols = gt.cols;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
-  hipLaunchKernelGGL(k_pobtaf_dag, dim3((unsigned)std::min<int>(g.ntasks, cap)), dim3(256), 0, h->stream, g);
+  const dim3 gm((unsigned)std::min<int>(g.ntasks, cap));
+  if (g.cols) hipLaunchKernelGGL(k_pobtaf_dag<true>, gm, dim3(256), 0, h->stream, g);
+  else hipLaunchKernelGGL(k_pobtaf_dag<false>, gm, dim3(256), 0, h->stream, g);
   if (timed) {
     (void)hipEventRecord(h->ev[2 * h->nrec + 1], h->stream);
     h->nrec++;
@@ -2858,7 +2994,8 @@ static void dag_launch(dh_handle* h, const DagArgs& g) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
   const int cap = h->dag_cap > 0 ? std::min(h->dag_cap, h->dag_wgs) : h->dag_wgs;
   const unsigned wgs = (unsigned)std::min<int>(g.ntasks, cap);
-  DH_LAUNCH(h, 7, k_pobtaf_dag, dim3(wgs), dim3(256), 0, g);
+  if (g.cols) DH_LAUNCH(h, 7, k_pobtaf_dag<true>, dim3(wgs), dim3(256), 0, g);
+  else DH_LAUNCH(h, 7, k_pobtaf_dag<false>, dim3(wgs), dim3(256), 0, g);
 }
 
 int dh_cholesky(dh_handle* h, int sparsity) {
@@ -3617,10 +3754,17 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
   else if (key == 7 && (value == 0 || value == 1)) {
     if (value == 1) {
       if (!h->diag || h->n < 3) return DH_EARG;
-      const int rc = build_twist(h);
+      int rc = h->dag_cols != 0 ? alloc_transposed(h) : DH_OK;
+      if (rc == DH_OK) rc = build_twist(h);
       if (rc != DH_OK) return rc;
     }
     h->twist = value;
+  } else if (key == 8 && value >= 0 && value <= 2) {
+    if (value == 1 || (value == 2 && h->twist)) {
+      const int rc = alloc_transposed(h);
+      if (rc != DH_OK) return rc;
+    }
+    h->dag_cols = value;
   }
   else if (key == 4 && value >= 0 && value <= 2) {
     if (value == 2 && !h->Wall) {  // per-block slots, kept until the handle is destroyed
@@ -3733,9 +3877,20 @@ static void ckpt_factor(dh_ckpt* c, bool arrow, int t0, int cnt, int slot0, doub
   const int FS = dag_fstride(h->nbt);
   const bool has_prev = t0 > 0, has_next = t0 + cnt < c->n;
   hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags + (long)slot0 * FS, 0, cnt * FS);
-  if (has_prev)
-    hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, h->stream, h->dagflags + (long)(slot0 - 1) * FS, 1, FS);
   DagArgs g = dag_args(h, arrow ? 1 : 0, slot0, cnt, has_prev, has_next, tip, t0 * h->b);
+  if (has_prev) {
+    hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, h->stream, h->dagflags + (long)(slot0 - 1) * FS, 1, FS);
+    if (g.cols) {  // the restored boundary factor's transposed copies, which the launch reads
+      const long bb = (long)h->bp * h->bp;
+      hipLaunchKernelGGL(k_transpose_oop, dim3(h->nbt, h->nbt), dim3(256), 0, h->stream,
+                         (const double*)(h->lower + (slot0 - 1) * bb), (long)h->bp, h->lowerT + (slot0 - 1) * bb,
+                         (long)h->bp);
+      if (arrow)
+        hipLaunchKernelGGL(k_transpose_oop, dim3(h->nbt, 1), dim3(256), 0, h->stream,
+                           (const double*)(h->arrow + (long)(slot0 - 1) * 64 * h->bp), (long)h->bp,
+                           h->arrowT + (long)(slot0 - 1) * 64 * h->bp, (long)DH_TILE);
+    }
+  }
   dag_launch(h, g);
   for (int g0 = slot0; g0 < slot0 + cnt; g0 += h->G) inverse_chunk(h, g0, std::min(h->G, slot0 + cnt - g0), h->stream, false);
 }
@@ -3952,6 +4107,7 @@ struct dh_multi {
   int ntotal[2];
   int* counter;
   int wgs;
+  int cols;  // the launch reads operands by columns (every handle has its transposed copies)
 };
 
 extern "C" {
@@ -3978,6 +4134,11 @@ int dh_multi_create(dh_handle* const* hs, int count, dh_multi** out) {
   mc->counter = nullptr;
   mc->wgs = 0;
   for (int k = 0; k < count; ++k) mc->wgs = std::max(mc->wgs, hs[k]->dag_wgs);
+  // a batch's factorizations run side by side, so the launch is bound by its products in bulk:
+  // by columns when every handle has (or can have) the transposed copies and allows them
+  mc->cols = 1;
+  for (int k = 0; k < count; ++k)
+    if (hs[k]->dag_cols == 0 || alloc_transposed(hs[k]) != DH_OK) mc->cols = 0;
   int rc = DH_OK;
   for (int sp = 0; sp < 2 && rc == DH_OK; ++sp) {
     std::vector<DagArgs> ga(count);
@@ -3990,6 +4151,7 @@ int dh_multi_create(dh_handle* const* hs, int count, dh_multi** out) {
       dh_handle* h = hs[k];
       const int ar = (sp == DH_SPARSITY_BTA && h->a > 0) ? 1 : 0;
       ga[k] = dag_args(h, ar, 0, h->n, false, false, h->tip, 0);
+      ga[k].cols = mc->cols;
       const int nt = ga[k].ntasks;
       for (int j = 0; j < nt; ++j) items.push_back(Item{(j + 0.5) / nt, k, j});
     }
@@ -4031,8 +4193,12 @@ int dh_multi_cholesky(dh_multi* mc, int sparsity, void* hip_stream) {
   }
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, mc->counter, 0);
   const unsigned wgs = (unsigned)std::min(mc->ntotal[sparsity], mc->wgs);
-  hipLaunchKernelGGL(k_pobtaf_dag_multi, dim3(wgs), dim3(256), 0, st, (const DagArgs*)mc->args[sparsity],
-                     (const int2*)mc->order[sparsity], mc->ntotal[sparsity], mc->counter);
+  if (mc->cols)
+    hipLaunchKernelGGL(k_pobtaf_dag_multi<true>, dim3(wgs), dim3(256), 0, st, (const DagArgs*)mc->args[sparsity],
+                       (const int2*)mc->order[sparsity], mc->ntotal[sparsity], mc->counter);
+  else
+    hipLaunchKernelGGL(k_pobtaf_dag_multi<false>, dim3(wgs), dim3(256), 0, st, (const DagArgs*)mc->args[sparsity],
+                       (const int2*)mc->order[sparsity], mc->ntotal[sparsity], mc->counter);
   for (dh_handle* h : mc->hs) {
     if (sparsity == DH_SPARSITY_BTA && h->a > 0)
       hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, st, h->tip, (long)h->ap, h->tipdinv, h->tipinv, h->info,
