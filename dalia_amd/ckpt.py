@@ -30,14 +30,16 @@ from dalia_amd.inla import STInla
 from dalia_amd.model import DeviceSTModel
 
 
-def full_handle_bytes(n: int, b: int, a: int) -> int:
+def full_handle_bytes(n: int, b: int, a: int, twisted: bool = False) -> int:
     """Device bytes of a full `dh_create` handle (factor, inverse, scratch) for n blocks of b, an
     upper bound: two sets of G = min(n, 8) selected-inversion slots (W, S, Wa) and the
-    factorization's own."""
+    factorization's own; twisted: with the transposed operand copies of the two-ended
+    factorization (2 n bp^2 + 64 n bp doubles)."""
     bp = (b + 63) // 64 * 64
     bb = bp * bp
     G = min(n, 8)
-    return 8 * ((3 * n + 3 * G) * bb + 2 * G * bp * (bp + 8) + 2 * G * 64 * bp + (n + 2) * 64 * bp)
+    extra = (2 * n * bb + 64 * n * bp) if twisted else 0
+    return 8 * ((3 * n + 3 * G) * bb + 2 * G * bp * (bp + 8) + 2 * G * 64 * bp + (n + 2) * 64 * bp + extra)
 
 
 def ckpt_bytes(n: int, b: int, a: int, K: int) -> int:
@@ -48,14 +50,20 @@ def ckpt_bytes(n: int, b: int, a: int, K: int) -> int:
 
 
 def make_engine(data, memory_budget: float | None = None, checkpoint: int | None = None, **kw):
-    """An `STInla` when the full handle fits `memory_budget` bytes (or no budget is given), else an
-    `STInlaCkpt` with segments of 16 time blocks (or fewer, down to the K that needs the least
-    memory, when 16 does not fit).  checkpoint=K forces the checkpointed engine with segments of K."""
+    """An `STInla` when the full handle fits `memory_budget` bytes (or no budget is given; two-ended
+    when its transposed operand copies fit too), else an `STInlaCkpt` with segments of 16 time
+    blocks (or fewer, down to the K that needs the least memory, when 16 does not fit).
+    checkpoint=K forces the checkpointed engine with segments of K."""
     n, b, a = data.nt, data.ns, data.nb
     if checkpoint is not None:
         return STInlaCkpt(data, K=int(checkpoint), **kw)
-    if memory_budget is None or full_handle_bytes(n, b, a) <= memory_budget:
+    if memory_budget is None:
         return STInla(data, **kw)
+    twisted = kw.get("twisted", True) and n >= 3
+    if full_handle_bytes(n, b, a, twisted) <= memory_budget:
+        return STInla(data, **kw)
+    if twisted and full_handle_bytes(n, b, a) <= memory_budget:  # one-ended: no transposed copies
+        return STInla(data, **{**kw, "twisted": False})
     ks = [K for K in range(1, min(16, n) + 1) if ckpt_bytes(n, b, a, K) <= memory_budget]
     if not ks:
         raise MemoryError(f"no checkpoint segment length fits {memory_budget / 1e9:.1f} GB "

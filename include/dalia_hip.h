@@ -246,7 +246,7 @@ long long dh_ckpt_memory_bytes(const dh_ckpt* c);
 dh_handle* dh_ckpt_vectors(dh_ckpt* c);
 int dh_ckpt_set_stream(dh_ckpt* c, void* hip_stream);
 int dh_ckpt_get_info(dh_ckpt* c, int* info);
-/* dh_set_tuning on the context's segment factorizations (keys 0, 2, 3, 5, 6; keys 4 and 7 are
+/* dh_set_tuning on the context's segment factorizations (keys 0, 2, 3, 5, 6, 8; keys 4 and 7 are
  * refused). */
 int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value);
 /* dh_timing_enable / dh_timing_read on the context's segment launches. */
@@ -312,6 +312,12 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
  *        dh_assemble writes blocks m .. n-2 of the sub-diagonal transposed for it, and a
  *        factorization that is not two-ended (or dh_multi_cholesky) transposes them back
  *        first.  0 = one-ended (default).
+ * key 8: operands of the persistent launch's products: 0 = read along the factor's rows, 1 = read
+ *        by columns from transposed copies of the final tiles that the launch also writes (they
+ *        take 2 n bp^2 + 64 n bp doubles, reserved on first use and kept until dh_destroy),
+ *        2 = by columns in two-ended launches only (default).  Columns are faster for products in
+ *        bulk and slower on the chain of diagonal tiles, so a one-ended launch, which that chain
+ *        bounds, reads by rows.  dh_multi_cholesky reads by columns unless a handle has 0.
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 

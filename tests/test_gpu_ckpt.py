@@ -84,6 +84,9 @@ def test_memory_budget_picks_the_engine(model17):
     eng = make_engine(d, memory_budget=0.7 * full)
     assert isinstance(eng, STInlaCkpt) and ckpt_bytes(d.nt, d.ns, d.nb, eng.K) <= 0.7 * full
     print(f"budget {0.7 * full / 1e6:.1f} MB -> K = {eng.K}, {eng.memory_bytes() / 1e6:.1f} MB", flush=True)
+    one = make_engine(d, memory_budget=full)  # the one-ended handle fits, the two-ended one's copies do not
+    assert type(one) is STInla and not one.twisted
+    assert full_handle_bytes(d.nt, d.ns, d.nb, twisted=True) > full
     forced = make_engine(d, checkpoint=2)
     assert isinstance(forced, STInlaCkpt) and forced.K == 2
     f_ref = oinla.f_theta(d, d.theta)
