@@ -44,6 +44,44 @@ def f_pobtaf(n, b, a):
     return n * b ** 3 / 3 + 2 * (n - 1) * b ** 3 + n * a * b ** 2 + 2 * (n - 1) * a * b ** 2 + n * a * a * b + a ** 3 / 3
 
 
+def tile_widths(b):
+    return np.array([min(64, b - 64 * t) for t in range((b + 63) // 64)], dtype=float)
+
+
+def f_pobtaf_profile(n, b, a, prof):
+    """Algorithmic flops of pobtaf when tile (R, K) of sub-diagonal block s is zero for K < prof[s][R]
+    (the zero-tile profile the factorization used; those tiles stay zero in the factor, and no
+    product with them is counted).  Equals f_pobtaf for an all-zero profile."""
+    w = tile_widths(b)
+    nbt = len(w)
+    suffix = np.concatenate([np.cumsum(w[::-1])[::-1], [0.0]])  # suffix[k] = sum_{K >= k} w[K]
+    F = n * b ** 3 / 3 + n * a * b * b + n * a * a * b + a ** 3 / 3
+    for s in range(n - 1):
+        f = np.minimum(np.asarray(prof[s], dtype=int), nbt)
+        for R in range(nbt):  # the coupling's TRSM: tiles (R, c >= f[R]), GEMM terms k in [f[R], c)
+            fr = f[R]
+            if fr >= nbt:
+                continue
+            cs = np.arange(fr, nbt)
+            pre = np.concatenate([[0.0], np.cumsum(w[fr:])])
+            F += float(np.sum(2 * w[R] * w[cs] * pre[cs - fr] + w[R] * w[cs] ** 2))
+        K = suffix[np.maximum.outer(f, f)]  # the next diagonal block's update: K >= max(f[r], f[c])
+        WW = np.outer(w, w)
+        F += float(np.sum(np.tril(2 * WW * K, -1)) + np.sum(np.diag(WW) * np.diag(K)))
+        F += float(2 * a * np.sum(w * suffix[f]))  # the arrow row against the coupling
+    return F
+
+
+def tile_profile(eng):
+    """The zero-tile profiles of the engine's last factorization, (n - 1) x nbt."""
+    from dalia_amd import _lib
+    lay = eng.solver.layout
+    nbt = lay.bp // 64
+    buf = np.zeros((lay.n - 1) * nbt, dtype=np.int32)
+    _lib.check(eng.lib.dh_get_tile_profile(eng.h, buf.ctypes.data, buf.size), "dh_get_tile_profile")
+    return buf.reshape(lay.n - 1, nbt)
+
+
 def f_pobtasi(n, b, a):
     return (n - 1) * (13.0 / 3.0) * b ** 3 + (2.0 / 3.0) * b ** 3 + (n - 1) * (6 * a * b * b + 4 * a * a * b)
 
@@ -584,15 +622,20 @@ def main():
         torch.cuda.synchronize()
         times.append(ev0.elapsed_time(ev1))
     t_pobtaf = float(np.median(times)) * 1e-3
-    F = f_pobtaf(nt, ns, nb)
+    # algorithmic flops of the factorization as run: the zero tiles of the sub-diagonal blocks
+    # (their profile, read back from the library) take part in no product
+    prof = tile_profile(eng)
+    F_dense = f_pobtaf(nt, ns, nb)
+    F = f_pobtaf_profile(nt, ns, nb, prof)
     pobtaf_tflops = F / t_pobtaf / 1e12
     log(f"pobtaf: {t_pobtaf * 1e3:.2f} ms, {pobtaf_tflops:.2f} TFLOP/s ({100 * pobtaf_tflops / FP64_PEAK_TFLOPS:.1f}% "
-        "of FP64 matrix peak)")
+        f"of FP64 matrix peak) on {F:.4g} flops ({F / F_dense:.3f} of the dense count; dense-equivalent "
+        f"{F_dense / t_pobtaf / 1e12:.2f} TFLOP/s)")
 
     # roofline of the dominant kernel, per launch, from its launches inside the timed steps
     avg_s = tot.value * 1e-3 / max(cnt.value, 1)
     if kname == "k_pobtaf_dag":
-        per = f_pobtaf(nt, ns, nb) - nb ** 3 / 3  # the whole factorization except the tip's own Cholesky
+        per = F - nb ** 3 / 3  # the whole factorization (as run, profile included) except the tip's own Cholesky
     elif kname == "k_syrk" and choice["syrk"] == 1:
         per = float(ns) ** 3  # k_syrk128: the SYRK alone (arrow and tip run in k_syrk_tail)
         kname = "k_syrk128"
@@ -677,6 +720,9 @@ def main():
         "pobtaf_ms": t_pobtaf * 1e3,
         "pobtaf_tflops": pobtaf_tflops,
         "pobtaf_frac_of_peak": pobtaf_tflops / FP64_PEAK_TFLOPS,
+        "pobtaf_flops": F,
+        "pobtaf_flops_dense": F_dense,
+        "pobtaf_dense_equiv_tflops": F_dense / t_pobtaf / 1e12,
         **fd_info,
         "roofline": roofline,
         "roofline_hbm": hbm,

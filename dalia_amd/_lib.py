@@ -81,6 +81,7 @@ SIGNATURES = {
     "dh_prior_eig": (_I, [_P, ctypes.POINTER(DhModel), _P, c_double_p, _P]),
     "dh_prior_eig_dev": (_I, [_P, ctypes.POINTER(DhModel), _P, _P, _P]),
     "dh_set_tuning": (_I, [_P, _I, _I]),
+    "dh_get_tile_profile": (_I, [_P, _P, _I]),
     "dh_dag_table": (_I, [_I, _I, _I, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "dh_multi_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P)]),
     "dh_multi_destroy": (_I, [_P]),

@@ -685,6 +685,11 @@ struct DagArgs {
   // blocks bp x 64, with the slot offsets of diag/lower/arrow; the second side's likewise
   double *diagT, *lowerT, *arrowT, *diagT2, *lowerT2, *arrowT2;
   int cols;  // 1: the launch runs the COLS variant (products read the transposed copies by columns)
+  // Zero-tile profiles of the sub-diagonal slots (same slot convention as lower / lower2): entry R of
+  // a slot is the first tile column of tile row R that holds a nonzero entry (nbt: none), read
+  // before the launch from the stored blocks.  Tiles left of it stay exactly zero in the factor, so
+  // the products skip them.  Null: no profile (every tile is treated as nonzero).
+  const int *prof, *prof2;
 };
 
 #define DH_DAG_SPIN (1 << 20)
@@ -924,6 +929,7 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
       g.diagT = g0.diagT2;
       g.lowerT = g0.lowerT2;
       g.arrowT = g0.arrowT2;
+      g.prof = g0.prof2;
     }
   }
   const int nbt = g.nbt, bp = g.bp;
@@ -960,6 +966,12 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   int* fLW = fX + nbt * nbt;  // OLD_LW(r, c)
   int* fXA = fLW + nbt * nbt; // OLD_A(c)
   int* fQ = fXA + nbt;        // QX(r): final partial sums of tiles (r, r) and (r, r-1)
+  // first nonzero tile column of tile row q of this block's sub-diagonal slot (po) and of the
+  // previous block's (pp); 0 without a profile
+  const int* pfo = g.prof ? g.prof + si * nbt : nullptr;
+  const int* pfp = g.prof ? g.prof + (si - g.dir) * nbt : nullptr;
+  auto po = [&](int q) { return pfo ? __builtin_amdgcn_readfirstlane(pfo[q]) : 0; };
+  auto pp = [&](int q) { return pfp ? __builtin_amdgcn_readfirstlane(pfp[q]) : 0; };
   const int kold = max(0, r - g.look);  // helpers of tile row r: own terms below kold are OLD's
   const int kpre = r >= 2 ? r - 2 : r - 1;  // PRE(r) stops before the term that QX(r) applies
   const int ks = max(0, c - g.slook);   // other tiles of column c: own terms below ks are the early task's
@@ -977,9 +989,8 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   } else if (kind == DK_OLD_D) {  // tile (r, r): previous block's terms and own terms k < kold
     dag_preload_neg(acc, D + r * T64 + r * DH_TILE, bp);
     if (prev)
-      dag_accumulate<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), r * DH_TILE, fLp + r * nbt, nullptr,
-                     r == 0 ? nbt - 1 : nbt,
-                     g.err, s_n);
+      dag_accumulate_from<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), r * DH_TILE, fLp + r * nbt,
+                                nullptr, pp(r), r == 0 ? nbt - 1 : nbt, g.err, s_n);
     dag_accumulate<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), r * DH_TILE, fD + r * nbt, nullptr, kold, g.err,
                    s_n);
     dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
@@ -987,9 +998,8 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   } else if (kind == DK_OLD_L) {  // tile (r, r-1): previous block's terms and own terms k < kold
     dag_preload_neg(acc, D + r * T64 + (r - 1) * DH_TILE, bp);
     if (prev)
-      dag_accumulate<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), (r - 1) * DH_TILE, fLp + r * nbt,
-                     fLp + (r - 1) * nbt, nbt,
-                     g.err, s_n);
+      dag_accumulate_from<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), (r - 1) * DH_TILE,
+                                fLp + r * nbt, fLp + (r - 1) * nbt, max(pp(r), pp(r - 1)), nbt, g.err, s_n);
     dag_accumulate<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), (r - 1) * DH_TILE, fD + r * nbt,
                    fD + (r - 1) * nbt, kold, g.err,
                    s_n);
@@ -1054,7 +1064,7 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
       dag_signal(fD + c * nbt + (c - 1));
       const Opnd xs{dh_lV, DH_LDS_LD, M_ROW, nullptr};
       dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, xs, 0, xs, 0, 0, DH_TILE);
-    } else if (prev) {
+    } else if (prev && pp(0) < nbt) {
       dag_ready(fLp + (nbt - 1), nullptr, 0, 1, g.err, s_n);
       const Opnd xl{Lp + (nbt - 1) * DH_TILE, bp, M_ROW, nullptr};
       dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, xl, 0, xl, 0, 0, DH_TILE);
@@ -1075,23 +1085,26 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   } else if (kind == DK_OLD_X) {  // diagonal-block tile (r, c): previous block's terms, own terms k < ks
     if (prev || ks > 0) dag_preload_neg(acc, D + r * T64 + c * DH_TILE, bp);
     if (prev)
-      dag_accumulate<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), c * DH_TILE, fLp + r * nbt,
-                     fLp + c * nbt, nbt, g.err, s_n);
+      dag_accumulate_from<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), c * DH_TILE, fLp + r * nbt,
+                                fLp + c * nbt, max(pp(r), pp(c)), nbt, g.err, s_n);
     dag_accumulate<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fD + r * nbt, fD + c * nbt,
                    ks, g.err, s_n);
     if (prev || ks > 0) dh_store(D + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fX + r * nbt + c;
   } else if (kind == DK_OLD_LW) {  // sub-diagonal tile (r, c): own terms k < ks
-    dag_preload_neg(acc, Lo + r * T64 + c * DH_TILE, bp);
-    dag_accumulate<COLS>(acc, DAG_OP(Lo, LoT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fL + r * nbt, fD + c * nbt,
-                   ks, g.err, s_n);
-    dh_store(Lo + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
+    const int z0 = po(r);
+    if (c >= z0 && ks > z0) {  // a tile left of its row's first nonzero tile stays zero
+      dag_preload_neg(acc, Lo + r * T64 + c * DH_TILE, bp);
+      dag_accumulate_from<COLS>(acc, DAG_OP(Lo, LoT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fL + r * nbt,
+                                fD + c * nbt, z0, ks, g.err, s_n);
+      dh_store(Lo + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
+    }
     fdone = fLW + r * nbt + c;
   } else if (kind == DK_OLD_A) {  // arrow tile c: previous block's terms, own terms k < ks
     if (prev || ks > 0) dag_preload_neg(acc, Ar + c * DH_TILE, bp);
     if (prev)
-      dag_accumulate<COLS>(acc, DAG_OP(Ap, ApT, DH_TILE), 0, DAG_OP(Lp, LpT, bp), c * DH_TILE, fAp, fLp + c * nbt, nbt,
-                     g.err, s_n);
+      dag_accumulate_from<COLS>(acc, DAG_OP(Ap, ApT, DH_TILE), 0, DAG_OP(Lp, LpT, bp), c * DH_TILE, fAp, fLp + c * nbt,
+                                pp(c), nbt, g.err, s_n);
     dag_accumulate<COLS>(acc, DAG_OP(Ar, ArT, DH_TILE), 0, DAG_OP(D, DT, bp), c * DH_TILE, fA, fD + c * nbt, ks, g.err, s_n);
     if (prev || ks > 0) dh_store(Ar + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fXA + c;
@@ -1101,6 +1114,7 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
     double* C;
     double* CT;  // its transposed position
     long ldt;
+    bool zero = false;  // a sub-diagonal tile left of its row's first nonzero tile: stays zero
     if (kind == DK_DIAG) {  // tile (r, c), r >= c + 2
       C = D + r * T64 + c * DH_TILE;
       CT = DT + c * T64 + r * DH_TILE;
@@ -1114,11 +1128,15 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
       C = Lo + r * T64 + c * DH_TILE;
       CT = LoT + c * T64 + r * DH_TILE;
       ldt = bp;
-      const int k0 = c > g.slook ? ks : 0;
-      if (c > g.slook) dag_ready(fLW + r * nbt + c, nullptr, 0, 1, g.err, s_n);
-      dag_preload_neg(acc, C, bp);
-      dag_accumulate_from<COLS>(acc, DAG_OP(Lo, LoT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fL + r * nbt,
-                          fD + c * nbt, k0, c, g.err, s_n);
+      const int z0 = po(r);
+      zero = c < z0;
+      if (!zero) {
+        const int k0 = max(z0, c > g.slook ? ks : 0);
+        if (c > g.slook) dag_ready(fLW + r * nbt + c, nullptr, 0, 1, g.err, s_n);
+        dag_preload_neg(acc, C, bp);
+        dag_accumulate_from<COLS>(acc, DAG_OP(Lo, LoT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fL + r * nbt,
+                                  fD + c * nbt, k0, c, g.err, s_n);
+      }
       fdone = fL + r * nbt + c;
     } else {  // arrow tile c
       C = Ar + c * DH_TILE;
@@ -1130,9 +1148,26 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
                           g.err, s_n);
       fdone = fA + c;
     }
-    dag_solve_store<COLS>(C, CT, ldt, acc, pinv + c * DH_TILE * DH_TILE, bp, fD + c * nbt + c, g.err, s_n);
+    if (!zero) dag_solve_store<COLS>(C, CT, ldt, acc, pinv + c * DH_TILE * DH_TILE, bp, fD + c * nbt + c, g.err, s_n);
   }
   if (fdone || g.bdone) dag_finish(fdone, g.bdone ? g.bdone + i : nullptr);  // wave-uniform
+}
+
+// Zero-tile profile of the sub-diagonal slots s0 .. s0 + gridDim.y - 1: prof[slot][R] = the first
+// tile column of tile row R (blockIdx.x) with a nonzero entry (NaN counts), nbt when there is none.
+// A block L_{i+1,i} = A_{i+1,i} L_ii^-T keeps A's zero tiles left of that column: tile (R, K) is
+// A(R, K) minus products with tiles (R, k < K) and then solved, so it is zero when they all are.
+__global__ __launch_bounds__(256) void k_tile_profile(const double* lower, int bp, int nbt, int s0, int* prof) {
+  const int R = blockIdx.x, s = s0 + blockIdx.y;
+  const double* T = lower + (long)s * bp * bp + (long)R * DH_TILE * bp;
+  int K = 0;
+  for (; K < nbt; ++K) {
+    bool nz = false;
+    for (int e = threadIdx.x; e < DH_TILE * DH_TILE; e += 256)
+      nz = nz || T[(long)(e >> 6) * bp + K * DH_TILE + (e & 63)] != 0.0;
+    if (__syncthreads_or(nz)) break;
+  }
+  if (threadIdx.x == 0) prof[(long)s * nbt + R] = K;
 }
 
 // One persistent launch over the factorizations of several handles: ticket t runs task order[t].y
@@ -2383,6 +2418,8 @@ struct dh_handle {
   int sel_slots;                 // workgroups of k_sel_s2w resident at once on the device
   double *diagT, *lowerT, *arrowT;  // transposed copies of the factor's operand tiles (reserved on first use)
   int dag_cols;                  // tile-task operands by columns: 0 never, 1 always, 2 in two-ended launches (key 8)
+  int* dagprof;                  // n * nbt zero-tile profiles of the sub-diagonal slots (k_tile_profile)
+  int no_prof;                   // 1: the tile tasks treat every sub-diagonal tile as nonzero (tuning key 9)
 };
 
 // Products of the tile-task launches read by columns from transposed copies of the operand tiles.
@@ -2601,6 +2638,7 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
   if (hipMalloc((void**)&h->err, sizeof(int)) != hipSuccess) rc = DH_EHIP;
   const size_t nflags = storage ? (size_t)n * dag_fstride(h->nbt) + n : 1;  // + finished tasks per block
   if (hipMalloc((void**)&h->dagflags, sizeof(int) * nflags) != hipSuccess) rc = DH_EHIP;
+  if (storage && hipMalloc((void**)&h->dagprof, sizeof(int) * (size_t)n * h->nbt) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->dagcounter, sizeof(int)) != hipSuccess) rc = DH_EHIP;
   if (rc == DH_OK && storage) {
     std::vector<int> tab;
@@ -2640,6 +2678,7 @@ int dh_destroy(dh_handle* h) {
   if (h->flags) (void)hipFree(h->flags);
   if (h->err) (void)hipFree(h->err);
   if (h->dagflags) (void)hipFree(h->dagflags);
+  if (h->dagprof) (void)hipFree(h->dagprof);
   if (h->dagcounter) (void)hipFree(h->dagcounter);
   if (h->dagtab) (void)hipFree(h->dagtab);
   for (int ar = 0; ar < 2; ++ar)
@@ -2873,7 +2912,17 @@ static DagArgs dag_args(const dh_handle* h, int ar, int slot0, int cnt, bool has
   g.flags2 = nullptr;
   g.n2 = g.row02 = 0;
   g.dir2 = 1;
+  g.prof = (h->dagprof && !h->no_prof) ? h->dagprof + (long)slot0 * h->nbt : nullptr;
+  g.prof2 = nullptr;
   return g;
+}
+
+// The zero-tile profiles of sub-diagonal slots [lo, hi) from the blocks as stored now (the
+// matrix before a factorization, or a final factor such as a restored boundary block).
+static void tile_profile(dh_handle* h, int lo, int hi) {
+  if (!h->dagprof || h->no_prof || !h->lower || hi <= lo) return;
+  hipLaunchKernelGGL(k_tile_profile, dim3(h->nbt, hi - lo), dim3(256), 0, h->stream, (const double*)h->lower, h->bp,
+                     h->nbt, lo, h->dagprof);
 }
 
 // The two-sided launch of the two-ended factorization (per arrow flag) and its merged task order:
@@ -2959,6 +3008,9 @@ static int cholesky_twisted(dh_handle* h, int sparsity, int ar) {
   const int cap = h->dag_cap > 0 ? std::min(h->dag_cap, h->dag_wgs) : h->dag_wgs;
   DagArgs gt = h->tw_g[ar];
   twist_set_t(h, gt);
+  tile_profile(h, 0, n - 1);
+  gt.prof = (h->dagprof && !h->no_prof) ? h->dagprof : nullptr;
+  gt.prof2 = gt.prof ? h->dagprof + (long)(n - 2) * nbt : nullptr;
   gt.cols = cols_for(h, true) ? 1 : 0;
   const dim3 gw((unsigned)std::min(h->tw_ntotal[ar], cap));
   if (gt.cols) hipLaunchKernelGGL(k_pobtaf_dag<true>, gw, dim3(256), 0, h->stream, gt);
@@ -3022,6 +3074,7 @@ int dh_cholesky(dh_handle* h, int sparsity) {
     const bool fp = ov && h->overlap == 2 && h->Wall;
     hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags, 0, n * dag_fstride(nbt) + n);
     const int ar = arrow ? 1 : 0;
+    tile_profile(h, 0, n - 1);
     DagArgs g = dag_args(h, ar, 0, n, false, false, h->tip, 0);
     g.bdone = ov ? h->dagflags + (long)n * dag_fstride(nbt) : nullptr;
     if (ov) {  // the side stream starts after the flags are cleared
@@ -3585,6 +3638,13 @@ int dh_get_info(dh_handle* h, int* info) {
   return DH_OK;
 }
 
+int dh_get_tile_profile(dh_handle* h, int* out, int cap) {
+  if (!h || !out || !h->dagprof || h->n < 2 || cap < (h->n - 1) * h->nbt) return DH_EARG;
+  HIPCHK(hipMemcpyAsync(out, h->dagprof, sizeof(int) * (size_t)(h->n - 1) * h->nbt, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return DH_OK;
+}
+
 int dh_qc_matvec(dh_handle* h, const dh_model* m, const double* params, const double* x, double* yp, double* yl) {
   if (!h || !params || !x || !yp || !yl || !model_ok(h, m)) return DH_EARG;
   const long nst = (long)m->ns * m->nt;
@@ -3750,6 +3810,7 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
   else if (key == 2 && (value == 0 || value == 1)) h->pobtaf_alg = value;
   else if (key == 3 && (value == 0 || value == 1)) h->side_on = value;
   else if (key == 5 && (value == 0 || value == 1)) h->force_giveup = value;
+  else if (key == 9 && (value == 0 || value == 1)) h->no_prof = value ? 0 : 1;
   else if (key == 6 && value >= 0) h->dag_cap = value;
   else if (key == 7 && (value == 0 || value == 1)) {
     if (value == 1) {
@@ -3878,6 +3939,7 @@ static void ckpt_factor(dh_ckpt* c, bool arrow, int t0, int cnt, int slot0, doub
   const bool has_prev = t0 > 0, has_next = t0 + cnt < c->n;
   hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags + (long)slot0 * FS, 0, cnt * FS);
   DagArgs g = dag_args(h, arrow ? 1 : 0, slot0, cnt, has_prev, has_next, tip, t0 * h->b);
+  tile_profile(h, slot0 - (has_prev ? 1 : 0), slot0 + cnt - 1 + (has_next ? 1 : 0));
   if (has_prev) {
     hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, h->stream, h->dagflags + (long)(slot0 - 1) * FS, 1, FS);
     if (g.cols) {  // the restored boundary factor's transposed copies, which the launch reads
@@ -4190,6 +4252,9 @@ int dh_multi_cholesky(dh_multi* mc, int sparsity, void* hip_stream) {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->info, INT_MAX);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->err, h->force_giveup ? 4 : 0);
     hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, st, h->dagflags, 0, h->n * dag_fstride(h->nbt) + h->n);
+    if (h->dagprof && h->lower && h->n > 1)  // the launch's arguments were fixed at creation: profile on, always
+      hipLaunchKernelGGL(k_tile_profile, dim3(h->nbt, h->n - 1), dim3(256), 0, st, (const double*)h->lower, h->bp,
+                         h->nbt, 0, h->dagprof);
   }
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, mc->counter, 0);
   const unsigned wgs = (unsigned)std::min(mc->ntotal[sparsity], mc->wgs);

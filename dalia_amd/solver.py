@@ -214,6 +214,12 @@ class HipBTASolver:
         then hold a factor of the matrix with the blocks after m in reverse order).  Needs n >= 3."""
         _lib.check(self.lib.dh_set_tuning(self.h, 7, 1 if on else 0), "dh_set_tuning")
 
+    def set_tile_profile(self, on: bool = True) -> None:
+        """Zero-tile profiles in the persistent factorization (`dh_set_tuning` key 9, on by default):
+        the sub-diagonal blocks are scanned for the first nonzero 64x64 tile of each tile row before
+        each launch, and the products with the tiles left of it, which stay zero, are skipped."""
+        _lib.check(self.lib.dh_set_tuning(self.h, 9, 1 if on else 0), "dh_set_tuning")
+
     def selected_inversion(self, sparsity: str = "bta") -> None:
         _lib.check(self.lib.dh_selected_inversion(self.h, _check_sparsity(sparsity)), "dh_selected_inversion")
 

@@ -162,6 +162,14 @@ int dh_selected_inversion(dh_handle* h, int sparsity);
  * inside a fused launch (results are then invalid). */
 int dh_get_info(dh_handle* h, int* info);
 
+/* The zero-tile profiles the last persistent factorization used (tuning key 9): out[s * nbt + R]
+ * = the first 64-column tile of tile row R of sub-diagonal slot s that held a nonzero entry (nbt:
+ * none), s = 0 .. n-2, as stored at that time (two-ended: slots m .. n-2 hold the transposed
+ * coupling).  cap >= (n - 1) * nbt ints.  Synchronises the handle's stream.  For the algorithmic
+ * flop count of a profiled factorization.  No counterpart in the reference (Serinv's pobtaf
+ * treats every block as dense). */
+int dh_get_tile_profile(dh_handle* h, int* out, int cap);
+
 /* y_prior = Q_prior x (spatio-temporal part + fe_prec on the fixed effects),
  * y_like = tau_scale A^T A x; vectors of length ns*nt + nb (device);
  * params as for dh_assemble (device). */
@@ -318,6 +326,13 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
  *        2 = by columns in two-ended launches only (default).  Columns are faster for products in
  *        bulk and slower on the chain of diagonal tiles, so a one-ended launch, which that chain
  *        bounds, reads by rows.  dh_multi_cholesky reads by columns unless a handle has 0.
+ * key 9: 1 = zero-tile profiles (default): before each persistent launch, every sub-diagonal
+ *        block as stored is scanned for the first tile column of each 64-row tile row that
+ *        holds a nonzero entry; the tiles left of it stay exactly zero in the factor, and the
+ *        launch skips every product with them (a banded spatial ordering leaves about half of
+ *        the factorization's tile products out).  The factor is the same.  0 = every tile is
+ *        treated as nonzero.  dh_multi_cholesky follows the setting its handles had when
+ *        dh_multi_create ran.
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 
