@@ -636,10 +636,11 @@ __global__ __launch_bounds__(256) void k_syrk_tail(BtaPtrs P, const double* Pbuf
 // previous block's sub-diagonal tile (0, nbt-1).  Each helper is split so
 // that it is never late: OLD_D/OLD_L(c) (the previous block's terms and own
 // terms k < c - look) are queued look columns ahead, PRE_D/PRE_L(c) (own
-// terms c - look <= k < c - 2) two columns ahead, and QX(c) applies the
-// term k = c - 2 as soon as L(c, c-2) is final, so that no helper that the
-// chain waits for has more than one rank-64 product left after the factor
-// two columns back.  The other tiles of column
+// terms c - look <= k < c - 2) two columns ahead, and CHAIN(c) applies the
+// term k = c - 2 itself, from L(c, c-2) and L(c-1, c-2) staged in LDS before
+// it waits for the factor of tile (c-1, c-1) (a separate helper task for it
+// had its wake-up, operand loads and stores on the chain's path in ~40 % of
+// the steps: pobtaf 50.9 -> 50.6 ms at ns = 2025, nt = 100).  The other tiles of column
 // c are split the same way: an early task queued slook columns ahead
 // (OLD_X for diagonal-block tiles, OLD_LW for sub-diagonal tiles, OLD_A for
 // the arrow tile) applies the previous block's terms and the own terms
@@ -652,7 +653,7 @@ __global__ __launch_bounds__(256) void k_syrk_tail(BtaPtrs P, const double* Pbuf
 // grid drains; the chain overlaps the bulk of the products of later columns
 // and of the next block.
 enum { DK_PRE_D = 0, DK_PRE_L = 1, DK_CHAIN = 2, DK_DIAG = 3, DK_LOWER = 4, DK_ARROW = 5, DK_TIP = 6,
-       DK_OLD_D = 7, DK_OLD_L = 8, DK_OLD_X = 9, DK_OLD_LW = 10, DK_OLD_A = 11, DK_QX = 12 };
+       DK_OLD_D = 7, DK_OLD_L = 8, DK_OLD_X = 9, DK_OLD_LW = 10, DK_OLD_A = 11 };
 
 struct DagArgs {
   double *diag, *lower, *arrow, *tip, *dinv;
@@ -705,12 +706,12 @@ struct DagArgs {
 #ifndef DAG_STAGES
 #define DAG_STAGES 2  // operand ring depth of the task products (measured: 1 -> 77.9 ms, 2 -> 77.2, 3 -> 78.8, 4 -> 80.3, 5 -> 80.9)
 #endif
-#define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed before QX(r) and after OLD(r))
+#define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed after OLD(r))
 #define DH_DAG_SLOOK 8  // measured at ns = 2025, nt = 100: 6 -> 83.7 ms, 7 -> 81.1, 8 -> 80.3, 10 -> 81.7, 12 -> 82.9
 
 // Flags of one block: final D tiles [nbt][nbt], final L tiles [nbt][nbt],
 // final arrow tiles [nbt], tip partial applied, PRE_D, PRE_L, OLD_D, OLD_L done [nbt each],
-// OLD_X [nbt][nbt], OLD_LW [nbt][nbt], OLD_A [nbt], QX [nbt] done.
+// OLD_X [nbt][nbt], OLD_LW [nbt][nbt], OLD_A [nbt], (unused) [nbt].
 __device__ __host__ __forceinline__ int dag_fstride(int nbt) { return 4 * nbt * nbt + 7 * nbt + 1; }
 
 // Number (1..64) of consecutive k in [k, kend) whose tiles are final in both
@@ -797,6 +798,32 @@ __device__ __forceinline__ void dag_preload_neg(dh_d4 acc[2][2], const double* C
   for (int x0 = 0; x0 < 2; ++x0)
 #pragma unroll
     for (int x1 = 0; x1 < 2; ++x1) acc[x0][x1] = -acc[x0][x1];
+}
+
+// A third LDS tile (with dh_lS and dh_lV) for the chain task's operands.
+__shared__ __align__(16) double dh_lW[64 * DH_LDS_LD];
+
+// NT row-major 64x64 tiles (leading dimensions ld[]) into LDS tiles (pitch DH_LDS_LD): every load
+// is issued before the first LDS store, so that the tiles take one memory round trip.  The caller
+// synchronises before reading them.
+template <int NT>
+__device__ __forceinline__ void dag_tiles_to_lds(double* const (&dst)[NT], const double* const (&srcp)[NT],
+                                                 const long (&ld)[NT]) {
+  double2 v[NT][8];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = 2 * (threadIdx.x + 256 * j);
+      v[t][j] = *reinterpret_cast<const double2*>(srcp[t] + (long)(e >> 6) * ld[t] + (e & 63));
+    }
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = 2 * (threadIdx.x + 256 * j);
+      *reinterpret_cast<double2*>(dst[t] + (e >> 6) * DH_LDS_LD + (e & 63)) = v[t][j];
+    }
 }
 
 // C = (-acc) inv(L)^T once the factor of the diagonal tile is final: -acc is
@@ -965,7 +992,6 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   int* fX = fOL + nbt;        // OLD_X(r, c) at r * nbt + c
   int* fLW = fX + nbt * nbt;  // OLD_LW(r, c)
   int* fXA = fLW + nbt * nbt; // OLD_A(c)
-  int* fQ = fXA + nbt;        // QX(r): final partial sums of tiles (r, r) and (r, r-1)
   // first nonzero tile column of tile row q of this block's sub-diagonal slot (po) and of the
   // previous block's (pp); 0 without a profile
   const int* pfo = g.prof ? g.prof + si * nbt : nullptr;
@@ -973,7 +999,7 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   auto po = [&](int q) { return pfo ? __builtin_amdgcn_readfirstlane(pfo[q]) : 0; };
   auto pp = [&](int q) { return pfp ? __builtin_amdgcn_readfirstlane(pfp[q]) : 0; };
   const int kold = max(0, r - g.look);  // helpers of tile row r: own terms below kold are OLD's
-  const int kpre = r >= 2 ? r - 2 : r - 1;  // PRE(r) stops before the term that QX(r) applies
+  const int kpre = r >= 2 ? r - 2 : r - 1;  // PRE(r) stops before the term that CHAIN(r) applies
   const int ks = max(0, c - g.slook);   // other tiles of column c: own terms below ks are the early task's
   const int* fLp = fL - FS;  // block i-1
   const int* fAp = fA - FS;
@@ -1020,39 +1046,50 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
                         kpre, g.err, s_n);
     dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fPL + r;
-  } else if (kind == DK_QX) {  // r >= 2: term k = r-2 of tiles (r, r) and (r, r-1), once L(r, r-2) is final
-    double* Pd = D + r * T64 + r * DH_TILE;
-    double* Pl = D + r * T64 + (r - 1) * DH_TILE;
-    dag_ready(fPD + r, nullptr, 0, 1, g.err, s_n);
-    dag_ready(fPL + r, nullptr, 0, 1, g.err, s_n);
-    dh_d4 a2[2][2];
-    dag_preload_neg(acc, Pd, bp);
-    dag_preload_neg(a2, Pl, bp);
-    // L(r, r-2) and L(r-1, r-2), from their transposed copies when COLS
-    const Opnd lr{COLS ? DT + (r - 2) * T64 + r * DH_TILE : D + r * T64 + (r - 2) * DH_TILE, bp, DAG_MODE, nullptr};
-    const Opnd lq{COLS ? DT + (r - 2) * T64 + (r - 1) * DH_TILE : D + (r - 1) * T64 + (r - 2) * DH_TILE, bp, DAG_MODE,
-                  nullptr};
-    dag_ready(fD + r * nbt + (r - 2), nullptr, 0, 1, g.err, s_n);
-    dh_mma<DAG_MODE, DAG_MODE, DAG_STAGES>(acc, lr, 0, lr, 0, 0, DH_TILE);
-    dag_ready(fD + (r - 1) * nbt + (r - 2), nullptr, 0, 1, g.err, s_n);
-    dh_mma<DAG_MODE, DAG_MODE, DAG_STAGES>(a2, lr, 0, lq, 0, 0, DH_TILE);
-    dh_store(Pd, bp, 0, 0, acc, -1.0, 0.0, false);
-    dh_store(Pl, bp, 0, 0, a2, -1.0, 0.0, false);
-    fdone = fQ + r;
   } else if (kind == DK_CHAIN) {  // tile (c, c)
     double* C = D + c * T64 + c * DH_TILE;
     double* pv = pinv + c * DH_TILE * DH_TILE;
     // the inverse slot was last read by block i-2's column c, whose arrow tile is the only reader
     // that is not already a dependency of this task
     if (i >= 2 && g.arrow_on) dag_ready(fA - 2 * FS + c, nullptr, 0, 1, g.err, s_n);
-    dag_ready(c == 0 ? fOD : (c == 1 ? fPD + c : fQ + c), nullptr, 0, 1, g.err, s_n);
+    dag_ready(c == 0 ? fOD : fPD + c, nullptr, 0, 1, g.err, s_n);
     dh_d4 part[2][2];
-    dh_load_acc(C, bp, 0, 0, part);  // partial sum of tile (c, c) without its last term
+    dh_load_acc(C, bp, 0, 0, part);  // partial sum of tile (c, c) without its last two terms
     if (c >= 1) {
       double* X = D + c * T64 + (c - 1) * DH_TILE;
-      if (c == 1) dag_ready(fPL + c, nullptr, 0, 1, g.err, s_n);
+      dag_ready(fPL + c, nullptr, 0, 1, g.err, s_n);
       double xa[2][2][2][4];
-      dh_load_a64<M_ROW>(Opnd{X, bp, M_ROW, nullptr}, 0, xa);
+      if (c >= 2) {
+        // The term k = c-2 of tiles (c, c) and (c, c-1), from L(c, c-2) and L(c-1, c-2) staged in
+        // LDS, before the chain dependency: they are final well before it (as a separate task, this
+        // work put its wake-up, operand loads and stores on the chain's path in ~40 % of the steps).
+        dh_d4 nx[2][2];
+        dag_preload_neg(nx, X, bp);
+        dag_ready(fD + c * nbt + (c - 2), nullptr, 0, 1, g.err, s_n);
+        dag_ready(fD + (c - 1) * nbt + (c - 2), nullptr, 0, 1, g.err, s_n);
+        {
+          double* const dst[2] = {dh_lS, dh_lW};
+          const double* const sp[2] = {D + c * T64 + (c - 2) * DH_TILE, D + (c - 1) * T64 + (c - 2) * DH_TILE};
+          const long ld[2] = {bp, bp};
+          dag_tiles_to_lds<2>(dst, sp, ld);
+        }
+        __syncthreads();
+        const Opnd l2{dh_lS, DH_LDS_LD, M_ROW, nullptr};
+        const Opnd l1{dh_lW, DH_LDS_LD, M_ROW, nullptr};
+        dh_d4 q[2][2];
+        dh_zero(q);
+        dh_mma<M_ROW, M_ROW, DAG_STAGES>(q, l2, 0, l2, 0, 0, DH_TILE);
+        dh_mma<M_ROW, M_ROW, DAG_STAGES>(nx, l2, 0, l1, 0, 0, DH_TILE);
+#pragma unroll
+        for (int x0 = 0; x0 < 2; ++x0)
+#pragma unroll
+          for (int x1 = 0; x1 < 2; ++x1) part[x0][x1] -= q[x0][x1];
+        dh_store(dh_lV, DH_LDS_LD, 0, 0, nx, -1.0, 0.0, false);  // tile (c, c-1) before its last term
+        __syncthreads();
+        dh_load_a64<M_ROW>(Opnd{dh_lV, DH_LDS_LD, M_ROW, nullptr}, 0, xa);
+      } else {
+        dh_load_a64<M_ROW>(Opnd{X, bp, M_ROW, nullptr}, 0, xa);
+      }
       dag_ready(fD + (c - 1) * nbt + (c - 1), nullptr, 0, 1, g.err, s_n);  // the chain dependency
       dh_d4 x[2][2];
       dh_zero(x);
@@ -2468,7 +2505,7 @@ static void build_nodes(int o, int s, int d, std::vector<NodeList>& lv) {
 // Task order of one block of the persistent pobtaf (nl sub-diagonal tile
 // rows: nbt, or 0 for the last block): OLD_D(0), OLD_D/OLD_L(1..look), the
 // early tasks of columns 0..slook-1, PRE_D/PRE_L(1), then per tile column c:
-// CHAIN(c), OLD_D/OLD_L(c+look), PRE_D/PRE_L(c+2), QX(c+1), the early tasks
+// CHAIN(c), OLD_D/OLD_L(c+look), PRE_D/PRE_L(c+2), the early tasks
 // of column c+slook, the diagonal-block tiles below (c+1, c), the
 // sub-diagonal tiles and the arrow tile of column c; then the tip partial.
 // Every task waits only for tasks listed before it (look >= 2).
@@ -2502,7 +2539,6 @@ static void dag_table(int nbt, int nl, int arrow, int look, int slook, std::vect
       tab.push_back(enc(DK_PRE_D, c + 2, c + 2));
       tab.push_back(enc(DK_PRE_L, c + 2, c + 1));
     }
-    if (c + 1 >= 2 && c + 1 < nbt) tab.push_back(enc(DK_QX, c + 1, c + 1));
     if (c + slook < nbt) dag_early(nbt, nl, arrow, slook, c + slook, tab);
     for (int r = c + 2; r < nbt; ++r) tab.push_back(enc(DK_DIAG, r, c));
     for (int r = 0; r < nl; ++r) tab.push_back(enc(DK_LOWER, r, c));

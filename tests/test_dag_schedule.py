@@ -3,8 +3,9 @@ by a task listed before it, so the oldest unfinished task can always proceed and
 
 The table is the library's own (`dh_dag_table`); the flags each task kind waits on and raises are
 restated from `dag_task` in dalia_amd/csrc/dalia_hip.hip (OLD_*: previous block's terms and the own
-terms below the lookahead; PRE/QX: the chain helpers; CHAIN: the diagonal tile; DIAG/LOWER/ARROW: the
-finishing tasks, which also wait for the factor of their column's diagonal tile).
+terms below the lookahead; PRE: the chain helpers; CHAIN: the diagonal tile, which also applies the
+term k = c-2 itself; DIAG/LOWER/ARROW: the finishing tasks, which also wait for the factor of their
+column's diagonal tile).
 """
 
 import ctypes
@@ -58,10 +59,11 @@ def waits_and_raises(i, kind, r, c, nbt, arrow, look, slook):
     if kind == CHAIN:
         if i >= 2 and arrow:
             w.append(A(c, i - 2))  # the tile-inverse slot's last reader in block i-2
-        w.append(("OD", i, 0) if c == 0 else (("PD", i, 1) if c == 1 else ("Q", i, c)))
+        w.append(("OD", i, 0) if c == 0 else ("PD", i, c))
         if c >= 1:
-            if c == 1:
-                w.append(("PL", i, 1))
+            w.append(("PL", i, c))
+            if c >= 2:  # the term k = c-2, applied by the chain task itself
+                w += [D(c, c - 2), D(c - 1, c - 2)]
             w.append(D(c - 1, c - 1))
             return w, [D(c, c - 1), D(c, c)]
         if i > 0:
