@@ -1195,16 +1195,28 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
 // A block L_{i+1,i} = A_{i+1,i} L_ii^-T keeps A's zero tiles left of that column: tile (R, K) is
 // A(R, K) minus products with tiles (R, k < K) and then solved, so it is zero when they all are.
 __global__ __launch_bounds__(256) void k_tile_profile(const double* lower, int bp, int nbt, int s0, int* prof) {
+  // wave w scans tiles w, w + 4, ... in order and stops at its first nonzero one, so that a row
+  // whose first nonzero tile is K reads about K / 4 tiles per wave
+  __shared__ int s_k[4];
   const int R = blockIdx.x, s = s0 + blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const double* T = lower + (long)s * bp * bp + (long)R * DH_TILE * bp;
-  int K = 0;
-  for (; K < nbt; ++K) {
-    bool nz = false;
-    for (int e = threadIdx.x; e < DH_TILE * DH_TILE; e += 256)
-      nz = nz || T[(long)(e >> 6) * bp + K * DH_TILE + (e & 63)] != 0.0;
-    if (__syncthreads_or(nz)) break;
+  int K = w;
+  for (; K < nbt; K += 4) {
+    double2 v[32];  // the whole tile in flight at once: one memory round trip per tile
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+      const int e = lane + 64 * j;  // two entries per load
+      v[j] = *reinterpret_cast<const double2*>(T + (long)(e >> 5) * bp + K * DH_TILE + 2 * (e & 31));
+    }
+    int nz = 0;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) nz |= (v[j].x != 0.0) | (v[j].y != 0.0);
+    if (__ballot(nz)) break;  // wave-uniform
   }
-  if (threadIdx.x == 0) prof[(long)s * nbt + R] = K;
+  if (lane == 0) s_k[w] = K < nbt ? K : nbt;
+  __syncthreads();
+  if (threadIdx.x == 0) prof[(long)s * nbt + R] = min(min(s_k[0], s_k[1]), min(s_k[2], s_k[3]));
 }
 
 // One persistent launch over the factorizations of several handles: ticket t runs task order[t].y
@@ -1549,6 +1561,7 @@ struct SelArgs {
   int bp, nbt, wld;
   int xt;              // 1: X_{i+1,i} is stored transposed into Lb (the upward side of a two-ended factor,
                        // whose coupling block holds the rows of block i's successor)
+  const int* prof;     // zero-tile profile of Lb as stored (tile (R, K) of L_{i+1,i} is zero for K < prof[R]), or null
 };
 
 #define DH_SELG 8  // blocks whose factor-only products are batched in one launch
@@ -1587,8 +1600,10 @@ __device__ __forceinline__ void sel_w_tile(const SelArgs& a, bool wa, int t, int
   dh_d4 acc[2][2];
   dh_zero(acc);
   if (!wa) {  // L_{i+1,i}^T is in the S slot (k_lb_transpose), read by columns
+    // k >= t (inv(L_ii) is lower triangular) and k >= the first nonzero tile of row J of L_{i+1,i}
+    const int k0 = a.prof ? max(t, min(a.prof[J], a.nbt)) : t;
     dh_mma<M_COL, M_COL, DH_MMA_STAGES, DESC>(acc, q, t * DH_TILE, Opnd{a.S, a.bp, M_COL, nullptr}, J * DH_TILE,
-                                              t * DH_TILE, a.bp);
+                                              k0 * DH_TILE, a.bp);
     dh_store(a.W, a.wld, t * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, true);  // tile (t, J) of W^T
   } else {
     dh_mma<M_ROW, M_COL, DH_MMA_STAGES, DESC>(acc, Opnd{a.R, a.bp, M_ROW, nullptr}, 0, q, t * DH_TILE, t * DH_TILE,
@@ -2456,6 +2471,7 @@ struct dh_handle {
   double *diagT, *lowerT, *arrowT;  // transposed copies of the factor's operand tiles (reserved on first use)
   int dag_cols;                  // tile-task operands by columns: 0 never, 1 always, 2 in two-ended launches (key 8)
   int* dagprof;                  // n * nbt zero-tile profiles of the sub-diagonal slots (k_tile_profile)
+  int prof_valid;                // 1: dagprof describes the current factor (its zero tiles), for pobtasi
   int no_prof;                   // 1: the tile tasks treat every sub-diagonal tile as nonzero (tuning key 9)
 };
 
@@ -2876,6 +2892,7 @@ static void sel_args_succ(const dh_handle* h, int i, int succ, bool arrow, doubl
   sa.nbt = h->nbt;
   sa.wld = bp + 8;
   sa.xt = xt;
+  sa.prof = (succ >= 0 && h->prof_valid) ? h->dagprof + (long)std::min(i, succ) * h->nbt : nullptr;
 }
 
 static void sel_args(const dh_handle* h, int i, bool arrow, double* W, double* Wa, double* S, SelArgs& sa) {
@@ -2956,7 +2973,8 @@ static DagArgs dag_args(const dh_handle* h, int ar, int slot0, int cnt, bool has
 // The zero-tile profiles of sub-diagonal slots [lo, hi) from the blocks as stored now (the
 // matrix before a factorization, or a final factor such as a restored boundary block).
 static void tile_profile(dh_handle* h, int lo, int hi) {
-  if (!h->dagprof || h->no_prof || !h->lower || hi <= lo) return;
+  h->prof_valid = (h->dagprof && !h->no_prof && h->lower) ? 1 : 0;
+  if (!h->prof_valid || hi <= lo) return;
   hipLaunchKernelGGL(k_tile_profile, dim3(h->nbt, hi - lo), dim3(256), 0, h->stream, (const double*)h->lower, h->bp,
                      h->nbt, lo, h->dagprof);
 }
@@ -3094,6 +3112,7 @@ int dh_cholesky(dh_handle* h, int sparsity) {
   const long bb = (long)bp * bp;
   h->fp_ready = -1;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->info, INT_MAX);
+  h->prof_valid = 0;  // set again by the launches that read the profiles
   // force_giveup: the flag that a timed-out readiness wait raises is set before the launch, so
   // that every wait of the persistent launch returns at once (tuning key 5, a test of the error path)
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->err, h->force_giveup ? 4 : 0);
@@ -4288,9 +4307,12 @@ int dh_multi_cholesky(dh_multi* mc, int sparsity, void* hip_stream) {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->info, INT_MAX);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->err, h->force_giveup ? 4 : 0);
     hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, st, h->dagflags, 0, h->n * dag_fstride(h->nbt) + h->n);
-    if (h->dagprof && h->lower && h->n > 1)  // the launch's arguments were fixed at creation: profile on, always
+    h->prof_valid = 0;
+    if (h->dagprof && h->lower && h->n > 1) {  // the launch's arguments were fixed at creation: profile on, always
       hipLaunchKernelGGL(k_tile_profile, dim3(h->nbt, h->n - 1), dim3(256), 0, st, (const double*)h->lower, h->bp,
                          h->nbt, 0, h->dagprof);
+      h->prof_valid = 1;  // the zero tiles of the stored blocks are those of the factor, whatever the launch skipped
+    }
   }
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, mc->counter, 0);
   const unsigned wgs = (unsigned)std::min(mc->ntotal[sparsity], mc->wgs);
