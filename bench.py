@@ -72,6 +72,24 @@ def f_pobtaf_profile(n, b, a, prof):
     return F
 
 
+def structural_profile(d):
+    """The zero-tile profile of the sub-diagonal blocks of a spatio-temporal model's Qc from its
+    structure (the union of the FEM patterns and the coupling A_st^T A_st makes between consecutive
+    time steps): first nonzero 64-column tile per 64-row tile row, as the library computes it."""
+    import scipy.sparse as sps
+    ns = d.ns
+    nbt = (ns + 63) // 64
+    pat = (abs(d.c0) + abs(d.g1) + abs(d.g2) + abs(d.g3)).tocsr()
+    nt = d.nt
+    a = d.a_st.tocsc()
+    cols = a[:, ns:2 * ns] if nt > 1 else None
+    off = (abs(a[:, 0:ns]).T @ abs(cols)).T if cols is not None else sps.csr_matrix((ns, ns))
+    P = (pat + off).tocoo()
+    f = np.full(nbt, nbt)
+    np.minimum.at(f, P.row // 64, P.col // 64)
+    return f
+
+
 def tile_profile(eng):
     """The zero-tile profiles of the engine's last factorization, (n - 1) x nbt."""
     from dalia_amd import _lib
@@ -233,7 +251,8 @@ def _f_pobtf(n, b):
 
 
 def _eager_pobtaf(eng, reps: int = 2):
-    """(ms, TFLOP/s) of the persistent factorization of the engine's Qc, HIP events on its stream."""
+    """(ms, TFLOP/s) of the persistent factorization of the engine's Qc, HIP events on its stream;
+    the flops are those of the factorization as run (zero tiles of the profile excluded)."""
     import torch
 
     from dalia_amd import _lib
@@ -248,7 +267,8 @@ def _eager_pobtaf(eng, reps: int = 2):
         ts.append(ev0.elapsed_time(ev1) * 1e-3)
     t = float(np.median(ts[1:]))
     lay = eng.solver.layout
-    return t * 1e3, f_pobtaf(lay.n, lay.b, lay.a) / t / 1e12
+    F = f_pobtaf_profile(lay.n, lay.b, lay.a, tile_profile(eng)) if lay.n > 1 else f_pobtaf(lay.n, lay.b, lay.a)
+    return t * 1e3, F / t / 1e12
 
 
 def _coreg_cpu_baseline(d, flops_total: float, reps: int = 3):
@@ -387,7 +407,7 @@ def bench_coreg(args):
     log(f"31-point diagonal Hessian stencil: {t_hess:.2f} s in one replay; f(theta) graph {fv[0]:.12g} "
         f"eager {f_eager:.12g}")
     ms, tf = _eager_pobtaf(eng)
-    flops = f_pobtaf(d.nt, d.b, d.a)
+    flops = f_pobtaf(d.nt, d.b, d.a)  # dense count: the CPU baseline (oracle) factors dense blocks
     cpu = None
     if not args.no_cpu:
         small = synth.make_coreg_model(20, 16, d.nb, d.nv, 4)
@@ -459,10 +479,14 @@ def bench_ckpt(args):
                    "adjoint_vs_fd_speedup": t_fd / t_adj, "adjoint_vs_fd_relerr": relerr}
         log(f"central differences (checkpointed f): {t_fd * 1e3:.0f} ms, speed-up {t_fd / t_adj:.2f}x, "
             f"rel. difference {relerr:.2e}")
-    # per evaluation: the forward pass factors every segment, the backward pass all but the last
+    # per evaluation: the forward pass factors every segment, the backward pass all but the last;
+    # flops as run: the zero tiles of the sub-diagonal blocks (the same profile for every block of
+    # this model, read from its structure) take part in no product
     S = (nt + K - 1) // K
-    per_block = ns ** 3 / 3 + 2 * ns ** 3 + 3 * nb * ns ** 2 + nb * nb * ns
-    flops_eval = (f_pobtaf(nt, ns, nb) - nb ** 3 / 3) + (S - 1) * K * per_block
+    prof = structural_profile(data)
+    f_prof = lambda n: f_pobtaf_profile(n, ns, nb, [prof] * (n - 1))
+    per_block = f_prof(3) - f_prof(2)
+    flops_eval = (f_prof(nt) - nb ** 3 / 3) + (S - 1) * K * per_block
     t_launch = tot.value * 1e-3 / max(cnt.value, 1)
     achieved = flops_eval * args.steps / (tot.value * 1e-3) / 1e12
     roofline = {"kernel": "k_pobtaf_dag", "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
@@ -506,6 +530,8 @@ def main():
                     help="factor checkpointed every K time blocks and recomputed in the backward pass (0: stored)")
     ap.add_argument("--twisted", type=int, default=1, choices=(0, 1),
                     help="two-ended factorizations (blocks eliminated from both ends, meeting in the middle)")
+    ap.add_argument("--profile", type=int, default=1, choices=(0, 1),
+                    help="zero-tile profiles of the sub-diagonal blocks in the factorization (0: every tile dense)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device-to-device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -544,6 +570,7 @@ def main():
     t0 = time.perf_counter()
     data = synth.make_st_model(m, nt, nb, seed)
     eng = STInla(data, overlap=args.overlap, twisted=bool(args.twisted))
+    eng.solver.set_tile_profile(bool(args.profile))
     torch.cuda.synchronize()
     log(f"set-up {time.perf_counter() - t0:.1f} s (closed-form prior: {eng.dm.closed_form})")
     theta = data.theta.copy()
@@ -624,9 +651,8 @@ def main():
     t_pobtaf = float(np.median(times)) * 1e-3
     # algorithmic flops of the factorization as run: the zero tiles of the sub-diagonal blocks
     # (their profile, read back from the library) take part in no product
-    prof = tile_profile(eng)
     F_dense = f_pobtaf(nt, ns, nb)
-    F = f_pobtaf_profile(nt, ns, nb, prof)
+    F = f_pobtaf_profile(nt, ns, nb, tile_profile(eng)) if args.profile else F_dense
     pobtaf_tflops = F / t_pobtaf / 1e12
     log(f"pobtaf: {t_pobtaf * 1e3:.2f} ms, {pobtaf_tflops:.2f} TFLOP/s ({100 * pobtaf_tflops / FP64_PEAK_TFLOPS:.1f}% "
         f"of FP64 matrix peak) on {F:.4g} flops ({F / F_dense:.3f} of the dense count; dense-equivalent "
@@ -649,9 +675,10 @@ def main():
     tw = kname == "k_pobtaf_dag" and eng.twisted
     roofline = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                # the committed counter summary is of the one-ended launch; the profiler's counter
-                # mode does not complete on this build, so the two-ended launch has none
-                "traffic": None if tw else committed_traffic(kname, args.workload),
+                # the committed counter summary is of the one-ended launch without zero-tile profiles;
+                # the profiler's counter mode does not complete on the two-ended build, so the other
+                # configurations have none
+                "traffic": None if (tw or args.profile) else committed_traffic(kname, args.workload),
                 "launches": int(cnt.value), "avg_launch_us": avg_s * 1e6, "flops_per_launch": per}
     if tw:
         roofline["timed_span"] = ("two-ended factorization: the two-sided k_pobtaf_dag launch, k_twist_join "
@@ -716,7 +743,8 @@ def main():
         "config": {"workload": args.workload, "ns": ns, "nt": nt, "nb": nb, "seed": seed, "theta_dim": 4,
                    "n_obs": data.n_obs, "kernel_variants": {"pobtaf_schedule": "tile-task" if choice["alg"] == 1 else "per-panel",
                                                         "syrk": choice["syrk"]},
-                   "overlap": args.overlap, "graph": bool(args.graph), "twisted": bool(eng.twisted)},
+                   "overlap": args.overlap, "graph": bool(args.graph), "twisted": bool(eng.twisted),
+                   "tile_profile": bool(args.profile)},
         "pobtaf_ms": t_pobtaf * 1e3,
         "pobtaf_tflops": pobtaf_tflops,
         "pobtaf_frac_of_peak": pobtaf_tflops / FP64_PEAK_TFLOPS,

@@ -3695,6 +3695,7 @@ int dh_get_info(dh_handle* h, int* info) {
 
 int dh_get_tile_profile(dh_handle* h, int* out, int cap) {
   if (!h || !out || !h->dagprof || h->n < 2 || cap < (h->n - 1) * h->nbt) return DH_EARG;
+  if (!h->prof_valid) return DH_ESTATE;  // the last factorization read no profiles
   HIPCHK(hipMemcpyAsync(out, h->dagprof, sizeof(int) * (size_t)(h->n - 1) * h->nbt, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   return DH_OK;

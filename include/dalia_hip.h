@@ -165,7 +165,8 @@ int dh_get_info(dh_handle* h, int* info);
 /* The zero-tile profiles the last persistent factorization used (tuning key 9): out[s * nbt + R]
  * = the first 64-column tile of tile row R of sub-diagonal slot s that held a nonzero entry (nbt:
  * none), s = 0 .. n-2, as stored at that time (two-ended: slots m .. n-2 hold the transposed
- * coupling).  cap >= (n - 1) * nbt ints.  Synchronises the handle's stream.  For the algorithmic
+ * coupling).  cap >= (n - 1) * nbt ints.  Synchronises the handle's stream.  DH_ESTATE when the
+ * last factorization read no profiles (key 9 off, or the per-panel schedule).  For the algorithmic
  * flop count of a profiled factorization.  No counterpart in the reference (Serinv's pobtaf
  * treats every block as dense). */
 int dh_get_tile_profile(dh_handle* h, int* out, int cap);
