@@ -1191,32 +1191,49 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
 }
 
 // Zero-tile profile of the sub-diagonal slots s0 .. s0 + gridDim.y - 1: prof[slot][R] = the first
-// tile column of tile row R (blockIdx.x) with a nonzero entry (NaN counts), nbt when there is none.
-// A block L_{i+1,i} = A_{i+1,i} L_ii^-T keeps A's zero tiles left of that column: tile (R, K) is
+// tile column of tile row R (blockIdx.x) with a nonzero entry (NaN counts), nbt when there is none;
+// profr[slot][R] = the last one (-1 when there is none).
+// A block L_{i+1,i} = A_{i+1,i} L_ii^-T keeps A's zero tiles left of the first column: tile (R, K) is
 // A(R, K) minus products with tiles (R, k < K) and then solved, so it is zero when they all are.
-__global__ __launch_bounds__(256) void k_tile_profile(const double* lower, int bp, int nbt, int s0, int* prof) {
-  // wave w scans tiles w, w + 4, ... in order and stops at its first nonzero one, so that a row
-  // whose first nonzero tile is K reads about K / 4 tiles per wave
-  __shared__ int s_k[4];
+// The last column bounds the tiles of the coupling where the matrix itself has entries (the tiles of
+// X_{i+1,i} that a selected inversion on the matrix's pattern needs).
+__device__ __forceinline__ bool dh_tile_nonzero(const double* T, int bp, int K) {
+  const int lane = threadIdx.x & 63;
+  double2 v[32];  // the whole tile in flight at once: one memory round trip per tile
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const int e = lane + 64 * j;  // two entries per load
+    v[j] = *reinterpret_cast<const double2*>(T + (long)(e >> 5) * bp + K * DH_TILE + 2 * (e & 31));
+  }
+  int nz = 0;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) nz |= (v[j].x != 0.0) | (v[j].y != 0.0);
+  return __ballot(nz) != 0;  // wave-uniform
+}
+
+__global__ __launch_bounds__(256) void k_tile_profile(const double* lower, int bp, int nbt, int s0, int* prof,
+                                                      int* profr) {
+  // wave w scans tiles w, w + 4, ... in order (from the left, then from the right) and stops at its
+  // first nonzero one, so that a row reads about (first + nbt - last) / 4 tiles per wave
+  __shared__ int s_k[8];
   const int R = blockIdx.x, s = s0 + blockIdx.y;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const double* T = lower + (long)s * bp * bp + (long)R * DH_TILE * bp;
   int K = w;
-  for (; K < nbt; K += 4) {
-    double2 v[32];  // the whole tile in flight at once: one memory round trip per tile
-#pragma unroll
-    for (int j = 0; j < 32; ++j) {
-      const int e = lane + 64 * j;  // two entries per load
-      v[j] = *reinterpret_cast<const double2*>(T + (long)(e >> 5) * bp + K * DH_TILE + 2 * (e & 31));
-    }
-    int nz = 0;
-#pragma unroll
-    for (int j = 0; j < 32; ++j) nz |= (v[j].x != 0.0) | (v[j].y != 0.0);
-    if (__ballot(nz)) break;  // wave-uniform
+  for (; K < nbt; K += 4)
+    if (dh_tile_nonzero(T, bp, K)) break;
+  int Kr = nbt - 1 - w;
+  for (; Kr >= 0; Kr -= 4)
+    if (dh_tile_nonzero(T, bp, Kr)) break;
+  if (lane == 0) {
+    s_k[w] = K < nbt ? K : nbt;
+    s_k[4 + w] = Kr >= 0 ? Kr : -1;
   }
-  if (lane == 0) s_k[w] = K < nbt ? K : nbt;
   __syncthreads();
-  if (threadIdx.x == 0) prof[(long)s * nbt + R] = min(min(s_k[0], s_k[1]), min(s_k[2], s_k[3]));
+  if (threadIdx.x == 0) {
+    prof[(long)s * nbt + R] = min(min(s_k[0], s_k[1]), min(s_k[2], s_k[3]));
+    profr[(long)s * nbt + R] = max(max(s_k[4], s_k[5]), max(s_k[6], s_k[7]));
+  }
 }
 
 // One persistent launch over the factorizations of several handles: ticket t runs task order[t].y
@@ -1758,8 +1775,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (g < fut.cnt) sel_w_work(fut.a[g], f - g * nwb);
 }
 
+// A diagonal tile of a symmetric result stored exactly symmetric: C[i0 + r][i0 + c] = alpha times
+// the accumulated entry (max(r, c), min(r, c)), mirrored through dh_lS.  The lower-tile-and-mirror
+// scheme makes every off-diagonal tile pair symmetric; a diagonal tile stored as accumulated keeps a
+// rounding-level antisymmetric part, which the congruence form's next block (X_qq read by columns,
+// M's tiles mirrored) turns into a symmetric error scaled by cond(L_ii)^2 that grows from block to
+// block.  Workgroup-wide (every thread calls it).
+__device__ __forceinline__ void dh_store_sym_tile(double* C, long ldc, int i0, const dh_d4 acc[2][2], double alpha) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int rbase = (w >> 1) * 32 + (lane >> 4);
+  const int cbase = (w & 1) * 32 + (lane & 15);
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dh_lS[(rbase + bi * 16 + 4 * r) * DH_LDS_LD + cbase + bj * 16] = acc[bi][bj][r];
+  __syncthreads();
+  for (int e = threadIdx.x; e < DH_TILE * DH_TILE; e += blockDim.x) {
+    const int i = e >> 6, j = e & 63;
+    const double v = i >= j ? dh_lS[i * DH_LDS_LD + j] : dh_lS[j * DH_LDS_LD + i];
+    C[(long)(i0 + i) * ldc + i0 + j] = alpha * v;
+  }
+}
+
 // X_ii = S - Wa^T X_{a,i} on lower tile (I, J) (S alone without the arrowhead), written to D at
-// (I, J) and mirrored to (J, I).
+// (I, J) and mirrored to (J, I) (a diagonal tile exactly symmetric).
 __device__ __forceinline__ void sel_sym_work(const SelArgs& a, double* D, int e) {
   int I = 0;
   while ((I + 1) * (I + 2) / 2 <= e) ++I;
@@ -1769,8 +1811,12 @@ __device__ __forceinline__ void sel_sym_work(const SelArgs& a, double* D, int e)
   if (a.R)
     dh_mma<M_COL, M_COL>(acc, Opnd{a.Wa, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.R, a.bp, M_COL, nullptr},
                          J * DH_TILE, 0, DH_TILE);
+  if (I == J) {
+    dh_store_sym_tile(D, a.bp, I * DH_TILE, acc, -1.0);
+    return;
+  }
   dh_store(D, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, false);
-  if (I != J) dh_store(D, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
+  dh_store(D, a.bp, I * DH_TILE, J * DH_TILE, acc, -1.0, 0.0, true);
 }
 
 __global__ __launch_bounds__(256) void k_sel_sym(SelArgs a, double* D) { sel_sym_work(a, D, blockIdx.x); }
@@ -1780,6 +1826,207 @@ __global__ __launch_bounds__(256) void k_sel_sym2(SelArgs a0, double* D0, int ns
   const int e = blockIdx.x;
   if (e < nsym) sel_sym_work(a0, D0, e);
   else sel_sym_work(a1, D1, e - nsym);
+}
+
+// ---------------------------------------------------------------------------
+// pobtasi in congruence form, the selected inverse on the matrix's coupling tiles
+// ---------------------------------------------------------------------------
+//
+// With q the successor of block i in the elimination, L = L_{q,i}, L_a = L_{a,i} and
+// Y = X_qq L + X_{a,q}^T L_a,  Ya = X_{a,q} L + X_aa L_a,  M = L^T Y + L_a^T Ya  (symmetric):
+//   X_{q,i} = -Y inv(L_ii),  X_{a,i} = -Ya inv(L_ii),  X_ii = inv(L_ii)^T (I + M) inv(L_ii)
+// (the Takahashi recurrences with W = L inv(L_ii) and Wa = L_a inv(L_ii) multiplied out).  Products
+// with L skip its zero tiles (the profile), and X_{q,i} is formed only on the tiles of the coupling
+// block where the matrix has entries (first..last nonzero tile of each tile row), which is all that
+// a caller reading the selected inverse on the matrix's pattern needs (the adjoint contractions,
+// marginal variances).  Per block 3.3 b^3 flops with the banded profile of a grid ordering, against
+// 4.1 b^3 for the W form, and no W or inv(L)^T inv(L).
+struct SelCArgs {
+  double* Lb;          // L_{q,i} as stored (rows of q); X_{q,i} on its band tiles on exit; null: no successor
+  double* La;          // L_{a,i} (64 x bp); X_{a,i} on exit; null: no arrowhead
+  const double* Dn;    // X_qq (full)
+  const double* Rn;    // X_{a,q}; null without successor or arrowhead
+  const double* tip;   // X_aa (ld 64)
+  const double* Q;     // inv(L_ii) row-major, zeros above the diagonal
+  double* D;           // block i's diagonal slot: X_ii (both triangles) on exit
+  double* Y;           // bp x bp (pitch yld)
+  double* Ya;          // 64 x bp
+  double* M;           // bp x bp, both triangles
+  double* T;           // bp x bp, lower tiles of (I + M) inv(L_ii)
+  const int* prof;     // first nonzero tile column of each tile row of Lb (null: 0)
+  const int* profr;    // last nonzero tile column of each tile row of Lb (null: nbt - 1)
+  int bp, nbt, yld;
+  int xt;              // 1: X_{q,i} stored transposed (the coupling slot holds the rows of q)
+};
+
+// Tile rows R of L with a nonzero tile in some column <= K: R < selc_rend(a, K) (a prefix that holds
+// them all; products with the zero tiles in it add nothing).
+__device__ __forceinline__ int selc_rend(const SelCArgs& a, int K) {
+  if (!a.prof) return a.nbt;
+  int r = 0;
+  for (int R = 0; R < a.nbt; ++R)
+    if (a.prof[R] <= K) r = R + 1;
+  return r;
+}
+
+// the lower tile of a triangular index
+__device__ __forceinline__ void selc_tri(int e, int& I, int& J) {
+  I = 0;
+  while ((I + 1) * (I + 2) / 2 <= e) ++I;
+  J = e - I * (I + 1) / 2;
+}
+
+// Workgroup e of a paired launch (k_selc_*2): with equal counts the two blocks' items alternate, so
+// that both are taken heaviest first; otherwise a0's items come first.  Returns true for a1's.
+__device__ __forceinline__ bool selc_side(int n0, int& e) {
+  e = blockIdx.x;
+  const int n1 = (int)gridDim.x - n0;
+  if (n1 == n0) {
+    const bool second = e & 1;
+    e >>= 1;
+    return second;
+  }
+  if (e < n0) return false;
+  e -= n0;
+  return true;
+}
+
+// Phase A: tile (I, K) of Y (with a successor) and tile K of Ya (with the arrowhead), tile columns K
+// in decreasing order: the k range of column K (the tile rows with a nonzero tile of L at or left of
+// K) does not shrink with K, so the longest products start first and the short ones fill the tail.
+__device__ __forceinline__ void selc_y_work(const SelCArgs& a, int e) {
+  const int nbt = a.nbt;
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  const int cy = a.Lb ? nbt : 0, per = cy + (a.La ? 1 : 0);
+  const int K = nbt - 1 - e / per, r = e % per;
+  if (r < cy) {
+    const int I = r;
+    const int rend = selc_rend(a, K);
+    // opM[i][r] = X_qq(r, i) = X_qq(i, r) and opM[k][r] = L(r, k): both read by columns
+    dh_mma<M_COL, M_COL>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
+                         K * DH_TILE, 0, rend * DH_TILE);
+    if (a.Rn && a.La)
+      dh_mma<M_COL, M_COL>(acc, Opnd{a.Rn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.La, a.bp, M_COL, nullptr},
+                           K * DH_TILE, 0, DH_TILE);
+    dh_store(a.Y, a.yld, I * DH_TILE, K * DH_TILE, acc, 1.0, 0.0, false);
+    return;
+  }
+  if (a.Lb && a.Rn)
+    dh_mma<M_ROW, M_COL>(acc, Opnd{a.Rn, a.bp, M_ROW, nullptr}, 0, Opnd{a.Lb, a.bp, M_COL, nullptr}, K * DH_TILE, 0,
+                         selc_rend(a, K) * DH_TILE);
+  dh_mma<M_ROW, M_COL>(acc, Opnd{a.tip, DH_TILE, M_ROW, nullptr}, 0, Opnd{a.La, a.bp, M_COL, nullptr}, K * DH_TILE,
+                       0, DH_TILE);
+  dh_store(a.Ya, a.bp, 0, K * DH_TILE, acc, 1.0, 0.0, false);
+}
+
+__device__ __host__ __forceinline__ int selc_ny(const SelCArgs& a) { return (a.Lb ? a.nbt * a.nbt : 0) + (a.La ? a.nbt : 0); }
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_selc_y2(SelCArgs a0, int n0,
+                                                                                     SelCArgs a1) {
+  int e;
+  if (selc_side(n0, e)) selc_y_work(a1, e);
+  else selc_y_work(a0, e);
+}
+
+// Phase B: lower tile (K', K) of M = L^T Y + L_a^T Ya, stored to both triangles (a diagonal tile
+// exactly symmetric).
+__device__ __forceinline__ void selc_m_work(const SelCArgs& a, int e) {
+  int Kp, K;
+  selc_tri(a.nbt * (a.nbt + 1) / 2 - 1 - e, Kp, K);  // tile rows K' in decreasing order: longest first
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  if (a.Lb)  // opM[k'][r] = L(r, k'), opM[k][r] = Y(r, k): both by columns
+    dh_mma<M_COL, M_COL>(acc, Opnd{a.Lb, a.bp, M_COL, nullptr}, Kp * DH_TILE, Opnd{a.Y, a.yld, M_COL, nullptr},
+                         K * DH_TILE, 0, selc_rend(a, Kp) * DH_TILE);
+  if (a.La)
+    dh_mma<M_COL, M_COL>(acc, Opnd{a.La, a.bp, M_COL, nullptr}, Kp * DH_TILE, Opnd{a.Ya, a.bp, M_COL, nullptr},
+                         K * DH_TILE, 0, DH_TILE);
+  if (Kp == K) {
+    dh_store_sym_tile(a.M, a.bp, K * DH_TILE, acc, 1.0);
+    return;
+  }
+  dh_store(a.M, a.bp, Kp * DH_TILE, K * DH_TILE, acc, 1.0, 0.0, false);
+  dh_store(a.M, a.bp, Kp * DH_TILE, K * DH_TILE, acc, 1.0, 0.0, true);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_selc_m2(SelCArgs a0, int n0,
+                                                                                     SelCArgs a1) {
+  int e;
+  if (selc_side(n0, e)) selc_m_work(a1, e);
+  else selc_m_work(a0, e);
+}
+
+// Phase C, by tile column c in increasing order (every product of column c runs over k tiles
+// c..nbt-1, so the longest start first): the lower tiles (K, c), K >= c, of T = (I + M) inv(L_ii);
+// tile c of X_{a,i} = -Ya inv(L_ii) (with the arrowhead); tile (R, c) of X_{q,i} = -Y inv(L_ii) for
+// each tile row R (with a successor) whose band of nonzero tiles holds c (the other R return).
+__device__ __forceinline__ void selc_t_work(const SelCArgs& a, bool has_m, int e) {
+  const int nbt = a.nbt, ca = a.La ? 1 : 0, cb = a.Lb ? nbt : 0;
+  int c = 0;
+  while (c < nbt - 1 && e >= nbt - c + ca + cb) {  // column c holds nbt - c + ca + cb items
+    e -= nbt - c + ca + cb;
+    ++c;
+  }
+  const Opnd q{a.Q, a.bp, M_COL, nullptr};  // opM[j][k'] = inv(L)(k', j), read by columns
+  dh_d4 acc[2][2];
+  if (e < nbt - c) {
+    const int K = c + e;
+    dh_load_acc(a.Q, a.bp, K * DH_TILE, c * DH_TILE, acc);  // the identity's term: inv(L)(K, c)
+    if (has_m)  // opM[k][k'] = M(k', k) = M(k, k')
+      dh_mma<M_COL, M_COL>(acc, Opnd{a.M, a.bp, M_COL, nullptr}, K * DH_TILE, q, c * DH_TILE, c * DH_TILE, a.bp);
+    dh_store(a.T, a.bp, K * DH_TILE, c * DH_TILE, acc, 1.0, 0.0, false);
+    return;
+  }
+  e -= nbt - c;
+  dh_zero(acc);
+  if (e < ca) {
+    dh_mma<M_ROW, M_COL>(acc, Opnd{a.Ya, a.bp, M_ROW, nullptr}, 0, q, c * DH_TILE, c * DH_TILE, a.bp);
+    dh_store(a.La, a.bp, 0, c * DH_TILE, acc, -1.0, 0.0, false);
+    return;
+  }
+  const int R = e - ca;
+  const int lo = a.prof ? a.prof[R] : 0, hi = a.profr ? a.profr[R] : nbt - 1;
+  if (c < lo || c > hi) return;  // no entry of the matrix in this tile: X not needed there
+  dh_mma<M_ROW, M_COL>(acc, Opnd{a.Y, a.yld, M_ROW, nullptr}, R * DH_TILE, q, c * DH_TILE, c * DH_TILE, a.bp);
+  dh_store(a.Lb, a.bp, R * DH_TILE, c * DH_TILE, acc, -1.0, 0.0, a.xt != 0);
+}
+
+__device__ __host__ __forceinline__ int selc_nt(const SelCArgs& a) {
+  return a.nbt * (a.nbt + 1) / 2 + (a.La ? a.nbt : 0) + (a.Lb ? a.nbt * a.nbt : 0);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_selc_t2(SelCArgs a0, int m0, int n0,
+                                                                                     SelCArgs a1, int m1) {
+  int e;
+  if (selc_side(n0, e)) selc_t_work(a1, m1 != 0, e);
+  else selc_t_work(a0, m0 != 0, e);
+}
+
+// Phase D: lower tile (I, J) of X_ii = inv(L_ii)^T T, written to D at (I, J) and mirrored to (J, I)
+// (a diagonal tile exactly symmetric); tile rows I in increasing order, the products (k over tiles
+// I..nbt-1) longest first.
+__device__ __forceinline__ void selc_x_work(const SelCArgs& a, int e) {
+  int I, J;
+  selc_tri(e, I, J);
+  dh_d4 acc[2][2];
+  dh_zero(acc);
+  // opM[i][k] = inv(L)(k, i) (k >= i) and opM[j][k] = T(k, j) (lower tiles, k >= i >= j): by columns
+  dh_mma<M_COL, M_COL>(acc, Opnd{a.Q, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.T, a.bp, M_COL, nullptr},
+                       J * DH_TILE, I * DH_TILE, a.bp);
+  if (I == J) {
+    dh_store_sym_tile(a.D, a.bp, I * DH_TILE, acc, 1.0);
+    return;
+  }
+  dh_store(a.D, a.bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
+  dh_store(a.D, a.bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, true);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_selc_x2(SelCArgs a0, int n0,
+                                                                                     SelCArgs a1) {
+  int e;
+  if (selc_side(n0, e)) selc_x_work(a1, e);
+  else selc_x_work(a0, e);
 }
 
 // ---------------------------------------------------------------------------
@@ -2471,8 +2718,10 @@ struct dh_handle {
   double *diagT, *lowerT, *arrowT;  // transposed copies of the factor's operand tiles (reserved on first use)
   int dag_cols;                  // tile-task operands by columns: 0 never, 1 always, 2 in two-ended launches (key 8)
   int* dagprof;                  // n * nbt zero-tile profiles of the sub-diagonal slots (k_tile_profile)
+  int* dagprofr;                 // n * nbt: the last nonzero tile column of each tile row, -1 for none
   int prof_valid;                // 1: dagprof describes the current factor (its zero tiles), for pobtasi
   int no_prof;                   // 1: the tile tasks treat every sub-diagonal tile as nonzero (tuning key 9)
+  int sel_cong;                  // 1: pobtasi in congruence form, couplings on the matrix's tiles only (key 10)
 };
 
 // Products of the tile-task launches read by columns from transposed copies of the operand tiles.
@@ -2691,6 +2940,7 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
   const size_t nflags = storage ? (size_t)n * dag_fstride(h->nbt) + n : 1;  // + finished tasks per block
   if (hipMalloc((void**)&h->dagflags, sizeof(int) * nflags) != hipSuccess) rc = DH_EHIP;
   if (storage && hipMalloc((void**)&h->dagprof, sizeof(int) * (size_t)n * h->nbt) != hipSuccess) rc = DH_EHIP;
+  if (storage && hipMalloc((void**)&h->dagprofr, sizeof(int) * (size_t)n * h->nbt) != hipSuccess) rc = DH_EHIP;
   if (hipMalloc((void**)&h->dagcounter, sizeof(int)) != hipSuccess) rc = DH_EHIP;
   if (rc == DH_OK && storage) {
     std::vector<int> tab;
@@ -2731,6 +2981,7 @@ int dh_destroy(dh_handle* h) {
   if (h->err) (void)hipFree(h->err);
   if (h->dagflags) (void)hipFree(h->dagflags);
   if (h->dagprof) (void)hipFree(h->dagprof);
+  if (h->dagprofr) (void)hipFree(h->dagprofr);
   if (h->dagcounter) (void)hipFree(h->dagcounter);
   if (h->dagtab) (void)hipFree(h->dagtab);
   for (int ar = 0; ar < 2; ++ar)
@@ -2976,7 +3227,7 @@ static void tile_profile(dh_handle* h, int lo, int hi) {
   h->prof_valid = (h->dagprof && !h->no_prof && h->lower) ? 1 : 0;
   if (!h->prof_valid || hi <= lo) return;
   hipLaunchKernelGGL(k_tile_profile, dim3(h->nbt, hi - lo), dim3(256), 0, h->stream, (const double*)h->lower, h->bp,
-                     h->nbt, lo, h->dagprof);
+                     h->nbt, lo, h->dagprof, h->dagprofr);
 }
 
 // The two-sided launch of the two-ended factorization (per arrow flag) and its merged task order:
@@ -3628,11 +3879,65 @@ static int selinv_range(dh_handle* h, int sparsity, bool arrow, int lo, int hi) 
   return DH_OK;
 }
 
+// The selected inversion in congruence form (k_selc_*), step by step (one block, or the two sides'
+// blocks of a two-ended factor together).  Scratch per side s: Y in W slot s, Ya in Wa slot s, M and
+// T in S slots 2s and 2s + 1.
+static void selc_args(const dh_handle* h, const SelBlk& k, bool arrow, int s, SelCArgs& c) {
+  const int bp = h->bp, nbt = h->nbt;
+  const long bb = (long)bp * bp;
+  const int slot = k.succ >= 0 ? std::min(k.i, k.succ) : 0;
+  c.Lb = k.succ >= 0 ? h->lower + slot * bb : nullptr;
+  c.La = arrow ? h->arrow + (long)k.i * 64 * bp : nullptr;
+  c.Dn = k.succ >= 0 ? h->diag + k.succ * bb : nullptr;
+  c.Rn = (arrow && k.succ >= 0) ? h->arrow + (long)k.succ * 64 * bp : nullptr;
+  c.tip = h->tip;
+  c.Q = h->qinv + k.i * bb;
+  c.D = h->diag + k.i * bb;
+  c.Y = h->W + (long)s * bp * (bp + 8);
+  c.yld = bp + 8;
+  c.Ya = h->Wa + (long)s * 64 * bp;
+  c.M = h->S + (long)(2 * s) * bb;
+  c.T = h->S + (long)(2 * s + 1) * bb;
+  c.prof = (k.succ >= 0 && h->prof_valid) ? h->dagprof + (long)slot * nbt : nullptr;
+  c.profr = (k.succ >= 0 && h->prof_valid) ? h->dagprofr + (long)slot * nbt : nullptr;
+  c.bp = bp;
+  c.nbt = nbt;
+  c.xt = k.xt;
+}
+
+static int selinv_cong(dh_handle* h, bool arrow, const std::vector<SelBlk>& blk, const std::vector<int>& step0) {
+  const int nbt = h->nbt, ntri = nbt * (nbt + 1) / 2;
+  for (size_t s = 0; s + 1 < step0.size(); ++s) {
+    const int c = step0[s + 1] - step0[s];
+    if (c < 1 || c > 2 || 2 * h->G < 2 * c) return DH_EARG;
+    SelCArgs a0, a1;
+    selc_args(h, blk[step0[s]], arrow, 0, a0);
+    if (c > 1) selc_args(h, blk[step0[s] + 1], arrow, 1, a1);
+    else a1 = a0;
+    const bool m0 = a0.Lb || a0.La, m1 = c > 1 && (a1.Lb || a1.La);
+    const int y0 = selc_ny(a0), y1 = c > 1 ? selc_ny(a1) : 0;
+    if (y0 + y1 > 0) {
+      if (y0 > 0) DH_LAUNCH(h, 2, k_selc_y2, dim3(y0 + y1), dim3(256), 0, a0, y0, a1);
+      else DH_LAUNCH(h, 2, k_selc_y2, dim3(y1), dim3(256), 0, a1, y1, a1);
+    }
+    const int b0 = m0 ? ntri : 0, b1 = m1 ? ntri : 0;
+    if (b0 + b1 > 0) {
+      if (b0 > 0) DH_LAUNCH(h, 2, k_selc_m2, dim3(b0 + b1), dim3(256), 0, a0, b0, a1);
+      else DH_LAUNCH(h, 2, k_selc_m2, dim3(b1), dim3(256), 0, a1, b1, a1);
+    }
+    const int t0 = selc_nt(a0), t1 = c > 1 ? selc_nt(a1) : 0;
+    DH_LAUNCH(h, 2, k_selc_t2, dim3(t0 + t1), dim3(256), 0, a0, m0 ? 1 : 0, t0, a1, m1 ? 1 : 0);
+    DH_LAUNCH(h, 2, k_selc_x2, dim3(ntri * c), dim3(256), 0, a0, ntri, a1);
+  }
+  HIPCHK(hipGetLastError());
+  return DH_OK;
+}
+
 // The Takahashi recursion of a two-ended factor with junction m: block m first (it has no
 // successor), then the two sides outward in lockstep, step k taking block m - k (successor
 // m - k + 1) and block m + k (successor m + k - 1, X stored transposed so that the result is in
 // the one-ended layout).
-static int selinv_twisted(dh_handle* h, bool arrow) {
+static int selinv_twisted(dh_handle* h, bool arrow, bool cong) {
   const int n = h->n, m = h->fact_tw;
   std::vector<SelBlk> blk{SelBlk{m, -1, 0}};
   std::vector<int> step0{0, 1};
@@ -3641,7 +3946,7 @@ static int selinv_twisted(dh_handle* h, bool arrow) {
     if (m + k <= n - 1) blk.push_back(SelBlk{m + k, m + k - 1, 1});
     step0.push_back((int)blk.size());
   }
-  return selinv_steps(h, arrow, blk, step0);
+  return cong ? selinv_cong(h, arrow, blk, step0) : selinv_steps(h, arrow, blk, step0);
 }
 
 extern "C" {
@@ -3654,7 +3959,21 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   if (rc) return rc;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
   if (arrow) selinv_tip(h);
-  rc = h->fact_tw >= 0 ? selinv_twisted(h, arrow) : selinv_range(h, sparsity, arrow, 0, h->n - 1);
+  // the congruence form needs the profiles of the current factor (they bound the tiles it forms)
+  const bool cong = h->sel_cong && h->prof_valid && h->fp_ready != sparsity;
+  if (h->fact_tw >= 0) {
+    rc = selinv_twisted(h, arrow, cong);
+  } else if (cong) {
+    std::vector<SelBlk> blk;
+    std::vector<int> step0{0};
+    for (int i = h->n - 1; i >= 0; --i) {
+      blk.push_back(SelBlk{i, i < h->n - 1 ? i + 1 : -1, 0});
+      step0.push_back((int)blk.size());
+    }
+    rc = selinv_cong(h, arrow, blk, step0);
+  } else {
+    rc = selinv_range(h, sparsity, arrow, 0, h->n - 1);
+  }
   h->fact_tw = -1;
   h->factored = -1;
   h->inverted = 0;
@@ -3867,6 +4186,7 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
   else if (key == 3 && (value == 0 || value == 1)) h->side_on = value;
   else if (key == 5 && (value == 0 || value == 1)) h->force_giveup = value;
   else if (key == 9 && (value == 0 || value == 1)) h->no_prof = value ? 0 : 1;
+  else if (key == 10 && (value == 0 || value == 1)) h->sel_cong = value;
   else if (key == 6 && value >= 0) h->dag_cap = value;
   else if (key == 7 && (value == 0 || value == 1)) {
     if (value == 1) {
@@ -4311,7 +4631,7 @@ int dh_multi_cholesky(dh_multi* mc, int sparsity, void* hip_stream) {
     h->prof_valid = 0;
     if (h->dagprof && h->lower && h->n > 1) {  // the launch's arguments were fixed at creation: profile on, always
       hipLaunchKernelGGL(k_tile_profile, dim3(h->nbt, h->n - 1), dim3(256), 0, st, (const double*)h->lower, h->bp,
-                         h->nbt, 0, h->dagprof);
+                         h->nbt, 0, h->dagprof, h->dagprofr);
       h->prof_valid = 1;  // the zero tiles of the stored blocks are those of the factor, whatever the launch skipped
     }
   }

@@ -44,14 +44,18 @@ from dalia_amd.solver import HipBTASolver
 
 class STInla:
     def __init__(self, data, device="cuda", closed_form_prior: bool = True, overlap: int = 0,
-                 twisted: bool = True):
+                 twisted: bool = True, pattern_selinv: bool = True):
         """overlap: work run beside each factorization on a second stream (`HipBTASolver.set_overlap`),
         at most this mode: 0 none (default), 1 the inverse of the diagonal factors, 2 also the
         selected inversion's products of the factor.  Measured at ns = 2025, nt = 100: the adjoint
         gradient gains about 2 % with 2, as the factorization slows down beside that work.
         twisted: two-ended factorizations (`HipBTASolver.set_twisted`, default, for nt >= 3;
         evaluations with overlap > 0 stay one-ended).  Measured at ns = 2025, nt = 100: pobtaf
-        77.9 -> 70.7 ms, the adjoint gradient 169.1 -> 162.3 ms."""
+        77.9 -> 70.7 ms, the adjoint gradient 169.1 -> 162.3 ms.
+        pattern_selinv: the selected inversion in congruence form with the couplings' blocks formed
+        only on the matrix's tiles (`HipBTASolver.set_selected_inversion_pattern`, default): every
+        consumer here reads the selected inverse on Qc's pattern (the adjoint contractions, the
+        marginal variances)."""
         self.data = data
         self.manifold = getattr(data, "manifold", "plane")
         if self.manifold not in hyper.MANIFOLDS:
@@ -64,6 +68,8 @@ class STInla:
         self.twisted = bool(twisted) and data.nt >= 3
         if self.twisted:
             self.solver.set_twisted(True)
+        self.pattern_selinv = bool(pattern_selinv)
+        self.solver.set_selected_inversion_pattern(self.pattern_selinv)
         N = self.dm.n_latent
         self.mu = torch.zeros(N, dtype=torch.float64, device="cuda")
         self.yp = torch.zeros(N, dtype=torch.float64, device="cuda")

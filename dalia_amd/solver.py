@@ -220,6 +220,16 @@ class HipBTASolver:
         each launch, and the products with the tiles left of it, which stay zero, are skipped."""
         _lib.check(self.lib.dh_set_tuning(self.h, 9, 1 if on else 0), "dh_set_tuning")
 
+    def set_selected_inversion_pattern(self, on: bool = True) -> None:
+        """The selected inversion in congruence form (`dh_set_tuning` key 10, off by default):
+        X_ii = inv(L_ii)^T (I + M) inv(L_ii) with M = L^T (X_qq L + X_aq^T L_a) + L_a^T Ya, and the
+        sub-diagonal blocks X_{i+1,i} formed only on the 64x64 tiles where the matrix's coupling block
+        has entries (other tiles keep stale values).  For callers that read the selected inverse on
+        the matrix's pattern only; the reference's selected_inversion returns whole blocks, which
+        stays the default.  Needs the zero-tile profiles (`set_tile_profile`), else the dense form
+        runs."""
+        _lib.check(self.lib.dh_set_tuning(self.h, 10, 1 if on else 0), "dh_set_tuning")
+
     def selected_inversion(self, sparsity: str = "bta") -> None:
         _lib.check(self.lib.dh_selected_inversion(self.h, _check_sparsity(sparsity)), "dh_selected_inversion")
 

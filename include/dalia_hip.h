@@ -334,6 +334,14 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
  *        the factorization's tile products out).  The factor is the same.  0 = every tile is
  *        treated as nonzero.  dh_multi_cholesky follows the setting its handles had when
  *        dh_multi_create ran.
+ * key 10: 1 = dh_selected_inversion in congruence form, for callers that read the selected
+ *        inverse on the matrix's pattern only: X_ii = inv(L_ii)^T (I + M) inv(L_ii) with
+ *        M = L^T Y + L_a^T Ya, Y = X_qq L + X_aq^T L_a, Ya = X_aq L + X_aa L_a (L = L_{q,i}, q the
+ *        successor), X_{a,i} = -Ya inv(L_ii), and X_{q,i} = -Y inv(L_ii) only on the 64x64 tiles
+ *        of each coupling block between the first and last tile of a tile row where the matrix
+ *        has entries (the others keep stale values).  Products with L skip its zero tiles.  Needs
+ *        the profiles of the current factor (key 9); otherwise, and with key 4 = 2, the default
+ *        form runs.  0 = the whole blocks (default, as the reference's selected inversion).
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 
