@@ -122,10 +122,14 @@ __global__ __launch_bounds__(256) void k_copy(const double* x, double* y, int le
 // those with local index >= tlo are written transposed, as block (t, t + 1) (the layout of the
 // upward side of a two-ended factorization).  The FEM and temporal matrices are symmetric and so is
 // A^T A, so row s of block (t, t + 1) is row (t, s) of Q restricted to time block t + 1.
+// prof, profr (optional): the zero-tile profiles of the coupling blocks as written (first and last
+// tile column of each 64-row tile row holding a nonzero entry), gathered from the rows in LDS with
+// the scan's test (v != 0.0); they must hold nbt and -1 on entry.
 __global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* lower, int n, int nl, int t0, int b,
                                                        int bp, dh_model m, const double* params, int with_like,
-                                                       int tlo) {
+                                                       int tlo, int* prof, int* profr) {
   extern __shared__ double row[];
+  __shared__ int s_lo, s_hi;
   const long r = blockIdx.x;
   const long ndiag = (long)n * bp;
   int t, s, trow, tcol;
@@ -149,6 +153,10 @@ __global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* low
     out = lower + (long)tl * bp * bp + (long)s * bp;
   }
   for (int c = threadIdx.x; c < bp; c += blockDim.x) row[c] = 0.0;
+  if (threadIdx.x == 0) {
+    s_lo = bp;
+    s_hi = -1;
+  }
   __syncthreads();
   if (s < b) {
     const int nt = m.nt;
@@ -186,6 +194,25 @@ __global__ __launch_bounds__(256) void k_assemble_rows(double* diag, double* low
   __syncthreads();
   double2* o2 = reinterpret_cast<double2*>(out);
   for (int c = threadIdx.x; c < bp / 2; c += blockDim.x) o2[c] = make_double2(row[2 * c], row[2 * c + 1]);
+  if (!isdiag && prof) {  // workgroup-uniform
+    int lo = bp, hi = -1;
+    for (int c = threadIdx.x; c < bp; c += blockDim.x)
+      if (row[c] != 0.0) {
+        lo = min(lo, c);
+        hi = max(hi, c);
+      }
+    if (hi >= 0) {
+      atomicMin(&s_lo, lo);
+      atomicMax(&s_hi, hi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_hi >= 0) {
+      const int nbt = bp / DH_TILE;
+      const long q = (r - ndiag) / bp * nbt + s / DH_TILE;
+      atomicMin(prof + q, s_lo / DH_TILE);
+      atomicMax(profr + q, s_hi / DH_TILE);
+    }
+  }
 }
 
 // Arrow rows of time blocks t0 .. t0 + n - 1 into arrow[0 .. n - 1].
@@ -1776,11 +1803,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // A diagonal tile of a symmetric result stored exactly symmetric: C[i0 + r][i0 + c] = alpha times
-// the accumulated entry (max(r, c), min(r, c)), mirrored through dh_lS.  The lower-tile-and-mirror
-// scheme makes every off-diagonal tile pair symmetric; a diagonal tile stored as accumulated keeps a
-// rounding-level antisymmetric part, which the congruence form's next block (X_qq read by columns,
-// M's tiles mirrored) turns into a symmetric error scaled by cond(L_ii)^2 that grows from block to
-// block.  Workgroup-wide (every thread calls it).
+// the accumulated entry (max(r, c), min(r, c)), its lower triangle mirrored through LDS.  The
+// lower-tile-and-mirror scheme makes every off-diagonal tile pair symmetric; a diagonal tile stored as
+// accumulated keeps a rounding-level antisymmetric part, which the congruence form's next block (X_qq
+// read by columns, M's tiles mirrored) turns into a symmetric error scaled by cond(L_ii)^2 that grows
+// from block to block.  Workgroup-wide (every thread calls it).  The packed lower triangle (16.6 KB)
+// leaves the kernels' occupancy to their registers.
+__shared__ double dh_lsym[DH_TILE * (DH_TILE + 1) / 2];  // lower triangle of a 64x64 tile, packed by rows
+
 __device__ __forceinline__ void dh_store_sym_tile(double* C, long ldc, int i0, const dh_d4 acc[2][2], double alpha) {
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -1791,11 +1821,14 @@ __device__ __forceinline__ void dh_store_sym_tile(double* C, long ldc, int i0, c
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) dh_lS[(rbase + bi * 16 + 4 * r) * DH_LDS_LD + cbase + bj * 16] = acc[bi][bj][r];
+      for (int r = 0; r < 4; ++r) {
+        const int i = rbase + bi * 16 + 4 * r, j = cbase + bj * 16;
+        if (j <= i) dh_lsym[i * (i + 1) / 2 + j] = acc[bi][bj][r];
+      }
   __syncthreads();
   for (int e = threadIdx.x; e < DH_TILE * DH_TILE; e += blockDim.x) {
     const int i = e >> 6, j = e & 63;
-    const double v = i >= j ? dh_lS[i * DH_LDS_LD + j] : dh_lS[j * DH_LDS_LD + i];
+    const double v = i >= j ? dh_lsym[i * (i + 1) / 2 + j] : dh_lsym[j * (j + 1) / 2 + i];
     C[(long)(i0 + i) * ldc + i0 + j] = alpha * v;
   }
 }
@@ -2722,6 +2755,8 @@ struct dh_handle {
   int prof_valid;                // 1: dagprof describes the current factor (its zero tiles), for pobtasi
   int no_prof;                   // 1: the tile tasks treat every sub-diagonal tile as nonzero (tuning key 9)
   int sel_cong;                  // 1: pobtasi in congruence form, couplings on the matrix's tiles only (key 10)
+  int prof_by_asm;               // 1: dh_assemble records the profiles as it writes the couplings (key 11)
+  int prof_asm;                  // 1: dagprof/dagprofr hold the profiles of the couplings dh_assemble wrote
 };
 
 // Products of the tile-task launches read by columns from transposed copies of the operand tiles.
@@ -3028,10 +3063,18 @@ int dh_assemble(dh_handle* h, const dh_model* m, const double* params, int spars
   const long rows = (long)(2 * h->n - 1) * h->bp;
   // for a two-ended factorization the upward side's coupling blocks are written transposed
   const bool tw = twist_active(h);
+  const bool rec = h->prof_by_asm && h->dagprof && !h->no_prof && h->n >= 2;
+  if (rec) {
+    const int cnt = (h->n - 1) * h->nbt;
+    hipLaunchKernelGGL(k_fill_int, dim3((cnt + 255) / 256), dim3(256), 0, h->stream, h->dagprof, h->nbt, cnt);
+    hipLaunchKernelGGL(k_fill_int, dim3((cnt + 255) / 256), dim3(256), 0, h->stream, h->dagprofr, -1, cnt);
+  }
   DH_LAUNCH(h, 5, k_assemble_rows, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double), h->diag,
-            h->lower, h->n, h->n - 1, 0, h->b, h->bp, *m, params, with_like, tw ? h->tw_m : h->n);
+            h->lower, h->n, h->n - 1, 0, h->b, h->bp, *m, params, with_like, tw ? h->tw_m : h->n,
+            rec ? h->dagprof : nullptr, rec ? h->dagprofr : nullptr);
   HIPCHK(hipGetLastError());
   h->asm_tw = tw ? 1 : 0;
+  h->prof_asm = rec ? 1 : 0;
   if (h->a > 0) {
     const int wl = (sparsity == DH_SPARSITY_BTA) ? with_like : 0;
     hipLaunchKernelGGL(k_assemble_arrow, dim3(h->n * h->ap), dim3(256), 0, h->stream, h->arrow, h->n, 0, h->b, h->a,
@@ -3060,6 +3103,7 @@ int dh_assemble_coreg(dh_handle* h, const dh_model* m, int nv, const double* par
             h->lower, h->n, h->bp, cm, params, with_like);
   HIPCHK(hipGetLastError());
   h->asm_tw = 0;
+  h->prof_asm = 0;
   if (h->a > 0) {
     const int wl = (sparsity == DH_SPARSITY_BTA) ? with_like : 0;
     hipLaunchKernelGGL(k_assemble_arrow_coreg, dim3(h->n * h->ap), dim3(256), 0, h->stream, h->arrow, h->n, h->bp,
@@ -3093,6 +3137,7 @@ int dh_scatter_coo(dh_handle* h, long long nnz, const long long* rows, const lon
                        (sparsity == DH_SPARSITY_BTA && h->a > 0) ? 1 : 0);
   HIPCHK(hipGetLastError());
   h->asm_tw = 0;
+  h->prof_asm = 0;
   h->factored = -1;
   h->inverted = 0;
   return DH_OK;
@@ -3223,9 +3268,10 @@ static DagArgs dag_args(const dh_handle* h, int ar, int slot0, int cnt, bool has
 
 // The zero-tile profiles of sub-diagonal slots [lo, hi) from the blocks as stored now (the
 // matrix before a factorization, or a final factor such as a restored boundary block).
-static void tile_profile(dh_handle* h, int lo, int hi) {
+// from_asm: dh_assemble recorded the profiles of all couplings as it wrote them (no scan).
+static void tile_profile(dh_handle* h, int lo, int hi, bool from_asm = false) {
   h->prof_valid = (h->dagprof && !h->no_prof && h->lower) ? 1 : 0;
-  if (!h->prof_valid || hi <= lo) return;
+  if (!h->prof_valid || hi <= lo || (from_asm && lo == 0 && hi == h->n - 1)) return;
   hipLaunchKernelGGL(k_tile_profile, dim3(h->nbt, hi - lo), dim3(256), 0, h->stream, (const double*)h->lower, h->bp,
                      h->nbt, lo, h->dagprof, h->dagprofr);
 }
@@ -3294,16 +3340,20 @@ static void twist_set_t(const dh_handle* h, DagArgs& g) {
 // Sub-diagonal blocks m .. n-2 transposed in place (into or out of the two-ended layout).
 static void twist_lower(dh_handle* h) {
   const int n = h->n, m = h->tw_m, nbt = h->nbt;
+  h->prof_asm = 0;  // the profiles describe the blocks as they were written
   hipLaunchKernelGGL(k_transpose_blocks, dim3(nbt * (nbt + 1) / 2, n - 1 - m), dim3(256), 0, h->stream,
                      h->lower + (long)m * h->bp * h->bp, h->bp);
 }
 
 // The two-ended factorization: both sides in one persistent launch, the upward side's terms
 // applied to block m, block m factored (the downward side's terms through has_prev), then the tip.
-static int cholesky_twisted(dh_handle* h, int sparsity, int ar) {
+static int cholesky_twisted(dh_handle* h, int sparsity, int ar, bool pa) {
   const int n = h->n, m = h->tw_m, bp = h->bp, nbt = h->nbt, FS = dag_fstride(nbt);
   const long bb = (long)bp * bp;
-  if (!h->asm_tw) twist_lower(h);
+  if (!h->asm_tw) {
+    twist_lower(h);
+    pa = false;
+  }
   h->asm_tw = 0;
   hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags, 0, n * FS + n);
   if (ar) hipLaunchKernelGGL(k_fill_dbl, dim3(16), dim3(256), 0, h->stream, h->tipB, 0.0, 64 * 64);
@@ -3313,7 +3363,7 @@ static int cholesky_twisted(dh_handle* h, int sparsity, int ar) {
   const int cap = h->dag_cap > 0 ? std::min(h->dag_cap, h->dag_wgs) : h->dag_wgs;
   DagArgs gt = h->tw_g[ar];
   twist_set_t(h, gt);
-  tile_profile(h, 0, n - 1);
+  tile_profile(h, 0, n - 1, pa);
   gt.prof = (h->dagprof && !h->no_prof) ? h->dagprof : nullptr;
   gt.prof2 = gt.prof ? h->dagprof + (long)(n - 2) * nbt : nullptr;
   gt.cols = cols_for(h, true) ? 1 : 0;
@@ -3364,13 +3414,16 @@ int dh_cholesky(dh_handle* h, int sparsity) {
   h->fp_ready = -1;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->info, INT_MAX);
   h->prof_valid = 0;  // set again by the launches that read the profiles
+  bool pa = h->prof_asm != 0;  // profiles recorded by the assembly, valid for this factorization only
+  h->prof_asm = 0;
   // force_giveup: the flag that a timed-out readiness wait raises is set before the launch, so
   // that every wait of the persistent launch returns at once (tuning key 5, a test of the error path)
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->err, h->force_giveup ? 4 : 0);
   h->fact_tw = -1;
-  if (twist_active(h)) return cholesky_twisted(h, sparsity, arrow ? 1 : 0);
+  if (twist_active(h)) return cholesky_twisted(h, sparsity, arrow ? 1 : 0, pa);
   if (h->asm_tw) {  // assembled for a two-ended factorization that is no longer selected
     twist_lower(h);
+    pa = false;
     h->asm_tw = 0;
   }
   if (h->pobtaf_alg == 1) {
@@ -3380,7 +3433,7 @@ int dh_cholesky(dh_handle* h, int sparsity) {
     const bool fp = ov && h->overlap == 2 && h->Wall;
     hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, h->stream, h->dagflags, 0, n * dag_fstride(nbt) + n);
     const int ar = arrow ? 1 : 0;
-    tile_profile(h, 0, n - 1);
+    tile_profile(h, 0, n - 1, pa);
     DagArgs g = dag_args(h, ar, 0, n, false, false, h->tip, 0);
     g.bdone = ov ? h->dagflags + (long)n * dag_fstride(nbt) : nullptr;
     if (ov) {  // the side stream starts after the flags are cleared
@@ -3474,7 +3527,10 @@ int dh_cholesky(dh_handle* h, int sparsity) {
 // Packed inverse of every diagonal factor, batched G blocks per launch.
 static int ensure_inverse(dh_handle* h) {
   if (h->inverted) return DH_OK;
-  for (int g0 = 0; g0 < h->n; g0 += h->G) inverse_chunk(h, g0, std::min(h->G, h->n - g0), h->stream, false);
+  // chunks of 2G blocks: the level kernels' scratch is bp^2 per block in the W slots (2G of them,
+  // idle here), and the small levels' launches are latency bound, so fewer and wider launches pay
+  const int G2 = 2 * h->G;
+  for (int g0 = 0; g0 < h->n; g0 += G2) inverse_chunk(h, g0, std::min(G2, h->n - g0), h->stream, false);
   HIPCHK(hipGetLastError());
   h->inverted = 1;
   return DH_OK;
@@ -4187,6 +4243,10 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
   else if (key == 5 && (value == 0 || value == 1)) h->force_giveup = value;
   else if (key == 9 && (value == 0 || value == 1)) h->no_prof = value ? 0 : 1;
   else if (key == 10 && (value == 0 || value == 1)) h->sel_cong = value;
+  else if (key == 11 && (value == 0 || value == 1)) {
+    h->prof_by_asm = value;
+    h->prof_asm = 0;
+  }
   else if (key == 6 && value >= 0) h->dag_cap = value;
   else if (key == 7 && (value == 0 || value == 1)) {
     if (value == 1) {
@@ -4300,7 +4360,9 @@ static int ckpt_assemble(dh_ckpt* c, const dh_model* m, const double* params, bo
   const int nl = cnt - 1 + ((t0 + cnt < c->n) ? 1 : 0);
   const long rows = (long)(cnt + nl) * h->bp;
   DH_LAUNCH(h, 5, k_assemble_rows, dim3((unsigned)rows), dim3(256), (size_t)h->bp * sizeof(double),
-            h->diag + slot0 * bb, h->lower + slot0 * bb, cnt, nl, t0, h->b, h->bp, *m, params, bta ? 1 : 0, nl);
+            h->diag + slot0 * bb, h->lower + slot0 * bb, cnt, nl, t0, h->b, h->bp, *m, params, bta ? 1 : 0, nl,
+            nullptr, nullptr);
+  h->prof_asm = 0;
   if (bta && h->a > 0)
     hipLaunchKernelGGL(k_assemble_arrow, dim3(cnt * h->ap), dim3(256), 0, h->stream,
                        h->arrow + (long)slot0 * h->ap * h->bp, cnt, t0, h->b, h->a, h->bp, h->ap, *m, params, 1);
@@ -4620,6 +4682,7 @@ int dh_multi_cholesky(dh_multi* mc, int sparsity, void* hip_stream) {
   for (dh_handle* h : mc->hs) {
     h->fp_ready = -1;
     h->fact_tw = -1;
+    h->prof_asm = 0;  // this launch scans the stored blocks itself
     if (h->asm_tw) {  // assembled for a two-ended factorization: back to the one-ended layout
       hipLaunchKernelGGL(k_transpose_blocks, dim3(h->nbt * (h->nbt + 1) / 2, h->n - 1 - h->tw_m), dim3(256), 0,
                          st, h->lower + (long)h->tw_m * h->bp * h->bp, h->bp);

@@ -44,7 +44,7 @@ from dalia_amd.solver import HipBTASolver
 
 class STInla:
     def __init__(self, data, device="cuda", closed_form_prior: bool = True, overlap: int = 0,
-                 twisted: bool = True, pattern_selinv: bool = True):
+                 twisted: bool = True, pattern_selinv: bool = True, profile_from_assembly: bool = True):
         """overlap: work run beside each factorization on a second stream (`HipBTASolver.set_overlap`),
         at most this mode: 0 none (default), 1 the inverse of the diagonal factors, 2 also the
         selected inversion's products of the factor.  Measured at ns = 2025, nt = 100: the adjoint
@@ -55,7 +55,10 @@ class STInla:
         pattern_selinv: the selected inversion in congruence form with the couplings' blocks formed
         only on the matrix's tiles (`HipBTASolver.set_selected_inversion_pattern`, default): every
         consumer here reads the selected inverse on Qc's pattern (the adjoint contractions, the
-        marginal variances)."""
+        marginal variances).
+        profile_from_assembly: the zero-tile profiles of Qc's couplings recorded by the assembly
+        (`HipBTASolver.set_profile_from_assembly`, default) instead of scanned before each
+        factorization (0.66 ms at ns = 2025, nt = 100); the same profiles."""
         self.data = data
         self.manifold = getattr(data, "manifold", "plane")
         if self.manifold not in hyper.MANIFOLDS:
@@ -70,6 +73,7 @@ class STInla:
             self.solver.set_twisted(True)
         self.pattern_selinv = bool(pattern_selinv)
         self.solver.set_selected_inversion_pattern(self.pattern_selinv)
+        self.solver.set_profile_from_assembly(bool(profile_from_assembly))
         N = self.dm.n_latent
         self.mu = torch.zeros(N, dtype=torch.float64, device="cuda")
         self.yp = torch.zeros(N, dtype=torch.float64, device="cuda")

@@ -230,6 +230,15 @@ class HipBTASolver:
         runs."""
         _lib.check(self.lib.dh_set_tuning(self.h, 10, 1 if on else 0), "dh_set_tuning")
 
+    def set_profile_from_assembly(self, on: bool = True) -> None:
+        """The zero-tile profiles recorded by the model assembly (`dh_assemble`) as it writes the
+        coupling blocks, instead of a scan of the blocks at the start of the next `cholesky`
+        (`dh_set_tuning` key 11, off by default).  Same profiles (the same test on the same values);
+        valid for the one factorization that follows the assembly, and dropped by any other write
+        through the library (`cholesky(A)`, a layout change).  Writing the blocks through the tensor
+        views between the assembly and the factorization is not seen: leave it off then."""
+        _lib.check(self.lib.dh_set_tuning(self.h, 11, 1 if on else 0), "dh_set_tuning")
+
     def selected_inversion(self, sparsity: str = "bta") -> None:
         _lib.check(self.lib.dh_selected_inversion(self.h, _check_sparsity(sparsity)), "dh_selected_inversion")
 

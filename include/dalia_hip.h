@@ -342,6 +342,12 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
  *        has entries (the others keep stale values).  Products with L skip its zero tiles.  Needs
  *        the profiles of the current factor (key 9); otherwise, and with key 4 = 2, the default
  *        form runs.  0 = the whole blocks (default, as the reference's selected inversion).
+ * key 11: 1 = dh_assemble records the zero-tile profiles of the coupling blocks (key 9) while it
+ *        writes them (the scan's test, v != 0.0, on the rows it builds), and the next
+ *        dh_cholesky uses them instead of scanning the blocks.  Valid for that one factorization;
+ *        dh_scatter_coo, dh_assemble_coreg, a change between the one- and two-ended layouts and
+ *        dh_multi_cholesky drop them.  Writes to the blocks by other means in between are not
+ *        seen.  0 = scan at each factorization (default).
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 

@@ -96,10 +96,12 @@ def test_profile_matches_dense_and_unprofiled(n, b, a, kind, twisted):
         assert rel(on["XT"], Xt) < 1e-9
 
 
+@pytest.mark.parametrize("by_asm", [False, True])
 @pytest.mark.parametrize("twisted", [False, True])
-def test_device_profile_of_the_model(twisted):
+def test_device_profile_of_the_model(twisted, by_asm):
     """The profile the library reads from the assembled Qc of the spatio-temporal model (ns = 2025,
-    45 x 45 grid in row-major node order) equals the one computed here from the oracle's sparse Qc:
+    45 x 45 grid in row-major node order), scanned before the factorization or recorded by the
+    assembly (`dh_set_tuning` key 11), equals the one computed here from the oracle's sparse Qc:
     per sub-diagonal block, the first 64-column tile of each tile row with a nonzero entry (the
     two-ended layout stores blocks m .. n-2 transposed).  On this grid it is max(0, R - 3)."""
     from dalia_amd import synth
@@ -107,8 +109,8 @@ def test_device_profile_of_the_model(twisted):
     from oracle import inla as oinla
     import bench
     d = synth.make_st_model(45, 6, 6, 1)
-    eng = STInla(d, twisted=twisted)
-    print(f"case device profile twisted={twisted}", flush=True)
+    eng = STInla(d, twisted=twisted, profile_from_assembly=by_asm)
+    print(f"case device profile twisted={twisted} by_asm={by_asm}", flush=True)
     eng.f(d.theta)
     got = bench.tile_profile(eng)
     Q = oinla.qcond_sparse(d, d.theta).tocsr()
@@ -125,3 +127,32 @@ def test_device_profile_of_the_model(twisted):
         np.minimum.at(exp, B.row[nz] // 64, B.col[nz] // 64)
         assert np.array_equal(got[s], exp), (s, got[s], exp)
     assert np.array_equal(got[0], np.maximum(0, np.arange(nbt) - 3))
+
+
+@pytest.mark.parametrize("twisted", [False, True])
+def test_profile_from_assembly_bitwise(twisted):
+    """The engine with the profiles recorded by the assembly and with the scan: f, the adjoint
+    gradient, mu and the latent marginal variances agree to the last bit (the same profiles, first and
+    last nonzero tile of each tile row, drive the factorization and the congruence-form selected
+    inversion), over two evaluations at different theta."""
+    from dalia_amd import synth
+    from dalia_amd.inla import STInla
+    d = synth.make_st_model(45, 7, 6, 3)
+    print(f"case profile from assembly twisted={twisted}", flush=True)
+    out = []
+    for by_asm in (True, False):
+        eng = STInla(d, twisted=twisted, profile_from_assembly=by_asm)
+        res = []
+        for th in (d.theta, d.theta + np.array([0.1, -0.05, 0.07, 0.2])):
+            f, g = eng.value_and_grad(th)
+            mu = eng.mu.cpu().numpy().copy()
+            v = eng.marginal_variances_latent(th, mu)
+            res.append((f, g, mu, v))
+        out.append(res)
+        del eng
+    for (f1, g1, mu1, v1), (f0, g0, mu0, v0) in zip(*out):
+        assert f1 == f0
+        assert np.array_equal(g1, g0)
+        assert np.array_equal(mu1, mu0)
+        assert np.array_equal(v1, v0)
+
