@@ -44,6 +44,22 @@ def f_pobtaf(n, b, a):
     return n * b ** 3 / 3 + 2 * (n - 1) * b ** 3 + n * a * b ** 2 + 2 * (n - 1) * a * b ** 2 + n * a * a * b + a ** 3 / 3
 
 
+def solve_launches(nt: int, nb: int, twisted: bool) -> int:
+    """Kernel launches of one dh_solve: per block step the row GEMV with the coupling (or a copy),
+    the column partials and their sum with the packed inverse, and the arrow partials (forward);
+    the column partials with the coupling and arrow rows, their sum and the packed-inverse row GEMV
+    (backward); plus packing, the arrow sums and the tip.  Two-ended: the two sides' steps share
+    their launches, and the junction block runs alone."""
+    a = 1 if nb else 0
+    if not twisted:
+        return 2 + nt * (3 + a) + a + 2 * a + 3 * nt - (0 if a else 1)
+    m = (nt - 1) // 2
+    steps = max(m, nt - 1 - m)
+    fwd = steps * (3 + a) + 2 * a + (4 + a) + a
+    bwd = (3 if a else 2) + 3 * steps
+    return 2 + fwd + 2 * a + bwd
+
+
 def tile_widths(b):
     return np.array([min(64, b - 64 * t) for t in range((b + 63) // 64)], dtype=float)
 
@@ -716,7 +732,7 @@ def main():
     sol_bytes = 16.0 * (nt * ns * (ns + 1) / 2 + (nt - 1) * ns * ns + nt * nb * ns + nb * (nb + 1) / 2)
     hbm["pobtas"] = {"bound": "hbm", "achieved": sol_bytes / sol_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": sol_bytes / sol_s / 1e9 / HBM_PEAK_GBS, "ms": sol_s * 1e3, "bytes": sol_bytes,
-                     "launches_per_solve": 5 + 7 * nt}
+                     "launches_per_solve": solve_launches(nt, nb, bool(eng.twisted))}
     log(f"assembly {asm_s * 1e3:.2f} ms ({hbm['k_assemble_rows']['achieved']:.0f} GB/s); "
         f"pobtas {sol_s * 1e3:.2f} ms ({hbm['pobtas']['achieved']:.0f} GB/s)")
 

@@ -1892,6 +1892,10 @@ struct SelCArgs {
   int xt;              // 1: X_{q,i} stored transposed (the coupling slot holds the rows of q)
 };
 
+#ifndef DH_SELC_S
+#define DH_SELC_S 2  // operand ring depth of the congruence form's main products
+#endif
+
 // Tile rows R of L with a nonzero tile in some column <= K: R < selc_rend(a, K) (a prefix that holds
 // them all; products with the zero tiles in it add nothing).
 __device__ __forceinline__ int selc_rend(const SelCArgs& a, int K) {
@@ -1937,7 +1941,7 @@ __device__ __forceinline__ void selc_y_work(const SelCArgs& a, int e) {
     const int I = r;
     const int rend = selc_rend(a, K);
     // opM[i][r] = X_qq(r, i) = X_qq(i, r) and opM[k][r] = L(r, k): both read by columns
-    dh_mma<M_COL, M_COL>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
+    dh_mma<M_COL, M_COL, DH_SELC_S>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
                          K * DH_TILE, 0, rend * DH_TILE);
     if (a.Rn && a.La)
       dh_mma<M_COL, M_COL>(acc, Opnd{a.Rn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.La, a.bp, M_COL, nullptr},
@@ -1970,7 +1974,7 @@ __device__ __forceinline__ void selc_m_work(const SelCArgs& a, int e) {
   dh_d4 acc[2][2];
   dh_zero(acc);
   if (a.Lb)  // opM[k'][r] = L(r, k'), opM[k][r] = Y(r, k): both by columns
-    dh_mma<M_COL, M_COL>(acc, Opnd{a.Lb, a.bp, M_COL, nullptr}, Kp * DH_TILE, Opnd{a.Y, a.yld, M_COL, nullptr},
+    dh_mma<M_COL, M_COL, DH_SELC_S>(acc, Opnd{a.Lb, a.bp, M_COL, nullptr}, Kp * DH_TILE, Opnd{a.Y, a.yld, M_COL, nullptr},
                          K * DH_TILE, 0, selc_rend(a, Kp) * DH_TILE);
   if (a.La)
     dh_mma<M_COL, M_COL>(acc, Opnd{a.La, a.bp, M_COL, nullptr}, Kp * DH_TILE, Opnd{a.Ya, a.bp, M_COL, nullptr},
@@ -2007,7 +2011,8 @@ __device__ __forceinline__ void selc_t_work(const SelCArgs& a, bool has_m, int e
     const int K = c + e;
     dh_load_acc(a.Q, a.bp, K * DH_TILE, c * DH_TILE, acc);  // the identity's term: inv(L)(K, c)
     if (has_m)  // opM[k][k'] = M(k', k) = M(k, k')
-      dh_mma<M_COL, M_COL>(acc, Opnd{a.M, a.bp, M_COL, nullptr}, K * DH_TILE, q, c * DH_TILE, c * DH_TILE, a.bp);
+      dh_mma<M_COL, M_COL, DH_SELC_S>(acc, Opnd{a.M, a.bp, M_COL, nullptr}, K * DH_TILE, q, c * DH_TILE, c * DH_TILE,
+                                      a.bp);
     dh_store(a.T, a.bp, K * DH_TILE, c * DH_TILE, acc, 1.0, 0.0, false);
     return;
   }
@@ -2045,7 +2050,7 @@ __device__ __forceinline__ void selc_x_work(const SelCArgs& a, int e) {
   dh_d4 acc[2][2];
   dh_zero(acc);
   // opM[i][k] = inv(L)(k, i) (k >= i) and opM[j][k] = T(k, j) (lower tiles, k >= i >= j): by columns
-  dh_mma<M_COL, M_COL>(acc, Opnd{a.Q, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.T, a.bp, M_COL, nullptr},
+  dh_mma<M_COL, M_COL, DH_SELC_S>(acc, Opnd{a.Q, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.T, a.bp, M_COL, nullptr},
                        J * DH_TILE, I * DH_TILE, a.bp);
   if (I == J) {
     dh_store_sym_tile(a.D, a.bp, I * DH_TILE, acc, 1.0);
@@ -2097,8 +2102,8 @@ __global__ __launch_bounds__(256) void k_unpack(const double* z, double* rhs, in
 // y[r] = yin[r] + alpha * sum_c M[r][c] x[c]; one wave per row.
 // y[r] = yin[r] + alpha * M[r, :ncols] x (one wave per row, ncols even; double2 loads issued in
 // groups of four so that each wave keeps several KB in flight).
-__global__ __launch_bounds__(256) void k_gemv_row(const double* M, long ld, int nrows, int ncols, const double* x,
-                                                  const double* yin, double* y, double alpha) {
+__device__ __forceinline__ void gemv_row_body(const double* M, long ld, int nrows, int ncols, const double* x,
+                                              const double* yin, double* y, double alpha) {
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (r >= nrows) return;
@@ -2122,12 +2127,35 @@ __global__ __launch_bounds__(256) void k_gemv_row(const double* M, long ld, int 
   if (lane == 0) y[r] = (yin ? yin[r] : 0.0) + alpha * s;
 }
 
+__global__ __launch_bounds__(256) void k_gemv_row(const double* M, long ld, int nrows, int ncols, const double* x,
+                                                  const double* yin, double* y, double alpha) {
+  gemv_row_body(M, ld, nrows, ncols, x, yin, y, alpha);
+}
+
+// Operands of one side of a paired sweep step (the two sides of a two-ended factor, blockIdx.z).
+struct GvRow {
+  const double* M;    // null: y = yin (a copy)
+  const double* x;
+  const double* yin;
+  double* y;
+};
+
+__global__ __launch_bounds__(256) void k_gemv_row2(GvRow a0, GvRow a1, long ld, int nrows, int ncols, double alpha) {
+  const GvRow& a = blockIdx.z ? a1 : a0;
+  if (a.M) {
+    gemv_row_body(a.M, ld, nrows, ncols, a.x, a.yin, a.y, alpha);
+    return;
+  }
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r < nrows && (threadIdx.x & 63) == 0) a.y[r] = a.yin[r];
+}
+
 // Packed inverse factor, row form: y[c] = dinv[c] w[c] + sum_{r>c} D[c][r] w[r]
 // (y = inv(L)^T w).  One wave per row c.
 // y[c] = dinv[c] w[c] + sum_{r > c} D[c][r] w[r]: x = inv(L)^T w with inv(L)^T packed in the strict
 // upper triangle (one wave per row, double2 loads in groups of four).
-__global__ __launch_bounds__(256) void k_upk_row(const double* D, const double* dinv, int bp, const double* w,
-                                                 double* y) {
+__device__ __forceinline__ void upk_row_body(const double* D, const double* dinv, int bp, const double* w,
+                                             double* y) {
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (c >= bp) return;
@@ -2151,6 +2179,23 @@ __global__ __launch_bounds__(256) void k_upk_row(const double* D, const double* 
   }
   s = dh_wave_sum(s);
   if (lane == 0) y[c] = dinv[c] * w[c] + s;
+}
+
+__global__ __launch_bounds__(256) void k_upk_row(const double* D, const double* dinv, int bp, const double* w,
+                                                 double* y) {
+  upk_row_body(D, dinv, bp, w, y);
+}
+
+struct Upk {
+  const double* D;
+  const double* dinv;
+  const double* w;
+  double* y;
+};
+
+__global__ __launch_bounds__(256) void k_upk_row2(Upk a0, Upk a1, int bp) {
+  const Upk& a = blockIdx.z ? a1 : a0;
+  upk_row_body(a.D, a.dinv, bp, a.w, a.y);
 }
 
 #define DH_CCH 64  // rows per chunk of the column-oriented GEMV partials
@@ -2207,8 +2252,8 @@ __device__ __forceinline__ void gv_col_tile(const double* M, long ld, int r0, in
 
 // part[chunk][c] = sum_{r in chunk} M[r][c] x[r] (upk: only r < c; M is then the packed inverse
 // inv(L)^T, whose columns are the rows of inv(L)).  Grid (ceil(ncols/128), ceil(nrows/DH_CCH)).
-__global__ __launch_bounds__(256) void k_gemv_col_part(const double* M, long ld, int nrows, int ncols,
-                                                       const double* x, double* part, int upk) {
+__device__ __forceinline__ void col_part_body(const double* M, long ld, int nrows, int ncols, const double* x,
+                                              double* part, int upk) {
   const int chunk = blockIdx.y;
   const int r0 = chunk * DH_CCH, r1 = min(nrows, r0 + DH_CCH);
   if (upk)
@@ -2217,12 +2262,29 @@ __global__ __launch_bounds__(256) void k_gemv_col_part(const double* M, long ld,
     gv_col_tile<false>(M, ld, r0, r1, ncols, x, part + (long)chunk * ncols);
 }
 
+__global__ __launch_bounds__(256) void k_gemv_col_part(const double* M, long ld, int nrows, int ncols,
+                                                       const double* x, double* part, int upk) {
+  col_part_body(M, ld, nrows, ncols, x, part, upk);
+}
+
+struct GvCol {
+  const double* M;
+  const double* x;
+  double* part;
+};
+
+__global__ __launch_bounds__(256) void k_gemv_col_part_2s(GvCol a0, GvCol a1, long ld, int nrows, int ncols,
+                                                          int upk) {
+  const GvCol& a = blockIdx.z ? a1 : a0;
+  col_part_body(a.M, ld, nrows, ncols, a.x, a.part, upk);
+}
+
 // y[c] = base(c) + sign * sum_chunks part[chunk][c]; base = dinv[c] v[c] (fwd) or v[c] (bwd)
 // y[c] = base[c] + sign * sum_k part[k][c] (base = dinv[c] v[c], or v[c] without dinv): 64 columns
 // per workgroup, wave w adding the partials k = w, w + 4, ... (eight loads in flight), the four
 // wave sums added in LDS in a fixed order.
-__global__ __launch_bounds__(256) void k_fin_col(const double* part, int nch, int ncols, const double* v,
-                                                 const double* dinv, double sign, double* y) {
+__device__ __forceinline__ void fin_col_body(const double* part, int nch, int ncols, const double* v,
+                                             const double* dinv, double sign, double* y) {
   __shared__ double red[4][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int c = blockIdx.x * 64 + lane;
@@ -2248,11 +2310,27 @@ __global__ __launch_bounds__(256) void k_fin_col(const double* part, int nch, in
   }
 }
 
+__global__ __launch_bounds__(256) void k_fin_col(const double* part, int nch, int ncols, const double* v,
+                                                 const double* dinv, double sign, double* y) {
+  fin_col_body(part, nch, ncols, v, dinv, sign, y);
+}
+
+struct Fin {
+  const double* part;
+  const double* v;
+  const double* dinv;
+  double* y;
+};
+
+__global__ __launch_bounds__(256) void k_fin_col2(Fin a0, Fin a1, int nch, int ncols, double sign) {
+  const Fin& a = blockIdx.z ? a1 : a0;
+  fin_col_body(a.part, nch, ncols, a.v, a.dinv, sign, a.y);
+}
+
 // Two-source column partials: rows of L (nrows1) then of R (nrows2).
 // The same over two matrices stacked by rows: chunks of M1 (nr1 rows) then of M2 (nr2 rows).
-__global__ __launch_bounds__(256) void k_gemv_col_part2(const double* M1, int nr1, const double* x1,
-                                                        const double* M2, int nr2, const double* x2, long ld,
-                                                        int ncols, double* part) {
+__device__ __forceinline__ void col_part2_body(const double* M1, int nr1, const double* x1, const double* M2,
+                                               int nr2, const double* x2, long ld, int ncols, double* part) {
   const int chunk = blockIdx.y;
   const int nch1 = (nr1 + DH_CCH - 1) / DH_CCH;
   double* prow = part + (long)chunk * ncols;
@@ -2263,6 +2341,26 @@ __global__ __launch_bounds__(256) void k_gemv_col_part2(const double* M1, int nr
     const int r0 = (chunk - nch1) * DH_CCH;
     gv_col_tile<false>(M2, ld, r0, min(nr2, r0 + DH_CCH), ncols, x2, prow);
   }
+}
+
+__global__ __launch_bounds__(256) void k_gemv_col_part2(const double* M1, int nr1, const double* x1,
+                                                        const double* M2, int nr2, const double* x2, long ld,
+                                                        int ncols, double* part) {
+  col_part2_body(M1, nr1, x1, M2, nr2, x2, ld, ncols, part);
+}
+
+struct GvCol2 {
+  const double* M1;
+  const double* x1;
+  const double* M2;
+  const double* x2;
+  double* part;
+};
+
+__global__ __launch_bounds__(256) void k_gemv_col_part2_2s(GvCol2 a0, GvCol2 a1, int nr1, int nr2, long ld,
+                                                           int ncols) {
+  const GvCol2& a = blockIdx.z ? a1 : a0;
+  col_part2_body(a.M1, nr1, a.x1, a.M2, nr2, a.x2, ld, ncols, a.part);
 }
 
 // Arrowhead tip solves on a 64-vector: y = inv(L_aa) x (trans=0) or inv(L_aa)^T x.
@@ -2707,6 +2805,7 @@ struct dh_handle {
   double* part;     // partial sums
   long part_len;
   double* arrpart;  // n x 64
+  double *v2, *part2, *arrpart2;  // the second side's v, column partials and arrow partials (paired sweeps)
   double* eigbuf;   // 2 x n x bp
   int* info;        // device
   int* flags;       // n * nbt readiness flags of fused launches
@@ -2965,6 +3064,9 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
                 (long long)((h->bp + 64) / DH_CCH + 2) * h->bp + (long long)n * h->bp / 256 * 4;
   rc |= dalloc(h, &h->part, h->part_len);
   rc |= dalloc(h, &h->arrpart, (long long)n * 64);
+  rc |= dalloc(h, &h->v2, h->bp);
+  rc |= dalloc(h, &h->part2, (long long)((h->bp + DH_CCH - 1) / DH_CCH + 1) * h->bp);
+  rc |= dalloc(h, &h->arrpart2, (long long)n * 64);
   rc |= dalloc(h, &h->eigbuf, 2LL * n * h->bp);
   rc |= dalloc(h, &h->dagpinv, sb * 4 * h->nbt * 64 * 64);  // [instance][parity][nbt][64][64]
   rc |= dalloc(h, &h->tipB, sb * 64 * 64);
@@ -3007,7 +3109,8 @@ int dh_create(int n, int b, int a, void* stream, dh_handle** out) {
 int dh_destroy(dh_handle* h) {
   if (!h) return DH_EARG;
   double* ps[] = {h->diag, h->lower, h->arrow, h->tip, h->dinv, h->pinv, h->tipinv, h->tipdinv,
-                  h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->eigbuf, h->dagpinv, h->qinv,
+                  h->W, h->S, h->Wa, h->z, h->v, h->part, h->arrpart, h->v2, h->part2, h->arrpart2,
+                  h->eigbuf, h->dagpinv, h->qinv,
                   h->Wall, h->Sall, h->Waall, h->tipB, h->diagT, h->lowerT, h->arrowT};
   for (double* p : ps)
     if (p) (void)hipFree(p);
@@ -3538,40 +3641,76 @@ static int ensure_inverse(dh_handle* h) {
 
 }  // extern "C"
 
-// Forward sweep of pobtas over blocks slot0, slot0 + dir, ..., cnt blocks of the handle's factor
-// (dir = -1: the upward side of a two-ended factor): y_i = inv(L_ii) (b_i - L_{i,p} y_p) with p the
-// previous block, in place in z (padded, block pitch bp; z[0] is block slot0's part, z[-dir bp..]
-// the previous block's y when has_prev), and za -= sum_i L_{a,i} y_i.  The coupling of two
-// neighbouring slots sits at the lower one.  L2, z2: a second previous block for block slot0
-// (the junction of a two-ended factor), y -= L2 z2.
-static void solve_forward(dh_handle* h, double* z, int slot0, int cnt, bool has_prev, bool arrow, double* za,
-                          int dir = 1, const double* L2 = nullptr, const double* z2 = nullptr) {
+// One side of a pobtas sweep: blocks slot0, slot0 + dir, ..., cnt blocks of the handle's factor
+// (dir = -1: the upward side of a two-ended factor), their part of z (padded, block pitch bp; z[0] is
+// block slot0's part), edge: the previous block's y sits at z[-dir bp..] (forward) or the next
+// block's x at z[dir cnt bp..] (backward), and the scratch of this side (v, column partials, arrow
+// partials).  Two sides run paired: every step launches its kernels once for both (blockIdx.z),
+// each side with its own scratch, and a side with more blocks finishes alone.
+struct Sweep {
+  double* z;
+  int slot0, cnt, dir;
+  bool edge;
+  double* v;
+  double* part;
+  double* arrpart;
+};
+
+// Forward sweep: y_i = inv(L_ii) (b_i - L_{i,p} y_p) with p the previous block, in place in z, and
+// za -= sum_i L_{a,i} y_i (the sides' sums subtracted in side order).  The coupling of two
+// neighbouring slots sits at the lower one.  L2, z2 (one side): a second previous block for its
+// first block (the junction of a two-ended factor), y -= L2 z2.
+static void sweep_forward(dh_handle* h, const Sweep* sw, int ns, bool arrow, double* za,
+                          const double* L2 = nullptr, const double* z2 = nullptr) {
   const int bp = h->bp;
   const long bb = (long)bp * bp;
   const int ncb = (bp + 127) / 128;  // column blocks of the partial GEMVs
   const int nfb = (bp + 63) / 64;    // column blocks of k_fin_col
   const int nch = (bp + DH_CCH - 1) / DH_CCH;
-  for (int l = 0; l < cnt; ++l) {
-    const int i = slot0 + dir * l;
-    double* zi = z + (long)dir * l * bp;
-    if (l > 0 || has_prev) {
-      hipLaunchKernelGGL(k_gemv_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream,
-                         h->lower + std::min(i, i - dir) * bb, (long)bp, bp, bp, zi - dir * bp, zi, h->v, -1.0);
-    } else {
-      hipLaunchKernelGGL(k_copy, dim3((bp + 255) / 256), dim3(256), 0, h->stream, zi, h->v, bp);
+  const int lmax = ns > 1 ? std::max(sw[0].cnt, sw[1].cnt) : sw[0].cnt;
+  for (int l = 0; l < lmax; ++l) {
+    GvRow r[2], ra[2];
+    GvCol c[2];
+    Fin f[2];
+    int k = 0;
+    for (int q = 0; q < ns; ++q) {
+      const Sweep& w = sw[q];
+      if (l >= w.cnt) continue;
+      const int i = w.slot0 + w.dir * l;
+      double* zi = w.z + (long)w.dir * l * bp;
+      const bool prev = l > 0 || w.edge;
+      r[k] = GvRow{prev ? h->lower + std::min(i, i - w.dir) * bb : nullptr, zi - w.dir * bp, zi, w.v};
+      c[k] = GvCol{h->diag + i * bb, w.v, w.part};
+      f[k] = Fin{w.part, w.v, h->dinv + (long)i * bp, zi};
+      ra[k] = GvRow{h->arrow + (long)i * h->ap * bp, zi, nullptr, w.arrpart + (long)l * 64};
+      ++k;
     }
+    if (k == 1) {
+      r[1] = r[0];
+      c[1] = c[0];
+      f[1] = f[0];
+      ra[1] = ra[0];
+    }
+    hipLaunchKernelGGL(k_gemv_row2, dim3((bp + 3) / 4, 1, k), dim3(256), 0, h->stream, r[0], r[1], (long)bp, bp, bp,
+                       -1.0);
     if (l == 0 && L2)
       hipLaunchKernelGGL(k_gemv_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, L2, (long)bp, bp, bp, z2,
-                         (const double*)h->v, h->v, -1.0);
-    hipLaunchKernelGGL(k_gemv_col_part, dim3(ncb, nch), dim3(256), 0, h->stream, h->diag + i * bb, (long)bp, bp,
-                       bp, (const double*)h->v, h->part, 1);
-    hipLaunchKernelGGL(k_fin_col, dim3(nfb), dim3(256), 0, h->stream, h->part, nch, bp, (const double*)h->v,
-                       h->dinv + (long)i * bp, 1.0, zi);
+                         (const double*)sw[0].v, sw[0].v, -1.0);
+    hipLaunchKernelGGL(k_gemv_col_part_2s, dim3(ncb, nch, k), dim3(256), 0, h->stream, c[0], c[1], (long)bp, bp, bp,
+                       1);
+    hipLaunchKernelGGL(k_fin_col2, dim3(nfb, 1, k), dim3(256), 0, h->stream, f[0], f[1], nch, bp, 1.0);
     if (arrow)
-      hipLaunchKernelGGL(k_gemv_row, dim3(16), dim3(256), 0, h->stream, h->arrow + (long)i * h->ap * bp, (long)bp, 64,
-                         bp, (const double*)zi, (const double*)nullptr, h->arrpart + (long)l * 64, 1.0);
+      hipLaunchKernelGGL(k_gemv_row2, dim3(16, 1, k), dim3(256), 0, h->stream, ra[0], ra[1], (long)bp, 64, bp, 1.0);
   }
-  if (arrow) hipLaunchKernelGGL(k_arrow_acc, dim3(1), dim3(64), 0, h->stream, h->arrpart, cnt, za);
+  if (arrow)
+    for (int q = 0; q < ns; ++q)
+      hipLaunchKernelGGL(k_arrow_acc, dim3(1), dim3(64), 0, h->stream, sw[q].arrpart, sw[q].cnt, za);
+}
+
+static void solve_forward(dh_handle* h, double* z, int slot0, int cnt, bool has_prev, bool arrow, double* za,
+                          int dir = 1, const double* L2 = nullptr, const double* z2 = nullptr) {
+  const Sweep w{z, slot0, cnt, dir, has_prev, h->v, h->part, h->arrpart};
+  sweep_forward(h, &w, 1, arrow, za, L2, z2);
 }
 
 // za = inv(L_aa)^T inv(L_aa) za (the tip of both sweeps).
@@ -3580,36 +3719,74 @@ static void solve_tip(dh_handle* h, double* za) {
   hipLaunchKernelGGL(k_tip_apply, dim3(1), dim3(64), 0, h->stream, h->tipinv, h->v, za, 1);
 }
 
-// Backward sweep over the blocks of solve_forward in reverse: x_i = inv(L_ii)^T (y_i - L_{q,i}^T x_q
-// - L_{a,i}^T x_a) with q the next block (slot i + dir), z as for solve_forward (z[dir cnt bp..]
-// holds x of the next block when has_next).
-static void solve_backward(dh_handle* h, double* z, int slot0, int cnt, bool has_next, bool arrow, const double* za,
-                           int dir = 1) {
+// Backward sweep over the blocks of the forward sweep in reverse: x_i = inv(L_ii)^T (y_i - L_{q,i}^T
+// x_q - L_{a,i}^T x_a) with q the next block (slot i + dir).  Paired sides take their steps from
+// their last block (next to the junction) outward; a step whose sides differ in having a coupling
+// term runs as two launches.
+static void sweep_backward(dh_handle* h, const Sweep* sw, int ns, bool arrow, const double* za) {
   const int bp = h->bp;
   const long bb = (long)bp * bp;
   const int ncb = (bp + 127) / 128;
   const int nfb = (bp + 63) / 64;
   const int nch = (bp + DH_CCH - 1) / DH_CCH;
-  for (int l = cnt - 1; l >= 0; --l) {
-    const int i = slot0 + dir * l;
-    double* zi = z + (long)dir * l * bp;
-    const bool hasL = l < cnt - 1 || has_next;
-    if (hasL || arrow) {
-      const int nch1 = hasL ? nch : 0;
-      const int nch2 = arrow ? 1 : 0;
-      hipLaunchKernelGGL(k_gemv_col_part2, dim3(ncb, nch1 + nch2), dim3(256), 0, h->stream,
-                         (const double*)(hasL ? h->lower + std::min(i, i + dir) * bb : h->diag), hasL ? bp : 0,
-                         (const double*)(zi + dir * bp),
-                         (const double*)(arrow ? h->arrow + (long)i * h->ap * bp : h->diag), arrow ? 64 : 0, za,
-                         (long)bp, bp, h->part);
-      hipLaunchKernelGGL(k_fin_col, dim3(nfb), dim3(256), 0, h->stream, h->part, nch1 + nch2, bp, (const double*)zi,
-                         (const double*)nullptr, -1.0, h->v);
-    } else {
-      hipLaunchKernelGGL(k_copy, dim3((bp + 255) / 256), dim3(256), 0, h->stream, zi, h->v, bp);
+  const int kmax = ns > 1 ? std::max(sw[0].cnt, sw[1].cnt) : sw[0].cnt;
+  for (int kk = 0; kk < kmax; ++kk) {
+    const Sweep* act[2];
+    int l[2], na = 0;
+    for (int q = 0; q < ns; ++q)
+      if (kk < sw[q].cnt) {
+        act[na] = &sw[q];
+        l[na] = sw[q].cnt - 1 - kk;
+        ++na;
+      }
+    bool hasL[2];
+    for (int q = 0; q < na; ++q) hasL[q] = l[q] < act[q]->cnt - 1 || act[q]->edge;
+    // sides launched together share the row counts of the column partials
+    const int groups = (na == 2 && hasL[0] != hasL[1]) ? 2 : 1;
+    for (int g = 0; g < groups; ++g) {
+      const int q0 = groups == 2 ? g : 0, k = groups == 2 ? 1 : na;
+      GvCol2 c[2];
+      Fin f[2];
+      Upk u[2];
+      GvRow cp[2];
+      for (int j = 0; j < k; ++j) {
+        const Sweep& w = *act[q0 + j];
+        const int lj = l[q0 + j];
+        const int i = w.slot0 + w.dir * lj;
+        double* zi = w.z + (long)w.dir * lj * bp;
+        const bool hl = hasL[q0 + j];
+        c[j] = GvCol2{hl ? h->lower + std::min(i, i + w.dir) * bb : h->diag, zi + w.dir * bp,
+                      arrow ? h->arrow + (long)i * h->ap * bp : h->diag, za, w.part};
+        f[j] = Fin{w.part, zi, nullptr, w.v};
+        u[j] = Upk{h->diag + i * bb, h->dinv + (long)i * bp, w.v, zi};
+        cp[j] = GvRow{nullptr, nullptr, zi, w.v};
+      }
+      if (k == 1) {
+        c[1] = c[0];
+        f[1] = f[0];
+        u[1] = u[0];
+        cp[1] = cp[0];
+      }
+      const bool hl = hasL[q0];
+      if (hl || arrow) {
+        const int nch1 = hl ? nch : 0;
+        const int nch2 = arrow ? 1 : 0;
+        hipLaunchKernelGGL(k_gemv_col_part2_2s, dim3(ncb, nch1 + nch2, k), dim3(256), 0, h->stream, c[0], c[1],
+                           hl ? bp : 0, arrow ? 64 : 0, (long)bp, bp);
+        hipLaunchKernelGGL(k_fin_col2, dim3(nfb, 1, k), dim3(256), 0, h->stream, f[0], f[1], nch1 + nch2, bp, -1.0);
+      } else {
+        hipLaunchKernelGGL(k_gemv_row2, dim3((bp + 3) / 4, 1, k), dim3(256), 0, h->stream, cp[0], cp[1], (long)bp, bp,
+                           bp, 0.0);
+      }
+      hipLaunchKernelGGL(k_upk_row2, dim3((bp + 3) / 4, 1, k), dim3(256), 0, h->stream, u[0], u[1], bp);
     }
-    hipLaunchKernelGGL(k_upk_row, dim3((bp + 3) / 4), dim3(256), 0, h->stream, h->diag + i * bb,
-                       h->dinv + (long)i * bp, bp, (const double*)h->v, zi);
   }
+}
+
+static void solve_backward(dh_handle* h, double* z, int slot0, int cnt, bool has_next, bool arrow, const double* za,
+                           int dir = 1) {
+  const Sweep w{z, slot0, cnt, dir, has_next, h->v, h->part, h->arrpart};
+  sweep_backward(h, &w, 1, arrow, za);
 }
 
 extern "C" {
@@ -3630,13 +3807,15 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
   if (m >= 0) {  // two-ended factor: both sides inward, the junction block m, and back outward
     const long bb = (long)bp * bp;
     double* z = h->z;
-    solve_forward(h, z, 0, m, false, arrow, za);
-    solve_forward(h, z + (long)(n - 1) * bp, n - 1, n - 1 - m, false, arrow, za, -1);
+    const Sweep two[2] = {Sweep{z, 0, m, 1, false, h->v, h->part, h->arrpart},
+                          Sweep{z + (long)(n - 1) * bp, n - 1, n - 1 - m, -1, false, h->v2, h->part2, h->arrpart2}};
+    sweep_forward(h, two, 2, arrow, za);
     solve_forward(h, z + (long)m * bp, m, 1, true, arrow, za, 1, h->lower + m * bb, z + (long)(m + 1) * bp);
     if (arrow) solve_tip(h, za);
     solve_backward(h, z + (long)m * bp, m, 1, false, arrow, za);
-    solve_backward(h, z, 0, m, true, arrow, za);
-    solve_backward(h, z + (long)(n - 1) * bp, n - 1, n - 1 - m, true, arrow, za, -1);
+    const Sweep back[2] = {Sweep{z, 0, m, 1, true, h->v, h->part, h->arrpart},
+                           Sweep{z + (long)(n - 1) * bp, n - 1, n - 1 - m, -1, true, h->v2, h->part2, h->arrpart2}};
+    sweep_backward(h, back, 2, arrow, za);
   } else {
     solve_forward(h, h->z, 0, n, false, arrow, za);
     if (arrow) solve_tip(h, za);
