@@ -136,6 +136,68 @@ __device__ __forceinline__ void dh_mma(dh_d4 acc[2][2], const Opnd& A, int i0, c
   }
 }
 
+// Two 64x64 tiles that share one operand panel: acc0 += opA[i0..] . opB[j0..] and acc1 += opA[i1..] .
+// opB[j1..] over k in [klo, khi), with i0 == i1 (SHARE_A: one row panel of opA against two of opB) or
+// j0 == j1 (the converse).  The shared panel's chunk is loaded once for both tiles: 24 loads per lane
+// for 32 MFMAs per wave and chunk, against 16 for 16 with dh_mma.  Same k order and ring as dh_mma,
+// so each tile's sum is bitwise the one dh_mma forms.
+template <int MA, int MB, bool SHARE_A>
+__device__ __forceinline__ void dh_mma_pair(dh_d4 acc0[2][2], dh_d4 acc1[2][2], const Opnd& A, int i0, int i1,
+                                            const Opnd& B, int j0, int j1, int klo, int khi) {
+  if (klo >= khi) return;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int ra0 = i0 + (w >> 1) * 32 + (lane & 15), ra1 = i1 + (w >> 1) * 32 + (lane & 15);
+  const int rb0 = j0 + (w & 1) * 32 + (lane & 15), rb1 = j1 + (w & 1) * 32 + (lane & 15);
+  const int kq = (lane >> 4) * 2;
+  const int nck = (khi - klo) >> 4;
+  const int k0 = klo + kq;
+  // x: the shared panel (A rows ra0, or B rows rb0), y0 / y1: the two others
+  double x[2][2][4], y0[2][2][4], y1[2][2][4];
+  auto load = [&](int st, int k) {
+    if (SHARE_A) {
+      dh_load4<MA>(A, ra0, k, x[st][0]);
+      dh_load4<MA>(A, ra0 + 16, k, x[st][1]);
+      dh_load4<MB>(B, rb0, k, y0[st][0]);
+      dh_load4<MB>(B, rb0 + 16, k, y0[st][1]);
+      dh_load4<MB>(B, rb1, k, y1[st][0]);
+      dh_load4<MB>(B, rb1 + 16, k, y1[st][1]);
+    } else {
+      dh_load4<MB>(B, rb0, k, x[st][0]);
+      dh_load4<MB>(B, rb0 + 16, k, x[st][1]);
+      dh_load4<MA>(A, ra0, k, y0[st][0]);
+      dh_load4<MA>(A, ra0 + 16, k, y0[st][1]);
+      dh_load4<MA>(A, ra1, k, y1[st][0]);
+      dh_load4<MA>(A, ra1 + 16, k, y1[st][1]);
+    }
+  };
+  load(0, k0);
+  for (int q = 0; q < nck; q += 2) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int ck = q + s;
+      if (ck < nck) {
+        if (ck + 1 < nck) load(s ^ 1, k0 + 16 * (ck + 1));
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+              if (SHARE_A) {
+                acc0[u][v] = dh_mfma(x[s][u][m], y0[s][v][m], acc0[u][v]);
+                acc1[u][v] = dh_mfma(x[s][u][m], y1[s][v][m], acc1[u][v]);
+              } else {
+                acc0[u][v] = dh_mfma(y0[s][u][m], x[s][v][m], acc0[u][v]);
+                acc1[u][v] = dh_mfma(y1[s][u][m], x[s][v][m], acc1[u][v]);
+              }
+            }
+        }
+      }
+    }
+  }
+}
+
 // 128x128 workgroup tile: wave w computes the 64x64 quadrant (w>>1, w&1) as
 // 4x4 MFMA blocks; one 16-wide k chunk is prefetched ahead of 64 MFMAs.
 template <int MA, int MB>

@@ -1892,10 +1892,6 @@ struct SelCArgs {
   int xt;              // 1: X_{q,i} stored transposed (the coupling slot holds the rows of q)
 };
 
-#ifndef DH_SELC_S
-#define DH_SELC_S 2  // operand ring depth of the congruence form's main products
-#endif
-
 // Tile rows R of L with a nonzero tile in some column <= K: R < selc_rend(a, K) (a prefix that holds
 // them all; products with the zero tiles in it add nothing).
 __device__ __forceinline__ int selc_rend(const SelCArgs& a, int K) {
@@ -1941,7 +1937,7 @@ __device__ __forceinline__ void selc_y_work(const SelCArgs& a, int e) {
     const int I = r;
     const int rend = selc_rend(a, K);
     // opM[i][r] = X_qq(r, i) = X_qq(i, r) and opM[k][r] = L(r, k): both read by columns
-    dh_mma<M_COL, M_COL, DH_SELC_S>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
+    dh_mma<M_COL, M_COL>(acc, Opnd{a.Dn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.Lb, a.bp, M_COL, nullptr},
                          K * DH_TILE, 0, rend * DH_TILE);
     if (a.Rn && a.La)
       dh_mma<M_COL, M_COL>(acc, Opnd{a.Rn, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.La, a.bp, M_COL, nullptr},
@@ -1966,28 +1962,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   else selc_y_work(a0, e);
 }
 
-// Phase B: lower tile (K', K) of M = L^T Y + L_a^T Ya, stored to both triangles (a diagonal tile
-// exactly symmetric).
-__device__ __forceinline__ void selc_m_work(const SelCArgs& a, int e) {
-  int Kp, K;
-  selc_tri(a.nbt * (a.nbt + 1) / 2 - 1 - e, Kp, K);  // tile rows K' in decreasing order: longest first
-  dh_d4 acc[2][2];
-  dh_zero(acc);
-  if (a.Lb)  // opM[k'][r] = L(r, k'), opM[k][r] = Y(r, k): both by columns
-    dh_mma<M_COL, M_COL, DH_SELC_S>(acc, Opnd{a.Lb, a.bp, M_COL, nullptr}, Kp * DH_TILE, Opnd{a.Y, a.yld, M_COL, nullptr},
-                         K * DH_TILE, 0, selc_rend(a, Kp) * DH_TILE);
-  if (a.La)
-    dh_mma<M_COL, M_COL>(acc, Opnd{a.La, a.bp, M_COL, nullptr}, Kp * DH_TILE, Opnd{a.Ya, a.bp, M_COL, nullptr},
-                         K * DH_TILE, 0, DH_TILE);
+// A tile (K', K) of a symmetric result from its lower triangle: stored at (K', K) and mirrored to
+// (K, K'), or exactly symmetric on the diagonal.  Workgroup-uniform.
+__device__ __forceinline__ void selc_store_lower(double* C, int bp, int Kp, int K, const dh_d4 acc[2][2]) {
   if (Kp == K) {
-    dh_store_sym_tile(a.M, a.bp, K * DH_TILE, acc, 1.0);
+    dh_store_sym_tile(C, bp, K * DH_TILE, acc, 1.0);
     return;
   }
-  dh_store(a.M, a.bp, Kp * DH_TILE, K * DH_TILE, acc, 1.0, 0.0, false);
-  dh_store(a.M, a.bp, Kp * DH_TILE, K * DH_TILE, acc, 1.0, 0.0, true);
+  dh_store(C, bp, Kp * DH_TILE, K * DH_TILE, acc, 1.0, 0.0, false);
+  dh_store(C, bp, Kp * DH_TILE, K * DH_TILE, acc, 1.0, 0.0, true);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_selc_m2(SelCArgs a0, int n0,
+// items of tile row R of a lower triangle taken in pairs (K, K + 1), K = 0, 2, ..
+__device__ __host__ __forceinline__ int selc_rowpairs(int R) { return (R + 2) / 2; }
+
+__device__ __host__ __forceinline__ int selc_ntripairs(int nbt) {
+  int t = 0;
+  for (int R = 0; R < nbt; ++R) t += selc_rowpairs(R);
+  return t;
+}
+
+// Phase B: tiles (K', K) and (K', K + 1) of M = L^T Y + L_a^T Ya, K + 1 <= K', which share L's column
+// panel K'; stored to both triangles (a diagonal tile exactly symmetric); tile rows K' in decreasing
+// order, longest first.
+__device__ __forceinline__ void selc_m_work(const SelCArgs& a, int e) {
+  int Kp = a.nbt - 1;
+  while (Kp > 0 && e >= selc_rowpairs(Kp)) {
+    e -= selc_rowpairs(Kp);
+    --Kp;
+  }
+  const int K0 = 2 * e, K1 = min(2 * e + 1, Kp);
+  dh_d4 acc0[2][2], acc1[2][2];
+  dh_zero(acc0);
+  dh_zero(acc1);
+  if (a.Lb)  // opM[k'][r] = L(r, k'), opM[k][r] = Y(r, k): both by columns
+    dh_mma_pair<M_COL, M_COL, true>(acc0, acc1, Opnd{a.Lb, a.bp, M_COL, nullptr}, Kp * DH_TILE, Kp * DH_TILE,
+                                    Opnd{a.Y, a.yld, M_COL, nullptr}, K0 * DH_TILE, K1 * DH_TILE, 0,
+                                    selc_rend(a, Kp) * DH_TILE);
+  if (a.La)
+    dh_mma_pair<M_COL, M_COL, true>(acc0, acc1, Opnd{a.La, a.bp, M_COL, nullptr}, Kp * DH_TILE, Kp * DH_TILE,
+                                    Opnd{a.Ya, a.bp, M_COL, nullptr}, K0 * DH_TILE, K1 * DH_TILE, 0, DH_TILE);
+  selc_store_lower(a.M, a.bp, Kp, K0, acc0);
+  if (K1 != K0) selc_store_lower(a.M, a.bp, Kp, K1, acc1);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_selc_m2(SelCArgs a0, int n0,
                                                                                      SelCArgs a1) {
   int e;
   if (selc_side(n0, e)) selc_m_work(a1, e);
@@ -2011,7 +2030,7 @@ __device__ __forceinline__ void selc_t_work(const SelCArgs& a, bool has_m, int e
     const int K = c + e;
     dh_load_acc(a.Q, a.bp, K * DH_TILE, c * DH_TILE, acc);  // the identity's term: inv(L)(K, c)
     if (has_m)  // opM[k][k'] = M(k', k) = M(k, k')
-      dh_mma<M_COL, M_COL, DH_SELC_S>(acc, Opnd{a.M, a.bp, M_COL, nullptr}, K * DH_TILE, q, c * DH_TILE, c * DH_TILE,
+      dh_mma<M_COL, M_COL>(acc, Opnd{a.M, a.bp, M_COL, nullptr}, K * DH_TILE, q, c * DH_TILE, c * DH_TILE,
                                       a.bp);
     dh_store(a.T, a.bp, K * DH_TILE, c * DH_TILE, acc, 1.0, 0.0, false);
     return;
@@ -2050,7 +2069,7 @@ __device__ __forceinline__ void selc_x_work(const SelCArgs& a, int e) {
   dh_d4 acc[2][2];
   dh_zero(acc);
   // opM[i][k] = inv(L)(k, i) (k >= i) and opM[j][k] = T(k, j) (lower tiles, k >= i >= j): by columns
-  dh_mma<M_COL, M_COL, DH_SELC_S>(acc, Opnd{a.Q, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.T, a.bp, M_COL, nullptr},
+  dh_mma<M_COL, M_COL>(acc, Opnd{a.Q, a.bp, M_COL, nullptr}, I * DH_TILE, Opnd{a.T, a.bp, M_COL, nullptr},
                        J * DH_TILE, I * DH_TILE, a.bp);
   if (I == J) {
     dh_store_sym_tile(a.D, a.bp, I * DH_TILE, acc, 1.0);
@@ -4141,7 +4160,7 @@ static void selc_args(const dh_handle* h, const SelBlk& k, bool arrow, int s, Se
 }
 
 static int selinv_cong(dh_handle* h, bool arrow, const std::vector<SelBlk>& blk, const std::vector<int>& step0) {
-  const int nbt = h->nbt, ntri = nbt * (nbt + 1) / 2;
+  const int nbt = h->nbt;
   for (size_t s = 0; s + 1 < step0.size(); ++s) {
     const int c = step0[s + 1] - step0[s];
     if (c < 1 || c > 2 || 2 * h->G < 2 * c) return DH_EARG;
@@ -4155,14 +4174,15 @@ static int selinv_cong(dh_handle* h, bool arrow, const std::vector<SelBlk>& blk,
       if (y0 > 0) DH_LAUNCH(h, 2, k_selc_y2, dim3(y0 + y1), dim3(256), 0, a0, y0, a1);
       else DH_LAUNCH(h, 2, k_selc_y2, dim3(y1), dim3(256), 0, a1, y1, a1);
     }
-    const int b0 = m0 ? ntri : 0, b1 = m1 ? ntri : 0;
+    const int b0 = m0 ? selc_ntripairs(nbt) : 0, b1 = m1 ? selc_ntripairs(nbt) : 0;
     if (b0 + b1 > 0) {
       if (b0 > 0) DH_LAUNCH(h, 2, k_selc_m2, dim3(b0 + b1), dim3(256), 0, a0, b0, a1);
       else DH_LAUNCH(h, 2, k_selc_m2, dim3(b1), dim3(256), 0, a1, b1, a1);
     }
     const int t0 = selc_nt(a0), t1 = c > 1 ? selc_nt(a1) : 0;
     DH_LAUNCH(h, 2, k_selc_t2, dim3(t0 + t1), dim3(256), 0, a0, m0 ? 1 : 0, t0, a1, m1 ? 1 : 0);
-    DH_LAUNCH(h, 2, k_selc_x2, dim3(ntri * c), dim3(256), 0, a0, ntri, a1);
+    const int nx = nbt * (nbt + 1) / 2;
+    DH_LAUNCH(h, 2, k_selc_x2, dim3(nx * c), dim3(256), 0, a0, nx, a1);
   }
   HIPCHK(hipGetLastError());
   return DH_OK;
