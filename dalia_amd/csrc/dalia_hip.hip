@@ -3662,7 +3662,7 @@ static int ensure_inverse(dh_handle* h) {
   if (h->inverted) return DH_OK;
   // chunks of 2G blocks: the level kernels' scratch is bp^2 per block in the W slots (2G of them,
   // idle here), and the small levels' launches are latency bound, so fewer and wider launches pay
-  const int G2 = 2 * h->G;
+  const int G2 = 2 * h->G;  // measured at ns = 2025: 7.59 ms for the inverse, 7.75 with 12, 7.86 with 8
   for (int g0 = 0; g0 < h->n; g0 += G2) inverse_chunk(h, g0, std::min(G2, h->n - g0), h->stream, false);
   HIPCHK(hipGetLastError());
   h->inverted = 1;
