@@ -77,7 +77,10 @@ class STInlaCkpt(STInla):
     Same results and interface for f, value_and_grad, fd_gradient, hessian, the autograd node and
     the latent marginal variances; not capturable in a HIP graph."""
 
-    def __init__(self, data, K: int = 16, device="cuda", closed_form_prior: bool = True):
+    def __init__(self, data, K: int = 16, device="cuda", closed_form_prior: bool = True,
+                 pattern_selinv: bool = True):
+        """pattern_selinv: each segment's Takahashi recursion in congruence form with the coupling
+        blocks formed only on the matrix's tiles (`dh_ckpt_set_tuning` key 10), as `STInla`."""
         self.data = data
         self.manifold = getattr(data, "manifold", "plane")
         if self.manifold not in hyper.MANIFOLDS:
@@ -90,6 +93,7 @@ class STInlaCkpt(STInla):
         _lib.check(self.lib.dh_ckpt_create(data.nt, data.ns, data.nb, self.K, ctypes.c_void_p(stream.cuda_stream),
                                            ctypes.byref(ck)), "dh_ckpt_create")
         self.ck = ck
+        _lib.check(self.lib.dh_ckpt_set_tuning(ck, 10, 1 if pattern_selinv else 0), "dh_ckpt_set_tuning")
         self.h = ctypes.c_void_p(self.lib.dh_ckpt_vectors(ck))
         self.solver = None
         N = self.dm.n_latent

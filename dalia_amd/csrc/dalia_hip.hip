@@ -4215,6 +4215,22 @@ static int selinv_twisted(dh_handle* h, bool arrow, bool cong) {
   return cong ? selinv_cong(h, arrow, blk, step0) : selinv_steps(h, arrow, blk, step0);
 }
 
+// The one-ended recursion over blocks hi down to lo, each with successor i + 1 when it exists (the
+// next block, or the slot after a segment holding X of the segment above): in congruence form when
+// it is selected and the profiles of the current factor are valid, else in the W form.
+static int selinv_blocks(dh_handle* h, int sparsity, bool arrow, int lo, int hi) {
+  if (h->sel_cong && h->prof_valid && h->fp_ready != sparsity) {
+    std::vector<SelBlk> blk;
+    std::vector<int> step0{0};
+    for (int i = hi; i >= lo; --i) {
+      blk.push_back(SelBlk{i, i < h->n - 1 ? i + 1 : -1, 0});
+      step0.push_back((int)blk.size());
+    }
+    return selinv_cong(h, arrow, blk, step0);
+  }
+  return selinv_range(h, sparsity, arrow, lo, hi);
+}
+
 extern "C" {
 
 int dh_selected_inversion(dh_handle* h, int sparsity) {
@@ -4227,19 +4243,8 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   if (arrow) selinv_tip(h);
   // the congruence form needs the profiles of the current factor (they bound the tiles it forms)
   const bool cong = h->sel_cong && h->prof_valid && h->fp_ready != sparsity;
-  if (h->fact_tw >= 0) {
-    rc = selinv_twisted(h, arrow, cong);
-  } else if (cong) {
-    std::vector<SelBlk> blk;
-    std::vector<int> step0{0};
-    for (int i = h->n - 1; i >= 0; --i) {
-      blk.push_back(SelBlk{i, i < h->n - 1 ? i + 1 : -1, 0});
-      step0.push_back((int)blk.size());
-    }
-    rc = selinv_cong(h, arrow, blk, step0);
-  } else {
-    rc = selinv_range(h, sparsity, arrow, 0, h->n - 1);
-  }
+  if (h->fact_tw >= 0) rc = selinv_twisted(h, arrow, cong);
+  else rc = selinv_blocks(h, sparsity, arrow, 0, h->n - 1);
   h->fact_tw = -1;
   h->factored = -1;
   h->inverted = 0;
@@ -4768,7 +4773,7 @@ int dh_ckpt_backward(dh_ckpt* c, const dh_model* m, const double* params, double
     if (rc) return rc;
     solve_backward(h, c->z + (long)t0 * bp, slot0, cnt, has_next, arrow, za);
     if (selinv) {
-      rc = selinv_range(h, DH_SPARSITY_BTA, arrow, slot0, slot0 + cnt - 1);
+      rc = selinv_blocks(h, DH_SPARSITY_BTA, arrow, slot0, slot0 + cnt - 1);
       if (rc) return rc;
       const int nl = cnt - 1 + (has_next ? 1 : 0);
       const int nq = cnt + nl;

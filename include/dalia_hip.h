@@ -255,8 +255,8 @@ long long dh_ckpt_memory_bytes(const dh_ckpt* c);
 dh_handle* dh_ckpt_vectors(dh_ckpt* c);
 int dh_ckpt_set_stream(dh_ckpt* c, void* hip_stream);
 int dh_ckpt_get_info(dh_ckpt* c, int* info);
-/* dh_set_tuning on the context's segment factorizations (keys 0, 2, 3, 5, 6, 8; keys 4 and 7 are
- * refused). */
+/* dh_set_tuning on the context's segment factorizations and recursions (keys 0, 2, 3, 5, 6, 8, 9,
+ * 10; keys 4 and 7 are refused; key 11 has no effect: the segments are assembled by the context). */
 int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value);
 /* dh_timing_enable / dh_timing_read on the context's segment launches. */
 int dh_ckpt_timing_enable(dh_ckpt* c, int kind);

@@ -39,13 +39,14 @@ def reference17(model17):
     return th, f, mu, fg, g, mv
 
 
+@pytest.mark.parametrize("pattern", [True, False])
 @pytest.mark.parametrize("K", [2, 3, 5, 16, 17])
-def test_checkpointed_matches_stored_factor(model17, reference17, K):
+def test_checkpointed_matches_stored_factor(model17, reference17, K, pattern):
     from dalia_amd.ckpt import STInlaCkpt
     d = model17
     th, f_ref, mu_ref, fg_ref, g_ref, mv_ref = reference17
-    print(f"case ckpt n={d.nt} ns={d.ns} K={K}", flush=True)
-    eng = STInlaCkpt(d, K=K)
+    print(f"case ckpt n={d.nt} ns={d.ns} K={K} pattern={pattern}", flush=True)
+    eng = STInlaCkpt(d, K=K, pattern_selinv=pattern)
     f = eng.f(th)
     assert abs(f - f_ref) <= 1e-12 * abs(f_ref), (f, f_ref)
     assert rel(eng.mu.cpu().numpy(), mu_ref) < 1e-10
@@ -60,19 +61,24 @@ def test_checkpointed_matches_stored_factor(model17, reference17, K):
     assert eng.memory_bytes() > 0
 
 
-def test_checkpointed_large_tiles():
-    """ns = 900 (15 tile columns per block), nt = 7 in segments of 3: f and the gradient."""
+@pytest.mark.parametrize("pattern", [True, False])
+def test_checkpointed_large_tiles(pattern):
+    """ns = 900 (15 tile columns per block, a banded profile), nt = 7 in segments of 3: f, the
+    gradient and the latent marginal variances, with the segments' recursions in congruence form on
+    the coupling tiles or in the W form, against the stored-factor engine with whole blocks."""
     from dalia_amd.ckpt import STInlaCkpt
     from dalia_amd.inla import STInla
     d = synth.make_st_model(30, 7, 4, 8)
-    print(f"case ckpt large tiles ns={d.ns} nt={d.nt}", flush=True)
-    ref = STInla(d)
+    print(f"case ckpt large tiles ns={d.ns} nt={d.nt} pattern={pattern}", flush=True)
+    ref = STInla(d, pattern_selinv=False)
     f_ref = ref.f(d.theta)
     _, g_ref = ref.value_and_grad(d.theta)
-    eng = STInlaCkpt(d, K=3)
+    mv_ref = ref.marginal_variances_latent(d.theta, np.zeros(d.n_latent))
+    eng = STInlaCkpt(d, K=3, pattern_selinv=pattern)
     assert abs(eng.f(d.theta) - f_ref) <= 1e-12 * abs(f_ref)
     _, g = eng.value_and_grad(d.theta)
     assert rel(g, g_ref) < 1e-10
+    assert rel(eng.marginal_variances_latent(d.theta, np.zeros(d.n_latent)), mv_ref) < 1e-10
 
 
 def test_memory_budget_picks_the_engine(model17):
