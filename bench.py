@@ -321,8 +321,8 @@ def bench_coreg_batch(args):
     pts = [p for st in stencils for p in st]
     E = len(pts)
     diags, bf = batched_hessian_diags(engines, thetas, h=h, points=3, tune=True, fused=args.fused)  # + 1st replay
-    log(f"autotuned per instance: {[(c['alg'], c.get('wgs', 0)) for c in bf.choices]} (schedule, workgroups); "
-        f"{E} evaluations in one graph")
+    log(f"autotuned per instance: {[(c['alg'], c.get('wgs', 0), c.get('tw', 0)) for c in bf.choices]} "
+        f"(schedule, workgroups, two-ended); {E} evaluations in one graph")
     for _ in range(args.warmup):
         fv = bf(pts)
     torch.cuda.synchronize()

@@ -4,7 +4,11 @@ Knobs (`dh_set_tuning`): key 2 = the pobtaf schedule (one persistent launch
 of left-looking tile tasks, or one launch per 64-column panel), key 0 = the
 Schur-update kernel of the per-panel schedule (64x64 tiles, or 128x128 tiles
 with the k range split in two), key 6 = the number of workgroups of the persistent
-launch (for the instances of a batch).  `tune(engine)` times every
+launch (for the instances of a batch), and, for engines that leave it open
+(`twisted_auto`), key 7 = the two-ended factorization, taken when it is at least
+3 % faster than the one-ended one (a factorization bound by its products in bulk
+gains nothing from halving its chain, and the two-ended recursion of the W form
+is slower there).  `tune(engine)` times every
 value of every knob on the engine's own matrix with HIP events, applies the
 fastest and records it in a JSON file keyed by (padded block size, number of
 blocks, arrowhead size, device name); later calls read the file.
@@ -26,7 +30,7 @@ DEFAULT_PATH = os.environ.get("DALIA_AMD_TUNING",
 
 def _key(engine) -> str:
     lay = engine.solver.layout
-    return f"v4_bp{lay.bp}_n{lay.n}_a{lay.a}_{torch.cuda.get_device_name(0)}"
+    return f"v5_bp{lay.bp}_n{lay.n}_a{lay.a}_{torch.cuda.get_device_name(0)}"
 
 
 def _load(path):
@@ -42,6 +46,8 @@ def apply(engine, choice: dict) -> None:
     _lib.check(lib.dh_set_tuning(h, 2, int(choice["alg"])), "dh_set_tuning")
     _lib.check(lib.dh_set_tuning(h, 0, int(choice["syrk"])), "dh_set_tuning")
     _lib.check(lib.dh_set_tuning(h, 6, int(choice.get("wgs", 0))), "dh_set_tuning")
+    if getattr(engine, "twisted_auto", False) and "tw" in choice:
+        engine.solver.set_twisted(bool(choice["tw"]))
 
 
 def _time(fn, reps: int) -> float:
@@ -83,6 +89,15 @@ def tune(engine, path: str = DEFAULT_PATH, reps: int = 2, force: bool = False, w
     best_alg, best_syrk = cands[best]
     _lib.check(lib.dh_set_tuning(h, 2, best_alg), "dh_set_tuning")
     _lib.check(lib.dh_set_tuning(h, 0, best_syrk), "dh_set_tuning")
+    twist = None
+    if getattr(engine, "twisted_auto", False):
+        twist = 0
+        if best_alg == 1 and engine.data.nt >= 3:
+            engine.solver.set_twisted(True)
+            factor()
+            times["dag_twisted"] = _time(factor, reps)
+            twist = 1 if times["dag_twisted"] <= 0.97 * times[best] else 0
+        engine.solver.set_twisted(bool(twist))
     wgs = 0
     if best_alg == 1 and workgroups:
         # the fewest persistent workgroups (of the default, a half, a quarter, an eighth) that keep the
@@ -101,6 +116,8 @@ def tune(engine, path: str = DEFAULT_PATH, reps: int = 2, force: bool = False, w
         _lib.check(lib.dh_set_tuning(h, 6, wgs), "dh_set_tuning")
 
     choice = {"alg": best_alg, "syrk": best_syrk, "wgs": wgs, "ms": times}
+    if twist is not None:
+        choice["tw"] = twist
     apply(engine, choice)
     cache[key] = choice
     try:

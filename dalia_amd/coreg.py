@@ -135,6 +135,9 @@ class CoregInla:
     """Plain f(theta) of the coregional model; the engine interface of `STInla`
     used by `BatchedF` (params, param_row, _enqueue_f, _f_from_scalars, set_stream)."""
 
+    #: the autotuner chooses between the one- and two-ended factorization for this engine
+    twisted_auto = True
+
     def __init__(self, data, device="cuda"):
         self.data = data
         self.dm = DeviceCoregModel(data, device=device)

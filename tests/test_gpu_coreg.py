@@ -107,3 +107,28 @@ def test_coreg_hessian_from_adjoint_gradients():
     H = hessian_adjoint(eng, d.theta, h=1e-3)
     H_ref = oinla.fd_hessian(d, d.theta, 1e-3, fun=lambda th: ocoreg.f_theta_coreg(d, th))
     assert np.abs(H - H_ref).max() <= 1e-4 * np.abs(H_ref).max(), (H, H_ref)
+
+
+def test_coreg_twisted_autotuned(tmp_path):
+    """The coregional engine leaves the one- or two-ended factorization to the autotuner (tuning key
+    7): the choice is recorded and re-applied from the file, and f and the adjoint gradient agree with
+    the oracle whichever it is, and across both settings to 1e-12 / 1e-10."""
+    from dalia_amd import autotune
+    from dalia_amd.coreg import CoregInla
+    d = synth.make_coreg_model(6, 7, 2, 2, 9)  # b = 72 (two tile columns), nt = 7
+    print("case coreg twisted autotune", flush=True)
+    eng = CoregInla(d)
+    path = str(tmp_path / "tune.json")
+    c = autotune.tune(eng, path=path, reps=1)
+    assert c.get("tw") in (0, 1)
+    assert autotune.tune(eng, path=path)["tw"] == c["tw"]
+    ref = ocoreg.f_theta_coreg(d, d.theta)
+    out = {}
+    for tw in (False, True):
+        eng.solver.set_twisted(tw)
+        f = eng.f(d.theta)
+        assert abs(f - ref) <= 1e-10 * abs(ref), (tw, f, ref)
+        out[tw] = eng.value_and_grad(d.theta)
+    (f0, g0), (f1, g1) = out[False], out[True]
+    assert abs(f1 - f0) <= 1e-12 * abs(f0)
+    assert np.linalg.norm(g1 - g0) <= 1e-10 * np.linalg.norm(g0)
