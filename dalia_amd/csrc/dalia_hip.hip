@@ -733,8 +733,14 @@ struct DagArgs {
 #ifndef DAG_STAGES
 #define DAG_STAGES 2  // operand ring depth of the task products (measured: 1 -> 77.9 ms, 2 -> 77.2, 3 -> 78.8, 4 -> 80.3, 5 -> 80.9)
 #endif
-#define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed after OLD(r))
-#define DH_DAG_SLOOK 8  // measured at ns = 2025, nt = 100: 6 -> 83.7 ms, 7 -> 81.1, 8 -> 80.3, 10 -> 81.7, 12 -> 82.9
+#define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed after OLD(r)); 6, 8, 12 measured within 0.3 %
+
+// Columns ahead at which the early tasks (OLD_X, OLD_LW, OLD_A) of a column are listed.  Measured on
+// the two-ended launch with zero-tile profiles: at ns = 2025 (32 tile columns) 1 -> 57.3 ms,
+// 2 -> 51.6, 3 -> 49.3, 4 -> 49.2, 5 -> 49.4, 6 -> 49.9, 8 -> 50.9, 11 -> 52.6; at ns = 4096
+// (64 tile columns, bound by the products in bulk) 4 -> 579 ms, 8 -> 576 (8 had also been the
+// optimum of the dense one-ended launch at ns = 2025).
+__host__ __device__ inline int dag_slook(int nbt) { return nbt <= 48 ? 4 : 8; }
 
 // Flags of one block: final D tiles [nbt][nbt], final L tiles [nbt][nbt],
 // final arrow tiles [nbt], tip partial applied, PRE_D, PRE_L, OLD_D, OLD_L done [nbt each],
@@ -3114,7 +3120,7 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
     for (int ar = 0; ar < 2; ++ar)
       for (int last = 0; last < 2; ++last) {
         const size_t before = tab.size();
-        dag_table(h->nbt, last ? 0 : h->nbt, ar, DH_DAG_LOOK, DH_DAG_SLOOK, tab);
+        dag_table(h->nbt, last ? 0 : h->nbt, ar, DH_DAG_LOOK, dag_slook(h->nbt), tab);
         h->dagt[ar][last] = (int)(tab.size() - before);
       }
     if (hipMalloc((void**)&h->dagtab, sizeof(int) * tab.size()) != hipSuccess ||
@@ -3387,7 +3393,7 @@ static DagArgs dag_args(const dh_handle* h, int ar, int slot0, int cnt, bool has
   g.tfull = h->dagt[ar][0];
   g.ntasks = last_full ? cnt * h->dagt[ar][0] : (cnt - 1) * h->dagt[ar][0] + h->dagt[ar][1];
   g.look = DH_DAG_LOOK;
-  g.slook = DH_DAG_SLOOK;
+  g.slook = dag_slook(h->nbt);
   g.bdone = nullptr;
   g.has_prev = has_prev ? 1 : 0;
   g.last_full = last_full ? 1 : 0;
@@ -4449,10 +4455,10 @@ int dh_prior_eig_dev(dh_handle* h, const dh_model* m, const double* lam, const d
 int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int* slook) {
   if (nbt < 1 || nbt > 2 * DH_MAXNODES || cap < 0 || (cap > 0 && !out)) return DH_EARG;
   std::vector<int> tab;
-  dag_table(nbt, last ? 0 : nbt, arrow ? 1 : 0, DH_DAG_LOOK, DH_DAG_SLOOK, tab);
+  dag_table(nbt, last ? 0 : nbt, arrow ? 1 : 0, DH_DAG_LOOK, dag_slook(nbt), tab);
   for (int e = 0; e < (int)tab.size() && e < cap; ++e) out[e] = tab[e];
   if (look) *look = DH_DAG_LOOK;
-  if (slook) *slook = DH_DAG_SLOOK;
+  if (slook) *slook = dag_slook(nbt);
   return (int)tab.size();
 }
 
