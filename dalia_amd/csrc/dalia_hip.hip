@@ -730,9 +730,12 @@ struct DagArgs {
 #define DAG_MODE (COLS ? M_COL : M_ROW)
 #define DAG_OP(P, PT, LDT) (COLS ? (const double*)(PT) : (const double*)(P)), (COLS ? (long)(LDT) : (long)bp)
 
-#ifndef DAG_STAGES
-#define DAG_STAGES 2  // operand ring depth of the task products (measured: 1 -> 77.9 ms, 2 -> 77.2, 3 -> 78.8, 4 -> 80.3, 5 -> 80.9)
-#endif
+// Operand ring depth S of the task products (a template parameter of the launch).  Measured on the
+// dense one-ended launch at ns = 2025: 1 -> 77.9 ms, 2 -> 77.2, 3 -> 78.8, 4 -> 80.3, 5 -> 80.9; on the
+// two-ended launch with zero-tile profiles (early tasks 4 columns ahead): at ns = 2025 1 -> 48.3 ms,
+// 2 -> 49.1, 3 -> 49.2; at ns = 4096 (64 tile columns, bound by the products in bulk) 1 -> 596,
+// 2 -> 574.
+__host__ __device__ inline int dag_stages(int nbt) { return nbt <= 48 ? 1 : 2; }
 #define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed after OLD(r)); 6, 8, 12 measured within 0.3 %
 
 // Columns ahead at which the early tasks (OLD_X, OLD_LW, OLD_A) of a column are listed.  Measured on
@@ -795,7 +798,7 @@ __device__ int dag_ready(const int* fa, const int* fb, int k, int kend, int* err
 
 // acc += opA[ra.., k] opB[rb.., k] over tile columns k in [0, kend), each
 // tile column consumed once its flags are set.
-template <bool COLS>
+template <bool COLS, int S>
 __device__ __forceinline__ void dag_accumulate(dh_d4 acc[2][2], const double* A, long lda, int ra, const double* B,
                                                long ldb, int rb, const int* fa, const int* fb, int kend, int* err,
                                                int* s_n) {
@@ -803,13 +806,13 @@ __device__ __forceinline__ void dag_accumulate(dh_d4 acc[2][2], const double* A,
   const Opnd ob{B, ldb, DAG_MODE, nullptr};
   for (int k = 0; k < kend;) {
     const int cnt = dag_ready(fa, fb, k, kend, err, s_n);
-    dh_mma<DAG_MODE, DAG_MODE, DAG_STAGES>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
+    dh_mma<DAG_MODE, DAG_MODE, S>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
     k += cnt;
   }
 }
 
 // dag_accumulate over tile columns k in [k0, kend).
-template <bool COLS>
+template <bool COLS, int S>
 __device__ __forceinline__ void dag_accumulate_from(dh_d4 acc[2][2], const double* A, long lda, int ra,
                                                     const double* B, long ldb, int rb, const int* fa, const int* fb,
                                                     int k0, int kend, int* err, int* s_n) {
@@ -817,7 +820,7 @@ __device__ __forceinline__ void dag_accumulate_from(dh_d4 acc[2][2], const doubl
   const Opnd ob{B, ldb, DAG_MODE, nullptr};
   for (int k = k0; k < kend;) {
     const int cnt = dag_ready(fa, fb, k, kend, err, s_n);
-    dh_mma<DAG_MODE, DAG_MODE, DAG_STAGES>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
+    dh_mma<DAG_MODE, DAG_MODE, S>(acc, oa, ra, ob, rb, k * DH_TILE, (k + cnt) * DH_TILE);
     k += cnt;
   }
 }
@@ -861,14 +864,14 @@ __device__ __forceinline__ void dag_tiles_to_lds(double* const (&dst)[NT], const
 
 // C = (-acc) inv(L)^T once the factor of the diagonal tile is final: -acc is
 // staged in LDS before the wait, so that only inv(L) is loaded after it.
-template <bool COLS>
+template <bool COLS, int S>
 __device__ __forceinline__ void dag_solve_store(double* C, double* CT, long ldt, const dh_d4 acc[2][2],
                                                 const double* pinv, int bp, const int* fdiag, int* err, int* s_n) {
   dh_store(dh_lS, DH_LDS_LD, 0, 0, acc, -1.0, 0.0, false);
   dag_ready(fdiag, nullptr, 0, 1, err, s_n);  // also orders the LDS stores
   dh_d4 x[2][2];
   dh_zero(x);
-  dh_mma<M_ROW, M_ROW, DAG_STAGES>(x, Opnd{dh_lS, DH_LDS_LD, M_ROW, nullptr}, 0, Opnd{pinv, DH_TILE, M_ROW, nullptr}, 0, 0,
+  dh_mma<M_ROW, M_ROW, S>(x, Opnd{dh_lS, DH_LDS_LD, M_ROW, nullptr}, 0, Opnd{pinv, DH_TILE, M_ROW, nullptr}, 0, 0,
                        DH_TILE);
   dh_store(C, bp, 0, 0, x, 1.0, 0.0, false);
   if constexpr (COLS) dh_store(CT, ldt, 0, 0, x, 1.0, 0.0, true);  // the transposed copy, tile (c, r)
@@ -969,7 +972,7 @@ __device__ __forceinline__ void chain_store_t(double* C, long ldc, const dh_d4 a
 }
 
 // One task (t wave-uniform); every path ends at the single signal.
-template <bool COLS>
+template <bool COLS, int S>
 __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   DagArgs g = g0;
   if (g0.order) {  // uniform: the side and its task, and the side's arrays
@@ -1041,25 +1044,25 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   dh_zero(acc);
   int* fdone;
   if (kind == DK_TIP) {
-    dag_accumulate<COLS>(acc, DAG_OP(Ar, ArT, DH_TILE), 0, DAG_OP(Ar, ArT, DH_TILE), 0, fA, nullptr, nbt, g.err, s_n);
+    dag_accumulate<COLS, S>(acc, DAG_OP(Ar, ArT, DH_TILE), 0, DAG_OP(Ar, ArT, DH_TILE), 0, fA, nullptr, nbt, g.err, s_n);
     if (i > 0) dag_ready(fTp, nullptr, 0, 1, g.err, s_n);
     dh_store(g.tip, DH_TILE, 0, 0, acc, -1.0, 1.0, false);
     fdone = fT;
   } else if (kind == DK_OLD_D) {  // tile (r, r): previous block's terms and own terms k < kold
     dag_preload_neg(acc, D + r * T64 + r * DH_TILE, bp);
     if (prev)
-      dag_accumulate_from<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), r * DH_TILE, fLp + r * nbt,
+      dag_accumulate_from<COLS, S>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), r * DH_TILE, fLp + r * nbt,
                                 nullptr, pp(r), r == 0 ? nbt - 1 : nbt, g.err, s_n);
-    dag_accumulate<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), r * DH_TILE, fD + r * nbt, nullptr, kold, g.err,
+    dag_accumulate<COLS, S>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), r * DH_TILE, fD + r * nbt, nullptr, kold, g.err,
                    s_n);
     dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fOD + r;
   } else if (kind == DK_OLD_L) {  // tile (r, r-1): previous block's terms and own terms k < kold
     dag_preload_neg(acc, D + r * T64 + (r - 1) * DH_TILE, bp);
     if (prev)
-      dag_accumulate_from<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), (r - 1) * DH_TILE,
+      dag_accumulate_from<COLS, S>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), (r - 1) * DH_TILE,
                                 fLp + r * nbt, fLp + (r - 1) * nbt, max(pp(r), pp(r - 1)), nbt, g.err, s_n);
-    dag_accumulate<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), (r - 1) * DH_TILE, fD + r * nbt,
+    dag_accumulate<COLS, S>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), (r - 1) * DH_TILE, fD + r * nbt,
                    fD + (r - 1) * nbt, kold, g.err,
                    s_n);
     dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
@@ -1067,14 +1070,14 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   } else if (kind == DK_PRE_D) {  // tile (r, r), r >= 1: own terms kold <= k < kpre
     dag_ready(fOD + r, nullptr, 0, 1, g.err, s_n);
     dag_preload_neg(acc, D + r * T64 + r * DH_TILE, bp);
-    dag_accumulate_from<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), r * DH_TILE, fD + r * nbt, nullptr,
+    dag_accumulate_from<COLS, S>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), r * DH_TILE, fD + r * nbt, nullptr,
                         kold, kpre, g.err, s_n);
     dh_store(D + r * T64 + r * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fPD + r;
   } else if (kind == DK_PRE_L) {  // tile (r, r-1), r >= 1: own terms kold <= k < kpre
     dag_ready(fOL + r, nullptr, 0, 1, g.err, s_n);
     dag_preload_neg(acc, D + r * T64 + (r - 1) * DH_TILE, bp);
-    dag_accumulate_from<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), (r - 1) * DH_TILE, fD + r * nbt,
+    dag_accumulate_from<COLS, S>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), (r - 1) * DH_TILE, fD + r * nbt,
                         fD + (r - 1) * nbt, kold,
                         kpre, g.err, s_n);
     dh_store(D + r * T64 + (r - 1) * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
@@ -1111,8 +1114,8 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
         const Opnd l1{dh_lW, DH_LDS_LD, M_ROW, nullptr};
         dh_d4 q[2][2];
         dh_zero(q);
-        dh_mma<M_ROW, M_ROW, DAG_STAGES>(q, l2, 0, l2, 0, 0, DH_TILE);
-        dh_mma<M_ROW, M_ROW, DAG_STAGES>(nx, l2, 0, l1, 0, 0, DH_TILE);
+        dh_mma<M_ROW, M_ROW, S>(q, l2, 0, l2, 0, 0, DH_TILE);
+        dh_mma<M_ROW, M_ROW, S>(nx, l2, 0, l1, 0, 0, DH_TILE);
 #pragma unroll
         for (int x0 = 0; x0 < 2; ++x0)
 #pragma unroll
@@ -1133,11 +1136,11 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
       chain_store(dh_lV, DH_LDS_LD, x, ca, cb);
       dag_signal(fD + c * nbt + (c - 1));
       const Opnd xs{dh_lV, DH_LDS_LD, M_ROW, nullptr};
-      dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, xs, 0, xs, 0, 0, DH_TILE);
+      dh_mma<M_ROW, M_ROW, S>(acc, xs, 0, xs, 0, 0, DH_TILE);
     } else if (prev && pp(0) < nbt) {
       dag_ready(fLp + (nbt - 1), nullptr, 0, 1, g.err, s_n);
       const Opnd xl{Lp + (nbt - 1) * DH_TILE, bp, M_ROW, nullptr};
-      dh_mma<M_ROW, M_ROW, DAG_STAGES>(acc, xl, 0, xl, 0, 0, DH_TILE);
+      dh_mma<M_ROW, M_ROW, S>(acc, xl, 0, xl, 0, 0, DH_TILE);
     }
 #pragma unroll
     for (int x0 = 0; x0 < 2; ++x0)
@@ -1155,9 +1158,9 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   } else if (kind == DK_OLD_X) {  // diagonal-block tile (r, c): previous block's terms, own terms k < ks
     if (prev || ks > 0) dag_preload_neg(acc, D + r * T64 + c * DH_TILE, bp);
     if (prev)
-      dag_accumulate_from<COLS>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), c * DH_TILE, fLp + r * nbt,
+      dag_accumulate_from<COLS, S>(acc, DAG_OP(Lp, LpT, bp), r * DH_TILE, DAG_OP(Lp, LpT, bp), c * DH_TILE, fLp + r * nbt,
                                 fLp + c * nbt, max(pp(r), pp(c)), nbt, g.err, s_n);
-    dag_accumulate<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fD + r * nbt, fD + c * nbt,
+    dag_accumulate<COLS, S>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fD + r * nbt, fD + c * nbt,
                    ks, g.err, s_n);
     if (prev || ks > 0) dh_store(D + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fX + r * nbt + c;
@@ -1165,7 +1168,7 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
     const int z0 = po(r);
     if (c >= z0 && ks > z0) {  // a tile left of its row's first nonzero tile stays zero
       dag_preload_neg(acc, Lo + r * T64 + c * DH_TILE, bp);
-      dag_accumulate_from<COLS>(acc, DAG_OP(Lo, LoT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fL + r * nbt,
+      dag_accumulate_from<COLS, S>(acc, DAG_OP(Lo, LoT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fL + r * nbt,
                                 fD + c * nbt, z0, ks, g.err, s_n);
       dh_store(Lo + r * T64 + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     }
@@ -1173,9 +1176,9 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
   } else if (kind == DK_OLD_A) {  // arrow tile c: previous block's terms, own terms k < ks
     if (prev || ks > 0) dag_preload_neg(acc, Ar + c * DH_TILE, bp);
     if (prev)
-      dag_accumulate_from<COLS>(acc, DAG_OP(Ap, ApT, DH_TILE), 0, DAG_OP(Lp, LpT, bp), c * DH_TILE, fAp, fLp + c * nbt,
+      dag_accumulate_from<COLS, S>(acc, DAG_OP(Ap, ApT, DH_TILE), 0, DAG_OP(Lp, LpT, bp), c * DH_TILE, fAp, fLp + c * nbt,
                                 pp(c), nbt, g.err, s_n);
-    dag_accumulate<COLS>(acc, DAG_OP(Ar, ArT, DH_TILE), 0, DAG_OP(D, DT, bp), c * DH_TILE, fA, fD + c * nbt, ks, g.err, s_n);
+    dag_accumulate<COLS, S>(acc, DAG_OP(Ar, ArT, DH_TILE), 0, DAG_OP(D, DT, bp), c * DH_TILE, fA, fD + c * nbt, ks, g.err, s_n);
     if (prev || ks > 0) dh_store(Ar + c * DH_TILE, bp, 0, 0, acc, -1.0, 0.0, false);
     fdone = fXA + c;
   } else {
@@ -1191,7 +1194,7 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
       ldt = bp;
       dag_ready(fX + r * nbt + c, nullptr, 0, 1, g.err, s_n);
       dag_preload_neg(acc, C, bp);
-      dag_accumulate_from<COLS>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fD + r * nbt,
+      dag_accumulate_from<COLS, S>(acc, DAG_OP(D, DT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fD + r * nbt,
                           fD + c * nbt, ks, c, g.err, s_n);
       fdone = fD + r * nbt + c;
     } else if (kind == DK_LOWER) {  // sub-diagonal block tile (r, c)
@@ -1204,7 +1207,7 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
         const int k0 = max(z0, c > g.slook ? ks : 0);
         if (c > g.slook) dag_ready(fLW + r * nbt + c, nullptr, 0, 1, g.err, s_n);
         dag_preload_neg(acc, C, bp);
-        dag_accumulate_from<COLS>(acc, DAG_OP(Lo, LoT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fL + r * nbt,
+        dag_accumulate_from<COLS, S>(acc, DAG_OP(Lo, LoT, bp), r * DH_TILE, DAG_OP(D, DT, bp), c * DH_TILE, fL + r * nbt,
                                   fD + c * nbt, k0, c, g.err, s_n);
       }
       fdone = fL + r * nbt + c;
@@ -1214,11 +1217,11 @@ __device__ void dag_task(const DagArgs& g0, int t, int* s_n) {
       ldt = DH_TILE;
       dag_ready(fXA + c, nullptr, 0, 1, g.err, s_n);
       dag_preload_neg(acc, C, bp);
-      dag_accumulate_from<COLS>(acc, DAG_OP(Ar, ArT, DH_TILE), 0, DAG_OP(D, DT, bp), c * DH_TILE, fA, fD + c * nbt, ks, c,
+      dag_accumulate_from<COLS, S>(acc, DAG_OP(Ar, ArT, DH_TILE), 0, DAG_OP(D, DT, bp), c * DH_TILE, fA, fD + c * nbt, ks, c,
                           g.err, s_n);
       fdone = fA + c;
     }
-    if (!zero) dag_solve_store<COLS>(C, CT, ldt, acc, pinv + c * DH_TILE * DH_TILE, bp, fD + c * nbt + c, g.err, s_n);
+    if (!zero) dag_solve_store<COLS, S>(C, CT, ldt, acc, pinv + c * DH_TILE * DH_TILE, bp, fD + c * nbt + c, g.err, s_n);
   }
   if (fdone || g.bdone) dag_finish(fdone, g.bdone ? g.bdone + i : nullptr);  // wave-uniform
 }
@@ -1272,7 +1275,7 @@ __global__ __launch_bounds__(256) void k_tile_profile(const double* lower, int b
 // One persistent launch over the factorizations of several handles: ticket t runs task order[t].y
 // of instance order[t].x.  The order keeps every instance's own task order, so each task still
 // waits only on tasks with smaller tickets, and the grid drains as for one instance.
-template <bool COLS>
+template <bool COLS, int S>
 __global__ __launch_bounds__(256) void k_pobtaf_dag_multi(const DagArgs* gs, const int2* order, int ntotal,
                                                           int* counter) {
   __shared__ int s_task, s_n;
@@ -1288,7 +1291,7 @@ __global__ __launch_bounds__(256) void k_pobtaf_dag_multi(const DagArgs* gs, con
     const int2 o = order[t];
     const int inst = __builtin_amdgcn_readfirstlane(o.x);
     const int task = __builtin_amdgcn_readfirstlane(o.y);
-    dag_task<COLS>(gs[inst], task, &s_n);
+    dag_task<COLS, S>(gs[inst], task, &s_n);
   }
 }
 
@@ -1382,7 +1385,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // Tasks are claimed by wave 0 under a wave-uniform branch: lane 0 adds 1 and
 // the other lanes add 0, and every lane writes lane 0's ticket, so the loop
 // contains no lane-divergent branch around its barriers.
-template <bool COLS>
+template <bool COLS, int S>
 __global__ __launch_bounds__(256) void k_pobtaf_dag(DagArgs g) {
   __shared__ int s_task, s_n;
   for (;;) {
@@ -1394,7 +1397,7 @@ __global__ __launch_bounds__(256) void k_pobtaf_dag(DagArgs g) {
     const int t = __builtin_amdgcn_readfirstlane(s_task);
     __syncthreads();
     if (t >= g.ntasks) break;
-    dag_task<COLS>(g, t, &s_n);
+    dag_task<COLS, S>(g, t, &s_n);
   }
 }
 
@@ -3059,7 +3062,7 @@ static int create_handle(int n, int b, int a, void* stream, bool storage, dh_han
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
       cus = 256;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pobtaf_dag<true>, 256, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pobtaf_dag<true, 2>, 256, 0) != hipSuccess || per_cu < 1)
       per_cu = 1;
     h->dag_wgs = per_cu * cus;
     int per_cu_s = 0;
@@ -3493,6 +3496,18 @@ static void twist_lower(dh_handle* h) {
                      h->lower + (long)m * h->bp * h->bp, h->bp);
 }
 
+// The persistent launch with the operand ring depth for nbt tile columns (dag_stages).
+static void dag_kernel_launch(const DagArgs& g, int nbt, dim3 grid, hipStream_t st) {
+  const int S = dag_stages(nbt);
+  if (g.cols) {
+    if (S == 1) hipLaunchKernelGGL((k_pobtaf_dag<true, 1>), grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL((k_pobtaf_dag<true, 2>), grid, dim3(256), 0, st, g);
+  } else {
+    if (S == 1) hipLaunchKernelGGL((k_pobtaf_dag<false, 1>), grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL((k_pobtaf_dag<false, 2>), grid, dim3(256), 0, st, g);
+  }
+}
+
 // The two-ended factorization: both sides in one persistent launch, the upward side's terms
 // applied to block m, block m factored (the downward side's terms through has_prev), then the tip.
 static int cholesky_twisted(dh_handle* h, int sparsity, int ar, bool pa) {
@@ -3516,8 +3531,7 @@ static int cholesky_twisted(dh_handle* h, int sparsity, int ar, bool pa) {
   gt.prof2 = gt.prof ? h->dagprof + (long)(n - 2) * nbt : nullptr;
   gt.cols = cols_for(h, true) ? 1 : 0;
   const dim3 gw((unsigned)std::min(h->tw_ntotal[ar], cap));
-  if (gt.cols) hipLaunchKernelGGL(k_pobtaf_dag<true>, gw, dim3(256), 0, h->stream, gt);
-  else hipLaunchKernelGGL(k_pobtaf_dag<false>, gw, dim3(256), 0, h->stream, gt);
+  dag_kernel_launch(gt, nbt, gw, h->stream);
   const int ntri = nbt * (nbt + 1) / 2;
   hipLaunchKernelGGL(k_twist_join, dim3(ntri + (ar ? nbt + 1 : 0)), dim3(256), 0, h->stream,
                      (const double*)(h->lower + m * bb), (const double*)(ar ? h->arrow + (long)(m + 1) * 64 * bp : nullptr),
@@ -3528,8 +3542,7 @@ static int cholesky_twisted(dh_handle* h, int sparsity, int ar, bool pa) {
   g.cols = gt.cols;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
   const dim3 gm((unsigned)std::min<int>(g.ntasks, cap));
-  if (g.cols) hipLaunchKernelGGL(k_pobtaf_dag<true>, gm, dim3(256), 0, h->stream, g);
-  else hipLaunchKernelGGL(k_pobtaf_dag<false>, gm, dim3(256), 0, h->stream, g);
+  dag_kernel_launch(g, nbt, gm, h->stream);
   if (timed) {
     (void)hipEventRecord(h->ev[2 * h->nrec + 1], h->stream);
     h->nrec++;
@@ -3549,8 +3562,13 @@ static void dag_launch(dh_handle* h, const DagArgs& g) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, h->stream, h->dagcounter, 0);
   const int cap = h->dag_cap > 0 ? std::min(h->dag_cap, h->dag_wgs) : h->dag_wgs;
   const unsigned wgs = (unsigned)std::min<int>(g.ntasks, cap);
-  if (g.cols) DH_LAUNCH(h, 7, k_pobtaf_dag<true>, dim3(wgs), dim3(256), 0, g);
-  else DH_LAUNCH(h, 7, k_pobtaf_dag<false>, dim3(wgs), dim3(256), 0, g);
+  const bool _t = h->tkind == 7 && h->nrec < DH_MAXREC;  // DH_LAUNCH's event pair
+  if (_t) (void)hipEventRecord(h->ev[2 * h->nrec], h->stream);
+  dag_kernel_launch(g, h->nbt, dim3(wgs), h->stream);
+  if (_t) {
+    (void)hipEventRecord(h->ev[2 * h->nrec + 1], h->stream);
+    h->nrec++;
+  }
 }
 
 int dh_cholesky(dh_handle* h, int sparsity) {
@@ -4927,12 +4945,18 @@ int dh_multi_cholesky(dh_multi* mc, int sparsity, void* hip_stream) {
   }
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, mc->counter, 0);
   const unsigned wgs = (unsigned)std::min(mc->ntotal[sparsity], mc->wgs);
-  if (mc->cols)
-    hipLaunchKernelGGL(k_pobtaf_dag_multi<true>, dim3(wgs), dim3(256), 0, st, (const DagArgs*)mc->args[sparsity],
-                       (const int2*)mc->order[sparsity], mc->ntotal[sparsity], mc->counter);
-  else
-    hipLaunchKernelGGL(k_pobtaf_dag_multi<false>, dim3(wgs), dim3(256), 0, st, (const DagArgs*)mc->args[sparsity],
-                       (const int2*)mc->order[sparsity], mc->ntotal[sparsity], mc->counter);
+  int nbt_max = 1;  // the ring depth of the largest instance
+  for (dh_handle* h : mc->hs) nbt_max = std::max(nbt_max, h->nbt);
+  const DagArgs* ga = (const DagArgs*)mc->args[sparsity];
+  const int2* go = (const int2*)mc->order[sparsity];
+  const int nt = mc->ntotal[sparsity];
+  if (dag_stages(nbt_max) == 1) {
+    if (mc->cols) hipLaunchKernelGGL((k_pobtaf_dag_multi<true, 1>), dim3(wgs), dim3(256), 0, st, ga, go, nt, mc->counter);
+    else hipLaunchKernelGGL((k_pobtaf_dag_multi<false, 1>), dim3(wgs), dim3(256), 0, st, ga, go, nt, mc->counter);
+  } else {
+    if (mc->cols) hipLaunchKernelGGL((k_pobtaf_dag_multi<true, 2>), dim3(wgs), dim3(256), 0, st, ga, go, nt, mc->counter);
+    else hipLaunchKernelGGL((k_pobtaf_dag_multi<false, 2>), dim3(wgs), dim3(256), 0, st, ga, go, nt, mc->counter);
+  }
   for (dh_handle* h : mc->hs) {
     if (sparsity == DH_SPARSITY_BTA && h->a > 0)
       hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, st, h->tip, (long)h->ap, h->tipdinv, h->tipinv, h->info,
