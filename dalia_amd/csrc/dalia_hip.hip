@@ -735,7 +735,14 @@ struct DagArgs {
 // two-ended launch with zero-tile profiles (early tasks 4 columns ahead): at ns = 2025 1 -> 48.3 ms,
 // 2 -> 49.1, 3 -> 49.2; at ns = 4096 (64 tile columns, bound by the products in bulk) 1 -> 596,
 // 2 -> 574.
-__host__ __device__ inline int dag_stages(int nbt) { return nbt <= 48 ? 1 : 2; }
+// Blocks of at most this many tile columns take the shallow schedule (early tasks 4 columns ahead, a
+// one-stage operand ring), larger ones the deep one (8 columns, two stages).  Two-ended launch with
+// zero-tile profiles, shallow against deep: 32 tile columns (ns = 2025) 48.3 against 50.9 ms;
+// 36 (ns = 2304) 50.0 against 50.0; 44 (ns = 2809) 69.9 against 67.2; 49 (ns = 3136) 77.0 against
+// 73.7; 50 (bivariate, b = 3200) 173.4 against 164.4; 64 (ns = 4096), each knob alone: early tasks 4
+// columns ahead 579 against 576, one stage 596 against 574.
+#define DH_DAG_SMALL 36
+__host__ __device__ inline int dag_stages(int nbt) { return nbt <= DH_DAG_SMALL ? 1 : 2; }
 #define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed after OLD(r)); 6, 8, 12 measured within 0.3 %
 
 // Columns ahead at which the early tasks (OLD_X, OLD_LW, OLD_A) of a column are listed.  Measured on
@@ -743,7 +750,7 @@ __host__ __device__ inline int dag_stages(int nbt) { return nbt <= 48 ? 1 : 2; }
 // 2 -> 51.6, 3 -> 49.3, 4 -> 49.2, 5 -> 49.4, 6 -> 49.9, 8 -> 50.9, 11 -> 52.6; at ns = 4096
 // (64 tile columns, bound by the products in bulk) 4 -> 579 ms, 8 -> 576 (8 had also been the
 // optimum of the dense one-ended launch at ns = 2025).
-__host__ __device__ inline int dag_slook(int nbt) { return nbt <= 48 ? 4 : 8; }
+__host__ __device__ inline int dag_slook(int nbt) { return nbt <= DH_DAG_SMALL ? 4 : 8; }
 
 // Flags of one block: final D tiles [nbt][nbt], final L tiles [nbt][nbt],
 // final arrow tiles [nbt], tip partial applied, PRE_D, PRE_L, OLD_D, OLD_L done [nbt each],
