@@ -743,7 +743,7 @@ struct DagArgs {
 // columns ahead 579 against 576, one stage 596 against 574.
 #define DH_DAG_SMALL 36
 __host__ __device__ inline int dag_stages(int nbt) { return nbt <= DH_DAG_SMALL ? 1 : 2; }
-#define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed after OLD(r)); 6, 8, 12 measured within 0.3 %
+#define DH_DAG_LOOK 10  // >= 2 (PRE(r) is listed after OLD(r)); 6, 8, 12, 14 measured within 0.3 % (4: +5 %)
 
 // Columns ahead at which the early tasks (OLD_X, OLD_LW, OLD_A) of a column are listed.  Measured on
 // the two-ended launch with zero-tile profiles: at ns = 2025 (32 tile columns) 1 -> 57.3 ms,
