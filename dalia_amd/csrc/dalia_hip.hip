@@ -1888,8 +1888,8 @@ __global__ __launch_bounds__(256) void k_sel_sym2(SelArgs a0, double* D0, int ns
 // with L skip its zero tiles (the profile), and X_{q,i} is formed only on the tiles of the coupling
 // block where the matrix has entries (first..last nonzero tile of each tile row), which is all that
 // a caller reading the selected inverse on the matrix's pattern needs (the adjoint contractions,
-// marginal variances).  Per block 3.3 b^3 flops with the banded profile of a grid ordering, against
-// 4.1 b^3 for the W form, and no W or inv(L)^T inv(L).
+// marginal variances).  Per block 3.4 b^3 flops with the banded profile of a grid ordering (28.9
+// GFLOP at b = 2048), against 13/3 b^3 for the W form, and no W or inv(L)^T inv(L).
 struct SelCArgs {
   double* Lb;          // L_{q,i} as stored (rows of q); X_{q,i} on its band tiles on exit; null: no successor
   double* La;          // L_{a,i} (64 x bp); X_{a,i} on exit; null: no arrowhead
