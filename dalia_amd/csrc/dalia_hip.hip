@@ -2041,14 +2041,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 }
 
 // Phase C, by tile column c in increasing order (every product of column c runs over k tiles
-// c..nbt-1, so the longest start first): the lower tiles (K, c), K >= c, of T = (I + M) inv(L_ii);
-// tile c of X_{a,i} = -Ya inv(L_ii) (with the arrowhead); tile (R, c) of X_{q,i} = -Y inv(L_ii) for
-// each tile row R (with a successor) whose band of nonzero tiles holds c (the other R return).
+// c..nbt-1, so the longest start first): the lower tiles (K, c), K >= c, of T = (I + M) inv(L_ii),
+// and tile c of X_{a,i} = -Ya inv(L_ii) (with the arrowhead).
 __device__ __forceinline__ void selc_t_work(const SelCArgs& a, bool has_m, int e) {
-  const int nbt = a.nbt, ca = a.La ? 1 : 0, cb = a.Lb ? nbt : 0;
+  const int nbt = a.nbt, ca = a.La ? 1 : 0;
   int c = 0;
-  while (c < nbt - 1 && e >= nbt - c + ca + cb) {  // column c holds nbt - c + ca + cb items
-    e -= nbt - c + ca + cb;
+  while (c < nbt - 1 && e >= nbt - c + ca) {  // column c holds nbt - c + ca items
+    e -= nbt - c + ca;
     ++c;
   }
   const Opnd q{a.Q, a.bp, M_COL, nullptr};  // opM[j][k'] = inv(L)(k', j), read by columns
@@ -2057,28 +2056,16 @@ __device__ __forceinline__ void selc_t_work(const SelCArgs& a, bool has_m, int e
     const int K = c + e;
     dh_load_acc(a.Q, a.bp, K * DH_TILE, c * DH_TILE, acc);  // the identity's term: inv(L)(K, c)
     if (has_m)  // opM[k][k'] = M(k', k) = M(k, k')
-      dh_mma<M_COL, M_COL>(acc, Opnd{a.M, a.bp, M_COL, nullptr}, K * DH_TILE, q, c * DH_TILE, c * DH_TILE,
-                                      a.bp);
+      dh_mma<M_COL, M_COL>(acc, Opnd{a.M, a.bp, M_COL, nullptr}, K * DH_TILE, q, c * DH_TILE, c * DH_TILE, a.bp);
     dh_store(a.T, a.bp, K * DH_TILE, c * DH_TILE, acc, 1.0, 0.0, false);
     return;
   }
-  e -= nbt - c;
-  dh_zero(acc);
-  if (e < ca) {
-    dh_mma<M_ROW, M_COL>(acc, Opnd{a.Ya, a.bp, M_ROW, nullptr}, 0, q, c * DH_TILE, c * DH_TILE, a.bp);
-    dh_store(a.La, a.bp, 0, c * DH_TILE, acc, -1.0, 0.0, false);
-    return;
-  }
-  const int R = e - ca;
-  const int lo = a.prof ? a.prof[R] : 0, hi = a.profr ? a.profr[R] : nbt - 1;
-  if (c < lo || c > hi) return;  // no entry of the matrix in this tile: X not needed there
-  dh_mma<M_ROW, M_COL>(acc, Opnd{a.Y, a.yld, M_ROW, nullptr}, R * DH_TILE, q, c * DH_TILE, c * DH_TILE, a.bp);
-  dh_store(a.Lb, a.bp, R * DH_TILE, c * DH_TILE, acc, -1.0, 0.0, a.xt != 0);
+  dh_zero(acc);  // the column's arrow tile (e == nbt - c, with the arrowhead)
+  dh_mma<M_ROW, M_COL>(acc, Opnd{a.Ya, a.bp, M_ROW, nullptr}, 0, q, c * DH_TILE, c * DH_TILE, a.bp);
+  dh_store(a.La, a.bp, 0, c * DH_TILE, acc, -1.0, 0.0, false);
 }
 
-__device__ __host__ __forceinline__ int selc_nt(const SelCArgs& a) {
-  return a.nbt * (a.nbt + 1) / 2 + (a.La ? a.nbt : 0) + (a.Lb ? a.nbt * a.nbt : 0);
-}
+__device__ __host__ __forceinline__ int selc_nt(const SelCArgs& a) { return a.nbt * (a.nbt + 1) / 2 + (a.La ? a.nbt : 0); }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_selc_t2(SelCArgs a0, int m0, int n0,
                                                                                      SelCArgs a1, int m1) {
@@ -2089,8 +2076,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // Phase D: lower tile (I, J) of X_ii = inv(L_ii)^T T, written to D at (I, J) and mirrored to (J, I)
 // (a diagonal tile exactly symmetric); tile rows I in increasing order, the products (k over tiles
-// I..nbt-1) longest first.
+// I..nbt-1) longest first.  After them (with a successor), by tile column c: tile (R, c) of
+// X_{q,i} = -Y inv(L_ii) for each tile row R whose band of nonzero tiles holds c (the other R
+// return).  It needs only Y and inv(L_ii); in this launch, whose items all run in one round, it
+// fills free slots instead of lengthening the T phase (selected inversion 58.5 -> 57.7 ms).
 __device__ __forceinline__ void selc_x_work(const SelCArgs& a, int e) {
+  const int ntri = a.nbt * (a.nbt + 1) / 2;
+  if (e >= ntri) {  // band tile (R, c) of X_{q,i} = -Y inv(L_ii), columns c increasing
+    const int c = (e - ntri) / a.nbt, R = (e - ntri) % a.nbt;
+    const int lo = a.prof ? a.prof[R] : 0, hi = a.profr ? a.profr[R] : a.nbt - 1;
+    if (c < lo || c > hi) return;
+    dh_d4 acc[2][2];
+    dh_zero(acc);
+    const Opnd q{a.Q, a.bp, M_COL, nullptr};
+    dh_mma<M_ROW, M_COL>(acc, Opnd{a.Y, a.yld, M_ROW, nullptr}, R * DH_TILE, q, c * DH_TILE, c * DH_TILE, a.bp);
+    dh_store(a.Lb, a.bp, R * DH_TILE, c * DH_TILE, acc, -1.0, 0.0, a.xt != 0);
+    return;
+  }
   int I, J;
   selc_tri(e, I, J);
   dh_d4 acc[2][2];
@@ -2104,6 +2106,10 @@ __device__ __forceinline__ void selc_x_work(const SelCArgs& a, int e) {
   }
   dh_store(a.D, a.bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, false);
   dh_store(a.D, a.bp, I * DH_TILE, J * DH_TILE, acc, 1.0, 0.0, true);
+}
+
+__device__ __host__ __forceinline__ int selc_nx(const SelCArgs& a) {
+  return a.nbt * (a.nbt + 1) / 2 + (a.Lb ? a.nbt * a.nbt : 0);
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_selc_x2(SelCArgs a0, int n0,
@@ -4229,8 +4235,8 @@ static int selinv_cong(dh_handle* h, bool arrow, const std::vector<SelBlk>& blk,
     }
     const int t0 = selc_nt(a0), t1 = c > 1 ? selc_nt(a1) : 0;
     DH_LAUNCH(h, 2, k_selc_t2, dim3(t0 + t1), dim3(256), 0, a0, m0 ? 1 : 0, t0, a1, m1 ? 1 : 0);
-    const int nx = nbt * (nbt + 1) / 2;
-    DH_LAUNCH(h, 2, k_selc_x2, dim3(nx * c), dim3(256), 0, a0, nx, a1);
+    const int nx0 = selc_nx(a0), nx1 = c > 1 ? selc_nx(a1) : 0;
+    DH_LAUNCH(h, 2, k_selc_x2, dim3(nx0 + nx1), dim3(256), 0, a0, nx0, a1);
   }
   HIPCHK(hipGetLastError());
   return DH_OK;
