@@ -2070,7 +2070,7 @@ __device__ __host__ __forceinline__ int selc_nt(const SelCArgs& a) { return a.nb
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_selc_t2(SelCArgs a0, int m0, int n0,
                                                                                      SelCArgs a1, int m1) {
   int e;
-  if (selc_side(n0, e)) selc_t_work(a1, m1 != 0, e);
+  if (selc_side<256>(n0, e)) selc_t_work(a1, m1 != 0, e);
   else selc_t_work(a0, m0 != 0, e);
 }
 
