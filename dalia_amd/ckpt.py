@@ -207,6 +207,10 @@ class STInlaCkpt(STInla):
                            dld_p=dld_p, lp=lp, glp=glp, lik=lik)
         return float(0.5 * ld_c - 0.5 * (quad_p + quad_l) - (lp + lik + 0.5 * ld_p))
 
+    def value_and_grad(self, theta):
+        self.forward(theta)
+        return self.backward()
+
     def backward(self):
         """Gradient of the last `forward` (its contractions are already on the device)."""
         st = self._state

@@ -93,6 +93,7 @@ class STInla:
         self._state = None
         self._graph = None
         self._ggraph = None
+        self._aux = None  # second stream of the adjoint evaluation (default priority)
         self.n_f_evals = 0
         self._theta_last = None  # last theta evaluated (the reference keeps it in model.theta)
         self._ov_max = int(overlap)
@@ -128,7 +129,7 @@ class STInla:
             self.solver.set_overlap(mode)
             self._ov = mode
 
-    def _conditional(self, theta, out=None, grad: bool = False):
+    def _conditional(self, theta, out=None, grad: bool = False, terms: bool = True):
         """Assemble Qc, factor, solve for mu; queue logdet and the quadratic-form pieces.
 
         theta None: the parameters already on the device are used (graph replay).
@@ -136,15 +137,16 @@ class STInla:
         mu^T A^T y, logdet Qp) (default: the engine's own slots).
         grad: a selected inversion of Qc follows (its products of the factor run beside the
         factorization).
+        terms: False leaves the quadratic-form pieces to the caller (`_mu_terms`).
         """
         out = self.scal if out is None else out
         self._assemble(theta, "bta", True)
         self._overlap(2 if grad else 1)
         _lib.check(self.lib.dh_cholesky(self.h, 1), "dh_cholesky")
-        self._after_qc(out)
+        self._after_qc(out, terms)
 
-    def _after_qc(self, out):
-        """After the factorization of Qc: mu, logdet Qc and the quadratic-form pieces into out."""
+    def _after_qc(self, out, terms: bool = True):
+        """After the factorization of Qc: mu, logdet Qc and (terms) the quadratic-form pieces into out."""
         L = self.lib
         N = self.dm.n_latent
         # mu = e^tau A^T y: scale by params[12] on the device (a 1x1 "scale" kernel via dot-free copy)
@@ -152,6 +154,14 @@ class STInla:
                                   self.mu.data_ptr(), N), "dh_scale_dev")
         _lib.check(L.dh_solve(self.h, self.mu.data_ptr(), 1), "dh_solve")
         _lib.check(L.dh_logdet(self.h, 1, out[0:1].data_ptr()), "dh_logdet")
+        if terms:
+            self._mu_terms(out)
+
+    def _mu_terms(self, out):
+        """mu^T Qp mu, mu^T tau A^T A mu and mu^T A^T y into out[1:4] (they read only mu and the
+        model's data, not the factor)."""
+        L = self.lib
+        N = self.dm.n_latent
         _lib.check(L.dh_qc_matvec(self.h, self.dm.ref(), self.params.data_ptr(),
                                   self.mu.data_ptr(), self.yp.data_ptr(), self.yl.data_ptr()), "dh_qc_matvec")
         _lib.check(L.dh_dot(self.h, self.mu.data_ptr(), self.yp.data_ptr(), N, out[1:2].data_ptr()), "dh_dot")
@@ -372,8 +382,17 @@ class STInla:
         return ld_p, d
 
     def value_and_grad(self, theta):
-        self.forward(theta)
-        return self.backward()
+        """f(theta) and its adjoint gradient: the whole evaluation enqueued at once (as the captured
+        graph runs it), then one read-back; `forward` + `backward` without the closed-form prior."""
+        if not self.dm.closed_form:
+            self.forward(theta)
+            return self.backward()
+        theta = np.asarray(theta, dtype=np.float64)
+        self._stage_grad_params(theta)
+        self.params.copy_(self.params_h, non_blocking=True)
+        self.gam.copy_(self.gam_h, non_blocking=True)
+        self._enqueue_grad()
+        return self._grad_readback(theta)
 
     # ------------------------------------------------------------------
     # the whole adjoint evaluation as one HIP graph
@@ -382,11 +401,25 @@ class STInla:
         forward half, the closed-form prior term, mu^T G_m mu, the selected inversion and its
         contractions (no host synchronisation in between)."""
         L, m = self.lib, self.dm.ref()
-        self._conditional(None, grad=True)
+        main = torch.cuda.current_stream()
+        self.set_stream(main)
+        self._conditional(None, grad=True, terms=False)
+        # The work that reads only mu and the model's data runs on a second stream beside the
+        # selected inversion, which overwrites the factor (and so follows the solve and logdet on
+        # the main stream).  Both use the reduction scratch of the handle: the contractions of the
+        # selected inverse wait for the second stream.
+        if self._aux is None:
+            self._aux = torch.cuda.Stream()
+        aux = self._aux
+        aux.wait_stream(main)
+        self.set_stream(aux)
+        self._mu_terms(self.scal)
         _lib.check(L.dh_prior_eig_dev(self.h, m, self.dm.lam.data_ptr(), self.gam.data_ptr(), self.eig.data_ptr()),
                    "dh_prior_eig_dev")
         _lib.check(L.dh_contract_fem_vec(self.h, m, self.mu.data_ptr(), self.cmu.data_ptr()), "dh_contract_fem_vec")
+        self.set_stream(main)
         _lib.check(L.dh_selected_inversion(self.h, 1), "dh_selected_inversion")
+        main.wait_stream(aux)
         _lib.check(L.dh_contract_fem(self.h, m, self.cx.data_ptr()), "dh_contract_fem")
         _lib.check(L.dh_contract_ata(self.h, m, self.scal[5:6].data_ptr()), "dh_contract_ata")
 
@@ -430,6 +463,10 @@ class STInla:
         theta = np.asarray(theta, dtype=np.float64)
         self._stage_grad_params(theta)
         self._ggraph.replay()
+        return self._grad_readback(theta)
+
+    def _grad_readback(self, theta):
+        """f and the gradient from the device results of one enqueued adjoint evaluation."""
         s = self.scal[:4].cpu().numpy()
         e = self.eig.cpu().numpy()
         ld_c, quad_p, quad_l, bmu = s
