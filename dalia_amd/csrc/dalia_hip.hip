@@ -2908,6 +2908,9 @@ struct dh_handle {
   int sel_cong;                  // 1: pobtasi in congruence form, couplings on the matrix's tiles only (key 10)
   int prof_by_asm;               // 1: dh_assemble records the profiles as it writes the couplings (key 11)
   int prof_asm;                  // 1: dagprof/dagprofr hold the profiles of the couplings dh_assemble wrote
+  int inv_beside;                // 1: dh_solve forms the inverse of the diagonal factors on the side stream,
+                                 // chunk by chunk, beside its forward sweep (key 12)
+  std::vector<hipEvent_t> iev;   // readiness of those chunks
 };
 
 // Products of the tile-task launches read by columns from transposed copies of the operand tiles.
@@ -3177,6 +3180,7 @@ int dh_destroy(dh_handle* h) {
   for (int ar = 0; ar < 2; ++ar)
     if (h->tw_order[ar]) (void)hipFree(h->tw_order[ar]);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->iev) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->sev)
     if (e) (void)hipEventDestroy(e);
   if (h->side) (void)hipStreamDestroy(h->side);
@@ -3712,6 +3716,42 @@ static int ensure_inverse(dh_handle* h) {
   return DH_OK;
 }
 
+// The same on the side stream beside the solve's forward sweep (tuning key 12): chunks of up to 2G
+// blocks in the order in which the sweep reaches them (from both ends inward for a two-ended
+// factor), each followed by an event; chunk_of maps a block to its chunk.  The chunks are as many
+// and as large as `inverse_blocks` makes, and the level kernels' scratch (the W slots) is used by
+// nothing else until the main stream has waited for the last chunk.
+static int inverse_beside(dh_handle* h, std::vector<int>& chunk_of) {
+  const int n = h->n, G2 = 2 * h->G;
+  std::vector<std::pair<int, int>> ch;  // (first block, blocks)
+  if (h->fact_tw >= 0) {
+    int lo = 0, hi = n;
+    for (bool low = true; lo < hi; low = !low) {
+      const int c = std::min(G2, hi - lo);
+      ch.push_back(low ? std::make_pair(lo, c) : std::make_pair(hi - c, c));
+      if (low) lo += c;
+      else hi -= c;
+    }
+  } else {
+    for (int g0 = 0; g0 < n; g0 += G2) ch.push_back(std::make_pair(g0, std::min(G2, n - g0)));
+  }
+  while (h->iev.size() < ch.size()) {
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    h->iev.push_back(e);
+  }
+  chunk_of.assign(n, 0);
+  HIPCHK(hipEventRecord(h->sev[0], h->stream));  // the factor is complete
+  HIPCHK(hipStreamWaitEvent(h->side, h->sev[0], 0));
+  for (size_t c = 0; c < ch.size(); ++c) {
+    inverse_chunk(h, ch[c].first, ch[c].second, h->side, false);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(h->iev[c], h->side));
+    for (int i = ch[c].first; i < ch[c].first + ch[c].second; ++i) chunk_of[i] = (int)c;
+  }
+  return DH_OK;
+}
+
 }  // extern "C"
 
 // One side of a pobtas sweep: blocks slot0, slot0 + dir, ..., cnt blocks of the handle's factor
@@ -3733,8 +3773,25 @@ struct Sweep {
 // za -= sum_i L_{a,i} y_i (the sides' sums subtracted in side order).  The coupling of two
 // neighbouring slots sits at the lower one.  L2, z2 (one side): a second previous block for its
 // first block (the junction of a two-ended factor), y -= L2 z2.
+// The inverse of the diagonal factors formed on the side stream chunk by chunk (`inverse_beside`):
+// before the first sweep step that reads block i's inverse, the main stream waits for the event of
+// i's chunk (once per chunk).
+struct InvGate {
+  const std::vector<int>* chunk_of;
+  std::vector<char>* waited;
+  const std::vector<hipEvent_t>* ev;
+  mutable hipError_t err;
+  void wait(hipStream_t st, int i) const {
+    const int c = (*chunk_of)[i];
+    if ((*waited)[c]) return;
+    const hipError_t e = hipStreamWaitEvent(st, (*ev)[c], 0);
+    if (e != hipSuccess) err = e;
+    (*waited)[c] = 1;
+  }
+};
+
 static void sweep_forward(dh_handle* h, const Sweep* sw, int ns, bool arrow, double* za,
-                          const double* L2 = nullptr, const double* z2 = nullptr) {
+                          const double* L2 = nullptr, const double* z2 = nullptr, const InvGate* gate = nullptr) {
   const int bp = h->bp;
   const long bb = (long)bp * bp;
   const int ncb = (bp + 127) / 128;  // column blocks of the partial GEMVs
@@ -3750,6 +3807,7 @@ static void sweep_forward(dh_handle* h, const Sweep* sw, int ns, bool arrow, dou
       const Sweep& w = sw[q];
       if (l >= w.cnt) continue;
       const int i = w.slot0 + w.dir * l;
+      if (gate) gate->wait(h->stream, i);
       double* zi = w.z + (long)w.dir * l * bp;
       const bool prev = l > 0 || w.edge;
       r[k] = GvRow{prev ? h->lower + std::min(i, i - w.dir) * bb : nullptr, zi - w.dir * bp, zi, w.v};
@@ -3781,9 +3839,21 @@ static void sweep_forward(dh_handle* h, const Sweep* sw, int ns, bool arrow, dou
 }
 
 static void solve_forward(dh_handle* h, double* z, int slot0, int cnt, bool has_prev, bool arrow, double* za,
-                          int dir = 1, const double* L2 = nullptr, const double* z2 = nullptr) {
+                          int dir = 1, const double* L2 = nullptr, const double* z2 = nullptr,
+                          const InvGate* gate = nullptr) {
   const Sweep w{z, slot0, cnt, dir, has_prev, h->v, h->part, h->arrpart};
-  sweep_forward(h, &w, 1, arrow, za, L2, z2);
+  sweep_forward(h, &w, 1, arrow, za, L2, z2, gate);
+}
+
+// After the forward sweep (which reads every block's inverse): the main stream waits for any chunk
+// of the inverse formed beside it that it has not waited for yet, so every later reader on it sees
+// the whole inverse.
+static int inverse_joined(dh_handle* h, const InvGate& gate, bool beside) {
+  if (!beside) return DH_OK;
+  for (int i = 0; i < h->n; ++i) gate.wait(h->stream, i);
+  if (gate.err != hipSuccess) return DH_EHIP;
+  h->inverted = 1;
+  return DH_OK;
 }
 
 // za = inv(L_aa)^T inv(L_aa) za (the tip of both sweeps).
@@ -3868,8 +3938,15 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
   if (!h || !rhs) return DH_EARG;
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
   if (h->factored != sparsity) return DH_ESTATE;
-  int rc = ensure_inverse(h);
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  const bool beside = !h->inverted && h->inv_beside && h->side &&
+                      hipStreamIsCapturing(h->stream, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
+  std::vector<int> chunk_of;
+  int rc = beside ? inverse_beside(h, chunk_of) : ensure_inverse(h);
   if (rc) return rc;
+  std::vector<char> waited(beside ? h->iev.size() : 0, 0);
+  const InvGate gate{&chunk_of, &waited, &h->iev, hipSuccess};
+  const InvGate* g = beside ? &gate : nullptr;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
   const int n = h->n, b = h->b, bp = h->bp, a = arrow ? h->a : 0;
   const long N = (long)n * bp + 64;
@@ -3882,15 +3959,19 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
     double* z = h->z;
     const Sweep two[2] = {Sweep{z, 0, m, 1, false, h->v, h->part, h->arrpart},
                           Sweep{z + (long)(n - 1) * bp, n - 1, n - 1 - m, -1, false, h->v2, h->part2, h->arrpart2}};
-    sweep_forward(h, two, 2, arrow, za);
-    solve_forward(h, z + (long)m * bp, m, 1, true, arrow, za, 1, h->lower + m * bb, z + (long)(m + 1) * bp);
+    sweep_forward(h, two, 2, arrow, za, nullptr, nullptr, g);
+    solve_forward(h, z + (long)m * bp, m, 1, true, arrow, za, 1, h->lower + m * bb, z + (long)(m + 1) * bp, g);
+    rc = inverse_joined(h, gate, beside);
+    if (rc) return rc;
     if (arrow) solve_tip(h, za);
     solve_backward(h, z + (long)m * bp, m, 1, false, arrow, za);
     const Sweep back[2] = {Sweep{z, 0, m, 1, true, h->v, h->part, h->arrpart},
                            Sweep{z + (long)(n - 1) * bp, n - 1, n - 1 - m, -1, true, h->v2, h->part2, h->arrpart2}};
     sweep_backward(h, back, 2, arrow, za);
   } else {
-    solve_forward(h, h->z, 0, n, false, arrow, za);
+    solve_forward(h, h->z, 0, n, false, arrow, za, 1, nullptr, nullptr, g);
+    rc = inverse_joined(h, gate, beside);
+    if (rc) return rc;
     if (arrow) solve_tip(h, za);
     solve_backward(h, h->z, 0, n, false, arrow, za);
   }
@@ -4505,6 +4586,7 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
     h->prof_by_asm = value;
     h->prof_asm = 0;
   }
+  else if (key == 12 && (value == 0 || value == 1)) h->inv_beside = value;
   else if (key == 6 && value >= 0) h->dag_cap = value;
   else if (key == 7 && (value == 0 || value == 1)) {
     if (value == 1) {
@@ -4720,8 +4802,8 @@ int dh_ckpt_timing_read(dh_ckpt* c, double* total_ms, long long* count) {
 }
 
 int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value) {
-  // nothing runs beside the segment factorizations, and segments are factored one-ended
-  if (!c || key == 4 || key == 7) return DH_EARG;
+  // nothing runs beside the segment factorizations or solves, and segments are factored one-ended
+  if (!c || key == 4 || key == 7 || key == 12) return DH_EARG;
   return dh_set_tuning(c->ring, key, value);
 }
 

@@ -44,7 +44,8 @@ from dalia_amd.solver import HipBTASolver
 
 class STInla:
     def __init__(self, data, device="cuda", closed_form_prior: bool = True, overlap: int = 0,
-                 twisted: bool = True, pattern_selinv: bool = True, profile_from_assembly: bool = True):
+                 twisted: bool = True, pattern_selinv: bool = True, profile_from_assembly: bool = True,
+                 inverse_beside_solve: bool = True):
         """overlap: work run beside each factorization on a second stream (`HipBTASolver.set_overlap`),
         at most this mode: 0 none (default), 1 the inverse of the diagonal factors, 2 also the
         selected inversion's products of the factor.  Measured at ns = 2025, nt = 100: the adjoint
@@ -58,7 +59,10 @@ class STInla:
         marginal variances).
         profile_from_assembly: the zero-tile profiles of Qc's couplings recorded by the assembly
         (`HipBTASolver.set_profile_from_assembly`, default) instead of scanned before each
-        factorization (0.66 ms at ns = 2025, nt = 100); the same profiles."""
+        factorization (0.66 ms at ns = 2025, nt = 100); the same profiles.
+        inverse_beside_solve: the inverse of the diagonal factors formed on a second stream beside
+        the solve's forward sweep, which waits for each chunk of it before reading it
+        (`HipBTASolver.set_inverse_beside_solve`, default); bitwise the same results."""
         self.data = data
         self.manifold = getattr(data, "manifold", "plane")
         if self.manifold not in hyper.MANIFOLDS:
@@ -74,6 +78,7 @@ class STInla:
         self.pattern_selinv = bool(pattern_selinv)
         self.solver.set_selected_inversion_pattern(self.pattern_selinv)
         self.solver.set_profile_from_assembly(bool(profile_from_assembly))
+        self.solver.set_inverse_beside_solve(bool(inverse_beside_solve))
         N = self.dm.n_latent
         self.mu = torch.zeros(N, dtype=torch.float64, device="cuda")
         self.yp = torch.zeros(N, dtype=torch.float64, device="cuda")

@@ -239,6 +239,14 @@ class HipBTASolver:
         views between the assembly and the factorization is not seen: leave it off then."""
         _lib.check(self.lib.dh_set_tuning(self.h, 11, 1 if on else 0), "dh_set_tuning")
 
+    def set_inverse_beside_solve(self, on: bool = True) -> None:
+        """The first `solve` after a factorization forms the inverse of the diagonal factors on a
+        second stream, in chunks of blocks ordered as its forward sweep reaches them, and the sweep
+        waits for each chunk before reading it (`dh_set_tuning` key 12, off by default; not under
+        graph capture, where the inverse runs first on the capturing stream).  The same kernels on
+        the same blocks: bitwise the same results."""
+        _lib.check(self.lib.dh_set_tuning(self.h, 12, 1 if on else 0), "dh_set_tuning")
+
     def selected_inversion(self, sparsity: str = "bta") -> None:
         _lib.check(self.lib.dh_selected_inversion(self.h, _check_sparsity(sparsity)), "dh_selected_inversion")
 

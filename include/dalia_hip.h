@@ -256,7 +256,7 @@ dh_handle* dh_ckpt_vectors(dh_ckpt* c);
 int dh_ckpt_set_stream(dh_ckpt* c, void* hip_stream);
 int dh_ckpt_get_info(dh_ckpt* c, int* info);
 /* dh_set_tuning on the context's segment factorizations and recursions (keys 0, 2, 3, 5, 6, 8, 9,
- * 10; keys 4 and 7 are refused; key 11 has no effect: the segments are assembled by the context). */
+ * 10; keys 4, 7 and 12 are refused; key 11 has no effect: the segments are assembled by the context). */
 int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value);
 /* dh_timing_enable / dh_timing_read on the context's segment launches. */
 int dh_ckpt_timing_enable(dh_ckpt* c, int kind);
@@ -348,6 +348,12 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
  *        dh_scatter_coo, dh_assemble_coreg, a change between the one- and two-ended layouts and
  *        dh_multi_cholesky drop them.  Writes to the blocks by other means in between are not
  *        seen.  0 = scan at each factorization (default).
+ * key 12: 1 = the first dh_solve after a factorization forms the inverse of the diagonal factors
+ *        on the handle's second stream, in chunks of blocks in the order its forward sweep reaches
+ *        them (from both ends inward for a two-ended factor), and the sweep waits for each chunk's
+ *        event before its first step that reads it; the calling stream has waited for every chunk
+ *        when dh_solve returns.  Not under stream capture (the inverse then runs first on the
+ *        calling stream).  Bitwise the same results.  0 = the inverse first (default).
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 
