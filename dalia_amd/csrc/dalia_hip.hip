@@ -2911,7 +2911,20 @@ struct dh_handle {
   int inv_beside;                // 1: dh_solve forms the inverse of the diagonal factors on the side stream,
                                  // chunk by chunk, beside its forward sweep (key 12)
   std::vector<hipEvent_t> iev;   // readiness of those chunks
+  int sel_beside;                // 1: dh_selected_inversion runs on the side stream beside the backward
+                                 // sweep of the dh_solve before it (key 13)
+  int bw_pending;                // 1: bev, bstep and bw_start describe the last dh_solve's backward sweep
+                                 // and nothing has touched the factor since
+  std::vector<hipEvent_t> bev;   // recorded after each step of that sweep (step 0: its first block)
+  std::vector<int> bstep;        // block -> the step of that sweep that reads it
+  hipEvent_t bw_start, sel_done; // before that sweep; after the recursion on the side stream
+  bool sel_wait;                 // the recursion's steps wait for bev (set while it runs beside)
 };
+
+// An entry point that reads or writes the factor's storage: a later selected inversion must not run
+// beside the last solve's backward sweep (it would overwrite what this call reads, or read what it
+// writes, out of order with it).
+static inline void bw_cancel(dh_handle* h) { h->bw_pending = 0; }
 
 // Products of the tile-task launches read by columns from transposed copies of the operand tiles.
 static bool cols_for(const dh_handle* h, bool twisted) {
@@ -3181,6 +3194,9 @@ int dh_destroy(dh_handle* h) {
     if (h->tw_order[ar]) (void)hipFree(h->tw_order[ar]);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->iev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->bev) (void)hipEventDestroy(e);
+  if (h->bw_start) (void)hipEventDestroy(h->bw_start);
+  if (h->sel_done) (void)hipEventDestroy(h->sel_done);
   for (hipEvent_t e : h->sev)
     if (e) (void)hipEventDestroy(e);
   if (h->side) (void)hipStreamDestroy(h->side);
@@ -3217,6 +3233,7 @@ static bool model_ok(const dh_handle* h, const dh_model* m) {
 
 int dh_assemble(dh_handle* h, const dh_model* m, const double* params, int sparsity, int with_like) {
   if (!h || !params || !model_ok(h, m)) return DH_EARG;
+  bw_cancel(h);
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
   if (sparsity == DH_SPARSITY_BTA && m->nb != h->a) return DH_EARG;
   const long rows = (long)(2 * h->n - 1) * h->bp;
@@ -3248,6 +3265,7 @@ int dh_assemble(dh_handle* h, const dh_model* m, const double* params, int spars
 
 int dh_assemble_coreg(dh_handle* h, const dh_model* m, int nv, const double* params, int sparsity, int with_like) {
   if (!h || !m || !params || nv < 2 || nv > 3) return DH_EARG;
+  bw_cancel(h);
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
   DhCoreg cm;
   memset(&cm, 0, sizeof(cm));
@@ -3287,6 +3305,7 @@ int dh_scale_indexed(dh_handle* h, const double* x, const int* idx, const double
 int dh_scatter_coo(dh_handle* h, long long nnz, const long long* rows, const long long* cols, const double* vals,
                    int sparsity) {
   if (!h || nnz < 0 || (nnz > 0 && (!rows || !cols || !vals))) return DH_EARG;
+  bw_cancel(h);
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
   hipLaunchKernelGGL(k_fill_bta, dim3(4096), dim3(256), 0, h->stream, h->diag, h->lower, h->arrow, h->tip, h->n,
                      h->b, h->bp, h->a, h->ap);
@@ -3590,6 +3609,7 @@ static void dag_launch(dh_handle* h, const DagArgs& g) {
 
 int dh_cholesky(dh_handle* h, int sparsity) {
   if (!h) return DH_EARG;
+  bw_cancel(h);
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
   const int n = h->n, bp = h->bp, nbt = h->nbt;
@@ -3866,7 +3886,8 @@ static void solve_tip(dh_handle* h, double* za) {
 // x_q - L_{a,i}^T x_a) with q the next block (slot i + dir).  Paired sides take their steps from
 // their last block (next to the junction) outward; a step whose sides differ in having a coupling
 // term runs as two launches.
-static void sweep_backward(dh_handle* h, const Sweep* sw, int ns, bool arrow, const double* za) {
+static int sweep_backward(dh_handle* h, const Sweep* sw, int ns, bool arrow, const double* za,
+                          const hipEvent_t* rec = nullptr) {
   const int bp = h->bp;
   const long bb = (long)bp * bp;
   const int ncb = (bp + 127) / 128;
@@ -3923,13 +3944,29 @@ static void sweep_backward(dh_handle* h, const Sweep* sw, int ns, bool arrow, co
       }
       hipLaunchKernelGGL(k_upk_row2, dim3((bp + 3) / 4, 1, k), dim3(256), 0, h->stream, u[0], u[1], bp);
     }
+    if (rec) HIPCHK(hipEventRecord(rec[kk], h->stream));  // step kk has read its blocks
   }
+  return DH_OK;
 }
 
-static void solve_backward(dh_handle* h, double* z, int slot0, int cnt, bool has_next, bool arrow, const double* za,
-                           int dir = 1) {
+static int solve_backward(dh_handle* h, double* z, int slot0, int cnt, bool has_next, bool arrow, const double* za,
+                          int dir = 1, const hipEvent_t* rec = nullptr) {
   const Sweep w{z, slot0, cnt, dir, has_next, h->v, h->part, h->arrpart};
-  sweep_backward(h, &w, 1, arrow, za);
+  return sweep_backward(h, &w, 1, arrow, za, rec);
+}
+
+// Events of the backward sweep that a selected inversion may run beside (tuning key 13): one per
+// step, plus the one before the sweep and the one after the recursion.
+static int bw_events(dh_handle* h, int steps) {
+  if (!h->bw_start) HIPCHK(hipEventCreateWithFlags(&h->bw_start, hipEventDisableTiming));
+  if (!h->sel_done) HIPCHK(hipEventCreateWithFlags(&h->sel_done, hipEventDisableTiming));
+  while ((int)h->bev.size() < steps) {
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    h->bev.push_back(e);
+  }
+  h->bstep.assign(h->n, 0);
+  return DH_OK;
 }
 
 extern "C" {
@@ -3939,8 +3976,10 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
   if (sparsity != DH_SPARSITY_BT && sparsity != DH_SPARSITY_BTA) return DH_EARG;
   if (h->factored != sparsity) return DH_ESTATE;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  const bool beside = !h->inverted && h->inv_beside && h->side &&
-                      hipStreamIsCapturing(h->stream, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
+  const bool free_run = hipStreamIsCapturing(h->stream, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
+  const bool beside = !h->inverted && h->inv_beside && h->side && free_run;
+  const bool track = h->sel_beside && h->side && free_run;  // events for a selected inversion beside
+  h->bw_pending = 0;
   std::vector<int> chunk_of;
   int rc = beside ? inverse_beside(h, chunk_of) : ensure_inverse(h);
   if (rc) return rc;
@@ -3964,25 +4003,47 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
     rc = inverse_joined(h, gate, beside);
     if (rc) return rc;
     if (arrow) solve_tip(h, za);
-    solve_backward(h, z + (long)m * bp, m, 1, false, arrow, za);
+    // backward step 0 reads block m, step k blocks m - k and m + k (as the recursion's step k writes)
+    if (track) {
+      rc = bw_events(h, 1 + std::max(m, n - 1 - m));
+      if (rc) return rc;
+      for (int k = 0; k <= std::max(m, n - 1 - m); ++k) {
+        if (k <= m) h->bstep[m - k] = k;
+        if (m + k <= n - 1) h->bstep[m + k] = k;
+      }
+      HIPCHK(hipEventRecord(h->bw_start, h->stream));
+    }
+    rc = solve_backward(h, z + (long)m * bp, m, 1, false, arrow, za, 1, track ? h->bev.data() : nullptr);
+    if (rc) return rc;
     const Sweep back[2] = {Sweep{z, 0, m, 1, true, h->v, h->part, h->arrpart},
                            Sweep{z + (long)(n - 1) * bp, n - 1, n - 1 - m, -1, true, h->v2, h->part2, h->arrpart2}};
-    sweep_backward(h, back, 2, arrow, za);
+    rc = sweep_backward(h, back, 2, arrow, za, track ? h->bev.data() + 1 : nullptr);
+    if (rc) return rc;
   } else {
     solve_forward(h, h->z, 0, n, false, arrow, za, 1, nullptr, nullptr, g);
     rc = inverse_joined(h, gate, beside);
     if (rc) return rc;
     if (arrow) solve_tip(h, za);
-    solve_backward(h, h->z, 0, n, false, arrow, za);
+    // backward step k reads block n - 1 - k (as the recursion's step k writes)
+    if (track) {
+      rc = bw_events(h, n);
+      if (rc) return rc;
+      for (int k = 0; k < n; ++k) h->bstep[n - 1 - k] = k;
+      HIPCHK(hipEventRecord(h->bw_start, h->stream));
+    }
+    rc = solve_backward(h, h->z, 0, n, false, arrow, za, 1, track ? h->bev.data() : nullptr);
+    if (rc) return rc;
   }
   const long Nl = (long)n * b + a;
   hipLaunchKernelGGL(k_unpack, dim3((unsigned)((Nl + 255) / 256)), dim3(256), 0, h->stream, h->z, rhs, n, b, bp, a);
   HIPCHK(hipGetLastError());
+  if (track) h->bw_pending = 1;
   return DH_OK;
 }
 
 int dh_logdet(dh_handle* h, int sparsity, double* out_dev) {
   if (!h || !out_dev) return DH_EARG;
+  bw_cancel(h);
   if (h->factored < 0) return DH_ESTATE;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
   hipLaunchKernelGGL(k_logdet_part, dim3(h->n), dim3(256), 0, h->stream, h->diag, (long)h->bp * h->bp, h->bp,
@@ -4299,6 +4360,8 @@ static int selinv_cong(dh_handle* h, bool arrow, const std::vector<SelBlk>& blk,
   for (size_t s = 0; s + 1 < step0.size(); ++s) {
     const int c = step0[s + 1] - step0[s];
     if (c < 1 || c > 2 || 2 * h->G < 2 * c) return DH_EARG;
+    if (h->sel_wait)  // beside the solve's backward sweep: its step that reads these blocks is done
+      for (int q = step0[s]; q < step0[s + 1]; ++q) HIPCHK(hipStreamWaitEvent(h->stream, h->bev[h->bstep[blk[q].i]], 0));
     SelCArgs a0, a1;
     selc_args(h, blk[step0[s]], arrow, 0, a0);
     if (c > 1) selc_args(h, blk[step0[s] + 1], arrow, 1, a1);
@@ -4364,11 +4427,30 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
   int rc = ensure_inverse(h);
   if (rc) return rc;
   const bool arrow = (sparsity == DH_SPARSITY_BTA) && h->a > 0;
-  if (arrow) selinv_tip(h);
   // the congruence form needs the profiles of the current factor (they bound the tiles it forms)
   const bool cong = h->sel_cong && h->prof_valid && h->fp_ready != sparsity;
+  // Beside the backward sweep of the dh_solve before it (tuning key 13), on the side stream: after
+  // everything the calling stream held before that sweep, each step after the sweep's step that
+  // reads its blocks (the recursion overwrites them); the calling stream waits for the end.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  const bool beside = h->bw_pending && cong && h->side && hipStreamIsCapturing(h->stream, &cap) == hipSuccess &&
+                      cap == hipStreamCaptureStatusNone;
+  h->bw_pending = 0;
+  hipStream_t caller = h->stream;
+  if (beside) {
+    HIPCHK(hipStreamWaitEvent(h->side, h->bw_start, 0));
+    h->stream = h->side;
+    h->sel_wait = true;
+  }
+  if (arrow) selinv_tip(h);
   if (h->fact_tw >= 0) rc = selinv_twisted(h, arrow, cong);
   else rc = selinv_blocks(h, sparsity, arrow, 0, h->n - 1);
+  if (beside) {
+    h->sel_wait = false;
+    h->stream = caller;
+    if (hipEventRecord(h->sel_done, h->side) != hipSuccess || hipStreamWaitEvent(caller, h->sel_done, 0) != hipSuccess)
+      rc = DH_EHIP;
+  }
   h->fact_tw = -1;
   h->factored = -1;
   h->inverted = 0;
@@ -4378,6 +4460,7 @@ int dh_selected_inversion(dh_handle* h, int sparsity) {
 
 int dh_marginal_variances(dh_handle* h, double* out) {
   if (!h || !out) return DH_EARG;
+  bw_cancel(h);
   const long len = (long)h->n * h->b + h->a;
   hipLaunchKernelGGL(k_diag_extract, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, h->stream, h->diag, h->tip,
                      h->n, h->b, h->bp, h->a, out);
@@ -4389,6 +4472,7 @@ int dh_obs_variances(dh_handle* h, long long n_obs, const long long* rowptr, con
                      const double* xcov, double* out) {
   if (!h || n_obs < 0 || (n_obs > 0 && (!rowptr || !col || !val || !out)) || (h->a > 0 && n_obs > 0 && !xcov))
     return DH_EARG;
+  bw_cancel(h);
   if (n_obs == 0) return DH_OK;
   hipLaunchKernelGGL(k_obs_variances, dim3((unsigned)((n_obs + 255) / 256)), dim3(256), 0, h->stream, h->diag,
                      h->lower, h->arrow, h->tip, h->n, h->b, h->bp, h->a, (long)n_obs, rowptr, col, val, xcov, out);
@@ -4497,11 +4581,13 @@ static bool model_ok_v(const dh_handle* h, const dh_model* m, int voff, int aoff
 
 int dh_contract_fem(dh_handle* h, const dh_model* m, double* out_dev) {
   if (!h || !out_dev || !model_ok(h, m)) return DH_EARG;
+  bw_cancel(h);
   return contract_fem(h, m, 0, 0, out_dev);
 }
 
 int dh_contract_fem_v(dh_handle* h, const dh_model* m, int roff, int coff, double* out_dev) {
   if (!h || !out_dev || !model_ok_v(h, m, roff, 0) || !model_ok_v(h, m, coff, 0)) return DH_EARG;
+  bw_cancel(h);
   return contract_fem(h, m, roff, coff, out_dev);
 }
 
@@ -4519,11 +4605,13 @@ int dh_contract_fem_vec_v(dh_handle* h, const dh_model* m, const double* x, int 
 
 int dh_contract_ata(dh_handle* h, const dh_model* m, double* out_dev) {
   if (!h || !out_dev || !model_ok(h, m)) return DH_EARG;
+  bw_cancel(h);
   return contract_ata(h, m, 0, 0, h->a > 0 && m->nb == h->a, out_dev);
 }
 
 int dh_contract_ata_v(dh_handle* h, const dh_model* m, int voff, int aoff, double* out_dev) {
   if (!h || !out_dev || !model_ok_v(h, m, voff, aoff)) return DH_EARG;
+  bw_cancel(h);
   return contract_ata(h, m, voff, aoff, h->a > 0 && m->nb > 0, out_dev);
 }
 
@@ -4576,6 +4664,7 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
 
 int dh_set_tuning(dh_handle* h, int key, int value) {
   if (!h) return DH_EARG;
+  bw_cancel(h);
   if (key == 0 && (value == 0 || value == 1)) h->syrk_variant = value;
   else if (key == 2 && (value == 0 || value == 1)) h->pobtaf_alg = value;
   else if (key == 3 && (value == 0 || value == 1)) h->side_on = value;
@@ -4587,6 +4676,7 @@ int dh_set_tuning(dh_handle* h, int key, int value) {
     h->prof_asm = 0;
   }
   else if (key == 12 && (value == 0 || value == 1)) h->inv_beside = value;
+  else if (key == 13 && (value == 0 || value == 1)) h->sel_beside = value;
   else if (key == 6 && value >= 0) h->dag_cap = value;
   else if (key == 7 && (value == 0 || value == 1)) {
     if (value == 1) {
@@ -4803,7 +4893,7 @@ int dh_ckpt_timing_read(dh_ckpt* c, double* total_ms, long long* count) {
 
 int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value) {
   // nothing runs beside the segment factorizations or solves, and segments are factored one-ended
-  if (!c || key == 4 || key == 7 || key == 12) return DH_EARG;
+  if (!c || key == 4 || key == 7 || key == 12 || key == 13) return DH_EARG;
   return dh_set_tuning(c->ring, key, value);
 }
 
@@ -5057,6 +5147,7 @@ int dh_multi_cholesky(dh_multi* mc, int sparsity, void* hip_stream) {
       hipLaunchKernelGGL(k_leaf, dim3(1), dim3(256), 0, st, h->tip, (long)h->ap, h->tipdinv, h->tipinv, h->info,
                          h->n * h->b);
     h->factored = sparsity;
+    bw_cancel(h);
     h->inverted = 0;
   }
   HIPCHK(hipGetLastError());

@@ -45,7 +45,7 @@ from dalia_amd.solver import HipBTASolver
 class STInla:
     def __init__(self, data, device="cuda", closed_form_prior: bool = True, overlap: int = 0,
                  twisted: bool = True, pattern_selinv: bool = True, profile_from_assembly: bool = True,
-                 inverse_beside_solve: bool = True):
+                 inverse_beside_solve: bool = True, selinv_beside_solve: bool = True):
         """overlap: work run beside each factorization on a second stream (`HipBTASolver.set_overlap`),
         at most this mode: 0 none (default), 1 the inverse of the diagonal factors, 2 also the
         selected inversion's products of the factor.  Measured at ns = 2025, nt = 100: the adjoint
@@ -62,7 +62,10 @@ class STInla:
         factorization (0.66 ms at ns = 2025, nt = 100); the same profiles.
         inverse_beside_solve: the inverse of the diagonal factors formed on a second stream beside
         the solve's forward sweep, which waits for each chunk of it before reading it
-        (`HipBTASolver.set_inverse_beside_solve`, default); bitwise the same results."""
+        (`HipBTASolver.set_inverse_beside_solve`, default); bitwise the same results.
+        selinv_beside_solve: the selected inversion of the gradient on a second stream beside the
+        solve's backward sweep, each recursion step after the sweep's step that reads its blocks
+        (`HipBTASolver.set_selinv_beside_solve`, default); bitwise the same results."""
         self.data = data
         self.manifold = getattr(data, "manifold", "plane")
         if self.manifold not in hyper.MANIFOLDS:
@@ -79,6 +82,7 @@ class STInla:
         self.solver.set_selected_inversion_pattern(self.pattern_selinv)
         self.solver.set_profile_from_assembly(bool(profile_from_assembly))
         self.solver.set_inverse_beside_solve(bool(inverse_beside_solve))
+        self.solver.set_selinv_beside_solve(bool(selinv_beside_solve))
         N = self.dm.n_latent
         self.mu = torch.zeros(N, dtype=torch.float64, device="cuda")
         self.yp = torch.zeros(N, dtype=torch.float64, device="cuda")
@@ -157,8 +161,10 @@ class STInla:
         # mu = e^tau A^T y: scale by params[12] on the device (a 1x1 "scale" kernel via dot-free copy)
         _lib.check(L.dh_scale_dev(self.h, self.dm.aty.data_ptr(), self.params[12:13].data_ptr(),
                                   self.mu.data_ptr(), N), "dh_scale_dev")
-        _lib.check(L.dh_solve(self.h, self.mu.data_ptr(), 1), "dh_solve")
+        # logdet first: it reads the factor, so after the solve it would keep a selected inversion
+        # from running beside the solve's backward sweep
         _lib.check(L.dh_logdet(self.h, 1, out[0:1].data_ptr()), "dh_logdet")
+        _lib.check(L.dh_solve(self.h, self.mu.data_ptr(), 1), "dh_solve")
         if terms:
             self._mu_terms(out)
 

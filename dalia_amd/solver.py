@@ -247,6 +247,15 @@ class HipBTASolver:
         the same blocks: bitwise the same results."""
         _lib.check(self.lib.dh_set_tuning(self.h, 12, 1 if on else 0), "dh_set_tuning")
 
+    def set_selinv_beside_solve(self, on: bool = True) -> None:
+        """A `selected_inversion` after a `solve` runs on a second stream beside the solve's backward
+        sweep, each recursion step after the sweep's step that reads the blocks it overwrites
+        (`dh_set_tuning` key 13, off by default; congruence form only, not under graph capture).
+        Work queued on the stream between the two calls runs beside the recursion and must not read
+        the factor: `logdet` and the other library calls that do cancel the arrangement (the
+        recursion then follows them), reads of the block views in between are not seen."""
+        _lib.check(self.lib.dh_set_tuning(self.h, 13, 1 if on else 0), "dh_set_tuning")
+
     def selected_inversion(self, sparsity: str = "bta") -> None:
         _lib.check(self.lib.dh_selected_inversion(self.h, _check_sparsity(sparsity)), "dh_selected_inversion")
 

@@ -256,7 +256,7 @@ dh_handle* dh_ckpt_vectors(dh_ckpt* c);
 int dh_ckpt_set_stream(dh_ckpt* c, void* hip_stream);
 int dh_ckpt_get_info(dh_ckpt* c, int* info);
 /* dh_set_tuning on the context's segment factorizations and recursions (keys 0, 2, 3, 5, 6, 8, 9,
- * 10; keys 4, 7 and 12 are refused; key 11 has no effect: the segments are assembled by the context). */
+ * 10; keys 4, 7, 12 and 13 are refused; key 11 has no effect: the segments are assembled by the context). */
 int dh_ckpt_set_tuning(dh_ckpt* c, int key, int value);
 /* dh_timing_enable / dh_timing_read on the context's segment launches. */
 int dh_ckpt_timing_enable(dh_ckpt* c, int kind);
@@ -354,6 +354,17 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
  *        event before its first step that reads it; the calling stream has waited for every chunk
  *        when dh_solve returns.  Not under stream capture (the inverse then runs first on the
  *        calling stream).  Bitwise the same results.  0 = the inverse first (default).
+ * key 13: 1 = dh_solve records an event after each step of its backward sweep, and a
+ *        dh_selected_inversion that follows it (congruence form, key 10) runs on the handle's second
+ *        stream: after everything the calling stream held before that sweep, each recursion step
+ *        after the sweep's step that reads the blocks it overwrites; the calling stream waits for
+ *        the recursion before the call returns.  Work that the caller enqueues on its stream
+ *        between the two calls runs beside the recursion, so it must not touch the factor's
+ *        storage: the entry points that do (dh_logdet, dh_marginal_variances, dh_obs_variances,
+ *        the dh_contract_fem / dh_contract_ata forms, the assemblies, dh_scatter_coo, dh_cholesky,
+ *        dh_multi_cholesky, dh_set_tuning) cancel the arrangement, and the recursion then runs on
+ *        the calling stream after them; reads through the block views by other means are not
+ *        seen.  Not under stream capture.  Bitwise the same results.  0 = off (default).
  * Other keys or values return DH_EARG.  No counterpart in the reference. */
 int dh_set_tuning(dh_handle* h, int key, int value);
 
