@@ -3886,8 +3886,14 @@ static void solve_tip(dh_handle* h, double* za) {
 // x_q - L_{a,i}^T x_a) with q the next block (slot i + dir).  Paired sides take their steps from
 // their last block (next to the junction) outward; a step whose sides differ in having a coupling
 // term runs as two launches.
+// The steps of a backward sweep after which an event is recorded for a selected inversion beside it
+// (step k of steps in all): the first two, every fourth, the last.  The recursion falls behind the
+// sweep after its first steps, so its later steps need not wait for exactly their own.
+__host__ inline bool bw_recorded(int k, int steps) { return k < 2 || k % 4 == 3 || k == steps - 1; }
+
+// rec: events indexed by the step over the whole sweep, rec0 + kk for this call's step kk (of steps).
 static int sweep_backward(dh_handle* h, const Sweep* sw, int ns, bool arrow, const double* za,
-                          const hipEvent_t* rec = nullptr) {
+                          const hipEvent_t* rec = nullptr, int rec0 = 0, int steps = 0) {
   const int bp = h->bp;
   const long bb = (long)bp * bp;
   const int ncb = (bp + 127) / 128;
@@ -3944,15 +3950,15 @@ static int sweep_backward(dh_handle* h, const Sweep* sw, int ns, bool arrow, con
       }
       hipLaunchKernelGGL(k_upk_row2, dim3((bp + 3) / 4, 1, k), dim3(256), 0, h->stream, u[0], u[1], bp);
     }
-    if (rec) HIPCHK(hipEventRecord(rec[kk], h->stream));  // step kk has read its blocks
+    if (rec && bw_recorded(rec0 + kk, steps)) HIPCHK(hipEventRecord(rec[rec0 + kk], h->stream));
   }
   return DH_OK;
 }
 
 static int solve_backward(dh_handle* h, double* z, int slot0, int cnt, bool has_next, bool arrow, const double* za,
-                          int dir = 1, const hipEvent_t* rec = nullptr) {
+                          int dir = 1, const hipEvent_t* rec = nullptr, int rec0 = 0, int steps = 0) {
   const Sweep w{z, slot0, cnt, dir, has_next, h->v, h->part, h->arrpart};
-  return sweep_backward(h, &w, 1, arrow, za, rec);
+  return sweep_backward(h, &w, 1, arrow, za, rec, rec0, steps);
 }
 
 // Events of the backward sweep that a selected inversion may run beside (tuning key 13): one per
@@ -4007,17 +4013,21 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
     if (track) {
       rc = bw_events(h, 1 + std::max(m, n - 1 - m));
       if (rc) return rc;
-      for (int k = 0; k <= std::max(m, n - 1 - m); ++k) {
-        if (k <= m) h->bstep[m - k] = k;
-        if (m + k <= n - 1) h->bstep[m + k] = k;
+      const int steps = 1 + std::max(m, n - 1 - m);
+      for (int k = 0; k < steps; ++k) {
+        int r = k;  // the first step at or after k with an event
+        while (!bw_recorded(r, steps)) ++r;
+        if (k <= m) h->bstep[m - k] = r;
+        if (m + k <= n - 1) h->bstep[m + k] = r;
       }
       HIPCHK(hipEventRecord(h->bw_start, h->stream));
     }
-    rc = solve_backward(h, z + (long)m * bp, m, 1, false, arrow, za, 1, track ? h->bev.data() : nullptr);
+    const int bsteps = 1 + std::max(m, n - 1 - m);
+    rc = solve_backward(h, z + (long)m * bp, m, 1, false, arrow, za, 1, track ? h->bev.data() : nullptr, 0, bsteps);
     if (rc) return rc;
     const Sweep back[2] = {Sweep{z, 0, m, 1, true, h->v, h->part, h->arrpart},
                            Sweep{z + (long)(n - 1) * bp, n - 1, n - 1 - m, -1, true, h->v2, h->part2, h->arrpart2}};
-    rc = sweep_backward(h, back, 2, arrow, za, track ? h->bev.data() + 1 : nullptr);
+    rc = sweep_backward(h, back, 2, arrow, za, track ? h->bev.data() : nullptr, 1, bsteps);
     if (rc) return rc;
   } else {
     solve_forward(h, h->z, 0, n, false, arrow, za, 1, nullptr, nullptr, g);
@@ -4028,10 +4038,14 @@ int dh_solve(dh_handle* h, double* rhs, int sparsity) {
     if (track) {
       rc = bw_events(h, n);
       if (rc) return rc;
-      for (int k = 0; k < n; ++k) h->bstep[n - 1 - k] = k;
+      for (int k = 0; k < n; ++k) {
+        int r = k;  // the first step at or after k with an event
+        while (!bw_recorded(r, n)) ++r;
+        h->bstep[n - 1 - k] = r;
+      }
       HIPCHK(hipEventRecord(h->bw_start, h->stream));
     }
-    rc = solve_backward(h, h->z, 0, n, false, arrow, za, 1, track ? h->bev.data() : nullptr);
+    rc = solve_backward(h, h->z, 0, n, false, arrow, za, 1, track ? h->bev.data() : nullptr, 0, n);
     if (rc) return rc;
   }
   const long Nl = (long)n * b + a;

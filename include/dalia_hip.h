@@ -354,10 +354,12 @@ int dh_dag_table(int nbt, int last, int arrow, int* out, int cap, int* look, int
  *        event before its first step that reads it; the calling stream has waited for every chunk
  *        when dh_solve returns.  Not under stream capture (the inverse then runs first on the
  *        calling stream).  Bitwise the same results.  0 = the inverse first (default).
- * key 13: 1 = dh_solve records an event after each step of its backward sweep, and a
+ * key 13: 1 = dh_solve records events during its backward sweep (after its first two steps, every
+ *        fourth and the last), and a
  *        dh_selected_inversion that follows it (congruence form, key 10) runs on the handle's second
  *        stream: after everything the calling stream held before that sweep, each recursion step
- *        after the sweep's step that reads the blocks it overwrites; the calling stream waits for
+ *        after the first recorded sweep step at or after the one that reads the blocks it
+ *        overwrites; the calling stream waits for
  *        the recursion before the call returns.  Work that the caller enqueues on its stream
  *        between the two calls runs beside the recursion, so it must not touch the factor's
  *        storage: the entry points that do (dh_logdet, dh_marginal_variances, dh_obs_variances,
