@@ -502,19 +502,13 @@ def bench_ckpt(args):
     prof = structural_profile(data)
     f_prof = lambda n: f_pobtaf_profile(n, ns, nb, [prof] * (n - 1))
     per_block = f_prof(3) - f_prof(2)
-    exec_eval = (f_prof(nt) - nb ** 3 / 3) + (S - 1) * K * per_block
-    # the algorithmic count: the dense BTA Cholesky F_pobtaf of the factorizations run
-    flops_eval = (f_pobtaf(nt, ns, nb) - nb ** 3 / 3) + (S - 1) * K * (f_pobtaf(3, ns, nb) - f_pobtaf(2, ns, nb))
+    flops_eval = (f_prof(nt) - nb ** 3 / 3) + (S - 1) * K * per_block
     t_launch = tot.value * 1e-3 / max(cnt.value, 1)
     achieved = flops_eval * args.steps / (tot.value * 1e-3) / 1e12
-    achieved_exec = exec_eval * args.steps / (tot.value * 1e-3) / 1e12
     roofline = {"kernel": "k_pobtaf_dag", "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                 "launches": int(cnt.value), "avg_launch_us": t_launch * 1e6,
-                "flops_per_launch": flops_eval * args.steps / max(cnt.value, 1),
-                "flops_count": "F_pobtaf(n, b, a), the dense BTA Cholesky, of the segments factored",
-                "executed_flops_per_launch": exec_eval * args.steps / max(cnt.value, 1),
-                "achieved_executed": achieved_exec, "frac_executed": achieved_exec / FP64_PEAK_TFLOPS}
+                "flops_per_launch": flops_eval * args.steps / max(cnt.value, 1)}
     cpu = None
     if not args.no_cpu:
         threads = min(len(os.sched_getaffinity(0)), 16)
@@ -673,23 +667,17 @@ def main():
     t_pobtaf = float(np.median(times)) * 1e-3
     # algorithmic flops of the factorization as run: the zero tiles of the sub-diagonal blocks
     # (their profile, read back from the library) take part in no product
-    # pobtaf TFLOP/s on the algorithmic count, the dense BTA Cholesky F_pobtaf(n, b, a); with the
-    # profile the launch skips the products with zero tiles, and its rate on the flops it executes
-    # is reported beside
     F_dense = f_pobtaf(nt, ns, nb)
     F = f_pobtaf_profile(nt, ns, nb, tile_profile(eng)) if args.profile else F_dense
-    pobtaf_tflops = F_dense / t_pobtaf / 1e12
-    exec_tflops = F / t_pobtaf / 1e12
+    pobtaf_tflops = F / t_pobtaf / 1e12
     log(f"pobtaf: {t_pobtaf * 1e3:.2f} ms, {pobtaf_tflops:.2f} TFLOP/s ({100 * pobtaf_tflops / FP64_PEAK_TFLOPS:.1f}% "
-        f"of FP64 matrix peak) on F_pobtaf = {F_dense:.4g} flops; it executes {F:.4g} ({F / F_dense:.3f} of them): "
-        f"{exec_tflops:.2f} TFLOP/s on those")
+        f"of FP64 matrix peak) on {F:.4g} flops ({F / F_dense:.3f} of the dense count; dense-equivalent "
+        f"{F_dense / t_pobtaf / 1e12:.2f} TFLOP/s)")
 
     # roofline of the dominant kernel, per launch, from its launches inside the timed steps
     avg_s = tot.value * 1e-3 / max(cnt.value, 1)
-    per_exec = None
     if kname == "k_pobtaf_dag":
-        per = F_dense - nb ** 3 / 3  # the whole factorization except the tip's own Cholesky
-        per_exec = F - nb ** 3 / 3   # the same, as run: without the products with zero tiles
+        per = F - nb ** 3 / 3  # the whole factorization (as run, profile included) except the tip's own Cholesky
     elif kname == "k_syrk" and choice["syrk"] == 1:
         per = float(ns) ** 3  # k_syrk128: the SYRK alone (arrow and tip run in k_syrk_tail)
         kname = "k_syrk128"
@@ -708,10 +696,6 @@ def main():
                 # configurations have none
                 "traffic": None if (tw or args.profile) else committed_traffic(kname, args.workload),
                 "launches": int(cnt.value), "avg_launch_us": avg_s * 1e6, "flops_per_launch": per}
-    if per_exec is not None:
-        roofline.update({"flops_count": "F_pobtaf(n, b, a), the dense BTA Cholesky, less the tip's own Cholesky",
-                         "executed_flops_per_launch": per_exec, "achieved_executed": per_exec / avg_s / 1e12,
-                         "frac_executed": per_exec / avg_s / 1e12 / FP64_PEAK_TFLOPS})
     if tw:
         roofline["timed_span"] = ("two-ended factorization: the two-sided k_pobtaf_dag launch, k_twist_join "
                                   "and block m's k_pobtaf_dag launch, between one event pair")
@@ -780,9 +764,9 @@ def main():
         "pobtaf_ms": t_pobtaf * 1e3,
         "pobtaf_tflops": pobtaf_tflops,
         "pobtaf_frac_of_peak": pobtaf_tflops / FP64_PEAK_TFLOPS,
-        "pobtaf_flops": F_dense,
-        "pobtaf_flops_executed": F,
-        "pobtaf_executed_tflops": exec_tflops,
+        "pobtaf_flops": F,
+        "pobtaf_flops_dense": F_dense,
+        "pobtaf_dense_equiv_tflops": F_dense / t_pobtaf / 1e12,
         **fd_info,
         "roofline": roofline,
         "roofline_hbm": hbm,
