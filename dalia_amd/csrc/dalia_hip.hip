@@ -3765,8 +3765,10 @@ static int inverse_beside(dh_handle* h, std::vector<int>& chunk_of) {
   HIPCHK(hipStreamWaitEvent(h->side, h->sev[0], 0));
   for (size_t c = 0; c < ch.size(); ++c) {
     inverse_chunk(h, ch[c].first, ch[c].second, h->side, false);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(h->iev[c], h->side));
+    if (hipGetLastError() != hipSuccess || hipEventRecord(h->iev[c], h->side) != hipSuccess) {
+      (void)hipStreamSynchronize(h->side);  // no forked work outlives the error
+      return DH_EHIP;
+    }
     for (int i = ch[c].first; i < ch[c].first + ch[c].second; ++i) chunk_of[i] = (int)c;
   }
   return DH_OK;
@@ -3871,7 +3873,10 @@ static void solve_forward(dh_handle* h, double* z, int slot0, int cnt, bool has_
 static int inverse_joined(dh_handle* h, const InvGate& gate, bool beside) {
   if (!beside) return DH_OK;
   for (int i = 0; i < h->n; ++i) gate.wait(h->stream, i);
-  if (gate.err != hipSuccess) return DH_EHIP;
+  if (gate.err != hipSuccess) {
+    (void)hipStreamSynchronize(h->side);  // no forked work outlives the error
+    return DH_EHIP;
+  }
   h->inverted = 1;
   return DH_OK;
 }
